@@ -1,0 +1,203 @@
+/*
+ * qem.h -- C ABI of libqem_hip.so, the MI355X (gfx950) implementation of the
+ * QEM4Molly quasi-Ewald force path (short range: Gauss-Legendre quadrature of
+ * the image-charge Sommerfeld integrals over a 2-D cell list; long range:
+ * random-batch k-space sum with z-sorted prefix sums and the k = 0 term).
+ *
+ * Every entry point replaces a piece of the reference's Molly plugin
+ * interface; the reference file:line it stands in for is given beside it
+ * (paths relative to the QEM4Molly repository root).
+ *
+ * Conventions
+ *   - Every function returns an int status (QEM_OK = 0); no exception or
+ *     abort crosses the ABI.  qem_last_error(ctx) gives the message.
+ *   - Indices are 0-based (a Julia host adds/subtracts 1).
+ *   - Coordinates are N x 3 row-major doubles (x0,y0,z0,x1,...), the memory
+ *     layout of Julia's Vector{SVector{3,Float64}}; charges are N doubles;
+ *     forces are N x 3 doubles, overwritten.
+ *   - Host-pointer entry points are synchronous and validate their inputs.
+ *     *_dev entry points take device pointers and a hipStream_t passed as
+ *     void* (NULL = the context's own stream); they enqueue work and return
+ *     without synchronising, and do not validate array contents.
+ *   - A context is not thread-safe: use one per host thread or stream.
+ *     Device workspaces belong to the context and grow on demand.
+ */
+#ifndef QEM_H
+#define QEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define QEM_OK                 0
+#define QEM_ERR_INVALID_ARG    1  /* null pointer, negative size, bad enum   */
+#define QEM_ERR_DOMAIN         2  /* NaN/Inf input, z outside (0, Lz), ...   */
+#define QEM_ERR_HIP            3  /* HIP runtime error                        */
+#define QEM_ERR_NOMEM          4  /* device or host allocation failed         */
+#define QEM_ERR_UNSUPPORTED    5  /* parameters outside what is implemented   */
+#define QEM_ERR_CAPACITY       6  /* caller buffer too small                  */
+
+/* ---- precision modes ------------------------------------------------------ */
+#define QEM_PREC_FP64   0  /* every operation in double precision             */
+#define QEM_PREC_MIXED  1  /* FP32 nodes, Bessel, exp, sincos; FP64 sums      */
+
+/* ---- semantics switches (0 = the reference as written) --------------------
+ * QEM_SEM_SWAP_LIMITS   gaussian integrals on k_f2, point-charge integrals on
+ *                       k_f1 (src/greens_function.jl:26-27 assigns them the
+ *                       other way round, src/Force_short.jl:48,51,64,67)
+ * QEM_SEM_MINIMAGE_DIR  xy force direction from the minimum-image separation
+ *                       (src/Force_short.jl:57-58 uses raw x_i - x_j)
+ * QEM_SEM_NEWTON_IMAGES z force on j from d/dz_j: images 2 and 3 push i and j
+ *                       the same way (src/QEM_short.jl:47 negates all of F_ij) */
+#define QEM_SEM_SWAP_LIMITS    1u
+#define QEM_SEM_MINIMAGE_DIR   2u
+#define QEM_SEM_NEWTON_IMAGES  4u
+#define QEM_SEM_CORRECTED      7u
+
+/* ---- long-range parts (qem_long_terms) ------------------------------------ */
+#define QEM_LONG_NONSORT  1u  /* Fl_non_sort  src/Force_long.jl:5-70    */
+#define QEM_LONG_SORT     2u  /* Fl_sort      src/Force_long.jl:72-127  */
+#define QEM_LONG_SELF     4u  /* Flz_self     src/Force_long.jl:164-184 */
+#define QEM_LONG_K0       8u  /* Flz_sort_k0  src/Force_long.jl:130-161 */
+#define QEM_LONG_ALL     15u
+
+/* Parameters: the union of the QEM_short and QEM_long keyword arguments
+ * (src/QEM_short.jl:19, src/QEM_long.jl:19) plus precision and semantics. */
+typedef struct qem_params {
+    double Lx, Ly, Lz;      /* box; periodic in x, y; interfaces at z=0, Lz */
+    double r_cutoff;        /* QEM_short r_cutoff                            */
+    double alpha;           /* splitting parameter                           */
+    double gamma_1;         /* dielectric contrast at z = 0  (gamma_down)    */
+    double gamma_2;         /* dielectric contrast at z = Lz (gamma_up)      */
+    double eps_0;
+    double accuracy;        /* sets the quadrature limits and the K set      */
+    int32_t N_t;            /* Gauss-Legendre order, 1..256                  */
+    int32_t iter_period;    /* neighbour-list rebuild period (>= 1)          */
+    int32_t rbe_p;          /* random-batch size P                           */
+    int32_t precision;      /* QEM_PREC_FP64 | QEM_PREC_MIXED                */
+    uint32_t semantics;     /* QEM_SEM_* bitmask                             */
+    int32_t device;         /* HIP device ordinal                            */
+} qem_params;
+
+typedef struct qem_ctx qem_ctx;
+
+/* Reference keyword defaults: iter_period=100, gamma=0, eps_0=1,
+ * accuracy=1e-6, N_t=30 (src/QEM_short.jl:19), rbe_p=30, alpha=1
+ * (src/QEM_long.jl:19); Lx=Ly=Lz=1, r_cutoff=1, FP64, as written. */
+void qem_default_params(qem_params *p);
+
+/* Stands in for the QEM_short / QEM_long constructors
+ * (src/QEM_short.jl:19-25, src/QEM_long.jl:19-30).  The K set
+ * (src/K_set_generator.jl:5-45) is built here when rbe_p > 0. */
+int qem_create(const qem_params *p, qem_ctx **out);
+int qem_destroy(qem_ctx *ctx);
+const char *qem_last_error(const qem_ctx *ctx);
+const char *qem_status_string(int status);
+int qem_get_params(const qem_ctx *ctx, qem_params *p);
+
+/* QEM_inter's derivation of alpha, r_cutoff and k_cutoff from n_atoms and s
+ * (src/Quasi_Ewald_Method.jl:3-24). */
+int qem_inter_params(int64_t n_atoms, double Lx, double Ly, double s,
+                     int32_t rbe_mode, double *alpha, double *r_cutoff,
+                     double *k_cutoff);
+
+/* ---- K set / random batch (src/K_set_generator.jl:5-45,
+ *      src/Force_long.jl:235) ------------------------------------------- */
+int qem_kset_size(qem_ctx *ctx, int64_t *n);
+/* kxkyk: n x 3 (k_x, k_y, |k|); prob: n Float32 weights; sum_K scalar */
+int qem_kset_copy(qem_ctx *ctx, double *kxkyk, float *prob, double *sum_K);
+/* Draws P indices with replacement, probability prob[i]/sum(prob), with a
+ * seeded alias sampler (stands in for StatsBase.sample(K_set, Prob, rbe_p),
+ * src/Force_long.jl:235, whose stream cannot be reproduced outside Julia). */
+int qem_sample_k(qem_ctx *ctx, uint64_t seed, int64_t P, int64_t *k_idx);
+
+/* ---- short range --------------------------------------------------------- */
+/* The 2-D neighbour list (CellListMap.neighborlist at src/QEM_short.jl:34-35):
+ * pairs i < j with minimum-image xy distance <= r_cutoff.  Writes
+ * min(npairs, cap) pairs (2 x int32 each) and sets *npairs to the total. */
+int qem_neighbor_list(qem_ctx *ctx, int64_t n, const double *xyz, int64_t cap,
+                      int32_t *pairs, int64_t *npairs);
+
+/* Molly.forces(::QEM_short, sys) (src/QEM_short.jl:27-59): advances the
+ * context's step counter, rebuilds the neighbour list when
+ * (n_steps - 1) % iter_period == 0 (or when n changes), evaluates every
+ * listed pair still within r_cutoff, adds the self-image term. */
+int qem_short_forces(qem_ctx *ctx, int64_t n, const double *xyz,
+                     const double *q, double *f);
+/* Same accumulation over a caller-given pair list (no step counter). */
+int qem_short_forces_pairs(qem_ctx *ctx, int64_t n, const double *xyz,
+                           const double *q, int64_t npairs,
+                           const int32_t *pairs, double *f);
+/* F_short_ij (src/Force_short.jl:25-78) for m independent pairs:
+ * f_i is m x 3 (force on i), fz_j is m (z force on j; may be NULL). */
+int qem_short_pair(qem_ctx *ctx, int64_t m, const double *qi, const double *qj,
+                   const double *ri, const double *rj, int32_t single,
+                   double *f_i, double *fz_j);
+/* F_short_i (src/Force_short.jl:82-107) for n atoms: f is n x 3. */
+int qem_short_self(qem_ctx *ctx, int64_t n, const double *q,
+                   const double *xyz, int32_t single, double *f);
+
+/* ---- long range ---------------------------------------------------------- */
+/* sortperm of z (src/QEM_long.jl:39-40): ascending, ties by index, 0-based. */
+int qem_sortperm_z(qem_ctx *ctx, int64_t n, const double *xyz, int64_t *zlist);
+/* Molly.forces(::QEM_long, sys) (src/QEM_long.jl:32-44 -> F_long,
+ * src/Force_long.jl:201-242, random-batch branch) with the batch given as
+ * P indices into the K set: F = F_k0 + (sum_K / P) sum_p F_l_total(k_p). */
+int qem_long_forces(qem_ctx *ctx, int64_t n, const double *xyz,
+                    const double *q, const int64_t *k_idx, int64_t P,
+                    double *f);
+/* Explicit k vectors (P x 3: k_x, k_y, |k|): F = [K0] + weight *
+ * sum_p (selected parts of F_l_total(k_p)); weight < 0 requests
+ * exp(-k^2/(4 alpha)) per k (the full-sum weighting).  Covers Fl_non_sort,
+ * Fl_sort, Flz_self, Flz_sort_k0 and F_l_total individually. */
+int qem_long_terms(qem_ctx *ctx, int64_t n, const double *xyz,
+                   const double *q, const double *kvecs, int64_t P,
+                   double weight, uint32_t parts, double *f);
+
+/* ---- both halves (one MD step's electrostatics) ------------------------ */
+int qem_forces(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+               const int64_t *k_idx, int64_t P, double *f);
+
+/* ---- device-pointer variants -------------------------------------------- */
+int qem_short_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
+                         const double *d_q, double *d_f, void *stream);
+/* k_idx is a host array (P is small); it is staged through pinned memory. */
+int qem_long_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
+                        const double *d_q, const int64_t *k_idx, int64_t P,
+                        double *d_f, void *stream);
+int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
+                   const double *d_q, const int64_t *k_idx, int64_t P,
+                   double *d_f, void *stream);
+
+/* ---- introspection ------------------------------------------------------ */
+/* pairs in the current neighbour list, its build count, and the step count */
+int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds,
+                    int64_t *n_steps);
+
+/* Gauss-Legendre rule on [-1, 1], ascending nodes (stands in for
+ * GaussQuadrature.legendre(n), used at src/Gaussian_int.jl:11); 1 <= n <= 256. */
+int qem_gauss_legendre(int32_t n, double *x, double *w);
+/* The FP32 J0/J1 pair used by QEM_PREC_MIXED kernels, evaluated on the GPU
+ * for n arguments in [0, table range] (no counterpart in the reference: it
+ * replaces SpecialFunctions.besselj0/besselj1 at src/Force_short.jl:5-21). */
+int qem_bessel01_mixed(qem_ctx *ctx, int64_t n, const double *x, double *j0,
+                       double *j1);
+
+/* ---- device memory utilities (no counterpart in the reference) ----------
+ * For hosts without their own GPU allocator (ctypes, Julia's ccall). */
+int qem_dev_malloc(void **ptr, size_t bytes);
+int qem_dev_free(void *ptr);
+int qem_memcpy_htod(void *dst, const void *src, size_t bytes);
+int qem_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int qem_device_synchronize(void);
+/* copies bytes with a plain load/store kernel (traffic calibration) */
+int qem_dev_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QEM_H */

@@ -1,0 +1,665 @@
+/*
+ * qem_oracle.c -- CPU FP64 restatement of the QEM4Molly quasi-Ewald force path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+ *
+ * Reference files restated here (QEM4Molly repository, Julia):
+ *   src/Gaussian_int.jl        Guass_int / Gauss_Legendre
+ *   src/greens_function.jl     greens_element*, Gamma_1/2, dz_Gamma_1/2
+ *   src/Force_short.jl         integrands, F_short_ij, F_short_i
+ *   src/QEM_short.jl           Molly.forces(::QEM_short) accumulation
+ *   src/K_set_generator.jl     K_set_generator
+ *   src/Force_long.jl          Fl_non_sort, Fl_sort, Flz_sort_k0, Flz_self,
+ *                              F_l_total, F_long (random-batch branch)
+ *   src/QEM_long.jl            sortperm of z
+ * Operation order follows the Julia expressions where it is cheap to do so;
+ * the only intentional numerical departure is documented at qo_Fl_sort.
+ *
+ * Build: see oracle/Makefile (gcc -O3 -ffp-contract=off -fopenmp).
+ */
+#define _DEFAULT_SOURCE 1 /* j0/j1 are XSI, not ISO C */
+#include "qem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define QO_TWO_PI 6.283185307179586 /* Float64(2pi), as Julia's 2π */
+
+/* ------------------------------------------------------------------------ */
+/* Gauss-Legendre nodes: stands in for GaussQuadrature.legendre(n)
+ * (GaussQuadrature 0.5.8, used at src/Gaussian_int.jl:11).  Newton iteration
+ * on the three-term recurrence of P_n in long double; nodes ascending. */
+int qo_legendre(int n, double *x, double *w)
+{
+    if (n < 1) return -1;
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int i = 0; i < n; ++i) {
+        long double z = cosl(pi * ((long double)i + 0.75L) / ((long double)n + 0.5L));
+        long double dp = 0.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / (long double)k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) { p0 = 1.0L; p1 = z; }
+            dp = (long double)n * (z * p1 - p0) / (z * z - 1.0L);
+            long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        /* recompute derivative at the converged root */
+        {
+            long double p0 = 1.0L, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / (long double)k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) p0 = 1.0L;
+            dp = (long double)n * (z * p1 - p0) / (z * z - 1.0L);
+        }
+        x[n - 1 - i] = (double)z;
+        w[n - 1 - i] = (double)(2.0L / ((1.0L - z * z) * dp * dp));
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* greens_element_ij_init -- src/greens_function.jl:17-30 */
+static double jl_sign(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
+
+static void elem_limits(qo_elem *e, double acc)
+{
+    /* k_f1 = -log(accuracy) ./ (2L_z .+ a)                      (line 26)
+     * k_f2 = sqrt.(4α² a² - 4α log(acc)) .- 2α a                 (line 27) */
+    double la = log(acc);
+    for (int l = 0; l < 4; ++l) {
+        e->k_f1[l] = -la / (2.0 * e->L_z + e->a[l]);
+        e->k_f2[l] = sqrt((4.0 * (e->alpha * e->alpha)) * (e->a[l] * e->a[l])
+                          - (4.0 * e->alpha) * la)
+                     - (2.0 * e->alpha) * e->a[l];
+    }
+}
+
+void qo_elem_ij(double g1, double g2, double zi, double zj, double rho,
+                double Lz, double alpha, double acc, qo_elem *e)
+{
+    double z_n = fabs(zi - zj), z_p = zi + zj;
+    e->gamma_1 = g1; e->gamma_2 = g2; e->rho = rho; e->L_z = Lz; e->alpha = alpha;
+    e->a[0] = z_n; e->a[1] = z_p; e->a[2] = 2.0 * Lz - z_p; e->a[3] = 2.0 * Lz - z_n;
+    e->b[0] = 1.0; e->b[1] = g1; e->b[2] = g2; e->b[3] = g1 * g2;
+    double sg = jl_sign(zi - zj);
+    e->s[0] = -sg; e->s[1] = -1.0; e->s[2] = 1.0; e->s[3] = sg;
+    elem_limits(e, acc);
+}
+
+/* greens_element_i_init -- src/greens_function.jl:33-48 */
+void qo_elem_i(double g1, double g2, double zi, double Lz, double alpha,
+               double acc, qo_elem *e)
+{
+    double z_p = 2.0 * zi;
+    e->gamma_1 = g1; e->gamma_2 = g2; e->rho = 0.0; e->L_z = Lz; e->alpha = alpha;
+    e->a[0] = 0.0; e->a[1] = z_p; e->a[2] = 2.0 * Lz - z_p; e->a[3] = 2.0 * Lz;
+    e->b[0] = 1.0; e->b[1] = g1; e->b[2] = g2; e->b[3] = g1 * g2;
+    e->s[0] = 0.0; e->s[1] = -1.0; e->s[2] = 1.0; e->s[3] = 0.0;
+    elem_limits(e, acc);
+}
+
+/* greens_element_init -- src/greens_function.jl:51-65 (limits unused) */
+void qo_elem_long(double g1, double g2, double Lz, double alpha, qo_elem *e)
+{
+    e->gamma_1 = g1; e->gamma_2 = g2; e->rho = 0.0; e->L_z = Lz; e->alpha = alpha;
+    e->a[0] = 0.0; e->a[1] = 0.0; e->a[2] = 2.0 * Lz; e->a[3] = 2.0 * Lz;
+    e->b[0] = 1.0; e->b[1] = g1; e->b[2] = g2; e->b[3] = g1 * g2;
+    e->s[0] = 0.0; e->s[1] = -1.0; e->s[2] = 1.0; e->s[3] = 0.0;
+    for (int l = 0; l < 4; ++l) { e->k_f1[l] = 0.0; e->k_f2[l] = 0.0; }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gamma_1 / Gamma_2 / dz_Gamma_1 / dz_Gamma_2 -- src/greens_function.jl:68-116 */
+static double green_d(double k, const qo_elem *e)
+{
+    return e->gamma_1 * e->gamma_2 * exp(-2.0 * k * e->L_z) - 1.0;
+}
+
+static double green_u(double k, const qo_elem *e, int l)
+{
+    if (l == 0) {
+        double s = 0.0;
+        for (int i = 0; i < 4; ++i) s += e->b[i] * exp(-k * e->a[i]);
+        return 0.5 * s;
+    }
+    return 0.5 * e->b[l - 1] * exp(-k * e->a[l - 1]);
+}
+
+static double dz_green_u(double k, const qo_elem *e, int l)
+{
+    if (l == 0) {
+        double s = 0.0;
+        for (int i = 0; i < 4; ++i) s += k * e->s[i] * e->b[i] * exp(-k * e->a[i]);
+        return 0.5 * s;
+    }
+    return 0.5 * k * e->s[l - 1] * e->b[l - 1] * exp(-k * e->a[l - 1]);
+}
+
+double qo_Gamma_1(double k, const qo_elem *e, int l)
+{
+    return green_u(k, e, l) / green_d(k, e);
+}
+
+double qo_Gamma_2(double k, const qo_elem *e, int l)
+{
+    return e->gamma_1 * e->gamma_2 * green_u(k, e, l) * exp(-2.0 * k * e->L_z) / green_d(k, e);
+}
+
+double qo_dz_Gamma_1(double k, const qo_elem *e, int l)
+{
+    return dz_green_u(k, e, l) / green_d(k, e);
+}
+
+double qo_dz_Gamma_2(double k, const qo_elem *e, int l)
+{
+    return e->gamma_1 * e->gamma_2 * dz_green_u(k, e, l) * exp(-2.0 * k * e->L_z) / green_d(k, e);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Integrands -- src/Force_short.jl:4-22 */
+enum { F_SR_GAUSS, F_SR_POINT, F_SZ_GAUSS, F_SZ_POINT };
+
+static double integrand(int kind, double k, const qo_elem *e, int l)
+{
+    switch (kind) {
+    case F_SR_GAUSS: return k * qo_Gamma_1(k, e, l) * exp(-(k * k) / (4.0 * e->alpha)) * j1(k * e->rho);
+    case F_SR_POINT: return k * qo_Gamma_2(k, e, l) * j1(k * e->rho);
+    case F_SZ_GAUSS: return qo_dz_Gamma_1(k, e, l) * exp(-(k * k) / (4.0 * e->alpha)) * j0(k * e->rho);
+    default:         return qo_dz_Gamma_2(k, e, l) * j0(k * e->rho);
+    }
+}
+
+/* Guass_int / Gauss_Legendre on region (0, K) -- src/Gaussian_int.jl:9-28.
+ * The nodes are generated once per N_t instead of once per call (line 11);
+ * the values are the same. */
+typedef struct { int n; double x[256]; double w[256]; } gl_rule;
+
+static const gl_rule *get_rule(int n)
+{
+    static gl_rule rules[257];
+    static int ready[257];
+    if (n < 1 || n > 256) return NULL;
+    if (!ready[n]) {
+#ifdef _OPENMP
+#pragma omp critical(qo_rule)
+#endif
+        {
+            if (!ready[n]) {
+                rules[n].n = n;
+                qo_legendre(n, rules[n].x, rules[n].w);
+                ready[n] = 1;
+            }
+        }
+    }
+    return &rules[n];
+}
+
+static double gauss_legendre(int kind, const qo_elem *e, double K, int l, const gl_rule *r)
+{
+    double s = 0.0;
+    for (int i = 0; i < r->n; ++i) {
+        double xi = K / 2.0 + (K * r->x[i]) / 2.0;
+        double wi = (K * r->w[i]) / 2.0;
+        s += integrand(kind, xi, e, l) * wi;
+    }
+    return s;
+}
+
+/* ------------------------------------------------------------------------ */
+static void min_image(double dx, double dy, double Lx, double Ly, double *mx, double *my)
+{
+    *mx = dx - Lx * rint(dx / Lx);
+    *my = dy - Ly * rint(dy / Ly);
+}
+
+/* F_short_ij -- src/Force_short.jl:25-78.  The per-pair 2-point
+ * neighborlist (line 36) reduces to: minimum-image distance, empty if
+ * beyond r_cutoff. */
+void qo_F_short_ij(const qo_params *p, double qi, double qj, const double *ri,
+                   const double *rj, int single, double *out, double *fz_j)
+{
+    out[0] = out[1] = out[2] = 0.0;
+    if (fz_j) *fz_j = 0.0;
+    double mx, my;
+    min_image(ri[0] - rj[0], ri[1] - rj[1], p->Lx, p->Ly, &mx, &my);
+    double d2 = mx * mx + my * my;
+    if (d2 > p->r_cutoff * p->r_cutoff) return;
+    double rho = sqrt(d2);
+
+    const gl_rule *r = get_rule(p->N_t);
+    qo_elem e;
+    qo_elem_ij(p->gamma_1, p->gamma_2, ri[2], rj[2], rho, p->Lz, p->alpha, p->accuracy, &e);
+    const int swap = (p->semantics & QO_SEM_SWAP_LIMITS) != 0;
+    const double *Kp = swap ? e.k_f1 : e.k_f2; /* point-charge integrals */
+    const double *Kg = swap ? e.k_f2 : e.k_f1; /* gaussian integrals     */
+
+    double Fsx = 0.0, Fsy = 0.0;
+    if (rho != 0.0) {
+        double p1 = 0.0, p2 = 0.0, g = 0.0;
+        for (int l = 1; l <= 4; ++l) p1 += gauss_legendre(F_SR_POINT, &e, Kp[l - 1], l, r);
+        for (int l = 0; l < 4; ++l) p2 += e.b[l] * rho / pow(e.a[l] * e.a[l] + rho * rho, 1.5);
+        p2 *= 0.5;
+        if (!single)
+            for (int l = 1; l <= 4; ++l) g += gauss_legendre(F_SR_GAUSS, &e, Kg[l - 1], l, r);
+        double Fsr = -p1 + p2 + g;
+        double dirx, diry;
+        if (p->semantics & QO_SEM_MINIMAGE_DIR) { dirx = mx; diry = my; }
+        else { dirx = ri[0] - rj[0]; diry = ri[1] - rj[1]; }
+        Fsx = Fsr * dirx / rho;
+        Fsy = Fsr * diry / rho;
+    }
+    /* z: keep the per-image parts for the j-side option */
+    double zp1[4], zp2[4], zg[4];
+    double P1 = 0.0, P2 = 0.0, G = 0.0;
+    for (int l = 1; l <= 4; ++l) {
+        zp1[l - 1] = gauss_legendre(F_SZ_POINT, &e, Kp[l - 1], l, r);
+        P1 += zp1[l - 1];
+    }
+    for (int l = 0; l < 4; ++l) {
+        zp2[l] = e.b[l] * e.a[l] * e.s[l] / pow(e.a[l] * e.a[l] + rho * rho, 1.5);
+        P2 += zp2[l];
+    }
+    P2 *= 0.5;
+    for (int l = 1; l <= 4; ++l) {
+        zg[l - 1] = single ? 0.0 : gauss_legendre(F_SZ_GAUSS, &e, Kg[l - 1], l, r);
+        G += zg[l - 1];
+    }
+    double Fsz = P1 - P2 - G;
+    double pref = qi * qj / (QO_TWO_PI * p->eps_0);
+    out[0] = pref * Fsx;
+    out[1] = pref * Fsy;
+    out[2] = pref * Fsz;
+    if (fz_j) {
+        if (p->semantics & QO_SEM_NEWTON_IMAGES) {
+            double part[4];
+            for (int l = 0; l < 4; ++l) part[l] = zp1[l] - 0.5 * zp2[l] - zg[l];
+            *fz_j = pref * (-part[0] + part[1] + part[2] - part[3]);
+        } else {
+            *fz_j = -out[2];
+        }
+    }
+}
+
+/* F_short_i -- src/Force_short.jl:82-107 */
+void qo_F_short_i(const qo_params *p, double qi, const double *ri, int single, double *out)
+{
+    const gl_rule *r = get_rule(p->N_t);
+    qo_elem e;
+    qo_elem_i(p->gamma_1, p->gamma_2, ri[2], p->Lz, p->alpha, p->accuracy, &e);
+    const int swap = (p->semantics & QO_SEM_SWAP_LIMITS) != 0;
+    const double *Kp = swap ? e.k_f1 : e.k_f2;
+    const double *Kg = swap ? e.k_f2 : e.k_f1;
+    double P1 = 0.0, P2 = 0.0, G = 0.0;
+    for (int l = 1; l <= 4; ++l) P1 += gauss_legendre(F_SZ_POINT, &e, Kp[l - 1], l, r);
+    for (int l = 1; l <= 2; ++l) { /* images 2 and 3 (0-based 1, 2) */
+        double a = e.a[l];
+        P2 += e.b[l] * a * e.s[l] / (a * a * a);
+    }
+    P2 *= 0.5;
+    if (!single)
+        for (int l = 1; l <= 4; ++l) G += gauss_legendre(F_SZ_GAUSS, &e, Kg[l - 1], l, r);
+    double Fsz = -P1 + P2 + G;
+    out[0] = 0.0;
+    out[1] = 0.0;
+    out[2] = -(qi * qi) / (QO_TWO_PI * p->eps_0) * Fsz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2-D neighbour pairs: stands in for CellListMap.neighborlist(xy, r_c;
+ * unitcell=[Lx,Ly]) (src/QEM_short.jl:34-35): all unordered pairs with
+ * minimum-image xy distance <= r_c, reported as i < j. */
+int64_t qo_neighbor_pairs(int64_t n, const double *xyz, double Lx, double Ly,
+                          double rc, int32_t *pairs, int64_t cap)
+{
+    int64_t np = 0;
+    double rc2 = rc * rc;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t j = i + 1; j < n; ++j) {
+            double mx, my;
+            min_image(xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1], Lx, Ly, &mx, &my);
+            double d2 = mx * mx + my * my;
+            if (d2 <= rc2) {
+                if (np < cap && pairs) {
+                    pairs[2 * np] = (int32_t)i;
+                    pairs[2 * np + 1] = (int32_t)j;
+                }
+                ++np;
+            }
+        }
+    }
+    return np;
+}
+
+/* ------------------------------------------------------------------------ */
+void qo_short_pair_forces(const qo_params *p, const double *xyz, const double *q,
+                          int64_t npairs, const int32_t *pairs, double *fpair,
+                          int nthreads)
+{
+    (void)get_rule(p->N_t);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int64_t t = 0; t < npairs; ++t) {
+        int32_t i = pairs[2 * t], j = pairs[2 * t + 1];
+        double out[3], fzj;
+        qo_F_short_ij(p, q[i], q[j], xyz + 3 * i, xyz + 3 * j, 0, out, &fzj);
+        fpair[4 * t] = out[0];
+        fpair[4 * t + 1] = out[1];
+        fpair[4 * t + 2] = out[2];
+        fpair[4 * t + 3] = fzj;
+    }
+    (void)nthreads;
+}
+
+/* Molly.forces(::QEM_short) -- src/QEM_short.jl:27-59, with the list given:
+ * F[i] += F_ij; F[j] -= F_ij (lines 41-48), then F[i] += F_short_i (51-56). */
+void qo_short_forces(const qo_params *p, int64_t n, const double *xyz,
+                     const double *q, int64_t npairs, const int32_t *pairs,
+                     double *f, int nthreads)
+{
+    double *fp = (double *)malloc(sizeof(double) * 4 * (size_t)(npairs > 0 ? npairs : 1));
+    double *fs = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    qo_short_pair_forces(p, xyz, q, npairs, pairs, fp, nthreads);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int64_t i = 0; i < n; ++i) qo_F_short_i(p, q[i], xyz + 3 * i, 0, fs + 3 * i);
+    memset(f, 0, sizeof(double) * 3 * (size_t)n);
+    for (int64_t t = 0; t < npairs; ++t) {
+        int32_t i = pairs[2 * t], j = pairs[2 * t + 1];
+        f[3 * i] += fp[4 * t];
+        f[3 * i + 1] += fp[4 * t + 1];
+        f[3 * i + 2] += fp[4 * t + 2];
+        f[3 * j] -= fp[4 * t];
+        f[3 * j + 1] -= fp[4 * t + 1];
+        f[3 * j + 2] += fp[4 * t + 3];
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        f[3 * i + 2] += fs[3 * i + 2];
+    }
+    free(fp);
+    free(fs);
+}
+
+/* ------------------------------------------------------------------------ */
+/* K_set_generator -- src/K_set_generator.jl:5-45 */
+int64_t qo_kset_size(double Lx, double Ly, double alpha, double acc,
+                     int32_t *mx_max, int32_t *my_max)
+{
+    double k_c = 2.0 * sqrt(-4.0 * alpha * log(acc));
+    int32_t mx = (int32_t)ceil(k_c * Lx / QO_TWO_PI) + 1;
+    int32_t my = (int32_t)ceil(k_c * Ly / QO_TWO_PI) + 1;
+    if (mx_max) *mx_max = mx;
+    if (my_max) *my_max = my;
+    return (int64_t)(2 * mx + 1) * (int64_t)(2 * my + 1) - 1;
+}
+
+void qo_kset(double Lx, double Ly, double alpha, double acc, double *kxkyk,
+             float *prob, double *sum_K)
+{
+    int32_t mx_max, my_max;
+    qo_kset_size(Lx, Ly, alpha, acc, &mx_max, &my_max);
+    double S_x = 0.0, S_y = 0.0;
+    for (int32_t m = -mx_max; m <= mx_max; ++m)
+        if (m != 0) {
+            double k_x = (double)m * QO_TWO_PI / Lx;
+            S_x += exp(-(k_x * k_x) / (4.0 * alpha));
+        }
+    for (int32_t m = -my_max; m <= my_max; ++m)
+        if (m != 0) {
+            double k_y = (double)m * QO_TWO_PI / Ly;
+            S_y += exp(-(k_y * k_y) / (4.0 * alpha));
+        }
+    double sK = S_x + S_y + S_x * S_y;
+    int64_t t = 0;
+    for (int32_t mx = -mx_max; mx <= mx_max; ++mx)
+        for (int32_t my = -my_max; my <= my_max; ++my) {
+            double k_x = (double)mx * QO_TWO_PI / Lx;
+            double k_y = (double)my * QO_TWO_PI / Ly;
+            double k = sqrt(k_x * k_x + k_y * k_y);
+            if (k != 0.0) {
+                kxkyk[3 * t] = k_x;
+                kxkyk[3 * t + 1] = k_y;
+                kxkyk[3 * t + 2] = k;
+                prob[t] = (float)(exp(-(k * k) / (4.0 * alpha)) / sK);
+                ++t;
+            }
+        }
+    *sum_K = sK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* sortperm(z) -- src/QEM_long.jl:39-40; ties in index order (stable). */
+static const double *g_sort_xyz;
+static int cmp_z(const void *a, const void *b)
+{
+    int64_t i = *(const int64_t *)a, j = *(const int64_t *)b;
+    double zi = g_sort_xyz[3 * i + 2], zj = g_sort_xyz[3 * j + 2];
+    if (zi < zj) return -1;
+    if (zi > zj) return 1;
+    return (i > j) - (i < j);
+}
+
+void qo_sortperm_z(int64_t n, const double *xyz, int64_t *zlist)
+{
+    for (int64_t i = 0; i < n; ++i) zlist[i] = i;
+#ifdef _OPENMP
+#pragma omp critical(qo_sort)
+#endif
+    {
+        g_sort_xyz = xyz;
+        qsort(zlist, (size_t)n, sizeof(int64_t), cmp_z);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fl_non_sort -- src/Force_long.jl:5-70 */
+void qo_Fl_non_sort(const double *k3, int64_t n, const double *xyz,
+                    const double *q, double g1, double g2, double Lz, double *f)
+{
+    double k_x = k3[0], k_y = k3[1], k = k3[2];
+    double f0[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < n; ++i) {
+        double qi = q[i], xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+        double th = k_x * xi + k_y * yi;
+        f0[0] += qi * cos(th) * exp(+k * zi);
+        f0[1] += qi * sin(th) * exp(+k * zi);
+        f0[2] += qi * cos(th) * exp(-k * zi);
+        f0[3] += qi * sin(th) * exp(-k * zi);
+    }
+    double e2 = exp(-2.0 * k * Lz);
+    double gamma_val = 0.5 / (g1 * g2 * e2 - 1.0);
+    double GP[4] = {g2, 1.0, g1, g1 * g2};
+    for (int64_t i = 0; i < n; ++i) {
+        double qi = q[i], xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+        double th = k_x * xi + k_y * yi;
+        double c = cos(th), s = sin(th), Ep = exp(k * zi), Em = exp(-k * zi);
+        double ff[4];
+        ff[0] = f0[0] - qi * c * exp(+k * zi);
+        ff[1] = f0[1] - qi * s * exp(+k * zi);
+        ff[2] = f0[2] - qi * c * exp(-k * zi);
+        ff[3] = f0[3] - qi * s * exp(-k * zi);
+        double sum_xy = qi * gamma_val *
+            (c * (GP[0] * e2 * Ep * ff[1] + GP[2] * Em * ff[3] + GP[3] * e2 * Em * ff[1] + GP[3] * e2 * Ep * ff[3])
+             - s * (GP[0] * e2 * Ep * ff[0] + GP[2] * Em * ff[2] + GP[3] * e2 * Em * ff[0] + GP[3] * e2 * Ep * ff[2]));
+        double sum_z = qi * gamma_val *
+            (c * (GP[0] * e2 * Ep * ff[0] - GP[2] * Em * ff[2] - GP[3] * e2 * Em * ff[0] + GP[3] * e2 * Ep * ff[2])
+             + s * (GP[0] * e2 * Ep * ff[1] - GP[2] * Em * ff[3] - GP[3] * e2 * Em * ff[1] + GP[3] * e2 * Ep * ff[3]));
+        f[3 * i] = (k_x / k) * sum_xy;
+        f[3 * i + 1] = (k_y / k) * sum_xy;
+        f[3 * i + 2] = sum_z;
+    }
+}
+
+/* Fl_sort -- src/Force_long.jl:72-127.
+ * A, B are the exclusive prefix sums of q cos/sin e^{+kz} in z order, as in
+ * lines 95-96.  C, D are the exclusive suffix sums of q cos/sin e^{-kz}.
+ * The reference forms C, D by subtracting from the total (lines 83-84,
+ * 97-98); in FP64 that loses every digit of C_i e^{+kz_i} once
+ * k (z_i - z_min) exceeds about 36.  qo_Fl_sort accumulates the suffix
+ * from the top (same sums, no cancellation); qo_Fl_sort_subtractive keeps
+ * the reference's rounding for comparison. */
+static void fl_sort_impl(const double *k3, int64_t n, const int64_t *zlist,
+                         const double *xyz, const double *q, double *f, int subtractive)
+{
+    double k_x = k3[0], k_y = k3[1], k = k3[2];
+    double *A = (double *)calloc((size_t)(4 * (n > 0 ? n : 1)), sizeof(double));
+    double *B = A + n, *C = B + n, *D = C + n;
+    if (n == 0) { free(A); return; }
+    if (subtractive) {
+        for (int64_t i = 1; i < n; ++i) {
+            int64_t l = zlist[i];
+            double th = k_x * xyz[3 * l] + k_y * xyz[3 * l + 1];
+            C[0] += q[l] * exp(-k * xyz[3 * l + 2]) * cos(th);
+            D[0] += q[l] * exp(-k * xyz[3 * l + 2]) * sin(th);
+        }
+    }
+    for (int64_t i = 0; i < n - 1; ++i) {
+        int64_t l = zlist[i], lp1 = zlist[i + 1];
+        double th = k_x * xyz[3 * l] + k_y * xyz[3 * l + 1];
+        double th1 = k_x * xyz[3 * lp1] + k_y * xyz[3 * lp1 + 1];
+        A[i + 1] = A[i] + q[l] * exp(+k * xyz[3 * l + 2]) * cos(th);
+        B[i + 1] = B[i] + q[l] * exp(+k * xyz[3 * l + 2]) * sin(th);
+        if (subtractive) {
+            C[i + 1] = C[i] - q[lp1] * exp(-k * xyz[3 * lp1 + 2]) * cos(th1);
+            D[i + 1] = D[i] - q[lp1] * exp(-k * xyz[3 * lp1 + 2]) * sin(th1);
+        }
+    }
+    if (!subtractive) {
+        C[n - 1] = 0.0;
+        D[n - 1] = 0.0;
+        for (int64_t i = n - 2; i >= 0; --i) {
+            int64_t lp1 = zlist[i + 1];
+            double th1 = k_x * xyz[3 * lp1] + k_y * xyz[3 * lp1 + 1];
+            C[i] = C[i + 1] + q[lp1] * exp(-k * xyz[3 * lp1 + 2]) * cos(th1);
+            D[i] = D[i + 1] + q[lp1] * exp(-k * xyz[3 * lp1 + 2]) * sin(th1);
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t l = zlist[i];
+        double qi = q[l], zi = xyz[3 * l + 2];
+        double th = k_x * xyz[3 * l] + k_y * xyz[3 * l + 1];
+        double c = cos(th), s = sin(th);
+        double sum_xy = s * exp(-k * zi) * A[i] - c * exp(-k * zi) * B[i]
+                      + s * exp(k * zi) * C[i] - c * exp(k * zi) * D[i];
+        f[3 * l] = qi * k_x / (2.0 * k) * sum_xy;
+        f[3 * l + 1] = qi * k_y / (2.0 * k) * sum_xy;
+        f[3 * l + 2] = (qi / 2.0) * (c * exp(-k * zi) * A[i] + s * exp(-k * zi) * B[i]
+                                     - c * exp(k * zi) * C[i] - s * exp(k * zi) * D[i]);
+    }
+    free(A);
+}
+
+void qo_Fl_sort(const double *k3, int64_t n, const int64_t *zlist,
+                const double *xyz, const double *q, double *f)
+{
+    fl_sort_impl(k3, n, zlist, xyz, q, f, 0);
+}
+
+void qo_Fl_sort_subtractive(const double *k3, int64_t n, const int64_t *zlist,
+                            const double *xyz, const double *q, double *f)
+{
+    fl_sort_impl(k3, n, zlist, xyz, q, f, 1);
+}
+
+/* Flz_sort_k0 -- src/Force_long.jl:130-161 (z component only; x, y = 0) */
+void qo_Flz_sort_k0(int64_t n, const int64_t *zlist, const double *q, double *f)
+{
+    if (n == 0) return;
+    double *Q1 = (double *)calloc((size_t)(2 * n), sizeof(double));
+    double *Q2 = Q1 + n;
+    for (int64_t i = 1; i < n; ++i) Q2[0] += q[zlist[i]];
+    for (int64_t i = 0; i < n - 1; ++i) Q1[i + 1] = Q1[i] + q[zlist[i]];
+    for (int64_t i = 1; i < n; ++i) Q2[i] = Q2[i - 1] - q[zlist[i]];
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t l = zlist[j];
+        f[3 * l] = 0.0;
+        f[3 * l + 1] = 0.0;
+        f[3 * l + 2] = q[l] * (Q1[j] - Q2[j]);
+    }
+    free(Q1);
+}
+
+/* Flz_self -- src/Force_long.jl:164-184 */
+void qo_Flz_self(const double *k3, int64_t n, const double *xyz,
+                 const double *q, double g1, double g2, double Lz, double *f)
+{
+    double k = k3[2];
+    double gamma_val = g1 * g2 * exp(-2.0 * k * Lz) - 1.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double zi = xyz[3 * i + 2];
+        f[3 * i] = 0.0;
+        f[3 * i + 1] = 0.0;
+        f[3 * i + 2] = q[i] * q[i] * (g2 * exp(-2.0 * k * (Lz - zi)) - g1 * exp(-2.0 * k * zi))
+                       / (2.0 * gamma_val);
+    }
+}
+
+/* F_l_total -- src/Force_long.jl:186-199 */
+void qo_F_l_total(const double *k3, int64_t n, const double *xyz,
+                  const double *q, const int64_t *zlist, const qo_params *p,
+                  double *f)
+{
+    double *a = (double *)malloc(sizeof(double) * 9 * (size_t)(n > 0 ? n : 1));
+    double *b = a + 3 * n, *c = b + 3 * n;
+    qo_Fl_non_sort(k3, n, xyz, q, p->gamma_1, p->gamma_2, p->Lz, a);
+    qo_Fl_sort(k3, n, zlist, xyz, q, b);
+    qo_Flz_self(k3, n, xyz, q, p->gamma_1, p->gamma_2, p->Lz, c);
+    double den = p->Lx * p->Ly * p->eps_0;
+    for (int64_t t = 0; t < 3 * n; ++t) f[t] = (a[t] + b[t] + c[t]) / den;
+    free(a);
+}
+
+/* F_long, random-batch branch -- src/Force_long.jl:201-218, 234-242 */
+void qo_long_forces(const qo_params *p, int64_t n, const double *xyz,
+                    const double *q, const double *kvecs, int64_t P,
+                    double weight, int full_sum, double *f, int nthreads)
+{
+    int64_t *zlist = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    qo_sortperm_z(n, xyz, zlist);
+    memset(f, 0, sizeof(double) * 3 * (size_t)n);
+    qo_Flz_sort_k0(n, zlist, q, f);
+    double den0 = 2.0 * p->Lx * p->Ly * p->eps_0;
+    for (int64_t t = 0; t < 3 * n; ++t) f[t] /= den0;
+    int nt = nthreads > 0 ? nthreads : 1;
+    double *part = (double *)calloc((size_t)nt * 3 * (size_t)n, sizeof(double));
+    /* per-thread partial sums over contiguous k blocks, added in thread order */
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        int tid = 0, nth = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        int64_t lo = P * tid / nth, hi = P * (tid + 1) / nth;
+        double *fk = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+        double *acc = part + (size_t)tid * 3 * (size_t)n;
+        for (int64_t t = lo; t < hi; ++t) {
+            const double *k3 = kvecs + 3 * t;
+            qo_F_l_total(k3, n, xyz, q, zlist, p, fk);
+            double wk = full_sum ? exp(-(k3[2] * k3[2]) / (4.0 * p->alpha)) : weight;
+            for (int64_t u = 0; u < 3 * n; ++u) acc[u] += fk[u] * wk;
+        }
+        free(fk);
+    }
+    for (int th = 0; th < nt; ++th)
+        for (int64_t u = 0; u < 3 * n; ++u) f[u] += part[(size_t)th * 3 * (size_t)n + u];
+    free(part);
+    free(zlist);
+}

@@ -1,0 +1,129 @@
+/*
+ * qem_oracle.h -- CPU FP64 restatement of the QEM4Molly quasi-Ewald force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
+ * loaded only by tests/, by __graft_entry__.smoke() (as the comparison side)
+ * and by bench.py's cpu_baseline leg.  The product library (libqem_hip.so)
+ * never calls it.
+ *
+ * Each function restates one reference function (cited file:line, paths
+ * relative to the QEM4Molly repository root).  The reference is Julia
+ * (Julia 1.8, packages pinned in Manifest.toml: CellListMap 0.8.10,
+ * SpecialFunctions 2.2.0, GaussQuadrature 0.5.8, StatsBase 0.33.21); Julia is
+ * not available, so nothing here is produced by running it.  Third-party
+ * numerics are replaced by published equivalents:
+ *   - GaussQuadrature.legendre(n)  -> Newton iteration on P_n in long double
+ *   - SpecialFunctions.besselj0/1  -> C library j0/j1
+ *   - CellListMap.neighborlist     -> O(N^2) minimum-image enumeration
+ *   - sortperm                     -> stable sort by z, ties by index
+ *   - StatsBase.sample             -> not restated: k indices are inputs
+ */
+#ifndef QEM_ORACLE_H
+#define QEM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Semantic switches.  0 = the reference as written (the parity target). */
+#define QO_SEM_SWAP_LIMITS   1u  /* gauss integrals on k_f2, point on k_f1   */
+#define QO_SEM_MINIMAGE_DIR  2u  /* xy direction from minimum-image dx/rho   */
+#define QO_SEM_NEWTON_IMAGES 4u  /* j-side z force uses d/dz_j (images 2,3)  */
+
+typedef struct qo_params {
+    double Lx, Ly, Lz;
+    double r_cutoff;
+    double alpha;
+    double gamma_1;   /* lower interface (z = 0)  */
+    double gamma_2;   /* upper interface (z = Lz) */
+    double eps_0;
+    double accuracy;
+    int32_t N_t;
+    uint32_t semantics;
+} qo_params;
+
+/* greens_element (src/greens_function.jl:4-15) */
+typedef struct qo_elem {
+    double gamma_1, gamma_2, rho;
+    double a[4], b[4], s[4];
+    double L_z, alpha;
+    double k_f1[4], k_f2[4];
+} qo_elem;
+
+/* Gauss-Legendre nodes/weights on [-1,1], ascending nodes. */
+int qo_legendre(int n, double *x, double *w);
+
+void qo_elem_ij(double g1, double g2, double zi, double zj, double rho,
+                double Lz, double alpha, double acc, qo_elem *e);
+void qo_elem_i(double g1, double g2, double zi, double Lz, double alpha,
+               double acc, qo_elem *e);
+void qo_elem_long(double g1, double g2, double Lz, double alpha, qo_elem *e);
+
+/* l = 0: sum over the four images; l = 1..4: one image (1-based as in Julia) */
+double qo_Gamma_1(double k, const qo_elem *e, int l);
+double qo_Gamma_2(double k, const qo_elem *e, int l);
+double qo_dz_Gamma_1(double k, const qo_elem *e, int l);
+double qo_dz_Gamma_2(double k, const qo_elem *e, int l);
+
+/* F_short_ij: force on i from j.  fz_j receives the z force on j
+ * (as written: -out[2]; with QO_SEM_NEWTON_IMAGES: from d/dz_j). */
+void qo_F_short_ij(const qo_params *p, double qi, double qj, const double *ri,
+                   const double *rj, int single, double *out, double *fz_j);
+void qo_F_short_i(const qo_params *p, double qi, const double *ri, int single,
+                  double *out);
+
+/* O(N^2) minimum-image 2-D pair enumeration, i < j, row-major order.
+ * Returns the number of pairs; writes at most cap pairs (2 int32 each). */
+int64_t qo_neighbor_pairs(int64_t n, const double *xyz, double Lx, double Ly,
+                          double rc, int32_t *pairs, int64_t cap);
+
+/* Molly.forces(::QEM_short) with an explicit pair list (src/QEM_short.jl:27-59).
+ * nthreads > 1 evaluates pairs with OpenMP; accumulation order is fixed. */
+void qo_short_forces(const qo_params *p, int64_t n, const double *xyz,
+                     const double *q, int64_t npairs, const int32_t *pairs,
+                     double *f, int nthreads);
+/* pair forces only (4 doubles per pair: fx, fy, fz_i, fz_j) for timing */
+void qo_short_pair_forces(const qo_params *p, const double *xyz,
+                          const double *q, int64_t npairs,
+                          const int32_t *pairs, double *fpair, int nthreads);
+
+/* K_set_generator (src/K_set_generator.jl:5-45) */
+int64_t qo_kset_size(double Lx, double Ly, double alpha, double acc,
+                     int32_t *mx_max, int32_t *my_max);
+void qo_kset(double Lx, double Ly, double alpha, double acc, double *kxkyk,
+             float *prob, double *sum_K);
+
+/* sortperm of z (src/QEM_long.jl:39-40), 0-based */
+void qo_sortperm_z(int64_t n, const double *xyz, int64_t *zlist);
+
+/* Long-range pieces for one k = (kx, ky, k); outputs N x 3 (overwritten) */
+void qo_Fl_non_sort(const double *k3, int64_t n, const double *xyz,
+                    const double *q, double g1, double g2, double Lz,
+                    double *f);
+void qo_Fl_sort(const double *k3, int64_t n, const int64_t *zlist,
+                const double *xyz, const double *q, double *f);
+/* the reference's subtractive suffix sums (src/Force_long.jl:83-84,97-98) */
+void qo_Fl_sort_subtractive(const double *k3, int64_t n, const int64_t *zlist,
+                            const double *xyz, const double *q, double *f);
+void qo_Flz_sort_k0(int64_t n, const int64_t *zlist, const double *q,
+                    double *f);
+void qo_Flz_self(const double *k3, int64_t n, const double *xyz,
+                 const double *q, double g1, double g2, double Lz, double *f);
+void qo_F_l_total(const double *k3, int64_t n, const double *xyz,
+                  const double *q, const int64_t *zlist, const qo_params *p,
+                  double *f);
+
+/* F_long restricted to its random-batch branch (src/Force_long.jl:201-218,
+ * 234-242) with the sampled k vectors given explicitly: kvecs is P x 3,
+ * weight = sum_K / rbe_p.  full_sum != 0 instead weights each k by
+ * exp(-k^2/(4 alpha)) (the expectation of the batch estimator). */
+void qo_long_forces(const qo_params *p, int64_t n, const double *xyz,
+                    const double *q, const double *kvecs, int64_t P,
+                    double weight, int full_sum, double *f, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

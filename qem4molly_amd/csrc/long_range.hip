@@ -1,0 +1,455 @@
+// QEM long range on gfx950: random-batch k-space sum with z-sorted prefix
+// sums and the k = 0 term.
+//
+// Reference: Molly.forces(::QEM_long) (src/QEM_long.jl:32-44: sortperm of z,
+// then F_long), F_long (src/Force_long.jl:201-242, random-batch branch),
+// F_l_total (186-199), Fl_non_sort (5-70), Fl_sort (72-127), Flz_sort_k0
+// (130-161), Flz_self (164-184).
+//
+// Layout and passes (atoms in ascending-z order, chunks of 256 = one block):
+//   sort   stable radix sort of z with the atom index as payload; gather
+//          x, y, z, q into z-ordered SoA arrays.
+//   A      per (k, chunk): the four chunk sums q cos e^{+kz}, q sin e^{+kz},
+//          q cos e^{-kz}, q sin e^{-kz} (block reduction, fixed order).
+//   S      per k: exclusive prefix over chunks of the e^{+kz} pair, exclusive
+//          suffix of the e^{-kz} pair (accumulated from the top, so no
+//          subtraction from a total), and the four structure factors.
+//   B      per (chunk, k tile): recompute the per-atom terms, in-block
+//          prefix/suffix scans plus chunk offsets give A, B, C, D of Fl_sort;
+//          the structure factors minus the atom's own term give Fl_non_sort;
+//          Flz_self; weighted and summed over the tile's k in registers.
+//   R      per atom: sum over k tiles, scatter back to input order.
+// The k = 0 term rides along as an extra row with (q, 0, q, 0).
+// Nothing of size N x P is stored.
+#include <hipcub/hipcub.hpp>
+
+#include "qem_internal.h"
+
+namespace qem {
+namespace {
+
+constexpr int CH = 256;  // atoms per chunk (= block size)
+constexpr double kInv2Pi = 0.15915494309189535;
+constexpr double kLog2eD = 1.4426950408889634;
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__global__ void k_fill_f64(double *__restrict__ p, int64_t n, double v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_extract_z(const double *__restrict__ xyz, int64_t n, double *__restrict__ z,
+                            int32_t *__restrict__ idx)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    z[i] = xyz[3 * i + 2];
+    idx[i] = (int32_t)i;
+}
+
+__global__ void k_gather_sorted(const double *__restrict__ xyz, const double *__restrict__ q,
+                                const int32_t *__restrict__ perm, int64_t n,
+                                double *__restrict__ sx, double *__restrict__ sy,
+                                double *__restrict__ sz, double *__restrict__ sq)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t a = perm[r];
+    sx[r] = xyz[3 * a];
+    sy[r] = xyz[3 * a + 1];
+    sz[r] = xyz[3 * a + 2];
+    sq[r] = q[a];
+}
+
+// Per-atom terms of one k: cos, sin of k.rho and e^{+kz}, e^{-kz}.
+template <int PREC>
+__device__ __forceinline__ void atom_terms(double kx, double ky, double k, double x, double y,
+                                           double z, double &c, double &s, double &Ep,
+                                           double &Em)
+{
+    const double th = kx * x + ky * y;
+    if (PREC == QEM_PREC_FP64) {
+        sincos(th, &s, &c);
+        Ep = exp(k * z);
+        Em = exp(-k * z);
+    } else {
+        // phase reduced in FP64 to [-1/2, 1/2] turns, then FP32 sin/cos;
+        // e^{+-kz} = 2^{+-(n+f)} with the fraction in FP32, exponent exact
+        double r = th * kInv2Pi;
+        r -= rint(r);
+        float sf, cf;
+        sincospif((float)(2.0 * r), &sf, &cf);
+        s = sf;
+        c = cf;
+        const double tz = k * z * kLog2eD;
+        const double nf = floor(tz);
+        const float f = (float)(tz - nf);
+        const int ni = (int)nf;
+        Ep = ldexp((double)__builtin_amdgcn_exp2f(f), ni);
+        Em = ldexp((double)__builtin_amdgcn_exp2f(-f), -ni);
+    }
+}
+
+__device__ __forceinline__ double4 add4(double4 a, double4 b)
+{
+    return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+// Pass A: chunk sums for the k rows [p0, p1).
+template <int PREC>
+__global__ __launch_bounds__(CH) void k_long_A(const double4 *__restrict__ kv, int64_t Ptot,
+                                               int kt, const double *__restrict__ sx,
+                                               const double *__restrict__ sy,
+                                               const double *__restrict__ sz,
+                                               const double *__restrict__ sq, int64_t n,
+                                               int nchunk, double4 *__restrict__ partA)
+{
+    __shared__ double4 red[2][CH / 64];
+    const int m = blockIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.y * kt;
+    const int64_t p1 = min(Ptot, p0 + kt);
+    const int64_t r = (int64_t)m * CH + threadIdx.x;
+    const bool live = r < n;
+    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0;
+    const double z = live ? sz[r] : 0.0, q = live ? sq[r] : 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t p = p0; p < p1; ++p) {
+        const double4 kk = kv[p];
+        double4 v;
+        if (kk.z == 0.0) {
+            v = make_double4(q, 0.0, q, 0.0);
+        } else {
+            double c, s, Ep, Em;
+            atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
+            v = make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em);
+            if (!live) v = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            v.x += __shfl_down(v.x, d);
+            v.y += __shfl_down(v.y, d);
+            v.z += __shfl_down(v.z, d);
+            v.w += __shfl_down(v.w, d);
+        }
+        const int b = (int)(p & 1);
+        if (lane == 0) red[b][wv] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double4 t = red[b][0];
+#pragma unroll
+            for (int w = 1; w < CH / 64; ++w) t = add4(t, red[b][w]);
+            partA[p * nchunk + m] = t;
+        }
+    }
+}
+
+// Pass S: per k row, chunk offsets and totals.  One block per row; thread t
+// owns a contiguous run of chunks; run sums are combined in thread order.
+__global__ __launch_bounds__(CH) void k_long_S(const double4 *__restrict__ partA, int64_t Ptot,
+                                               int nchunk, double4 *__restrict__ offA,
+                                               double4 *__restrict__ totA)
+{
+    __shared__ double2 pre[CH], suf[CH];
+    const int64_t p = blockIdx.x;
+    const double4 *pa = partA + p * nchunk;
+    double4 *oa = offA + p * nchunk;
+    const int seg = (nchunk + CH - 1) / CH;
+    const int m0 = min(nchunk, (int)threadIdx.x * seg), m1 = min(nchunk, m0 + seg);
+    double2 sp = make_double2(0.0, 0.0), ss = make_double2(0.0, 0.0);
+    for (int m = m0; m < m1; ++m) {
+        sp.x += pa[m].x;
+        sp.y += pa[m].y;
+    }
+    for (int m = m1 - 1; m >= m0; --m) {
+        ss.x += pa[m].z;
+        ss.y += pa[m].w;
+    }
+    pre[threadIdx.x] = sp;
+    suf[threadIdx.x] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // exclusive prefix over runs (ascending), exclusive suffix (descending)
+        double2 run = make_double2(0.0, 0.0);
+        for (int t = 0; t < CH; ++t) {
+            const double2 v = pre[t];
+            pre[t] = run;
+            run.x += v.x;
+            run.y += v.y;
+        }
+        double2 runs = make_double2(0.0, 0.0);
+        for (int t = CH - 1; t >= 0; --t) {
+            const double2 v = suf[t];
+            suf[t] = runs;
+            runs.x += v.x;
+            runs.y += v.y;
+        }
+        totA[p] = make_double4(run.x, run.y, runs.x, runs.y);
+    }
+    __syncthreads();
+    double2 a = pre[threadIdx.x];
+    for (int m = m0; m < m1; ++m) {
+        oa[m].x = a.x;
+        oa[m].y = a.y;
+        a.x += pa[m].x;
+        a.y += pa[m].y;
+    }
+    double2 b = suf[threadIdx.x];
+    for (int m = m1 - 1; m >= m0; --m) {
+        oa[m].z = b.x;
+        oa[m].w = b.y;
+        b.x += pa[m].z;
+        b.y += pa[m].w;
+    }
+}
+
+// Pass B: forces for one chunk and one tile of k rows.
+template <int PREC>
+__global__ __launch_bounds__(CH) void k_long_B(LongConsts lc, const double4 *__restrict__ kv,
+                                               int64_t Ptot, int kt,
+                                               const double *__restrict__ sx,
+                                               const double *__restrict__ sy,
+                                               const double *__restrict__ sz,
+                                               const double *__restrict__ sq, int64_t n,
+                                               int nchunk, const double4 *__restrict__ offA,
+                                               const double4 *__restrict__ totA,
+                                               double *__restrict__ partB)
+{
+    __shared__ double4 wsum[2][CH / 64];
+    const int m = blockIdx.x;
+    const int tile = blockIdx.y;
+    const int64_t p0 = (int64_t)tile * kt;
+    const int64_t p1 = min(Ptot, p0 + kt);
+    const int64_t r = (int64_t)m * CH + threadIdx.x;
+    const bool live = r < n;
+    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0;
+    const double z = live ? sz[r] : 0.0, q = live ? sq[r] : 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int64_t p = p0; p < p1; ++p) {
+        const double4 kk = kv[p];
+        const bool k0 = kk.z == 0.0;
+        double c = 1.0, s = 0.0, Ep = 1.0, Em = 1.0;
+        double4 v;
+        if (k0) {
+            v = make_double4(q, 0.0, q, 0.0);
+        } else {
+            atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
+            v = live ? make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em)
+                     : make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        // in-wave inclusive prefix (x, y) and inclusive suffix (z, w)
+        double4 inc = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double ux = __shfl_up(inc.x, d), uy = __shfl_up(inc.y, d);
+            const double dz = __shfl_down(inc.z, d), dw = __shfl_down(inc.w, d);
+            if (lane >= d) {
+                inc.x += ux;
+                inc.y += uy;
+            }
+            if (lane + d < 64) {
+                inc.z += dz;
+                inc.w += dw;
+            }
+        }
+        const int b = (int)(p & 1);
+        if (lane == 63) {
+            wsum[b][wv].x = inc.x;
+            wsum[b][wv].y = inc.y;
+        }
+        if (lane == 0) {
+            wsum[b][wv].z = inc.z;
+            wsum[b][wv].w = inc.w;
+        }
+        // exclusive by shifting one lane
+        double ex = __shfl_up(inc.x, 1), ey = __shfl_up(inc.y, 1);
+        double ez = __shfl_down(inc.z, 1), ew = __shfl_down(inc.w, 1);
+        if (lane == 0) ex = ey = 0.0;
+        if (lane == 63) ez = ew = 0.0;
+        __syncthreads();
+        for (int w = 0; w < wv; ++w) {
+            ex += wsum[b][w].x;
+            ey += wsum[b][w].y;
+        }
+        for (int w = CH / 64 - 1; w > wv; --w) {
+            ez += wsum[b][w].z;
+            ew += wsum[b][w].w;
+        }
+        const double4 off = offA[p * nchunk + m];
+        const double A = off.x + ex, B = off.y + ey, C = off.z + ez, D = off.w + ew;
+        if (k0) {
+            // Flz_sort_k0 (src/Force_long.jl:130-161): q (Q1 - Q2) / (2 Lx Ly eps0)
+            if (lc.parts & QEM_LONG_K0) fz += kk.w * (q * (A - C));
+            continue;
+        }
+        const double k = kk.z;
+        double gx = 0.0, gz = 0.0;  // xy magnitude (times k_x/k, k_y/k) and z
+        if (lc.parts & QEM_LONG_NONSORT) {
+            // Fl_non_sort (src/Force_long.jl:5-70), own term removed
+            const double4 T = totA[p];
+            const double f1 = T.x - v.x, f2 = T.y - v.y, f3 = T.z - v.z, f4 = T.w - v.w;
+            const double e2 = exp(-2.0 * k * lc.Lz);
+            const double g = 0.5 / (lc.g12 * e2 - 1.0);
+            const double u = lc.g2 * e2 * Ep + lc.g12 * e2 * Em;
+            const double w_ = lc.g1 * Em + lc.g12 * e2 * Ep;
+            const double uz = lc.g2 * e2 * Ep - lc.g12 * e2 * Em;
+            const double wz = -lc.g1 * Em + lc.g12 * e2 * Ep;
+            gx += q * g * (c * (u * f2 + w_ * f4) - s * (u * f1 + w_ * f3));
+            gz += q * g * (c * (uz * f1 + wz * f3) + s * (uz * f2 + wz * f4));
+        }
+        if (lc.parts & QEM_LONG_SORT) {
+            // Fl_sort (src/Force_long.jl:72-127)
+            const double sxy = s * Em * A - c * Em * B + s * Ep * C - c * Ep * D;
+            gx += 0.5 * q * sxy;
+            gz += 0.5 * q * (c * Em * A + s * Em * B - c * Ep * C - s * Ep * D);
+        }
+        if (lc.parts & QEM_LONG_SELF) {
+            // Flz_self (src/Force_long.jl:164-184)
+            const double e2 = exp(-2.0 * k * lc.Lz);
+            gz += q * q * (lc.g2 * (e2 * Ep * Ep) - lc.g1 * (Em * Em)) / (2.0 * (lc.g12 * e2 - 1.0));
+        }
+        const double wk = kk.w;
+        fx += wk * (kk.x / k) * gx;
+        fy += wk * (kk.y / k) * gx;
+        fz += wk * gz;
+    }
+    if (live) {
+        double *o = partB + ((int64_t)tile * n + r) * 3;
+        o[0] = fx;
+        o[1] = fy;
+        o[2] = fz;
+    }
+}
+
+// Pass R: sum the k tiles and scatter to input order.
+__global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
+                         const int32_t *__restrict__ perm, int accumulate,
+                         double *__restrict__ f)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int t = 0; t < ntile; ++t) {
+        const double *o = partB + ((int64_t)t * n + r) * 3;
+        fx += o[0];
+        fy += o[1];
+        fz += o[2];
+    }
+    const int64_t a = perm[r];
+    if (accumulate) {
+        f[3 * a] += fx;
+        f[3 * a + 1] += fy;
+        f[3 * a + 2] += fz;
+    } else {
+        f[3 * a] = fx;
+        f[3 * a + 1] = fy;
+        f[3 * a + 2] = fz;
+    }
+}
+
+#define LR_CHECK(expr)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            err = std::string("long range: ") + #expr + ": " +                \
+                  hipGetErrorString(e_);                                      \
+            return e_;                                                        \
+        }                                                                     \
+    } while (0)
+
+}  // namespace
+
+hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const double *d_q,
+                        hipStream_t s, std::string &err)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > INT32_MAX) {
+        err = "long range: more than 2^31-1 atoms";
+        return hipErrorInvalidValue;
+    }
+    const size_t nd = sizeof(double) * (size_t)n, ni = sizeof(int32_t) * (size_t)n;
+    LR_CHECK(ensure(lw.keys, nd));
+    LR_CHECK(ensure(lw.keys_sorted, nd));
+    LR_CHECK(ensure(lw.idx, ni));
+    LR_CHECK(ensure(lw.perm, ni));
+    LR_CHECK(ensure(lw.sx, nd));
+    LR_CHECK(ensure(lw.sy, nd));
+    LR_CHECK(ensure(lw.sz, nd));
+    LR_CHECK(ensure(lw.sq, nd));
+    k_extract_z<<<nblk(n, 256), 256, 0, s>>>(d_xyz, n, lw.keys.as<double>(), lw.idx.as<int32_t>());
+    size_t tbytes = 0;
+    LR_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, lw.keys.as<double>(),
+                                                lw.keys_sorted.as<double>(),
+                                                lw.idx.as<int32_t>(), lw.perm.as<int32_t>(),
+                                                (int)n, 0, 64, s));
+    LR_CHECK(ensure(lw.tmp, tbytes));
+    LR_CHECK(hipcub::DeviceRadixSort::SortPairs(lw.tmp.p, tbytes, lw.keys.as<double>(),
+                                                lw.keys_sorted.as<double>(),
+                                                lw.idx.as<int32_t>(), lw.perm.as<int32_t>(),
+                                                (int)n, 0, 64, s));
+    k_gather_sorted<<<nblk(n, 256), 256, 0, s>>>(d_xyz, d_q, lw.perm.as<int32_t>(), n,
+                                                 lw.sx.as<double>(), lw.sy.as<double>(),
+                                                 lw.sz.as<double>(), lw.sq.as<double>());
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                       double *d_f, int accumulate, hipStream_t s, std::string &err)
+{
+    if (n <= 0) return hipSuccess;
+    const int nchunk = (int)((n + CH - 1) / CH);
+    if (Ptot <= 0) {
+        // nothing to add; an overwrite still has to clear the output
+        if (!accumulate) {
+            k_fill_f64<<<nblk(3 * n, 256), 256, 0, s>>>(d_f, 3 * n, 0.0);
+            LR_CHECK(hipGetLastError());
+        }
+        return hipSuccess;
+    }
+    // k tiles: enough blocks to cover the chip, at most one tile per k row
+    int ntile = (int)std::min<int64_t>(Ptot, std::max<int64_t>(1, (2048 + nchunk - 1) / nchunk));
+    const int kt = (int)((Ptot + ntile - 1) / ntile);
+    ntile = (int)((Ptot + kt - 1) / kt);
+    LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * nchunk));
+    LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * nchunk));
+    LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
+    LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)ntile * (size_t)n));
+    const double4 *kv = lw.kv.as<double4>();
+    dim3 gA(nchunk, ntile), gB(nchunk, ntile);
+    if (c.precision == QEM_PREC_MIXED)
+        k_long_A<QEM_PREC_MIXED><<<gA, CH, 0, s>>>(kv, Ptot, kt, lw.sx.as<double>(),
+                                                   lw.sy.as<double>(), lw.sz.as<double>(),
+                                                   lw.sq.as<double>(), n, nchunk,
+                                                   lw.partA.as<double4>());
+    else
+        k_long_A<QEM_PREC_FP64><<<gA, CH, 0, s>>>(kv, Ptot, kt, lw.sx.as<double>(),
+                                                  lw.sy.as<double>(), lw.sz.as<double>(),
+                                                  lw.sq.as<double>(), n, nchunk,
+                                                  lw.partA.as<double4>());
+    LR_CHECK(hipGetLastError());
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
+                                           lw.offA.as<double4>(), lw.totA.as<double4>());
+    LR_CHECK(hipGetLastError());
+    if (c.precision == QEM_PREC_MIXED)
+        k_long_B<QEM_PREC_MIXED><<<gB, CH, 0, s>>>(c, kv, Ptot, kt, lw.sx.as<double>(),
+                                                   lw.sy.as<double>(), lw.sz.as<double>(),
+                                                   lw.sq.as<double>(), n, nchunk,
+                                                   lw.offA.as<double4>(), lw.totA.as<double4>(),
+                                                   lw.partB.as<double>());
+    else
+        k_long_B<QEM_PREC_FP64><<<gB, CH, 0, s>>>(c, kv, Ptot, kt, lw.sx.as<double>(),
+                                                  lw.sy.as<double>(), lw.sz.as<double>(),
+                                                  lw.sq.as<double>(), n, nchunk,
+                                                  lw.offA.as<double4>(), lw.totA.as<double4>(),
+                                                  lw.partB.as<double>());
+    LR_CHECK(hipGetLastError());
+    k_long_R<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ntile, n, lw.perm.as<int32_t>(),
+                                          accumulate, d_f);
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+}  // namespace qem

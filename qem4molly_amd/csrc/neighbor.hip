@@ -1,0 +1,316 @@
+// 2-D periodic cell list and half neighbour list for the QEM short range.
+//
+// Stands in for CellListMap.neighborlist(xy, r_cutoff; unitcell=[Lx, Ly])
+// as called at src/QEM_short.jl:34-35: all unordered pairs whose minimum-
+// image xy distance is <= r_cutoff, reported once with i < j.
+//
+// Layout: atoms are bucketed into an ncx x ncy grid of cells at least
+// r_cutoff wide (stable radix sort of cell ids, so atoms inside a cell stay
+// in index order); one thread per atom walks the 3 x 3 block of cells
+// around its own, first counting and then writing its partners j > i.
+// Pairs are therefore grouped by i, in a deterministic order.  A second
+// stable sort of the pair list by j gives, per atom, the pairs in which it
+// is the second endpoint; the force gather uses both groupings so that the
+// per-atom reduction order is fixed (no atomics).
+#include <hipcub/hipcub.hpp>
+
+#include "qem_internal.h"
+
+namespace qem {
+namespace {
+
+struct CellGrid {
+    double Lx, Ly;
+    int ncx, ncy;
+};
+
+__device__ __forceinline__ int cell_of(double x, double L, int nc)
+{
+    double xw = x - L * floor(x / L);
+    int c = (int)(xw / L * (double)nc);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+// Minimum-image squared xy distance.  Contraction is disabled so the value
+// (and hence the pair set) is bit-identical to a plain IEEE evaluation of
+// dx - L*rint(dx/L), dx*dx + dy*dy.
+__device__ __forceinline__ double pair_d2(double xi, double yi, double xj,
+                                          double yj, double Lx, double Ly)
+{
+#pragma clang fp contract(off)
+    double dx = xi - xj, dy = yi - yj;
+    double mx = dx - Lx * rint(dx / Lx);
+    double my = dy - Ly * rint(dy / Ly);
+    return mx * mx + my * my;
+}
+
+__global__ void k_cell_keys(const double *__restrict__ xyz, int64_t n, CellGrid g,
+                            uint32_t *__restrict__ key, int32_t *__restrict__ idx)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int cx = cell_of(xyz[3 * i], g.Lx, g.ncx);
+    int cy = cell_of(xyz[3 * i + 1], g.Ly, g.ncy);
+    key[i] = (uint32_t)(cy * g.ncx + cx);
+    idx[i] = (int32_t)i;
+}
+
+__global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i64(int64_t *__restrict__ p, int64_t n, int64_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// start/end of each run of equal keys in a sorted key array
+__global__ void k_run_bounds(const uint32_t *__restrict__ skey, int64_t n,
+                             int32_t *__restrict__ start, int32_t *__restrict__ end)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t k = skey[t];
+    if (t == 0 || skey[t - 1] != k) start[k] = (int32_t)t;
+    if (t == n - 1 || skey[t + 1] != k) end[k] = (int32_t)(t + 1);
+}
+
+template <bool FILL>
+__global__ void k_nl_pass(const double *__restrict__ xyz, int64_t n, CellGrid g, double rc2,
+                          const int32_t *__restrict__ cstart, const int32_t *__restrict__ cend,
+                          const int32_t *__restrict__ sidx, int64_t *__restrict__ counts,
+                          const int64_t *__restrict__ row_ptr, int32_t *__restrict__ pi,
+                          int32_t *__restrict__ pj)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = xyz[3 * i], yi = xyz[3 * i + 1];
+    const int cx = cell_of(xi, g.Lx, g.ncx), cy = cell_of(yi, g.Ly, g.ncy);
+    const int nxi = g.ncx >= 3 ? 3 : g.ncx, nyi = g.ncy >= 3 ? 3 : g.ncy;
+    int64_t cnt = 0;
+    const int64_t base = FILL ? row_ptr[i] : 0;
+    for (int oy = 0; oy < nyi; ++oy) {
+        const int cyy = g.ncy >= 3 ? (cy + oy - 1 + g.ncy) % g.ncy : oy;
+        for (int ox = 0; ox < nxi; ++ox) {
+            const int cxx = g.ncx >= 3 ? (cx + ox - 1 + g.ncx) % g.ncx : ox;
+            const int c = cyy * g.ncx + cxx;
+            const int t1 = cend[c];
+            for (int t = cstart[c]; t < t1; ++t) {
+                const int32_t j = sidx[t];
+                if (j <= i) continue;
+                const double d2 = pair_d2(xi, yi, xyz[3 * (int64_t)j], xyz[3 * (int64_t)j + 1],
+                                          g.Lx, g.Ly);
+                if (d2 <= rc2) {
+                    if (FILL) {
+                        pi[base + cnt] = (int32_t)i;
+                        pj[base + cnt] = j;
+                    }
+                    ++cnt;
+                }
+            }
+        }
+    }
+    if (!FILL) counts[i] = cnt;
+}
+
+__global__ void k_row_bounds(const int64_t *__restrict__ row_ptr, int64_t n,
+                             int32_t *__restrict__ start, int32_t *__restrict__ end)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    start[i] = (int32_t)row_ptr[i];
+    end[i] = (int32_t)row_ptr[i + 1];
+}
+
+__global__ void k_iota_u32key(const int32_t *__restrict__ src, int64_t m,
+                              uint32_t *__restrict__ key, int32_t *__restrict__ idx)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    key[t] = (uint32_t)src[t];
+    idx[t] = (int32_t)t;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+inline int bits_for(uint64_t maxval)
+{
+    int b = 1;
+    while (b < 32 && (1ull << b) <= maxval) ++b;
+    return b;
+}
+
+#define NL_CHECK(expr)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            err = std::string("neighbour list: ") + #expr + ": " +            \
+                  hipGetErrorString(e_);                                      \
+            return e_;                                                        \
+        }                                                                     \
+    } while (0)
+
+// Sorts m int32 endpoint values and records, for every atom, the run of
+// pair ids having that endpoint (stable: pair ids ascending inside a run).
+hipError_t group_by_endpoint(NeighborList &nl, const int32_t *d_end, int64_t m, int64_t n,
+                             DevBuf &idx_out, DevBuf &start, DevBuf &end, hipStream_t s,
+                             std::string &err)
+{
+    NL_CHECK(ensure(start, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+    NL_CHECK(ensure(end, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+    if (n > 0) {
+        k_fill_i32<<<nblk(n, 256), 256, 0, s>>>(start.as<int32_t>(), n, 0);
+        k_fill_i32<<<nblk(n, 256), 256, 0, s>>>(end.as<int32_t>(), n, 0);
+    }
+    if (m == 0) return hipSuccess;
+    NL_CHECK(ensure(nl.cell_key, sizeof(uint32_t) * (size_t)m));
+    NL_CHECK(ensure(nl.cell_key_sorted, sizeof(uint32_t) * (size_t)m));
+    NL_CHECK(ensure(nl.atom_idx, sizeof(int32_t) * (size_t)m));
+    NL_CHECK(ensure(idx_out, sizeof(int32_t) * (size_t)m));
+    k_iota_u32key<<<nblk(m, 256), 256, 0, s>>>(d_end, m, nl.cell_key.as<uint32_t>(),
+                                                nl.atom_idx.as<int32_t>());
+    int ebits = bits_for((uint64_t)(n > 0 ? n - 1 : 0));
+    size_t tbytes = 0;
+    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, nl.cell_key.as<uint32_t>(),
+                                                nl.cell_key_sorted.as<uint32_t>(),
+                                                nl.atom_idx.as<int32_t>(), idx_out.as<int32_t>(),
+                                                (int)m, 0, ebits, s));
+    NL_CHECK(ensure(nl.tmp, tbytes));
+    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nl.tmp.p, tbytes, nl.cell_key.as<uint32_t>(),
+                                                nl.cell_key_sorted.as<uint32_t>(),
+                                                nl.atom_idx.as<int32_t>(), idx_out.as<int32_t>(),
+                                                (int)m, 0, ebits, s));
+    k_run_bounds<<<nblk(m, 256), 256, 0, s>>>(nl.cell_key_sorted.as<uint32_t>(), m,
+                                              start.as<int32_t>(), end.as<int32_t>());
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz, double Lx,
+                               double Ly, double rc, hipStream_t s, std::string &err)
+{
+    nl.n = n;
+    nl.npairs = 0;
+    nl.has_i_idx = false;
+    ++nl.builds;
+    NL_CHECK(ensure(nl.i_start, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+    NL_CHECK(ensure(nl.i_end, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+    NL_CHECK(ensure(nl.j_start, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+    NL_CHECK(ensure(nl.j_end, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+    if (n == 0) return hipSuccess;
+    if (n > INT32_MAX) {
+        err = "neighbour list: more than 2^31-1 atoms";
+        return hipErrorInvalidValue;
+    }
+    // cells at least r_cutoff wide (a hair wider so rounding in the cell
+    // index cannot separate a pair by two cells)
+    CellGrid g;
+    g.Lx = Lx;
+    g.Ly = Ly;
+    double wx = rc * (1.0 + 1e-9), wy = rc * (1.0 + 1e-9);
+    g.ncx = (int)std::min(1024.0, std::max(1.0, std::floor(Lx / wx)));
+    g.ncy = (int)std::min(1024.0, std::max(1.0, std::floor(Ly / wy)));
+    const int64_t ncell = (int64_t)g.ncx * g.ncy;
+
+    NL_CHECK(ensure(nl.cell_key, sizeof(uint32_t) * (size_t)n));
+    NL_CHECK(ensure(nl.cell_key_sorted, sizeof(uint32_t) * (size_t)n));
+    NL_CHECK(ensure(nl.atom_idx, sizeof(int32_t) * (size_t)n));
+    NL_CHECK(ensure(nl.atom_idx_sorted, sizeof(int32_t) * (size_t)n));
+    NL_CHECK(ensure(nl.cell_start, sizeof(int32_t) * (size_t)ncell));
+    NL_CHECK(ensure(nl.cell_end, sizeof(int32_t) * (size_t)ncell));
+    NL_CHECK(ensure(nl.counts, sizeof(int64_t) * (size_t)(n + 1)));
+    NL_CHECK(ensure(nl.row_ptr, sizeof(int64_t) * (size_t)(n + 1)));
+
+    k_cell_keys<<<nblk(n, 256), 256, 0, s>>>(d_xyz, n, g, nl.cell_key.as<uint32_t>(),
+                                             nl.atom_idx.as<int32_t>());
+    int kbits = bits_for((uint64_t)(ncell - 1));
+    size_t tbytes = 0;
+    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, nl.cell_key.as<uint32_t>(),
+                                                nl.cell_key_sorted.as<uint32_t>(),
+                                                nl.atom_idx.as<int32_t>(),
+                                                nl.atom_idx_sorted.as<int32_t>(), (int)n, 0,
+                                                kbits, s));
+    NL_CHECK(ensure(nl.tmp, tbytes));
+    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nl.tmp.p, tbytes, nl.cell_key.as<uint32_t>(),
+                                                nl.cell_key_sorted.as<uint32_t>(),
+                                                nl.atom_idx.as<int32_t>(),
+                                                nl.atom_idx_sorted.as<int32_t>(), (int)n, 0,
+                                                kbits, s));
+    k_fill_i32<<<nblk(ncell, 256), 256, 0, s>>>(nl.cell_start.as<int32_t>(), ncell, 0);
+    k_fill_i32<<<nblk(ncell, 256), 256, 0, s>>>(nl.cell_end.as<int32_t>(), ncell, 0);
+    k_run_bounds<<<nblk(n, 256), 256, 0, s>>>(nl.cell_key_sorted.as<uint32_t>(), n,
+                                              nl.cell_start.as<int32_t>(),
+                                              nl.cell_end.as<int32_t>());
+    const double rc2 = rc * rc;
+    // counts[n] = 0 so the exclusive scan's last entry is the total
+    k_fill_i64<<<1, 64, 0, s>>>(nl.counts.as<int64_t>() + n, 1, 0);
+    k_nl_pass<false><<<nblk(n, 128), 128, 0, s>>>(
+        d_xyz, n, g, rc2, nl.cell_start.as<int32_t>(), nl.cell_end.as<int32_t>(),
+        nl.atom_idx_sorted.as<int32_t>(), nl.counts.as<int64_t>(), nullptr, nullptr, nullptr);
+    NL_CHECK(hipGetLastError());
+    tbytes = 0;
+    NL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, nl.counts.as<int64_t>(),
+                                              nl.row_ptr.as<int64_t>(), (int)(n + 1), s));
+    NL_CHECK(ensure(nl.tmp, tbytes));
+    NL_CHECK(hipcub::DeviceScan::ExclusiveSum(nl.tmp.p, tbytes, nl.counts.as<int64_t>(),
+                                              nl.row_ptr.as<int64_t>(), (int)(n + 1), s));
+    int64_t total = 0;
+    NL_CHECK(hipMemcpyAsync(&total, nl.row_ptr.as<int64_t>() + n, sizeof(int64_t),
+                            hipMemcpyDeviceToHost, s));
+    NL_CHECK(hipStreamSynchronize(s));
+    if (total > INT32_MAX) {
+        err = "neighbour list: more than 2^31-1 pairs";
+        return hipErrorInvalidValue;
+    }
+    nl.npairs = total;
+    NL_CHECK(ensure(nl.pi, sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
+    NL_CHECK(ensure(nl.pj, sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
+    if (total > 0) {
+        k_nl_pass<true><<<nblk(n, 128), 128, 0, s>>>(
+            d_xyz, n, g, rc2, nl.cell_start.as<int32_t>(), nl.cell_end.as<int32_t>(),
+            nl.atom_idx_sorted.as<int32_t>(), nullptr, nl.row_ptr.as<int64_t>(),
+            nl.pi.as<int32_t>(), nl.pj.as<int32_t>());
+        NL_CHECK(hipGetLastError());
+    }
+    k_row_bounds<<<nblk(n, 256), 256, 0, s>>>(nl.row_ptr.as<int64_t>(), n,
+                                              nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>());
+    NL_CHECK(hipGetLastError());
+    NL_CHECK(group_by_endpoint(nl, nl.pj.as<int32_t>(), total, n, nl.j_idx, nl.j_start,
+                               nl.j_end, s, err));
+    return hipSuccess;
+}
+
+hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs, int64_t npairs,
+                         hipStream_t s, std::string &err)
+{
+    nl.n = n;
+    nl.npairs = npairs;
+    nl.has_i_idx = true;
+    ++nl.builds;
+    std::vector<int32_t> a((size_t)npairs), b((size_t)npairs);
+    for (int64_t t = 0; t < npairs; ++t) {
+        a[(size_t)t] = h_pairs[2 * t];
+        b[(size_t)t] = h_pairs[2 * t + 1];
+    }
+    NL_CHECK(ensure(nl.pi, sizeof(int32_t) * (size_t)(npairs > 0 ? npairs : 1)));
+    NL_CHECK(ensure(nl.pj, sizeof(int32_t) * (size_t)(npairs > 0 ? npairs : 1)));
+    if (npairs > 0) {
+        NL_CHECK(hipMemcpyAsync(nl.pi.p, a.data(), sizeof(int32_t) * (size_t)npairs,
+                                hipMemcpyHostToDevice, s));
+        NL_CHECK(hipMemcpyAsync(nl.pj.p, b.data(), sizeof(int32_t) * (size_t)npairs,
+                                hipMemcpyHostToDevice, s));
+    }
+    NL_CHECK(group_by_endpoint(nl, nl.pi.as<int32_t>(), npairs, n, nl.i_idx, nl.i_start,
+                               nl.i_end, s, err));
+    NL_CHECK(group_by_endpoint(nl, nl.pj.as<int32_t>(), npairs, n, nl.j_idx, nl.j_start,
+                               nl.j_end, s, err));
+    // the host vectors must outlive the asynchronous copies
+    NL_CHECK(hipStreamSynchronize(s));
+    return hipSuccess;
+}
+
+}  // namespace qem

@@ -1,0 +1,1013 @@
+// C ABI of libqem_hip.so: contexts, parameter checks, host-side tables
+// (Gauss-Legendre rule, K set, alias sampler, Bessel table) and the entry
+// points declared in include/qem.h.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+
+#include "qem_internal.h"
+
+namespace qem {
+
+hipError_t ensure(DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 1;
+    if (b.cap >= bytes) return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    size_t want = bytes + bytes / 4;  // headroom for slowly growing inputs
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return e;
+    }
+    b.cap = want;
+    return hipSuccess;
+}
+
+void release(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// Gauss-Legendre nodes and weights on [-1, 1], ascending (the rule that
+// GaussQuadrature.legendre(n) returns, used at src/Gaussian_int.jl:11).
+// Newton's method on P_n from Tricomi's initial guesses, long double.
+GLRule make_gl_rule(int n)
+{
+    GLRule r;
+    r.n = n;
+    r.x.assign((size_t)n, 0.0);
+    r.w.assign((size_t)n, 0.0);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    const int half = (n + 1) / 2;
+    for (int i = 1; i <= half; ++i) {
+        // i-th largest root
+        long double th = PI * (4.0L * i - 1.0L) / (4.0L * n + 2.0L);
+        long double x = (1.0L - (n - 1.0L) / (8.0L * n * n * n)) * cosl(th);
+        long double pn = 0.0L, dpn = 1.0L;
+        for (int it = 0; it < 64; ++it) {
+            long double pm1 = 1.0L, p = x;  // P_0, P_1
+            for (int k = 2; k <= n; ++k) {
+                long double pk = ((2 * k - 1) * x * p - (k - 1) * pm1) / k;
+                pm1 = p;
+                p = pk;
+            }
+            if (n == 1) pm1 = 1.0L;
+            pn = p;
+            dpn = n * (pm1 - x * p) / (1.0L - x * x);
+            long double step = pn / dpn;
+            x -= step;
+            if (fabsl(step) <= 4e-20L * fabsl(x) + 1e-30L) break;
+        }
+        {
+            long double pm1 = 1.0L, p = x;
+            for (int k = 2; k <= n; ++k) {
+                long double pk = ((2 * k - 1) * x * p - (k - 1) * pm1) / k;
+                pm1 = p;
+                p = pk;
+            }
+            if (n == 1) pm1 = 1.0L;
+            dpn = n * (pm1 - x * p) / (1.0L - x * x);
+        }
+        const long double w = 2.0L / ((1.0L - x * x) * dpn * dpn);
+        // ascending order: largest root last
+        r.x[(size_t)(n - i)] = (double)x;
+        r.w[(size_t)(n - i)] = (double)w;
+        r.x[(size_t)(i - 1)] = (double)(-x);
+        r.w[(size_t)(i - 1)] = (double)w;
+    }
+    if (n % 2 == 1) r.x[(size_t)(n / 2)] = 0.0;
+    return r;
+}
+
+// J0 Taylor coefficients about c = (m + 1/2) dx, degree 7, from J0(c),
+// J0'(c) = -J1(c) and the Bessel equation x y'' + y' + x y = 0, which gives
+// c_{n+2} = -((n+1)^2 c_{n+1} + c c_n + c_{n-1}) / (c (n+1)(n+2)).
+std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals)
+{
+    const int nm = (int)std::ceil(x_max / dx) + 1;
+    std::vector<float> t((size_t)nm * 8);
+    for (int m = 0; m < nm; ++m) {
+        const long double c = ((long double)m + 0.5L) * (long double)dx;
+        long double co[9];
+        co[0] = (long double)::j0((double)c);
+        co[1] = -(long double)::j1((double)c);
+        for (int k = 0; k + 2 <= 7; ++k) {
+            const long double cm1 = k >= 1 ? co[k - 1] : 0.0L;
+            co[k + 2] = -(((long double)(k + 1) * (k + 1)) * co[k + 1] + c * co[k] + cm1) /
+                        (c * (long double)(k + 1) * (long double)(k + 2));
+        }
+        for (int k = 0; k < 8; ++k) t[(size_t)m * 8 + k] = (float)co[k];
+    }
+    *n_intervals = nm;
+    return t;
+}
+
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+__global__ void k_copy1(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    const size_t n16 = bytes / 16;
+    if (n16 > 0) {
+        size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 32);
+        k_copy16<<<(unsigned)blocks, 256, 0, s>>>(static_cast<const uint4 *>(src),
+                                                  static_cast<uint4 *>(dst), n16);
+    }
+    const size_t tail = bytes - n16 * 16;
+    if (tail > 0)
+        k_copy1<<<1, 256, 0, s>>>(static_cast<const uint8_t *>(src) + n16 * 16,
+                                  static_cast<uint8_t *>(dst) + n16 * 16, tail);
+    return hipGetLastError();
+}
+
+// evaluates the FP32 table Bessel pair used by the mixed-precision kernels
+__global__ void k_bessel_tab(const double *__restrict__ x, int64_t n,
+                             const float4 *__restrict__ tab, int ntab, float dx, float inv_dx,
+                             double *__restrict__ j0o, double *__restrict__ j1o)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float xf = (float)x[i];
+    int m = (int)(xf * inv_dx);
+    m = m < ntab - 1 ? m : ntab - 1;
+    const float t = xf - ((float)m + 0.5f) * dx;
+    const float4 c0 = tab[2 * m], c1 = tab[2 * m + 1];
+    float p = c1.w, dp = 0.f;
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.z);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.y);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.x);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.w);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.z);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.y);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.x);
+    j0o[i] = p;
+    j1o[i] = -dp;
+}
+
+}  // namespace qem
+
+using namespace qem;
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586;  // Float64(2pi)
+constexpr double kBesselDx = 0.5;
+constexpr int kMaxTabIntervals = 1536;        // 48 KiB of LDS
+
+int set_err(qem_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_err(qem_ctx *ctx, hipError_t e, const char *what)
+{
+    return set_err(ctx, QEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define API_HIP(ctx, expr)                                                    \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) return hip_err((ctx), e_, #expr);               \
+    } while (0)
+
+#define API_TRY(ctx, expr)                                                    \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            if ((ctx)->err.empty())                                           \
+                (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_); \
+            return QEM_ERR_HIP;                                               \
+        }                                                                     \
+    } while (0)
+
+bool finite_all(const double *a, int64_t m)
+{
+    for (int64_t i = 0; i < m; ++i)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+int check_params(const qem_params *p, std::string &msg)
+{
+    auto bad = [&](const char *m) {
+        msg = m;
+        return QEM_ERR_INVALID_ARG;
+    };
+    const double vals[] = {p->Lx, p->Ly, p->Lz, p->r_cutoff, p->alpha,
+                           p->gamma_1, p->gamma_2, p->eps_0, p->accuracy};
+    for (double v : vals)
+        if (!std::isfinite(v)) return bad("parameters must be finite");
+    if (!(p->Lx > 0 && p->Ly > 0 && p->Lz > 0)) return bad("box lengths must be positive");
+    if (!(p->r_cutoff > 0)) return bad("r_cutoff must be positive");
+    if (!(2.0 * p->r_cutoff < std::min(p->Lx, p->Ly)))
+        return bad("r_cutoff must be below min(Lx, Ly)/2 (one minimum image per pair)");
+    if (!(p->alpha > 0)) return bad("alpha must be positive");
+    if (!(p->accuracy > 0 && p->accuracy < 1)) return bad("accuracy must lie in (0, 1)");
+    if (p->eps_0 == 0.0) return bad("eps_0 must be non-zero");
+    if (p->N_t < 1 || p->N_t > 256) return bad("N_t must lie in 1..256");
+    if (p->iter_period < 1) return bad("iter_period must be >= 1");
+    if (p->rbe_p < 0) return bad("rbe_p must be >= 0");
+    if (p->precision != QEM_PREC_FP64 && p->precision != QEM_PREC_MIXED)
+        return bad("precision must be QEM_PREC_FP64 or QEM_PREC_MIXED");
+    if (p->semantics & ~QEM_SEM_CORRECTED) return bad("unknown semantics bits");
+    if (!(std::fabs(p->gamma_1 * p->gamma_2) < 1.0)) {
+        msg = "|gamma_1 gamma_2| must be < 1: the image-series denominator "
+              "gamma_1 gamma_2 exp(-2 k Lz) - 1 vanishes otherwise";
+        return QEM_ERR_UNSUPPORTED;
+    }
+    return QEM_OK;
+}
+
+// positions finite, z strictly inside the slab, charges finite
+int check_atoms(qem_ctx *ctx, int64_t n, const double *xyz, const double *q)
+{
+    if (n < 0) return set_err(ctx, QEM_ERR_INVALID_ARG, "n must be >= 0");
+    if (n > 0 && (!xyz)) return set_err(ctx, QEM_ERR_INVALID_ARG, "null coordinate pointer");
+    if (n > 0 && !finite_all(xyz, 3 * n))
+        return set_err(ctx, QEM_ERR_DOMAIN, "coordinates must be finite");
+    if (q && n > 0 && !finite_all(q, n))
+        return set_err(ctx, QEM_ERR_DOMAIN, "charges must be finite");
+    for (int64_t i = 0; i < n; ++i) {
+        const double z = xyz[3 * i + 2];
+        if (!(z > 0.0 && z < ctx->p.Lz)) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "atom %lld: z = %.17g outside (0, Lz = %.17g)",
+                          (long long)i, z, ctx->p.Lz);
+            return set_err(ctx, QEM_ERR_DOMAIN, buf);
+        }
+    }
+    return QEM_OK;
+}
+
+ShortConsts short_consts(const qem_ctx *ctx, int single)
+{
+    const qem_params &p = ctx->p;
+    ShortConsts c;
+    c.Lx = p.Lx;
+    c.Ly = p.Ly;
+    c.Lz = p.Lz;
+    c.rc2 = p.r_cutoff * p.r_cutoff;
+    c.alpha = p.alpha;
+    c.g1 = p.gamma_1;
+    c.g2 = p.gamma_2;
+    c.g12 = p.gamma_1 * p.gamma_2;
+    c.ln_acc = std::log(p.accuracy);
+    c.inv2pieps = 1.0 / (kTwoPi * p.eps_0);
+    c.sem = p.semantics;
+    c.Nt = p.N_t;
+    c.single = single;
+    c.jt_n = ctx->sw.jt_n;
+    c.jt_dx = ctx->sw.jt_dx;
+    c.jt_inv_dx = ctx->sw.jt_dx > 0 ? 1.0f / ctx->sw.jt_dx : 0.0f;
+    return c;
+}
+
+// K_set_generator (src/K_set_generator.jl:5-45): m_x outer, m_y inner,
+// (0,0) skipped, k_x = m_x * 2pi / L_x, Float32 weights.
+void build_kset(qem_ctx *ctx)
+{
+    const qem_params &p = ctx->p;
+    const double k_c = 2.0 * std::sqrt(-4.0 * p.alpha * std::log(p.accuracy));
+    const int64_t mxm = (int64_t)std::ceil(k_c * p.Lx / kTwoPi) + 1;
+    const int64_t mym = (int64_t)std::ceil(k_c * p.Ly / kTwoPi) + 1;
+    double Sx = 0.0, Sy = 0.0;
+    for (int64_t m = -mxm; m <= mxm; ++m)
+        if (m != 0) {
+            const double kx = (double)m * kTwoPi / p.Lx;
+            Sx += std::exp(-(kx * kx) / (4.0 * p.alpha));
+        }
+    for (int64_t m = -mym; m <= mym; ++m)
+        if (m != 0) {
+            const double ky = (double)m * kTwoPi / p.Ly;
+            Sy += std::exp(-(ky * ky) / (4.0 * p.alpha));
+        }
+    const double sK = Sx + Sy + Sx * Sy;
+    const size_t nk = (size_t)((2 * mxm + 1) * (2 * mym + 1) - 1);
+    ctx->kset.resize(3 * nk);
+    ctx->kprob.resize(nk);
+    size_t t = 0;
+    for (int64_t mx = -mxm; mx <= mxm; ++mx)
+        for (int64_t my = -mym; my <= mym; ++my) {
+            const double kx = (double)mx * kTwoPi / p.Lx;
+            const double ky = (double)my * kTwoPi / p.Ly;
+            const double k = std::sqrt(kx * kx + ky * ky);
+            if (k != 0.0) {
+                ctx->kset[3 * t] = kx;
+                ctx->kset[3 * t + 1] = ky;
+                ctx->kset[3 * t + 2] = k;
+                ctx->kprob[t] = (float)(std::exp(-(k * k) / (4.0 * p.alpha)) / sK);
+                ++t;
+            }
+        }
+    ctx->sum_K = sK;
+    // Vose alias table over prob[i] / sum(prob)
+    double tot = 0.0;
+    for (float v : ctx->kprob) tot += (double)v;
+    const size_t n = nk;
+    ctx->alias_prob.assign(n, 1.0);
+    ctx->alias_idx.assign(n, 0);
+    std::vector<double> sc(n);
+    std::vector<int64_t> small, large;
+    for (size_t i = 0; i < n; ++i) {
+        sc[i] = (double)ctx->kprob[i] / tot * (double)n;
+        ctx->alias_idx[i] = (int64_t)i;
+        (sc[i] < 1.0 ? small : large).push_back((int64_t)i);
+    }
+    while (!small.empty() && !large.empty()) {
+        const int64_t s = small.back(), l = large.back();
+        small.pop_back();
+        ctx->alias_prob[(size_t)s] = sc[(size_t)s];
+        ctx->alias_idx[(size_t)s] = l;
+        sc[(size_t)l] = (sc[(size_t)l] + sc[(size_t)s]) - 1.0;
+        if (sc[(size_t)l] < 1.0) {
+            large.pop_back();
+            small.push_back(l);
+        }
+    }
+    for (int64_t i : large) ctx->alias_prob[(size_t)i] = 1.0;
+    for (int64_t i : small) ctx->alias_prob[(size_t)i] = 1.0;
+}
+
+int ensure_kset(qem_ctx *ctx)
+{
+    if (!ctx->kset.empty()) return QEM_OK;
+    try {
+        build_kset(ctx);
+    } catch (const std::bad_alloc &) {
+        return set_err(ctx, QEM_ERR_NOMEM, "K set: host allocation failed");
+    }
+    return QEM_OK;
+}
+
+uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int upload(qem_ctx *ctx, DevBuf &b, const void *h, size_t bytes)
+{
+    API_HIP(ctx, ensure(b, bytes));
+    if (bytes > 0) API_HIP(ctx, hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return QEM_OK;
+}
+
+// Stages P k rows (plus the k = 0 row when requested) in the pinned buffer
+// and copies them to lw.kv: (k_x, k_y, |k|, weight / (Lx Ly eps_0)).
+int stage_kvecs(qem_ctx *ctx, const double *kvecs, int64_t P, double weight, bool with_k0,
+                hipStream_t s, int64_t *Ptot)
+{
+    const qem_params &p = ctx->p;
+    const int64_t rows = P + (with_k0 ? 1 : 0);
+    const size_t bytes = sizeof(double) * 4 * (size_t)(rows > 0 ? rows : 1);
+    LongWork &lw = ctx->lw;
+    // ring of pinned staging slots; a slot is rewritten only after the copy
+    // that last read it has completed
+    const int slot = lw.kv_slot;
+    lw.kv_slot = (lw.kv_slot + 1) % LongWork::kSlots;
+    if (lw.kv_ev[slot]) API_HIP(ctx, hipEventSynchronize(lw.kv_ev[slot]));
+    else API_HIP(ctx, hipEventCreateWithFlags(&lw.kv_ev[slot], hipEventDisableTiming));
+    if (lw.h_kv_cap[slot] < bytes) {
+        if (lw.h_kv[slot]) (void)hipHostFree(lw.h_kv[slot]);
+        lw.h_kv[slot] = nullptr;
+        lw.h_kv_cap[slot] = 0;
+        API_HIP(ctx, hipHostMalloc(&lw.h_kv[slot], bytes, hipHostMallocDefault));
+        lw.h_kv_cap[slot] = bytes;
+    }
+    double *h = static_cast<double *>(lw.h_kv[slot]);
+    const double inv_ae = 1.0 / (p.Lx * p.Ly * p.eps_0);
+    for (int64_t t = 0; t < P; ++t) {
+        const double kx = kvecs[3 * t], ky = kvecs[3 * t + 1], k = kvecs[3 * t + 2];
+        if (!(std::isfinite(kx) && std::isfinite(ky) && std::isfinite(k) && k > 0.0))
+            return set_err(ctx, QEM_ERR_DOMAIN, "k vectors must be finite with |k| > 0");
+        const double wk = weight >= 0.0 ? weight : std::exp(-(k * k) / (4.0 * p.alpha));
+        h[4 * t] = kx;
+        h[4 * t + 1] = ky;
+        h[4 * t + 2] = k;
+        h[4 * t + 3] = wk * inv_ae;
+    }
+    if (with_k0) {
+        h[4 * P] = 0.0;
+        h[4 * P + 1] = 0.0;
+        h[4 * P + 2] = 0.0;
+        h[4 * P + 3] = 1.0 / (2.0 * p.Lx * p.Ly * p.eps_0);
+    }
+    API_HIP(ctx, ensure(lw.kv, bytes));
+    API_HIP(ctx, hipMemcpyAsync(lw.kv.p, h, bytes, hipMemcpyHostToDevice, s));
+    API_HIP(ctx, hipEventRecord(lw.kv_ev[slot], s));
+    *Ptot = rows;
+    return QEM_OK;
+}
+
+int kidx_to_vecs(qem_ctx *ctx, const int64_t *k_idx, int64_t P, std::vector<double> &out)
+{
+    int st = ensure_kset(ctx);
+    if (st) return st;
+    const int64_t nk = (int64_t)ctx->kprob.size();
+    out.resize(3 * (size_t)P);
+    for (int64_t t = 0; t < P; ++t) {
+        const int64_t i = k_idx[t];
+        if (i < 0 || i >= nk) {
+            char buf[128];
+            std::snprintf(buf, sizeof buf, "k index %lld out of range [0, %lld)", (long long)i,
+                          (long long)nk);
+            return set_err(ctx, QEM_ERR_INVALID_ARG, buf);
+        }
+        out[3 * t] = ctx->kset[3 * (size_t)i];
+        out[3 * t + 1] = ctx->kset[3 * (size_t)i + 1];
+        out[3 * t + 2] = ctx->kset[3 * (size_t)i + 2];
+    }
+    return QEM_OK;
+}
+
+hipStream_t pick_stream(qem_ctx *ctx, void *stream)
+{
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+// device part of Molly.forces(::QEM_short) (src/QEM_short.jl:27-59)
+int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, double *d_f,
+              hipStream_t s, bool use_counter)
+{
+    if (n == 0) return QEM_OK;
+    if (use_counter) {
+        ctx->n_steps += 1;
+        const bool due = ((ctx->n_steps - 1) % ctx->p.iter_period) == 0;
+        if (due || ctx->nl.n != n || ctx->nl.has_i_idx) {
+            hipError_t e = build_neighbor_list(ctx->nl, n, d_xyz, ctx->p.Lx, ctx->p.Ly,
+                                               ctx->p.r_cutoff, s, ctx->err);
+            if (e != hipSuccess) return hip_err(ctx, e, ctx->err.c_str());
+        }
+    }
+    const ShortConsts c = short_consts(ctx, 0);
+    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    API_HIP(ctx, launch_pair_forces(c, ctx->p.precision, ctx->sw, ctx->nl.pi.as<int32_t>(),
+                                    ctx->nl.pj.as<int32_t>(), ctx->nl.npairs, d_xyz, d_q,
+                                    ctx->sw.fpair.as<double>(), s));
+    API_HIP(ctx, launch_gather_self(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
+                                    ctx->sw.fpair.as<double>(), d_f, s));
+    return QEM_OK;
+}
+
+int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+             const double *kvecs, int64_t P, double weight, uint32_t parts, double *d_f,
+             int accumulate, hipStream_t s)
+{
+    if (n == 0) return QEM_OK;
+    int64_t Ptot = 0;
+    int st = stage_kvecs(ctx, kvecs, P, weight, (parts & QEM_LONG_K0) != 0, s, &Ptot);
+    if (st) return st;
+    hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, "z sort");
+    LongConsts c;
+    c.Lz = ctx->p.Lz;
+    c.g1 = ctx->p.gamma_1;
+    c.g2 = ctx->p.gamma_2;
+    c.g12 = c.g1 * c.g2;
+    c.inv_area_eps = 1.0 / (ctx->p.Lx * ctx->p.Ly * ctx->p.eps_0);
+    c.weight = weight;
+    c.inv4alpha = 0.25 / ctx->p.alpha;
+    c.parts = parts;
+    c.precision = ctx->p.precision;
+    e = launch_long(c, ctx->lw, n, Ptot, d_f, accumulate, s, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, "long range");
+    return QEM_OK;
+}
+
+#define CTX_GUARD(ctx)                                                        \
+    do {                                                                      \
+        if (!(ctx)) return QEM_ERR_INVALID_ARG;                               \
+        (ctx)->err.clear();                                                   \
+        hipError_t e0_ = hipSetDevice((ctx)->p.device);                       \
+        if (e0_ != hipSuccess) return hip_err((ctx), e0_, "hipSetDevice");   \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+void qem_default_params(qem_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->Lx = p->Ly = p->Lz = 1.0;
+    p->r_cutoff = 0.25;
+    p->alpha = 1.0;
+    p->gamma_1 = 0.0;
+    p->gamma_2 = 0.0;
+    p->eps_0 = 1.0;
+    p->accuracy = 1e-6;
+    p->N_t = 30;
+    p->iter_period = 100;
+    p->rbe_p = 30;
+    p->precision = QEM_PREC_FP64;
+    p->semantics = 0;
+    p->device = 0;
+}
+
+const char *qem_status_string(int status)
+{
+    switch (status) {
+    case QEM_OK: return "ok";
+    case QEM_ERR_INVALID_ARG: return "invalid argument";
+    case QEM_ERR_DOMAIN: return "input outside the domain";
+    case QEM_ERR_HIP: return "HIP runtime error";
+    case QEM_ERR_NOMEM: return "out of memory";
+    case QEM_ERR_UNSUPPORTED: return "unsupported parameters";
+    case QEM_ERR_CAPACITY: return "buffer too small";
+    default: return "unknown status";
+    }
+}
+
+int qem_inter_params(int64_t n_atoms, double Lx, double Ly, double s, int32_t rbe_mode,
+                     double *alpha, double *r_cutoff, double *k_cutoff)
+{
+    if (!alpha || !r_cutoff || !k_cutoff || n_atoms <= 0 || !(Lx > 0) || !(Ly > 0) || !(s > 0))
+        return QEM_ERR_INVALID_ARG;
+    const double a = rbe_mode ? (double)n_atoms / (Lx * Ly) : std::sqrt((double)n_atoms) / (Lx * Ly);
+    *alpha = a;
+    *r_cutoff = s / std::sqrt(a);
+    *k_cutoff = 2.0 * s * std::sqrt(a);
+    return QEM_OK;
+}
+
+int qem_create(const qem_params *p, qem_ctx **out)
+{
+    if (!p || !out) return QEM_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::string msg;
+    int st = check_params(p, msg);
+    if (st) return st;
+    qem_ctx *ctx = new (std::nothrow) qem_ctx();
+    if (!ctx) return QEM_ERR_NOMEM;
+    ctx->p = *p;
+    auto fail = [&](int code) {
+        qem_destroy(ctx);
+        return code;
+    };
+    if (hipSetDevice(p->device) != hipSuccess) return fail(QEM_ERR_HIP);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(QEM_ERR_HIP);
+    ctx->gl = make_gl_rule(p->N_t);
+    std::vector<float> xf(ctx->gl.x.begin(), ctx->gl.x.end()), wf(ctx->gl.w.begin(), ctx->gl.w.end());
+    if (upload(ctx, ctx->sw.glx, ctx->gl.x.data(), sizeof(double) * p->N_t) ||
+        upload(ctx, ctx->sw.glw, ctx->gl.w.data(), sizeof(double) * p->N_t) ||
+        upload(ctx, ctx->sw.glxf, xf.data(), sizeof(float) * p->N_t) ||
+        upload(ctx, ctx->sw.glwf, wf.data(), sizeof(float) * p->N_t))
+        return fail(QEM_ERR_HIP);
+    if (p->precision == QEM_PREC_MIXED) {
+        // largest Bessel argument: r_cutoff times the largest limit
+        const double la = -std::log(p->accuracy);
+        const double kmax = std::max(std::sqrt(4.0 * p->alpha * la), la / (2.0 * p->Lz));
+        const double x_max = p->r_cutoff * kmax * (1.0 + 1e-6) + kBesselDx;
+        int nm = 0;
+        if (x_max / kBesselDx + 2 > kMaxTabIntervals) {
+            qem_destroy(ctx);
+            return QEM_ERR_UNSUPPORTED;
+        }
+        std::vector<float> tab = make_j0_table(x_max, kBesselDx, &nm);
+        ctx->sw.jt_n = nm;
+        ctx->sw.jt_dx = (float)kBesselDx;
+        if (upload(ctx, ctx->sw.jtab, tab.data(), sizeof(float) * tab.size()))
+            return fail(QEM_ERR_HIP);
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(QEM_ERR_HIP);
+    if (p->rbe_p > 0 && ensure_kset(ctx)) return fail(QEM_ERR_NOMEM);
+    *out = ctx;
+    return QEM_OK;
+}
+
+int qem_destroy(qem_ctx *ctx)
+{
+    if (!ctx) return QEM_OK;
+    (void)hipSetDevice(ctx->p.device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab,
+                      &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
+                      &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
+                      &ctx->nl.cell_key, &ctx->nl.cell_key_sorted, &ctx->nl.atom_idx,
+                      &ctx->nl.atom_idx_sorted, &ctx->nl.cell_start, &ctx->nl.cell_end,
+                      &ctx->nl.counts, &ctx->nl.row_ptr, &ctx->nl.tmp, &ctx->lw.keys,
+                      &ctx->lw.keys_sorted, &ctx->lw.idx, &ctx->lw.perm, &ctx->lw.tmp,
+                      &ctx->lw.sx, &ctx->lw.sy, &ctx->lw.sz, &ctx->lw.sq, &ctx->lw.kv,
+                      &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,
+                      &ctx->lw.q0off, &ctx->lw.partB, &ctx->d_xyz, &ctx->d_q, &ctx->d_f,
+                      &ctx->d_f2, &ctx->d_aux};
+    for (DevBuf *b : bufs) release(*b);
+    for (int t = 0; t < LongWork::kSlots; ++t) {
+        if (ctx->lw.h_kv[t]) (void)hipHostFree(ctx->lw.h_kv[t]);
+        if (ctx->lw.kv_ev[t]) (void)hipEventDestroy(ctx->lw.kv_ev[t]);
+    }
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return QEM_OK;
+}
+
+const char *qem_last_error(const qem_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+int qem_get_params(const qem_ctx *ctx, qem_params *p)
+{
+    if (!ctx || !p) return QEM_ERR_INVALID_ARG;
+    *p = ctx->p;
+    return QEM_OK;
+}
+
+int qem_kset_size(qem_ctx *ctx, int64_t *n)
+{
+    if (!ctx || !n) return QEM_ERR_INVALID_ARG;
+    int st = ensure_kset(ctx);
+    if (st) return st;
+    *n = (int64_t)ctx->kprob.size();
+    return QEM_OK;
+}
+
+int qem_kset_copy(qem_ctx *ctx, double *kxkyk, float *prob, double *sum_K)
+{
+    if (!ctx) return QEM_ERR_INVALID_ARG;
+    int st = ensure_kset(ctx);
+    if (st) return st;
+    if (kxkyk) std::memcpy(kxkyk, ctx->kset.data(), sizeof(double) * ctx->kset.size());
+    if (prob) std::memcpy(prob, ctx->kprob.data(), sizeof(float) * ctx->kprob.size());
+    if (sum_K) *sum_K = ctx->sum_K;
+    return QEM_OK;
+}
+
+int qem_sample_k(qem_ctx *ctx, uint64_t seed, int64_t P, int64_t *k_idx)
+{
+    if (!ctx || P < 0 || (P > 0 && !k_idx)) return QEM_ERR_INVALID_ARG;
+    int st = ensure_kset(ctx);
+    if (st) return st;
+    const uint64_t n = (uint64_t)ctx->kprob.size();
+    uint64_t s = seed;
+    for (int64_t t = 0; t < P; ++t) {
+        const double u1 = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+        const double u2 = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+        uint64_t j = (uint64_t)(u1 * (double)n);
+        if (j >= n) j = n - 1;
+        k_idx[t] = u2 < ctx->alias_prob[j] ? (int64_t)j : ctx->alias_idx[j];
+    }
+    return QEM_OK;
+}
+
+int qem_neighbor_list(qem_ctx *ctx, int64_t n, const double *xyz, int64_t cap, int32_t *pairs,
+                      int64_t *npairs)
+{
+    CTX_GUARD(ctx);
+    if (!npairs || cap < 0 || (cap > 0 && !pairs))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad output arguments");
+    int st = check_atoms(ctx, n, xyz, nullptr);
+    if (st) return st;
+    *npairs = 0;
+    if (n == 0) return QEM_OK;
+    st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n);
+    if (st) return st;
+    hipError_t e = build_neighbor_list(ctx->nl, n, ctx->d_xyz.as<double>(), ctx->p.Lx, ctx->p.Ly,
+                                       ctx->p.r_cutoff, ctx->stream, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, ctx->err.c_str());
+    const int64_t np = ctx->nl.npairs;
+    *npairs = np;
+    const int64_t m = std::min(np, cap);
+    if (m > 0) {
+        std::vector<int32_t> a((size_t)m), b((size_t)m);
+        API_HIP(ctx, hipMemcpyAsync(a.data(), ctx->nl.pi.p, sizeof(int32_t) * (size_t)m,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        API_HIP(ctx, hipMemcpyAsync(b.data(), ctx->nl.pj.p, sizeof(int32_t) * (size_t)m,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int64_t t = 0; t < m; ++t) {
+            pairs[2 * t] = a[(size_t)t];
+            pairs[2 * t + 1] = b[(size_t)t];
+        }
+    }
+    return np > cap ? set_err(ctx, QEM_ERR_CAPACITY, "pair buffer too small") : QEM_OK;
+}
+
+int qem_short_forces(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, double *f)
+{
+    CTX_GUARD(ctx);
+    if (n > 0 && (!q || !f)) return set_err(ctx, QEM_ERR_INVALID_ARG, "null pointer");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    if (n == 0) return QEM_OK;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
+    st = short_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), ctx->d_f.as<double>(),
+                   ctx->stream, true);
+    if (st) return st;
+    API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, sizeof(double) * 3 * (size_t)n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_short_forces_pairs(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                           int64_t npairs, const int32_t *pairs, double *f)
+{
+    CTX_GUARD(ctx);
+    if ((n > 0 && (!q || !f)) || npairs < 0 || (npairs > 0 && !pairs))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "null pointer or negative size");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    if (npairs > INT32_MAX) return set_err(ctx, QEM_ERR_UNSUPPORTED, "more than 2^31-1 pairs");
+    for (int64_t t = 0; t < npairs; ++t) {
+        const int32_t i = pairs[2 * t], j = pairs[2 * t + 1];
+        if (i < 0 || j < 0 || i >= n || j >= n || i == j)
+            return set_err(ctx, QEM_ERR_INVALID_ARG, "pair endpoints must be distinct atoms in [0, n)");
+    }
+    if (n == 0) return QEM_OK;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    hipError_t e = install_pairs(ctx->nl, n, pairs, npairs, ctx->stream, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, ctx->err.c_str());
+    API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
+    st = short_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), ctx->d_f.as<double>(),
+                   ctx->stream, false);
+    if (st) return st;
+    API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, sizeof(double) * 3 * (size_t)n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_short_pair(qem_ctx *ctx, int64_t m, const double *qi, const double *qj, const double *ri,
+                   const double *rj, int32_t single, double *f_i, double *fz_j)
+{
+    CTX_GUARD(ctx);
+    if (m < 0 || (m > 0 && (!qi || !qj || !ri || !rj || !f_i)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "null pointer or negative size");
+    int st;
+    if ((st = check_atoms(ctx, m, ri, qi)) || (st = check_atoms(ctx, m, rj, qj))) return st;
+    if (m == 0) return QEM_OK;
+    const size_t b3 = sizeof(double) * 3 * (size_t)m, b1 = sizeof(double) * (size_t)m;
+    API_HIP(ctx, ensure(ctx->d_aux, 2 * b3 + 2 * b1));
+    char *base = ctx->d_aux.as<char>();
+    API_HIP(ctx, hipMemcpyAsync(base, ri, b3, hipMemcpyHostToDevice, ctx->stream));
+    API_HIP(ctx, hipMemcpyAsync(base + b3, rj, b3, hipMemcpyHostToDevice, ctx->stream));
+    API_HIP(ctx, hipMemcpyAsync(base + 2 * b3, qi, b1, hipMemcpyHostToDevice, ctx->stream));
+    API_HIP(ctx, hipMemcpyAsync(base + 2 * b3 + b1, qj, b1, hipMemcpyHostToDevice, ctx->stream));
+    API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * 4 * (size_t)m));
+    const ShortConsts c = short_consts(ctx, single ? 1 : 0);
+    API_HIP(ctx, launch_pair_direct(c, ctx->p.precision, ctx->sw, m,
+                                    reinterpret_cast<const double *>(base),
+                                    reinterpret_cast<const double *>(base + b3),
+                                    reinterpret_cast<const double *>(base + 2 * b3),
+                                    reinterpret_cast<const double *>(base + 2 * b3 + b1),
+                                    ctx->d_f2.as<double>(), ctx->stream));
+    std::vector<double> h(4 * (size_t)m);
+    API_HIP(ctx, hipMemcpyAsync(h.data(), ctx->d_f2.p, sizeof(double) * 4 * (size_t)m,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t t = 0; t < m; ++t) {
+        f_i[3 * t] = h[4 * (size_t)t];
+        f_i[3 * t + 1] = h[4 * (size_t)t + 1];
+        f_i[3 * t + 2] = h[4 * (size_t)t + 2];
+        if (fz_j) fz_j[t] = h[4 * (size_t)t + 3];
+    }
+    return QEM_OK;
+}
+
+int qem_short_self(qem_ctx *ctx, int64_t n, const double *q, const double *xyz, int32_t single,
+                   double *f)
+{
+    CTX_GUARD(ctx);
+    if (n > 0 && (!q || !f)) return set_err(ctx, QEM_ERR_INVALID_ARG, "null pointer");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    if (n == 0) return QEM_OK;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
+    const ShortConsts c = short_consts(ctx, single ? 1 : 0);
+    API_HIP(ctx, launch_self_only(c, ctx->p.precision, ctx->sw, n, ctx->d_xyz.as<double>(),
+                                  ctx->d_q.as<double>(), ctx->d_f.as<double>(), ctx->stream));
+    API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, sizeof(double) * 3 * (size_t)n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_sortperm_z(qem_ctx *ctx, int64_t n, const double *xyz, int64_t *zlist)
+{
+    CTX_GUARD(ctx);
+    if (n > 0 && !zlist) return set_err(ctx, QEM_ERR_INVALID_ARG, "null pointer");
+    int st = check_atoms(ctx, n, xyz, nullptr);
+    if (st) return st;
+    if (n == 0) return QEM_OK;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n))) return st;
+    API_HIP(ctx, ensure(ctx->d_q, sizeof(double) * (size_t)n));
+    hipError_t e = launch_zsort(ctx->lw, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(),
+                                ctx->stream, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, "z sort");
+    std::vector<int32_t> h((size_t)n);
+    API_HIP(ctx, hipMemcpyAsync(h.data(), ctx->lw.perm.p, sizeof(int32_t) * (size_t)n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < n; ++i) zlist[i] = h[(size_t)i];
+    return QEM_OK;
+}
+
+int qem_long_terms(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                   const double *kvecs, int64_t P, double weight, uint32_t parts, double *f)
+{
+    CTX_GUARD(ctx);
+    if ((n > 0 && (!q || !f)) || P < 0 || (P > 0 && !kvecs) || (parts & ~QEM_LONG_ALL))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    if (n == 0) return QEM_OK;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
+    st = long_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kvecs, P, weight,
+                  parts, ctx->d_f.as<double>(), 0, ctx->stream);
+    if (st) return st;
+    API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, sizeof(double) * 3 * (size_t)n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_long_forces(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                    const int64_t *k_idx, int64_t P, double *f)
+{
+    CTX_GUARD(ctx);
+    if (P < 0 || (P > 0 && !k_idx)) return set_err(ctx, QEM_ERR_INVALID_ARG, "bad k indices");
+    std::vector<double> kv;
+    int st = kidx_to_vecs(ctx, k_idx, P, kv);
+    if (st) return st;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    return qem_long_terms(ctx, n, xyz, q, kv.data(), P, w, QEM_LONG_ALL, f);
+}
+
+int qem_forces(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, const int64_t *k_idx,
+               int64_t P, double *f)
+{
+    CTX_GUARD(ctx);
+    if ((n > 0 && (!q || !f)) || P < 0 || (P > 0 && !k_idx))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    std::vector<double> kv;
+    if ((st = kidx_to_vecs(ctx, k_idx, P, kv))) return st;
+    if (n == 0) return QEM_OK;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
+    st = short_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), ctx->d_f.as<double>(),
+                   ctx->stream, true);
+    if (st) return st;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    st = long_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv.data(), P, w,
+                  QEM_LONG_ALL, ctx->d_f.as<double>(), 1, ctx->stream);
+    if (st) return st;
+    API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, sizeof(double) * 3 * (size_t)n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_short_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                         double *d_f, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || (n > 0 && (!d_xyz || !d_q || !d_f)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    return short_dev(ctx, n, d_xyz, d_q, d_f, pick_stream(ctx, stream), true);
+}
+
+int qem_long_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                        const int64_t *k_idx, int64_t P, double *d_f, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || P < 0 || (P > 0 && !k_idx) || (n > 0 && (!d_xyz || !d_q || !d_f)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    std::vector<double> kv;
+    int st = kidx_to_vecs(ctx, k_idx, P, kv);
+    if (st) return st;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    return long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 0,
+                    pick_stream(ctx, stream));
+}
+
+int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                   const int64_t *k_idx, int64_t P, double *d_f, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || P < 0 || (P > 0 && !k_idx) || (n > 0 && (!d_xyz || !d_q || !d_f)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    std::vector<double> kv;
+    int st = kidx_to_vecs(ctx, k_idx, P, kv);
+    if (st) return st;
+    hipStream_t s = pick_stream(ctx, stream);
+    st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true);
+    if (st) return st;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    return long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 1, s);
+}
+
+int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds, int64_t *n_steps)
+{
+    if (!ctx) return QEM_ERR_INVALID_ARG;
+    if (npairs) *npairs = ctx->nl.npairs;
+    if (n_builds) *n_builds = ctx->nl.builds;
+    if (n_steps) *n_steps = ctx->n_steps;
+    return QEM_OK;
+}
+
+int qem_gauss_legendre(int32_t n, double *x, double *w)
+{
+    if (n < 1 || n > 256 || !x || !w) return QEM_ERR_INVALID_ARG;
+    GLRule r = make_gl_rule(n);
+    std::memcpy(x, r.x.data(), sizeof(double) * (size_t)n);
+    std::memcpy(w, r.w.data(), sizeof(double) * (size_t)n);
+    return QEM_OK;
+}
+
+int qem_bessel01_mixed(qem_ctx *ctx, int64_t n, const double *x, double *j0o, double *j1o)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || (n > 0 && (!x || !j0o || !j1o)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    if (ctx->sw.jt_n == 0)
+        return set_err(ctx, QEM_ERR_UNSUPPORTED, "context is not in mixed precision");
+    const double xmax = ctx->sw.jt_n * (double)ctx->sw.jt_dx;
+    for (int64_t i = 0; i < n; ++i)
+        if (!(x[i] >= 0.0 && x[i] <= xmax))
+            return set_err(ctx, QEM_ERR_DOMAIN, "argument outside the table range");
+    if (n == 0) return QEM_OK;
+    const size_t b = sizeof(double) * (size_t)n;
+    API_HIP(ctx, ensure(ctx->d_aux, 3 * b));
+    char *base = ctx->d_aux.as<char>();
+    API_HIP(ctx, hipMemcpyAsync(base, x, b, hipMemcpyHostToDevice, ctx->stream));
+    k_bessel_tab<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
+        reinterpret_cast<const double *>(base), n, ctx->sw.jtab.as<float4>(), ctx->sw.jt_n,
+        ctx->sw.jt_dx, 1.0f / ctx->sw.jt_dx, reinterpret_cast<double *>(base + b),
+        reinterpret_cast<double *>(base + 2 * b));
+    API_HIP(ctx, hipGetLastError());
+    API_HIP(ctx, hipMemcpyAsync(j0o, base + b, b, hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipMemcpyAsync(j1o, base + 2 * b, b, hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_dev_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return QEM_ERR_INVALID_ARG;
+    return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? QEM_OK : QEM_ERR_NOMEM;
+}
+
+int qem_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? QEM_OK : QEM_ERR_HIP; }
+
+int qem_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? QEM_OK : QEM_ERR_HIP;
+}
+
+int qem_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? QEM_OK : QEM_ERR_HIP;
+}
+
+int qem_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? QEM_OK : QEM_ERR_HIP; }
+
+int qem_dev_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!dst || !src) return QEM_ERR_INVALID_ARG;
+    return launch_copy(dst, src, bytes, static_cast<hipStream_t>(stream)) == hipSuccess
+               ? QEM_OK
+               : QEM_ERR_HIP;
+}
+
+}  // extern "C"

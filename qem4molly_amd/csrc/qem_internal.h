@@ -1,0 +1,165 @@
+// Internal declarations shared by the libqem_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/qem.h"
+
+namespace qem {
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+hipError_t ensure(DevBuf &b, size_t bytes);
+void release(DevBuf &b);
+
+// Constants of one short-range launch (kernel argument, passed by value).
+struct ShortConsts {
+    double Lx, Ly, Lz;
+    double rc2;          // r_cutoff^2
+    double alpha;
+    double g1, g2, g12;  // gamma_1, gamma_2, gamma_1*gamma_2
+    double ln_acc;       // log(accuracy)
+    double inv2pieps;    // 1 / (2 pi eps_0)
+    uint32_t sem;        // QEM_SEM_* bits
+    int32_t Nt;
+    int32_t single;      // point-charge part only (F_short_ij single=true)
+    int32_t jt_n;        // Bessel table intervals (mixed precision)
+    float jt_dx, jt_inv_dx;
+};
+
+// Gauss-Legendre rule on [-1,1] (ascending) plus its FP32 copy.
+struct GLRule {
+    int n = 0;
+    std::vector<double> x, w;
+};
+GLRule make_gl_rule(int n);
+
+// Taylor coefficients of J0 about interval centres: 8 floats per interval,
+// J0(c + t) = sum_k C[k] t^k, J1 = -d/dt of the same polynomial.
+std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals);
+
+// ---- neighbour list (neighbor.hip) ------------------------------------
+struct NeighborList {
+    int64_t n = -1;       // atom count the list was built for
+    int64_t npairs = 0;
+    int64_t builds = 0;
+    DevBuf pi, pj;        // int32 pair endpoints
+    // per atom a: pairs with a as first endpoint are ids
+    // i_idx[i_start[a] .. i_end[a]) (or the ids themselves when !has_i_idx),
+    // pairs with a as second endpoint are j_idx[j_start[a] .. j_end[a])
+    bool has_i_idx = false;
+    DevBuf i_start, i_end, i_idx;
+    DevBuf j_start, j_end, j_idx;
+    // scratch
+    DevBuf cell_key, cell_key_sorted, atom_idx, atom_idx_sorted;
+    DevBuf cell_start, cell_end, counts, row_ptr, tmp;
+};
+// Builds the list for device coordinates; synchronises the stream once
+// (to learn the pair count).  Returns hipSuccess or the first error.
+hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
+                               double Lx, double Ly, double rc, hipStream_t s,
+                               std::string &err);
+// Installs a caller-given pair list (device copies of pairs, i < j not required).
+hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs,
+                         int64_t npairs, hipStream_t s, std::string &err);
+
+// ---- short range (short_range.hip) ------------------------------------
+struct ShortWork {
+    DevBuf glx, glw;      // double nodes / weights
+    DevBuf glxf, glwf;    // float nodes / weights
+    DevBuf jtab;          // float4 pairs per interval
+    int jt_n = 0;
+    float jt_dx = 0.f;
+    DevBuf fpair;         // double4 per pair
+};
+// Evaluates every pair of the list into ws.fpair (fx, fy, fz_i, fz_j).
+hipError_t launch_pair_forces(const ShortConsts &c, int precision,
+                              const ShortWork &ws, const int32_t *d_pi,
+                              const int32_t *d_pj, int64_t npairs,
+                              const double *d_xyz, const double *d_q,
+                              double *d_fpair, hipStream_t s);
+// Per atom: gather pair forces (i side and j side) and add the self term.
+hipError_t launch_gather_self(const ShortConsts &c, int precision,
+                              const ShortWork &ws, const NeighborList &nl,
+                              int64_t n, const double *d_xyz,
+                              const double *d_q, const double *d_fpair,
+                              double *d_f, hipStream_t s);
+// Self term only (F_short_i), n atoms.
+hipError_t launch_self_only(const ShortConsts &c, int precision,
+                            const ShortWork &ws, int64_t n,
+                            const double *d_xyz, const double *d_q,
+                            double *d_f, hipStream_t s);
+// Independent pairs (F_short_ij): positions/charges given per pair.
+hipError_t launch_pair_direct(const ShortConsts &c, int precision,
+                              const ShortWork &ws, int64_t m,
+                              const double *d_ri, const double *d_rj,
+                              const double *d_qi, const double *d_qj,
+                              double *d_fpair, hipStream_t s);
+
+// ---- long range (long_range.hip) ---------------------------------------
+struct LongConsts {
+    double Lz, g1, g2, g12;
+    double inv_area_eps;  // 1 / (Lx Ly eps_0)
+    double weight;        // batch weight (sum_K / P), or < 0 for exp(-k^2/4a)
+    double inv4alpha;
+    uint32_t parts;       // QEM_LONG_* bits
+    int32_t precision;
+};
+struct LongWork {
+    DevBuf keys, keys_sorted, idx, perm, tmp;
+    DevBuf sx, sy, sz, sq;   // atoms in z order
+    DevBuf kv;               // P x 4 doubles (kx, ky, k, weight)
+    DevBuf partA;            // [P][nchunk][4] chunk sums
+    DevBuf offA;             // [P][nchunk][4] chunk offsets
+    DevBuf totA;             // [P][4] totals
+    DevBuf q0part, q0off;    // k = 0 chunk sums / offsets
+    DevBuf partB;            // [ntile][n][3]
+    static constexpr int kSlots = 4;   // pinned staging ring for k vectors
+    void *h_kv[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+    size_t h_kv_cap[kSlots] = {0, 0, 0, 0};
+    hipEvent_t kv_ev[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+    int kv_slot = 0;
+};
+// z-sort; fills lw.perm (int32, rank -> atom) and the sorted SoA arrays.
+hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz,
+                        const double *d_q, hipStream_t s, std::string &err);
+// Full long-range force for the k vectors already in lw.kv (P rows).
+// accumulate != 0 adds into d_f instead of overwriting.
+hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n,
+                       int64_t P, double *d_f, int accumulate, hipStream_t s,
+                       std::string &err);
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+}  // namespace qem
+
+// The context behind the opaque qem_ctx handle.
+struct qem_ctx {
+    qem_params p{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    // K set (host)
+    std::vector<double> kset;   // |K| x 3
+    std::vector<float> kprob;   // |K|
+    double sum_K = 0.0;
+    std::vector<double> alias_prob;
+    std::vector<int64_t> alias_idx;
+    // quadrature
+    qem::GLRule gl;
+    qem::ShortWork sw;
+    qem::NeighborList nl;
+    int64_t n_steps = 0;
+    qem::LongWork lw;
+    // host-pointer staging
+    qem::DevBuf d_xyz, d_q, d_f, d_f2, d_aux;
+};

@@ -1,0 +1,453 @@
+// QEM short range on gfx950: per-pair Gauss-Legendre quadrature of the
+// image-charge Sommerfeld integrals, the self-image term, and a
+// deterministic per-atom gather.
+//
+// Reference: F_short_ij (src/Force_short.jl:25-78), integrands
+// (src/Force_short.jl:4-22), Gamma_1/2 and dz_Gamma_1/2
+// (src/greens_function.jl:68-116), element set-up (17-48), Gauss_Legendre
+// (src/Gaussian_int.jl:9-28), F_short_i (src/Force_short.jl:82-107) and the
+// accumulation of Molly.forces(::QEM_short) (src/QEM_short.jl:41-56).
+//
+// Work mapping: one lane per pair.  All pairs cost the same (8 node sets of
+// N_t nodes), so a wave runs in lockstep with no divergence: quadrature
+// nodes and weights are wave-uniform (scalar loads), the per-pair limits
+// and image factors live in registers.  Per node the integrands of the
+// rho-force (k Gamma J1) and z-force (dz Gamma J0) share k, the exponentials,
+// the denominator and the Bessel pair, so each node set is one loop.
+//
+// Precision modes
+//   FP64:  everything double; J0/J1 from the device math library.
+//   MIXED: node arithmetic, exponentials (v_exp_f32), reciprocal
+//          (v_rcp_f32) and J0/J1 in FP32; J0/J1 come from a table of degree-7
+//          Taylor polynomials of J0 about the centres of 0.5-wide intervals
+//          (LDS resident; J1 = -J0' from the same polynomial, p and p' by one
+//          Horner pass).  Per-pair set-up, closed-form image terms and all
+//          sums over images, pairs and atoms stay FP64.
+#include "qem_internal.h"
+
+namespace qem {
+namespace {
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+__device__ __forceinline__ double jl_sign(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
+
+// ---------------------------------------------------------------------------
+// Node sums for one image l and one integral type.
+//   point: sum_n w k [e^{-k a} e^{-2kLz} / D(k)] {J1, J0}(k rho)
+//   gauss: sum_n w k [e^{-k a - k^2/(4 alpha)} / D(k)] {J1, J0}(k rho)
+// with D(k) = g12 e^{-2kLz} - 1 and the nodes mapped to [0, K].
+// The constant factors 0.5 b_l (g12 for point) and s_l are applied by the
+// caller.
+__device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool gauss,
+                                           const ShortConsts &c, const double *__restrict__ glx,
+                                           const double *__restrict__ glw, double &sr, double &sz)
+{
+    const double h = 0.5 * K;
+    const double inv4a = 0.25 / c.alpha;
+    double ar = 0.0, az = 0.0;
+    for (int n = 0; n < c.Nt; ++n) {
+        const double k = h + h * glx[n];
+        const double w = h * glw[n];
+        const double E = exp(-2.0 * k * c.Lz);
+        const double D = c.g12 * E - 1.0;
+        const double num = gauss ? exp(-k * a - k * k * inv4a) : exp(-k * a) * E;
+        const double wkt = w * k * (num / D);
+        const double x = k * rho;
+        ar += wkt * j1(x);
+        az += wkt * j0(x);
+    }
+    sr = ar;
+    sz = az;
+}
+
+// FP32 J0/J1 from the LDS table.
+__device__ __forceinline__ void bessel01_tab(float x, const float4 *__restrict__ tab, int ntab,
+                                             float dx, float inv_dx, float &J0, float &J1)
+{
+    int m = (int)(x * inv_dx);
+    m = m < ntab - 1 ? m : ntab - 1;
+    const float t = x - ((float)m + 0.5f) * dx;
+    const float4 c0 = tab[2 * m];
+    const float4 c1 = tab[2 * m + 1];
+    // p(t) and p'(t) together
+    float p = c1.w;
+    float dp = 0.f;
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.z);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.y);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.x);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.w);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.z);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.y);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.x);
+    J0 = p;
+    J1 = -dp;
+}
+
+__device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool gauss,
+                                            const ShortConsts &c, const float *__restrict__ glx,
+                                            const float *__restrict__ glw,
+                                            const float4 *__restrict__ tab, float &sr, float &sz)
+{
+    const float h = 0.5f * K;
+    const float c2 = -2.0f * (float)c.Lz * kLog2e;
+    const float ca = -a * kLog2e;
+    const float cg = -kLog2e * (float)(0.25 / c.alpha);
+    const float g12 = (float)c.g12;
+    float ar = 0.f, az = 0.f;
+    for (int n = 0; n < c.Nt; ++n) {
+        const float k = fmaf(h, glx[n], h);
+        const float w = h * glw[n];
+        const float E = __builtin_amdgcn_exp2f(k * c2);
+        const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+        const float num = gauss ? __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca))
+                                : __builtin_amdgcn_exp2f(k * ca) * E;
+        const float wkt = w * k * (num * rD);
+        float J0, J1;
+        bessel01_tab(k * rho, tab, c.jt_n, c.jt_dx, c.jt_inv_dx, J0, J1);
+        ar = fmaf(wkt, J1, ar);
+        az = fmaf(wkt, J0, az);
+    }
+    sr = ar;
+    sz = az;
+}
+
+struct PairForce {
+    double fx, fy, fz, fzj;
+};
+
+// F_short_ij for one pair (src/Force_short.jl:25-78).  Returns zeros when
+// the minimum-image distance exceeds r_cutoff (the reference's per-pair
+// 2-point neighbour list is empty then, line 36-41).
+template <int PREC>
+__device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__ glx,
+                                const double *__restrict__ glw, const float *__restrict__ glxf,
+                                const float *__restrict__ glwf, const float4 *__restrict__ tab,
+                                double qi, double qj, double xi, double yi, double zi, double xj,
+                                double yj, double zj)
+{
+    PairForce out{0.0, 0.0, 0.0, 0.0};
+    const double dx = xi - xj, dy = yi - yj;
+    double mx, my, d2;
+    {
+#pragma clang fp contract(off)
+        mx = dx - c.Lx * rint(dx / c.Lx);
+        my = dy - c.Ly * rint(dy / c.Ly);
+        d2 = mx * mx + my * my;
+    }
+    if (d2 > c.rc2) return out;
+    const double rho = sqrt(d2);
+    const double zn = fabs(zi - zj), zp = zi + zj;
+    const double sg = jl_sign(zi - zj);
+    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
+
+    double Fsr = 0.0, Fsz = 0.0;
+    double part[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const double a = l == 0 ? zn : (l == 1 ? zp : (l == 2 ? 2.0 * c.Lz - zp : 2.0 * c.Lz - zn));
+        const double b = l == 0 ? 1.0 : (l == 1 ? c.g1 : (l == 2 ? c.g2 : c.g12));
+        const double s = l == 0 ? -sg : (l == 1 ? -1.0 : (l == 2 ? 1.0 : sg));
+        // greens_element_ij_init limits (src/greens_function.jl:26-27)
+        const double kf1 = -c.ln_acc / (2.0 * c.Lz + a);
+        const double kf2 = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc) -
+                           2.0 * c.alpha * a;
+        const double Kp = swap ? kf1 : kf2;  // point-charge integral limit
+        const double Kg = swap ? kf2 : kf1;  // gaussian integral limit
+        // closed-form image sums (src/Force_short.jl:49,65)
+        const double r2 = a * a + rho * rho;
+        const double inv32 = 1.0 / (r2 * sqrt(r2));
+        const double cr = 0.5 * b * rho * inv32;
+        const double cz = 0.5 * b * a * s * inv32;
+        double pr = 0.0, pz = 0.0, gr = 0.0, gz = 0.0;
+        if (b != 0.0) {
+            if (PREC == QEM_PREC_FP64) {
+                nodes_fp64(Kp, a, rho, false, c, glx, glw, pr, pz);
+                if (!c.single) nodes_fp64(Kg, a, rho, true, c, glx, glw, gr, gz);
+            } else {
+                float fr, fz;
+                nodes_mixed((float)Kp, (float)a, (float)rho, false, c, glxf, glwf, tab, fr, fz);
+                pr = fr;
+                pz = fz;
+                if (!c.single) {
+                    nodes_mixed((float)Kg, (float)a, (float)rho, true, c, glxf, glwf, tab, fr, fz);
+                    gr = fr;
+                    gz = fz;
+                }
+            }
+        }
+        const double hb = 0.5 * b;
+        // F_sr: -sum GL[k Gamma_2 J1] + closed + sum GL[k Gamma_1 e^{-k^2/4a} J1]
+        Fsr += -(hb * c.g12) * pr + cr + hb * gr;
+        // F_sz: +sum GL[dz Gamma_2 J0] - closed - sum GL[dz Gamma_1 e^{..} J0]
+        part[l] = (hb * c.g12 * s) * pz - cz - (hb * s) * gz;
+        Fsz += part[l];
+    }
+    const double pref = qi * qj * c.inv2pieps;
+    if (rho != 0.0) {
+        const bool mi = (c.sem & QEM_SEM_MINIMAGE_DIR) != 0;
+        const double ux = mi ? mx : dx, uy = mi ? my : dy;
+        out.fx = pref * (Fsr * ux / rho);
+        out.fy = pref * (Fsr * uy / rho);
+    }
+    out.fz = pref * Fsz;
+    if (c.sem & QEM_SEM_NEWTON_IMAGES)
+        out.fzj = pref * (-part[0] + part[1] + part[2] - part[3]);
+    else
+        out.fzj = -out.fz;
+    return out;
+}
+
+// Self-image z force F_short_i (src/Force_short.jl:82-107).  Images 1 and 4
+// have sign_a = 0 and contribute exactly zero; they are skipped.
+template <int PREC>
+__device__ double self_force_z(const ShortConsts &c, const double *__restrict__ glx,
+                               const double *__restrict__ glw, const float *__restrict__ glxf,
+                               const float *__restrict__ glwf, double qi, double zi)
+{
+    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
+    double P1 = 0.0, P2 = 0.0, G = 0.0;
+#pragma unroll
+    for (int l = 1; l <= 2; ++l) {
+        const double a = l == 1 ? 2.0 * zi : 2.0 * c.Lz - 2.0 * zi;
+        const double b = l == 1 ? c.g1 : c.g2;
+        const double s = l == 1 ? -1.0 : 1.0;
+        const double kf1 = -c.ln_acc / (2.0 * c.Lz + a);
+        const double kf2 = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc) -
+                           2.0 * c.alpha * a;
+        const double Kp = swap ? kf1 : kf2;
+        const double Kg = swap ? kf2 : kf1;
+        double pz = 0.0, gz = 0.0;
+        if (b != 0.0) {
+            // J0(0) = 1: the node sums reduce to sum w k num / D
+            for (int pass = 0; pass < (c.single ? 1 : 2); ++pass) {
+                const bool gauss = pass == 1;
+                const double K = gauss ? Kg : Kp;
+                double acc = 0.0;
+                if (PREC == QEM_PREC_FP64) {
+                    const double h = 0.5 * K;
+                    const double inv4a = 0.25 / c.alpha;
+                    for (int n = 0; n < c.Nt; ++n) {
+                        const double k = h + h * glx[n];
+                        const double w = h * glw[n];
+                        const double E = exp(-2.0 * k * c.Lz);
+                        const double D = c.g12 * E - 1.0;
+                        const double num = gauss ? exp(-k * a - k * k * inv4a) : exp(-k * a) * E;
+                        acc += w * k * (num / D);
+                    }
+                } else {
+                    const float h = 0.5f * (float)K;
+                    const float c2 = -2.0f * (float)c.Lz * kLog2e;
+                    const float ca = -(float)a * kLog2e;
+                    const float cg = -kLog2e * (float)(0.25 / c.alpha);
+                    const float g12 = (float)c.g12;
+                    float af = 0.f;
+                    for (int n = 0; n < c.Nt; ++n) {
+                        const float k = fmaf(h, glxf[n], h);
+                        const float w = h * glwf[n];
+                        const float E = __builtin_amdgcn_exp2f(k * c2);
+                        const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+                        const float num = gauss ? __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca))
+                                                : __builtin_amdgcn_exp2f(k * ca) * E;
+                        af = fmaf(w * k, num * rD, af);
+                    }
+                    acc = af;
+                }
+                if (gauss) gz = acc; else pz = acc;
+            }
+        }
+        const double hb = 0.5 * b;
+        P1 += (hb * c.g12 * s) * pz;
+        P2 += b * a * s / (a * a * a);
+        G += (hb * s) * gz;
+    }
+    P2 *= 0.5;
+    const double Fsz = -P1 + P2 + G;
+    return -(qi * qi) * c.inv2pieps * Fsz;
+}
+
+// LDS copy of the Bessel table (mixed precision only).
+__device__ __forceinline__ const float4 *stage_table(const ShortConsts &c,
+                                                     const float4 *__restrict__ gtab,
+                                                     float4 *lds)
+{
+    const int n = 2 * c.jt_n;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) lds[t] = gtab[t];
+    __syncthreads();
+    return lds;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pair_list(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf,
+    const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
+    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
+    const double *__restrict__ q, double4 *__restrict__ fpair)
+{
+    extern __shared__ float4 lds_tab[];
+    const float4 *tab = nullptr;
+    if (PREC == QEM_PREC_MIXED) tab = stage_table(c, gtab, lds_tab);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+        const int64_t i = pi[p], j = pj[p];
+        PairForce f = pair_force<PREC>(c, glx, glw, glxf, glwf, tab, q[i], q[j], xyz[3 * i],
+                                       xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
+                                       xyz[3 * j + 1], xyz[3 * j + 2]);
+        fpair[p] = make_double4(f.fx, f.fy, f.fz, f.fzj);
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pair_direct(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf,
+    const float4 *__restrict__ gtab, int64_t m, const double *__restrict__ ri,
+    const double *__restrict__ rj, const double *__restrict__ qi,
+    const double *__restrict__ qj, double4 *__restrict__ fpair)
+{
+    extern __shared__ float4 lds_tab[];
+    const float4 *tab = nullptr;
+    if (PREC == QEM_PREC_MIXED) tab = stage_table(c, gtab, lds_tab);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += stride) {
+        PairForce f = pair_force<PREC>(c, glx, glw, glxf, glwf, tab, qi[p], qj[p], ri[3 * p],
+                                       ri[3 * p + 1], ri[3 * p + 2], rj[3 * p], rj[3 * p + 1],
+                                       rj[3 * p + 2]);
+        fpair[p] = make_double4(f.fx, f.fy, f.fz, f.fzj);
+    }
+}
+
+// Per atom: sum of pair forces with the atom as first endpoint, minus the
+// xy (and, as written, z) forces of pairs with it as second endpoint, plus
+// the self-image term (src/QEM_short.jl:41-56).  Fixed order, no atomics.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_gather_self(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf, int64_t n,
+    const double *__restrict__ xyz, const double *__restrict__ q,
+    const double4 *__restrict__ fpair, const int32_t *__restrict__ i_start,
+    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
+    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
+    const int32_t *__restrict__ j_idx, int with_pairs, double *__restrict__ f)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (with_pairs) {
+        const int32_t t0 = i_start[a], t1 = i_end[a];
+        for (int32_t t = t0; t < t1; ++t) {
+            const double4 v = fpair[i_idx ? i_idx[t] : t];
+            fx += v.x;
+            fy += v.y;
+            fz += v.z;
+        }
+        const int32_t u0 = j_start[a], u1 = j_end[a];
+        for (int32_t u = u0; u < u1; ++u) {
+            const double4 v = fpair[j_idx[u]];
+            fx -= v.x;
+            fy -= v.y;
+            fz += v.w;
+        }
+    }
+    fz += self_force_z<PREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2]);
+    f[3 * a] = fx;
+    f[3 * a + 1] = fy;
+    f[3 * a + 2] = fz;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// grid for grid-stride pair kernels: enough blocks to fill 256 CUs several
+// times over, without relaunching the LDS table stage per 256 pairs
+inline unsigned pair_grid(int64_t m)
+{
+    const int64_t want = (m + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
+}
+
+}  // namespace
+
+hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWork &ws,
+                              const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
+                              const double *d_xyz, const double *d_q, double *d_fpair,
+                              hipStream_t s)
+{
+    if (npairs <= 0) return hipSuccess;
+    if (precision == QEM_PREC_MIXED) {
+        const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+        k_pair_list<QEM_PREC_MIXED><<<pair_grid(npairs), 256, lds, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), ws.jtab.as<float4>(), d_pi, d_pj, npairs, d_xyz, d_q,
+            reinterpret_cast<double4 *>(d_fpair));
+    } else {
+        k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), nullptr, d_pi, d_pj, npairs, d_xyz, d_q,
+            reinterpret_cast<double4 *>(d_fpair));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_direct(const ShortConsts &c, int precision, const ShortWork &ws,
+                              int64_t m, const double *d_ri, const double *d_rj,
+                              const double *d_qi, const double *d_qj, double *d_fpair,
+                              hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    if (precision == QEM_PREC_MIXED) {
+        const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+        k_pair_direct<QEM_PREC_MIXED><<<pair_grid(m), 256, lds, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), ws.jtab.as<float4>(), m, d_ri, d_rj, d_qi, d_qj,
+            reinterpret_cast<double4 *>(d_fpair));
+    } else {
+        k_pair_direct<QEM_PREC_FP64><<<pair_grid(m), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), nullptr, m, d_ri, d_rj, d_qi, d_qj,
+            reinterpret_cast<double4 *>(d_fpair));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWork &ws,
+                              const NeighborList &nl, int64_t n, const double *d_xyz,
+                              const double *d_q, const double *d_fpair, double *d_f,
+                              hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
+    if (precision == QEM_PREC_MIXED)
+        k_gather_self<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, reinterpret_cast<const double4 *>(d_fpair),
+            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
+            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), 1, d_f);
+    else
+        k_gather_self<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, reinterpret_cast<const double4 *>(d_fpair),
+            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
+            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), 1, d_f);
+    return hipGetLastError();
+}
+
+hipError_t launch_self_only(const ShortConsts &c, int precision, const ShortWork &ws,
+                            int64_t n, const double *d_xyz, const double *d_q, double *d_f,
+                            hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (precision == QEM_PREC_MIXED)
+        k_gather_self<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, nullptr, 0, d_f);
+    else
+        k_gather_self<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, nullptr, 0, d_f);
+    return hipGetLastError();
+}
+
+}  // namespace qem
