@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""QEM force-path throughput on one MI355X (libqem_hip.so, gfx950).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload slab_n100k]
+
+A step is one full electrostatic force evaluation of the quasi-Ewald method
+(short range: neighbour list at its iter_period=100 frequency + pair
+quadrature + self term + gather; long range: z sort + random batch of P
+k-vectors + k = 0 term) on device-resident synthetic inputs, with a fresh
+k batch drawn per step.  Prints ONE JSON line (other output goes to stderr).
+
+Metric: atom-force evals/s = N / t_step, mixed precision (FP32 quadrature,
+Bessel, exp, sincos; FP64 accumulation), next to the FP64 plain GPU path on
+the same inputs and k batches, with the mixed result's rel-RMS error against
+the FP64 path.  The workload is slab_n100k (N = 100,000), the size at which
+the mixed-vs-FP64 target is stated; slab_n10k is reported as a secondary
+figure.  roofline: the pair-quadrature kernel (the dominant kernel) against
+the FP32 VALU peak, its duration measured with HIP events on the stream it
+runs on.  cpu_baseline: the C FP64 restatement of the reference algorithm
+(oracle/) on the host cores, timed on a bounded sample and extrapolated.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic flop count per unit (add/mul 1, FMA 2, exp/sin/cos/rcp/sqrt 1
+# plus argument arithmetic, joint J0+J1 evaluation 40)
+FLOP_NODE_POINT = 62      # one point-charge quadrature node (rho and z integrands)
+FLOP_NODE_GAUSS = 64      # one gaussian-screened quadrature node
+FLOP_NODE_SELF_POINT = 22
+FLOP_NODE_SELF_GAUSS = 24
+FLOP_ATOM_K = 90          # long range per (atom, k): structure factor 20 + force 70
+PEAK_FP32_TFLOPS = 157.3  # MI355X published vector FP32
+PEAK_FP64_TFLOPS = 78.6   # MI355X published vector FP64
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pair_flops(npairs: int, N_t: int) -> float:
+    return float(npairs) * 4 * N_t * (FLOP_NODE_POINT + FLOP_NODE_GAUSS)
+
+
+def run_precision(cfg, xyz_d, q_d, f_d, precision, steps, warmup, seed0=1000):
+    from qem4molly_amd import _lib as L
+    p = L.default_params()
+    p.Lx, p.Ly, p.Lz = cfg.L
+    p.r_cutoff, p.alpha = cfg.r_cutoff, cfg.alpha
+    p.gamma_1, p.gamma_2 = cfg.gamma_1, cfg.gamma_2
+    p.eps_0, p.accuracy = cfg.eps_0, cfg.accuracy
+    p.N_t, p.iter_period, p.rbe_p = cfg.N_t, 100, cfg.P
+    p.precision = L.PREC_MIXED if precision == "mixed" else L.PREC_FP64
+    ctx = L.Context(p)
+    ctx.timing_enable(True)
+    step = 0
+
+    def one():
+        nonlocal step
+        kidx = ctx.sample_k(seed0 + step, cfg.P)
+        ctx.forces_dev(cfg.n, xyz_d.ptr, q_d.ptr, kidx, f_d.ptr)
+        step += 1
+
+    for _ in range(warmup):
+        one()
+    L.synchronize()
+    ctx.timing_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    L.synchronize()
+    dt = time.perf_counter() - t0
+    phases = ctx.timing_read(reset=True)
+    npairs, builds, nsteps = ctx.short_state()
+    f = f_d.download()
+    ctx.close()
+    return {"ms_per_step": dt / steps * 1e3, "phases": phases, "npairs": npairs,
+            "builds": builds, "f": f}
+
+
+def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, target_s=8.0):
+    """Times the oracle on a bounded sample and extrapolates to one step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    p = O.params(cfg.L, cfg.r_cutoff, cfg.alpha, cfg.gamma_1, cfg.gamma_2, cfg.eps_0,
+                 cfg.accuracy, cfg.N_t, 0)
+    pairs = pairs_dev_ctx_pairs
+    rng = np.random.default_rng(0)
+    # calibrate on a small sample, then size the timed sample to ~target_s
+    m0 = min(len(pairs), 2000)
+    sel = rng.choice(len(pairs), m0, replace=False)
+    t = time.perf_counter()
+    O.short_pair_forces(p, xyz, q, pairs[sel], nthreads=threads)
+    per_pair = (time.perf_counter() - t) / m0
+    m = int(min(len(pairs), max(m0, 0.6 * target_s / max(per_pair, 1e-9))))
+    sel = rng.choice(len(pairs), m, replace=False)
+    t = time.perf_counter()
+    O.short_pair_forces(p, xyz, q, pairs[sel], nthreads=threads)
+    t_pairs = (time.perf_counter() - t) * len(pairs) / m
+    # self term on a subset of atoms
+    ns = min(cfg.n, 20000)
+    idx = rng.choice(cfg.n, ns, replace=False)
+    t = time.perf_counter()
+    O.short_forces(p, xyz[idx], q[idx], np.zeros((0, 2), np.int32), nthreads=threads)
+    t_self = (time.perf_counter() - t) * cfg.n / ns
+    # long range: all atoms, a subset of the batch
+    ps = max(1, min(len(kvecs), int(0.3 * target_s / max(cfg.n * 4e-7 / threads, 1e-9))))
+    t = time.perf_counter()
+    O.long_forces(p, xyz, q, kvecs[:ps], weight=weight, nthreads=threads)
+    t_long = (time.perf_counter() - t) * len(kvecs) / ps
+    t_step = t_pairs + t_self + t_long
+    return {
+        "value": cfg.n / t_step, "unit": "atom-force evals/s", "cores": threads,
+        "kind": "port",
+        "sample": (f"{cfg.name}: {m} of {len(pairs)} pairs, {ns} of {cfg.n} self terms, "
+                   f"{ps} of {len(kvecs)} k vectors over all atoms; each part scaled "
+                   f"linearly to one full step (C FP64 restatement, OpenMP)"),
+        "seconds_per_step_est": t_step,
+        "parts_s": {"pairs": t_pairs, "self": t_self, "long": t_long},
+    }
+
+
+def traffic_from_profiles(workload):
+    path = os.path.join(ROOT, "profiles", f"pair_kernel_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return float(d["hbm_bytes_per_dispatch"])
+    except Exception:
+        return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="slab_n100k",
+                    choices=["slab_n100", "slab_n10k", "slab_n100k"])
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="steps for the FP64 plain comparison (default: --steps)")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=None)
+    a = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print("bench.py: this build runs the QEM force path on one GPU only; "
+                  f"--gpus {a.gpus} (WORLD_SIZE={world}) is not supported. How the path "
+                  "would shard is described in DESIGN.md.", file=sys.stderr)
+        return 2
+
+    from qem4molly_amd import _lib as L
+    from qem4molly_amd.workloads import CONFIGS, make_system, rel_rms
+
+    def measure(name, steps, warmup, fp64_steps):
+        cfg = CONFIGS[name]
+        xyz, q = make_system(cfg, seed=0)
+        xd, qd, fd = L.DeviceArray(xyz), L.DeviceArray(q), L.DeviceArray(np.zeros_like(xyz))
+        log(f"[bench] {name}: N={cfg.n} mixed x {steps} steps")
+        mix = run_precision(cfg, xd, qd, fd, "mixed", steps, warmup)
+        log(f"[bench] {name}: mixed {mix['ms_per_step']:.3f} ms/step; fp64 x {fp64_steps} steps")
+        f64 = run_precision(cfg, xd, qd, fd, "fp64", fp64_steps, warmup)
+        log(f"[bench] {name}: fp64 {f64['ms_per_step']:.3f} ms/step")
+        # both runs end on the same k batch (same seeds, same step count)
+        same_batch = (steps + warmup) == (fp64_steps + warmup)
+        err = rel_rms(mix["f"], f64["f"]) if same_batch else None
+        return cfg, xyz, q, mix, f64, err
+
+    fp64_steps = a.fp64_steps or a.steps
+    cfg, xyz, q, mix, f64, err = measure(a.workload, a.steps, a.warmup, fp64_steps)
+    value = cfg.n / (mix["ms_per_step"] * 1e-3)
+    value64 = cfg.n / (f64["ms_per_step"] * 1e-3)
+
+    pair_ms, pair_calls = mix["phases"]["pair"]
+    pair_avg_ms = pair_ms / max(pair_calls, 1)
+    fl = pair_flops(mix["npairs"], cfg.N_t)
+    achieved = fl / (pair_avg_ms * 1e-3) / 1e12
+    p64_ms, p64_calls = f64["phases"]["pair"]
+    p64_avg = p64_ms / max(p64_calls, 1)
+    achieved64 = fl / (p64_avg * 1e-3) / 1e12
+
+    out = {
+        "metric": "atom-force evals/s, mixed vs FP64 plain on 1 GPU at 1e-6 rel RMS force err",
+        "value": value,
+        "unit": "atom-force evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": mix["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64",
+        "data": "synthetic (seeded uniform slab, see qem4molly_amd/workloads.py)",
+        "config": {
+            "workload": cfg.name, "N": cfg.n, "box": list(cfg.L), "gamma_down": cfg.gamma_1,
+            "gamma_up": cfg.gamma_2, "alpha": cfg.alpha, "r_cutoff": cfg.r_cutoff,
+            "N_t": cfg.N_t, "P": cfg.P, "accuracy": cfg.accuracy, "iter_period": 100,
+            "n_pairs": mix["npairs"], "precision": "mixed", "parallelism": "single GPU",
+        },
+        "fp64_plain": {"value": value64, "ms_per_step": f64["ms_per_step"]},
+        "mixed_over_fp64": value / value64,
+        "rel_rms_err_vs_fp64": err,
+        "phases_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in mix["phases"].items()},
+        "phases_ms_per_step_fp64": {k: v[0] / max(fp64_steps, 1)
+                                    for k, v in f64["phases"].items()},
+        "roofline": {
+            "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic_from_profiles(cfg.name),
+            "kernel": "k_pair_list<mixed>", "avg_ms": pair_avg_ms,
+            "flop_per_launch": fl,
+        },
+        "roofline_fp64": {
+            "bound": "valu", "achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved64 / PEAK_FP64_TFLOPS,
+            "kernel": "k_pair_list<fp64>", "avg_ms": p64_avg,
+        },
+    }
+
+    if not a.no_secondary and a.workload != "slab_n10k":
+        c2, _, _, m2, d2, e2 = measure("slab_n10k", 50, 3, 50)
+        out["secondary"] = {
+            "workload": "slab_n10k", "value": c2.n / (m2["ms_per_step"] * 1e-3),
+            "ms_per_step": m2["ms_per_step"],
+            "fp64_plain_value": c2.n / (d2["ms_per_step"] * 1e-3),
+            "mixed_over_fp64": d2["ms_per_step"] / m2["ms_per_step"],
+            "rel_rms_err_vs_fp64": e2,
+        }
+
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+            min(16, os.cpu_count() or 1)
+        pctx_p = L.default_params()
+        pctx_p.Lx, pctx_p.Ly, pctx_p.Lz = cfg.L
+        pctx_p.r_cutoff, pctx_p.alpha = cfg.r_cutoff, cfg.alpha
+        pctx_p.gamma_1, pctx_p.gamma_2 = cfg.gamma_1, cfg.gamma_2
+        pctx_p.N_t, pctx_p.rbe_p = cfg.N_t, cfg.P
+        with L.Context(pctx_p) as c:
+            pairs = c.neighbor_list(xyz)
+            K, Pk, sK = c.kset()
+            kidx = c.sample_k(1000, cfg.P)
+        log(f"[bench] cpu baseline on {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(cfg, xyz, q, pairs, K[kidx], sK / cfg.P, threads)
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

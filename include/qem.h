@@ -178,6 +178,18 @@ int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
 int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds,
                     int64_t *n_steps);
 
+/* Per-phase GPU time, measured with HIP events recorded on the stream each
+ * phase is launched on (no counterpart in the reference; for benchmarks).
+ * qem_timing_read synchronises, returns the summed milliseconds and launch
+ * counts per phase since the last reset, and optionally resets them. */
+#define QEM_PHASE_NEIGHBOR  0  /* neighbour-list build            */
+#define QEM_PHASE_PAIR      1  /* short-range pair quadrature     */
+#define QEM_PHASE_GATHER    2  /* per-atom gather + self term     */
+#define QEM_PHASE_LONG      3  /* z sort + k-space passes         */
+#define QEM_NPHASES         4
+int qem_timing_enable(qem_ctx *ctx, int32_t on);
+int qem_timing_read(qem_ctx *ctx, double *ms, int64_t *calls, int32_t reset);
+
 /* Gauss-Legendre rule on [-1, 1], ascending nodes (stands in for
  * GaussQuadrature.legendre(n), used at src/Gaussian_int.jl:11); 1 <= n <= 256. */
 int qem_gauss_legendre(int32_t n, double *x, double *w);

@@ -47,8 +47,11 @@ EXPORTS = (
     "qem_short_forces_dev", "qem_long_forces_dev", "qem_forces_dev",
     "qem_short_state", "qem_gauss_legendre", "qem_bessel01_mixed",
     "qem_dev_malloc", "qem_dev_free", "qem_memcpy_htod", "qem_memcpy_dtoh",
-    "qem_device_synchronize", "qem_dev_copy_kernel",
+    "qem_device_synchronize", "qem_dev_copy_kernel", "qem_timing_enable",
+    "qem_timing_read",
 )
+
+PHASES = ("neighbor", "pair", "gather", "long")
 
 
 class QEMError(RuntimeError):
@@ -107,6 +110,8 @@ _SIGS = {
     "qem_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
     "qem_device_synchronize": (C.c_int, []),
     "qem_dev_copy_kernel": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "qem_timing_enable": (C.c_int, [_vp, _i32]),
+    "qem_timing_read": (C.c_int, [_vp, _pd, _pi64, _i32]),
 }
 
 _lib = None
@@ -310,6 +315,15 @@ class Context:
         self._check(lib().qem_long_forces_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
                                               k_idx.shape[0], d_f, stream),
                     "qem_long_forces_dev")
+
+    def timing_enable(self, on=True):
+        self._check(lib().qem_timing_enable(self._h, int(bool(on))), "qem_timing_enable")
+
+    def timing_read(self, reset=True):
+        ms = np.zeros(4); calls = np.zeros(4, dtype=np.int64)
+        self._check(lib().qem_timing_read(self._h, ptr(ms), ptr(calls), int(bool(reset))),
+                    "qem_timing_read")
+        return {ph: (float(ms[i]), int(calls[i])) for i, ph in enumerate(PHASES)}
 
     def bessel01_mixed(self, x):
         x = f64(x)

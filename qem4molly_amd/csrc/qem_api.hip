@@ -448,6 +448,41 @@ hipStream_t pick_stream(qem_ctx *ctx, void *stream)
     return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
 }
 
+// phase timing: a start/stop event pair around each timed launch sequence
+hipEvent_t take_event(qem_ctx *ctx)
+{
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct PhaseScope {
+    qem_ctx *ctx;
+    int phase;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    PhaseScope(qem_ctx *c, int ph, hipStream_t st) : ctx(c), phase(ph), s(st)
+    {
+        if (ctx->timing && (a = take_event(ctx))) (void)hipEventRecord(a, s);
+    }
+    ~PhaseScope()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(ctx);
+        if (!b) {
+            ctx->ev_pool.push_back(a);
+            return;
+        }
+        (void)hipEventRecord(b, s);
+        ctx->marks.push_back({phase, a, b});
+    }
+};
+
 // device part of Molly.forces(::QEM_short) (src/QEM_short.jl:27-59)
 int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, double *d_f,
               hipStream_t s, bool use_counter)
@@ -457,6 +492,7 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
         ctx->n_steps += 1;
         const bool due = ((ctx->n_steps - 1) % ctx->p.iter_period) == 0;
         if (due || ctx->nl.n != n || ctx->nl.has_i_idx) {
+            PhaseScope ps(ctx, QEM_PHASE_NEIGHBOR, s);
             hipError_t e = build_neighbor_list(ctx->nl, n, d_xyz, ctx->p.Lx, ctx->p.Ly,
                                                ctx->p.r_cutoff, s, ctx->err);
             if (e != hipSuccess) return hip_err(ctx, e, ctx->err.c_str());
@@ -464,11 +500,17 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
     }
     const ShortConsts c = short_consts(ctx, 0);
     API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
-    API_HIP(ctx, launch_pair_forces(c, ctx->p.precision, ctx->sw, ctx->nl.pi.as<int32_t>(),
-                                    ctx->nl.pj.as<int32_t>(), ctx->nl.npairs, d_xyz, d_q,
-                                    ctx->sw.fpair.as<double>(), s));
-    API_HIP(ctx, launch_gather_self(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
-                                    ctx->sw.fpair.as<double>(), d_f, s));
+    {
+        PhaseScope ps_pair(ctx, QEM_PHASE_PAIR, s);
+        API_HIP(ctx, launch_pair_forces(c, ctx->p.precision, ctx->sw, ctx->nl.pi.as<int32_t>(),
+                                        ctx->nl.pj.as<int32_t>(), ctx->nl.npairs, d_xyz, d_q,
+                                        ctx->sw.fpair.as<double>(), s));
+    }
+    {
+        PhaseScope ps_g(ctx, QEM_PHASE_GATHER, s);
+        API_HIP(ctx, launch_gather_self(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
+                                        ctx->sw.fpair.as<double>(), d_f, s));
+    }
     return QEM_OK;
 }
 
@@ -480,6 +522,7 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     int64_t Ptot = 0;
     int st = stage_kvecs(ctx, kvecs, P, weight, (parts & QEM_LONG_K0) != 0, s, &Ptot);
     if (st) return st;
+    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
     hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
     if (e != hipSuccess) return hip_err(ctx, e, "z sort");
     LongConsts c;
@@ -621,6 +664,11 @@ int qem_destroy(qem_ctx *ctx)
         if (ctx->lw.h_kv[t]) (void)hipHostFree(ctx->lw.h_kv[t]);
         if (ctx->lw.kv_ev[t]) (void)hipEventDestroy(ctx->lw.kv_ev[t]);
     }
+    for (auto &m : ctx->marks) {
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
+    }
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return QEM_OK;
@@ -943,6 +991,37 @@ int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds, int64_t *n
     if (npairs) *npairs = ctx->nl.npairs;
     if (n_builds) *n_builds = ctx->nl.builds;
     if (n_steps) *n_steps = ctx->n_steps;
+    return QEM_OK;
+}
+
+int qem_timing_enable(qem_ctx *ctx, int32_t on)
+{
+    if (!ctx) return QEM_ERR_INVALID_ARG;
+    ctx->timing = on != 0;
+    return QEM_OK;
+}
+
+int qem_timing_read(qem_ctx *ctx, double *ms, int64_t *calls, int32_t reset)
+{
+    CTX_GUARD(ctx);
+    for (auto &m : ctx->marks) {
+        API_HIP(ctx, hipEventSynchronize(m.b));
+        float t = 0.f;
+        API_HIP(ctx, hipEventElapsedTime(&t, m.a, m.b));
+        ctx->phase_ms[m.phase] += (double)t;
+        ctx->phase_calls[m.phase] += 1;
+        ctx->ev_pool.push_back(m.a);
+        ctx->ev_pool.push_back(m.b);
+    }
+    ctx->marks.clear();
+    for (int i = 0; i < QEM_NPHASES; ++i) {
+        if (ms) ms[i] = ctx->phase_ms[i];
+        if (calls) calls[i] = ctx->phase_calls[i];
+        if (reset) {
+            ctx->phase_ms[i] = 0.0;
+            ctx->phase_calls[i] = 0;
+        }
+    }
     return QEM_OK;
 }
 

@@ -162,4 +162,14 @@ struct qem_ctx {
     qem::LongWork lw;
     // host-pointer staging
     qem::DevBuf d_xyz, d_q, d_f, d_f2, d_aux;
+    // optional per-phase timing (HIP events on the launch stream)
+    bool timing = false;
+    struct Mark {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;           // recorded, not yet read
+    std::vector<hipEvent_t> ev_pool;   // free events
+    double phase_ms[QEM_NPHASES] = {0, 0, 0, 0};
+    int64_t phase_calls[QEM_NPHASES] = {0, 0, 0, 0};
 };
