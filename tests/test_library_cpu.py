@@ -1,0 +1,133 @@
+"""Host-side checks of libqem_hip.so and the Python mirror (no GPU needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from qem4molly_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "qem.h")).read()
+    return sorted(set(re.findall(r"^(?:int|void|const char \*)\s*(qem_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = _declared()
+    assert len(names) >= 30
+    lib = L.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(L.EXPORTS)
+
+
+def test_library_is_gfx950_hip_code():
+    out = subprocess.run(["readelf", "-d", L.LIB_PATH], capture_output=True, text=True).stdout
+    assert "libamdhip64" in out
+    blob = open(L.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def _gl_mpmath(n):
+    import mpmath as mp
+    mp.mp.dps = 40
+    xs, ws = [], []
+    for x0 in np.polynomial.legendre.leggauss(n)[0]:
+        t = mp.findroot(lambda s: mp.legendre(n, s), mp.mpf(x0))
+        dp = n * (mp.legendre(n - 1, t) - t * mp.legendre(n, t)) / (1 - t * t)
+        xs.append(float(t))
+        ws.append(float(2 / ((1 - t * t) * dp * dp)))
+    return np.array(xs), np.array(ws)
+
+
+def test_gauss_legendre_host_rule():
+    # 40-digit reference for the orders the workloads use
+    for n in (3, 30, 40, 50):
+        x, w = L.gauss_legendre(n)
+        xm, wm = _gl_mpmath(n)
+        assert np.max(np.abs(x - xm)) <= 1.2e-16
+        assert np.max(np.abs(w - wm) / wm) <= 2.3e-16
+    for n in (1, 128, 256):
+        x, w = L.gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.max(np.abs(x - xr)) < 1e-15
+        assert np.max(np.abs(w - wr) / wr) < 1e-10   # numpy's own end weights err ~1e-11
+    with pytest.raises(L.QEMError):
+        L.gauss_legendre(0)
+    with pytest.raises(L.QEMError):
+        L.gauss_legendre(257)
+
+
+def test_inter_params_match_reference_factory():
+    # src/Quasi_Ewald_Method.jl:5-7 (rbe) and 14-16 (non-rbe)
+    a, rc, kc = L.inter_params(1000, 100.0, 100.0, 1.0, True)
+    assert a == 1000 / 1e4 and rc == 1.0 / np.sqrt(a) and kc == 2 * np.sqrt(a)
+    a, rc, kc = L.inter_params(1000, 100.0, 50.0, 2.0, False)
+    assert a == np.sqrt(1000) / 5000 and rc == 2.0 / np.sqrt(a) and kc == 4.0 * np.sqrt(a)
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("Lz", 0.0, L.QEM_ERR_INVALID_ARG),
+    ("r_cutoff", 11.0, L.QEM_ERR_INVALID_ARG),      # >= min(Lx, Ly)/2
+    ("alpha", -1.0, L.QEM_ERR_INVALID_ARG),
+    ("accuracy", 1.0, L.QEM_ERR_INVALID_ARG),
+    ("N_t", 0, L.QEM_ERR_INVALID_ARG),
+    ("N_t", 300, L.QEM_ERR_INVALID_ARG),
+    ("iter_period", 0, L.QEM_ERR_INVALID_ARG),
+    ("precision", 7, L.QEM_ERR_INVALID_ARG),
+    ("semantics", 8, L.QEM_ERR_INVALID_ARG),
+    ("gamma_1", 1.0, L.QEM_ERR_UNSUPPORTED),        # gamma_1 gamma_2 = 1: pole
+    ("eps_0", 0.0, L.QEM_ERR_INVALID_ARG),
+    ("Lx", float("nan"), L.QEM_ERR_INVALID_ARG),
+])
+def test_create_rejects_bad_parameters(field, value, status):
+    p = L.default_params()
+    p.Lx = p.Ly = 20.0
+    p.Lz = 5.0
+    p.r_cutoff = 5.0
+    p.gamma_1 = 0.9
+    p.gamma_2 = 1.0 if field == "gamma_1" else 0.9
+    setattr(p, field, value)
+    with pytest.raises(L.QEMError) as e:
+        L.Context(p)
+    assert e.value.status == status
+
+
+def test_default_params_are_reference_defaults():
+    p = L.default_params()
+    # src/QEM_short.jl:19 and src/QEM_long.jl:19 keyword defaults
+    assert (p.iter_period, p.N_t, p.rbe_p) == (100, 30, 30)
+    assert (p.gamma_1, p.gamma_2, p.eps_0, p.accuracy, p.alpha) == (0.0, 0.0, 1.0, 1e-6, 1.0)
+    assert p.precision == L.PREC_FP64 and p.semantics == 0
+
+
+def test_null_and_status_strings():
+    lib = L.lib()
+    assert lib.qem_create(None, None) == L.QEM_ERR_INVALID_ARG
+    assert lib.qem_destroy(None) == L.QEM_OK
+    assert lib.qem_status_string(L.QEM_ERR_DOMAIN) == b"input outside the domain"
+    assert lib.qem_last_error(None) == b"null context"
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_workload_configs():
+    from qem4molly_amd.workloads import CONFIGS, make_system
+    xyz, q = make_system("slab_n100k")
+    assert xyz.shape == (100_000, 3) and q.sum() == 0.0
+    assert (q == 2).sum() == 33_333 and (q == -1).sum() == 66_666
+    assert xyz[:, 2].min() >= 0.5 and xyz[:, 2].max() < 19.5
+    xyz, q = make_system("slab_n100")
+    assert q[0] == -1.0 and q[1] == 1.0 and q.sum() == 0.0   # (-1)^i, i = 1..N
+    for c in CONFIGS.values():
+        assert 2 * c.r_cutoff < min(c.Lx, c.Ly)
