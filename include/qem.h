@@ -173,6 +173,34 @@ int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
                    const double *d_q, const int64_t *k_idx, int64_t P,
                    double *d_f, void *stream);
 
+/* ---- adjoint (no counterpart in the reference, which has forces only) ----
+ * Energy E whose negative gradient at frozen quadrature limits is the force
+ * (with QEM_SEM_CORRECTED the physical slab energy, split by
+ * exp(-k^2/(4 alpha))): pair terms are the J0 transforms of Gamma_2 and
+ * Gamma_1 e^{-k^2/4a} (src/greens_function.jl:68-88) over the nodes and
+ * limits of src/Force_short.jl:48-67, self terms the images of each charge,
+ * long range the structure-factor energy of each sampled k and the k = 0
+ * term.  dE_dq (N, may be NULL) is the per-atom potential; dE_dgamma (2,
+ * may be NULL) = (dE/dgamma_1, dE/dgamma_2).  The short range uses the
+ * context's neighbour list (built if absent; the step counter is untouched). */
+int qem_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+               const int64_t *k_idx, int64_t P, double *E, double *dE_dq,
+               double *dE_dgamma);
+/* d_dE_dq is device memory (may be NULL); E and dE_dgamma are host outputs
+ * (the call synchronises the stream). */
+int qem_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
+                   const double *d_q, const int64_t *k_idx, int64_t P,
+                   double *d_dE_dq, double *E, double *dE_dgamma, void *stream);
+/* Hessian-vector product hv = -(dF/dx) v of the force as computed (short +
+ * long range for the k batch; the k = 0 term is piecewise constant in z),
+ * differentiating the quadrature through the z-dependent limits.  For a
+ * conservative force this is H v; v and hv are N x 3. */
+int qem_hvp(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+            const int64_t *k_idx, int64_t P, const double *v, double *hv);
+int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv,
+                void *stream);
+
 /* ---- introspection ------------------------------------------------------ */
 /* pairs in the current neighbour list, its build count, and the step count */
 int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds,

@@ -248,3 +248,27 @@ def long_forces(p: Params, xyz, q, kvecs, weight=1.0, full_sum=False, nthreads=1
                          kv.shape[0], float(weight), int(bool(full_sum)), _pd(f),
                          int(nthreads))
     return f
+
+
+def energy(p: Params, xyz, q, pairs, kvecs=None, weight=1.0, full_sum=False,
+           short=True, long=True, nthreads=1):
+    """E, phi = dE/dq, (dE/dgamma_1, dE/dgamma_2) -- see qo_energy."""
+    L = lib()
+    if not hasattr(L.qo_energy, "_sig"):
+        d, i64 = C.c_double, C.c_int64
+        pd = _P(d)
+        L.qo_energy.restype = None
+        L.qo_energy.argtypes = [_P(Params), i64, pd, pd, i64, _P(C.c_int32), pd, i64, d,
+                                C.c_int, C.c_int, C.c_int, pd, pd, pd, C.c_int]
+        L.qo_energy._sig = True
+    xyz = _c(xyz, np.float64); q = _c(q, np.float64)
+    pairs = _c(pairs, np.int32).reshape(-1, 2)
+    kv = _c(np.zeros((0, 3)) if kvecs is None else kvecs, np.float64).reshape(-1, 3)
+    E = C.c_double()
+    phi = np.zeros(xyz.shape[0])
+    dg = np.zeros(2)
+    L.qo_energy(C.byref(p), xyz.shape[0], _pd(xyz), _pd(q), pairs.shape[0],
+                pairs.ctypes.data_as(_P(C.c_int32)), _pd(kv), kv.shape[0], float(weight),
+                int(bool(full_sum)), int(bool(short)), int(bool(long)), C.byref(E), _pd(phi),
+                _pd(dg), int(nthreads))
+    return E.value, phi, dg

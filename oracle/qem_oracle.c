@@ -663,3 +663,220 @@ void qo_long_forces(const qo_params *p, int64_t n, const double *xyz,
     free(part);
     free(zlist);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Energy, dE/dq and dE/dgamma.  The reference has no energy; this is the
+ * functional whose negative gradient at frozen quadrature limits is the
+ * force above (with QO_SEM_CORRECTED it is the physical slab energy split
+ * by exp(-k^2/(4 alpha))).  Definitions (P = 1/(2 pi eps_0)):
+ *   pair   q_i q_j P [ 1/2 sum_l b_l / sqrt(a_l^2 + rho^2)
+ *                      - sum_l GL[Gamma_2,l J0; K_point,l]
+ *                      + sum_l GL[Gamma_1,l e^{-k^2/4a} J0; K_gauss,l] ]
+ *   self   q_i^2 P/2 [ 1/2 sum_{l=2..4} b_l / a_l - sum_{l=1..4} GL[Gamma_2,l]
+ *                      + GL[Gamma_2,1 e] + sum_{l=2..4} GL[Gamma_1,l e] ]
+ *          (images only: the charge's own 1/r is excluded, so image 1
+ *           keeps only its multiple reflections Gamma_2,1)
+ *   k = 0  -1/(2 Lx Ly eps_0) sum_{i<j} q_i q_j |z_i - z_j|
+ *   k      w_k/(Lx Ly eps_0 2k) [ sum_{i<j} q_i q_j cos(k.rho_ij) e^{-k|z_ij|}
+ *                       + 1/2 sum_{i,j} q_i q_j cos(k.rho_ij) R_ij/(1 - x) ]
+ *          R_ij = g1 e^{-k z+} + g2 e^{-k(2Lz - z+)} + g1 g2 (e^{-k(2Lz+dz)} + e^{-k(2Lz-dz)}),
+ *          x = g1 g2 e^{-2kLz}
+ * phi_i = dE/dq_i; E = 1/2 sum_i q_i phi_i (E is quadratic in q). */
+
+/* d Gamma / d gamma_w at fixed k (w = 0: gamma_1, w = 1: gamma_2) */
+static double dGamma_dg(double k, const qo_elem *e, int l, int which2, int w)
+{
+    const double E = exp(-2.0 * k * e->L_z);
+    const double g12 = e->gamma_1 * e->gamma_2;
+    const double D = g12 * E - 1.0;
+    const double dg12 = w == 0 ? e->gamma_2 : e->gamma_1;
+    double db[4] = {0.0, w == 0 ? 1.0 : 0.0, w == 1 ? 1.0 : 0.0, dg12};
+    const double ea = exp(-k * e->a[l - 1]);
+    const double b = e->b[l - 1];
+    if (!which2) return 0.5 * ea * (db[l - 1] / D - b * dg12 * E / (D * D));
+    return 0.5 * ea * (db[l - 1] * g12 * E / D - b * dg12 * E / (D * D));
+}
+
+/* integrand kinds for the energy */
+enum { U_POINT, U_GAUSS, U_GAUSS2, DU_POINT, DU_GAUSS, DU_GAUSS2 };
+
+static double integrand_e(int kind, double k, const qo_elem *e, int l, int w)
+{
+    const double g = exp(-(k * k) / (4.0 * e->alpha));
+    const double J = j0(k * e->rho);
+    switch (kind) {
+    case U_POINT:   return qo_Gamma_2(k, e, l) * J;
+    case U_GAUSS:   return qo_Gamma_1(k, e, l) * g * J;
+    case U_GAUSS2:  return qo_Gamma_2(k, e, l) * g * J;
+    case DU_POINT:  return dGamma_dg(k, e, l, 1, w) * J;
+    case DU_GAUSS:  return dGamma_dg(k, e, l, 0, w) * g * J;
+    default:        return dGamma_dg(k, e, l, 1, w) * g * J;
+    }
+}
+
+static double gl_e(int kind, const qo_elem *e, double K, int l, int w, const gl_rule *r)
+{
+    double s = 0.0;
+    for (int i = 0; i < r->n; ++i) {
+        double xi = K / 2.0 + (K * r->x[i]) / 2.0;
+        double wi = (K * r->w[i]) / 2.0;
+        s += integrand_e(kind, xi, e, l, w) * wi;
+    }
+    return s;
+}
+
+/* pair potential u_ij (without q_i q_j) and its gamma derivatives */
+void qo_pair_potential(const qo_params *p, const double *ri, const double *rj,
+                       double *u, double *du)
+{
+    *u = 0.0;
+    du[0] = du[1] = 0.0;
+    double mx, my;
+    min_image(ri[0] - rj[0], ri[1] - rj[1], p->Lx, p->Ly, &mx, &my);
+    double d2 = mx * mx + my * my;
+    if (d2 > p->r_cutoff * p->r_cutoff) return;
+    double rho = sqrt(d2);
+    const gl_rule *r = get_rule(p->N_t);
+    qo_elem e;
+    qo_elem_ij(p->gamma_1, p->gamma_2, ri[2], rj[2], rho, p->Lz, p->alpha, p->accuracy, &e);
+    const int swap = (p->semantics & QO_SEM_SWAP_LIMITS) != 0;
+    const double *Kp = swap ? e.k_f1 : e.k_f2;
+    const double *Kg = swap ? e.k_f2 : e.k_f1;
+    const double P = 1.0 / (QO_TWO_PI * p->eps_0);
+    double s = 0.0, ds[2] = {0.0, 0.0};
+    const double dbs[2][4] = {{0.0, 1.0, 0.0, p->gamma_2}, {0.0, 0.0, 1.0, p->gamma_1}};
+    for (int l = 1; l <= 4; ++l) {
+        double inv = 1.0 / sqrt(e.a[l - 1] * e.a[l - 1] + rho * rho);
+        s += 0.5 * e.b[l - 1] * inv - gl_e(U_POINT, &e, Kp[l - 1], l, 0, r)
+             + gl_e(U_GAUSS, &e, Kg[l - 1], l, 0, r);
+        for (int w = 0; w < 2; ++w)
+            ds[w] += 0.5 * dbs[w][l - 1] * inv - gl_e(DU_POINT, &e, Kp[l - 1], l, w, r)
+                     + gl_e(DU_GAUSS, &e, Kg[l - 1], l, w, r);
+    }
+    *u = P * s;
+    du[0] = P * ds[0];
+    du[1] = P * ds[1];
+}
+
+/* self energy factor s_i (E_self = q_i^2 s_i) and gamma derivatives */
+void qo_self_potential(const qo_params *p, const double *ri, double *s_out, double *ds_out)
+{
+    const gl_rule *r = get_rule(p->N_t);
+    qo_elem e;
+    qo_elem_i(p->gamma_1, p->gamma_2, ri[2], p->Lz, p->alpha, p->accuracy, &e);
+    const int swap = (p->semantics & QO_SEM_SWAP_LIMITS) != 0;
+    const double *Kp = swap ? e.k_f1 : e.k_f2;
+    const double *Kg = swap ? e.k_f2 : e.k_f1;
+    const double P = 1.0 / (QO_TWO_PI * p->eps_0);
+    const double dbs[2][4] = {{0.0, 1.0, 0.0, p->gamma_2}, {0.0, 0.0, 1.0, p->gamma_1}};
+    double s = 0.0, ds[2] = {0.0, 0.0};
+    for (int l = 2; l <= 4; ++l) {
+        s += 0.5 * e.b[l - 1] / e.a[l - 1];
+        for (int w = 0; w < 2; ++w) ds[w] += 0.5 * dbs[w][l - 1] / e.a[l - 1];
+    }
+    for (int l = 1; l <= 4; ++l) {
+        s -= gl_e(U_POINT, &e, Kp[l - 1], l, 0, r);
+        s += gl_e(l == 1 ? U_GAUSS2 : U_GAUSS, &e, Kg[l - 1], l, 0, r);
+        for (int w = 0; w < 2; ++w) {
+            ds[w] -= gl_e(DU_POINT, &e, Kp[l - 1], l, w, r);
+            ds[w] += gl_e(l == 1 ? DU_GAUSS2 : DU_GAUSS, &e, Kg[l - 1], l, w, r);
+        }
+    }
+    *s_out = 0.5 * P * s;
+    ds_out[0] = 0.5 * P * ds[0];
+    ds_out[1] = 0.5 * P * ds[1];
+}
+
+/* E, phi = dE/dq (may be NULL) and dE/dgamma_1,2 for the short range over
+ * a pair list plus self terms, and the long range over explicit k vectors
+ * (weight as in qo_long_forces).  O(N^2) per k: test sizes only. */
+void qo_energy(const qo_params *p, int64_t n, const double *xyz, const double *q,
+               int64_t npairs, const int32_t *pairs, const double *kvecs, int64_t P,
+               double weight, int full_sum, int with_short, int with_long,
+               double *E_out, double *phi_out, double *dEdg, int nthreads)
+{
+    double *phi = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    double *psi = (double *)calloc((size_t)(2 * (n > 0 ? n : 1)), sizeof(double));
+    double dg_long[2] = {0.0, 0.0};
+    (void)nthreads;
+    if (with_short) {
+        double *u = (double *)malloc(sizeof(double) * 3 * (size_t)(npairs > 0 ? npairs : 1));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 32) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+        for (int64_t t = 0; t < npairs; ++t) {
+            int32_t i = pairs[2 * t], j = pairs[2 * t + 1];
+            qo_pair_potential(p, xyz + 3 * i, xyz + 3 * j, u + 3 * t, u + 3 * t + 1);
+        }
+        for (int64_t t = 0; t < npairs; ++t) {
+            int32_t i = pairs[2 * t], j = pairs[2 * t + 1];
+            phi[i] += q[j] * u[3 * t];
+            phi[j] += q[i] * u[3 * t];
+            for (int w = 0; w < 2; ++w) {
+                psi[2 * i + w] += q[j] * u[3 * t + 1 + w];
+                psi[2 * j + w] += q[i] * u[3 * t + 1 + w];
+            }
+        }
+        free(u);
+        for (int64_t i = 0; i < n; ++i) {
+            double s, ds[2];
+            qo_self_potential(p, xyz + 3 * i, &s, ds);
+            phi[i] += 2.0 * q[i] * s;
+            psi[2 * i] += 2.0 * q[i] * ds[0];
+            psi[2 * i + 1] += 2.0 * q[i] * ds[1];
+        }
+    }
+    if (with_long && n > 0) {
+        const double A = p->Lx * p->Ly * p->eps_0;
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < n; ++j)
+                if (j != i) phi[i] -= q[j] * fabs(xyz[3 * i + 2] - xyz[3 * j + 2]) / (2.0 * A);
+        const double g1 = p->gamma_1, g2 = p->gamma_2, Lz = p->Lz;
+        for (int64_t t = 0; t < P; ++t) {
+            const double kx = kvecs[3 * t], ky = kvecs[3 * t + 1], k = kvecs[3 * t + 2];
+            const double wk = full_sum ? exp(-(k * k) / (4.0 * p->alpha)) : weight;
+            const double c = wk / (A * 2.0 * k);
+            const double e2 = exp(-2.0 * k * Lz), x = g1 * g2 * e2;
+            double dgk[2] = {0.0, 0.0};
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : dgk[:2])
+#endif
+            for (int64_t i = 0; i < n; ++i) {
+                double acc = 0.0, a1 = 0.0, a2 = 0.0;
+                const double thi = kx * xyz[3 * i] + ky * xyz[3 * i + 1], zi = xyz[3 * i + 2];
+                for (int64_t j = 0; j < n; ++j) {
+                    const double thj = kx * xyz[3 * j] + ky * xyz[3 * j + 1], zj = xyz[3 * j + 2];
+                    const double cs = cos(thi - thj);
+                    const double dz = zi - zj, zp = zi + zj;
+                    const double R = g1 * exp(-k * zp) + g2 * exp(-k * (2.0 * Lz - zp)) +
+                                     g1 * g2 * (exp(-k * (2.0 * Lz + dz)) + exp(-k * (2.0 * Lz - dz)));
+                    const double dR1 = exp(-k * zp) + g2 * (exp(-k * (2.0 * Lz + dz)) + exp(-k * (2.0 * Lz - dz)));
+                    const double dR2 = exp(-k * (2.0 * Lz - zp)) + g1 * (exp(-k * (2.0 * Lz + dz)) + exp(-k * (2.0 * Lz - dz)));
+                    if (j != i) acc += q[j] * cs * exp(-k * fabs(dz));
+                    acc += q[j] * cs * R / (1.0 - x);
+                    a1 += q[j] * cs * (dR1 / (1.0 - x) + R * g2 * e2 / ((1.0 - x) * (1.0 - x)));
+                    a2 += q[j] * cs * (dR2 / (1.0 - x) + R * g1 * e2 / ((1.0 - x) * (1.0 - x)));
+                }
+                phi[i] += c * acc;
+                dgk[0] += 0.5 * c * q[i] * a1;
+                dgk[1] += 0.5 * c * q[i] * a2;
+            }
+            dg_long[0] += dgk[0];
+            dg_long[1] += dgk[1];
+        }
+    }
+    double E = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        E += 0.5 * q[i] * phi[i];
+        d1 += 0.5 * q[i] * psi[2 * i];
+        d2 += 0.5 * q[i] * psi[2 * i + 1];
+    }
+    if (E_out) *E_out = E;
+    if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)n);
+    if (dEdg) {
+        dEdg[0] = d1 + dg_long[0];
+        dEdg[1] = d2 + dg_long[1];
+    }
+    free(phi);
+    free(psi);
+}

@@ -123,6 +123,16 @@ void qo_long_forces(const qo_params *p, int64_t n, const double *xyz,
                     const double *q, const double *kvecs, int64_t P,
                     double weight, int full_sum, double *f, int nthreads);
 
+/* Energy (no counterpart in the reference; see qem_oracle.c for the
+ * definition), phi = dE/dq, dE/dgamma_1, dE/dgamma_2. */
+void qo_pair_potential(const qo_params *p, const double *ri, const double *rj,
+                       double *u, double *du);
+void qo_self_potential(const qo_params *p, const double *ri, double *s, double *ds);
+void qo_energy(const qo_params *p, int64_t n, const double *xyz, const double *q,
+               int64_t npairs, const int32_t *pairs, const double *kvecs, int64_t P,
+               double weight, int full_sum, int with_short, int with_long,
+               double *E, double *phi, double *dEdg, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

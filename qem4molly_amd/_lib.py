@@ -48,7 +48,7 @@ EXPORTS = (
     "qem_short_state", "qem_gauss_legendre", "qem_bessel01_mixed",
     "qem_dev_malloc", "qem_dev_free", "qem_memcpy_htod", "qem_memcpy_dtoh",
     "qem_device_synchronize", "qem_dev_copy_kernel", "qem_timing_enable",
-    "qem_timing_read",
+    "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
 )
 
 PHASES = ("neighbor", "pair", "gather", "long")
@@ -112,10 +112,19 @@ _SIGS = {
     "qem_dev_copy_kernel": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     "qem_timing_enable": (C.c_int, [_vp, _i32]),
     "qem_timing_read": (C.c_int, [_vp, _pd, _pi64, _i32]),
+    "qem_energy": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd, _pd]),
+    "qem_energy_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _pd, _pd, _vp]),
+    "qem_hvp": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd]),
+    "qem_hvp_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _vp]),
 }
 
 _lib = None
 _lock = threading.Lock()
+# torch ships its own HIP runtime (DT_NEEDED "libamdhip64.so"); this library
+# needs "libamdhip64.so.7".  Loaded in that order the process shares torch's
+# runtime; the other order would map two runtimes, so torch users must import
+# torch first (qem4molly_amd.torch_qem checks this).
+_loaded_with_torch = False
 
 
 def lib() -> C.CDLL:
@@ -127,6 +136,9 @@ def lib() -> C.CDLL:
                 raise RuntimeError(
                     f"{LIB_PATH} is not built; run `python -m qem4molly_amd.build` "
                     "(or __graft_entry__.build())")
+            global _loaded_with_torch
+            import sys
+            _loaded_with_torch = "torch" in sys.modules
             L = C.CDLL(LIB_PATH)
             for name, (res, args) in _SIGS.items():
                 fn = getattr(L, name)
@@ -315,6 +327,37 @@ class Context:
         self._check(lib().qem_long_forces_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
                                               k_idx.shape[0], d_f, stream),
                     "qem_long_forces_dev")
+
+    # ---- adjoint ------------------------------------------------------------
+    def energy(self, xyz, q, k_idx):
+        """(E, dE/dq [N], (dE/dgamma_1, dE/dgamma_2))."""
+        xyz = f64(xyz, (-1, 3)); q = f64(q)
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        E = _d(); phi = np.zeros(xyz.shape[0]); dg = np.zeros(2)
+        self._check(lib().qem_energy(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
+                                     k_idx.shape[0], C.byref(E), ptr(phi), ptr(dg)), "qem_energy")
+        return E.value, phi, dg
+
+    def hvp(self, xyz, q, k_idx, v) -> np.ndarray:
+        """-(dF/dx) v of the force as computed (N x 3)."""
+        xyz = f64(xyz, (-1, 3)); q = f64(q); v = f64(v, (-1, 3))
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        hv = np.zeros_like(xyz)
+        self._check(lib().qem_hvp(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
+                                  k_idx.shape[0], ptr(v), ptr(hv)), "qem_hvp")
+        return hv
+
+    def energy_dev(self, n, d_xyz, d_q, k_idx, d_phi=None, stream=None):
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        E = _d(); dg = np.zeros(2)
+        self._check(lib().qem_energy_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx), k_idx.shape[0],
+                                         d_phi, C.byref(E), ptr(dg), stream), "qem_energy_dev")
+        return E.value, dg
+
+    def hvp_dev(self, n, d_xyz, d_q, k_idx, d_v, d_hv, stream=None):
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        self._check(lib().qem_hvp_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx), k_idx.shape[0],
+                                      d_v, d_hv, stream), "qem_hvp_dev")
 
     def timing_enable(self, on=True):
         self._check(lib().qem_timing_enable(self._h, int(bool(on))), "qem_timing_enable")

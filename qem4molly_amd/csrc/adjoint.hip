@@ -1,0 +1,607 @@
+// Short-range adjoint kernels on gfx950: pair and self energies with their
+// gamma derivatives (for E, dE/dq, dE/dgamma), and forward-mode tangents of
+// the pair and self forces (for Hessian-vector products).
+//
+// No reference function computes these: QEM4Molly has forces only.  The
+// energy is the functional whose negative gradient at frozen quadrature
+// limits is the force of src/Force_short.jl:25-107 (pair terms: the J0
+// transforms of Gamma_2 and Gamma_1 e^{-k^2/4a} of src/greens_function.jl
+// over the same nodes and limits; self terms: the images of the charge
+// itself).  The tangents differentiate the force as computed, including the
+// motion of the quadrature nodes through the z-dependent limits k_f1, k_f2
+// (src/greens_function.jl:26-27).
+//
+// Same work mapping as the force kernels (one lane per pair, wave-uniform
+// nodes, LDS Bessel table in mixed precision); nothing per pair is stored
+// beyond one double4 of output, the quadrature is recomputed.
+#include "qem_device.h"
+
+namespace qem {
+namespace {
+
+using dev::bessel01_tab;
+using dev::jl_sign;
+using dev::kLog2e;
+
+// ---- energy node sums -----------------------------------------------------
+// kind 0 (point):   t = e^{-ka} E / D,              v += w t J0,  d += w t / D J0
+// kind 1 (gauss):   t = e^{-ka - k^2/4a} / D,       v += w t J0,  d += w t E / D J0
+// kind 2 (gauss, multiple reflections of the charge itself, a = 0):
+//                   t = e^{-k^2/4a} E / D,          v += w t,     d += w t / D
+// with E = e^{-2kLz}, D = g12 E - 1.  d is what d/dgamma of D brings down.
+template <int PREC, bool SELF>
+__device__ __forceinline__ void nodes_energy(double K, double a, double rho, int kind,
+                                             const ShortConsts &c,
+                                             const double *__restrict__ glx,
+                                             const double *__restrict__ glw,
+                                             const float *__restrict__ glxf,
+                                             const float *__restrict__ glwf,
+                                             const float4 *__restrict__ tab, double &v, double &d)
+{
+    if (PREC == QEM_PREC_FP64) {
+        const double h = 0.5 * K, inv4a = 0.25 / c.alpha;
+        double av = 0.0, ad = 0.0;
+        for (int n = 0; n < c.Nt; ++n) {
+            const double k = h + h * glx[n];
+            const double w = h * glw[n];
+            const double E = exp(-2.0 * k * c.Lz);
+            const double rD = 1.0 / (c.g12 * E - 1.0);
+            double num, fd;
+            if (kind == 0) { num = exp(-k * a) * E; fd = rD; }
+            else if (kind == 1) { num = exp(-k * a - k * k * inv4a); fd = E * rD; }
+            else { num = exp(-k * k * inv4a) * E; fd = rD; }
+            const double J = SELF ? 1.0 : j0(k * rho);
+            const double wt = w * num * rD * J;
+            av += wt;
+            ad += wt * fd;
+        }
+        v = av;
+        d = ad;
+    } else {
+        const float h = 0.5f * (float)K;
+        const float c2 = -2.0f * (float)c.Lz * kLog2e;
+        const float ca = -(float)a * kLog2e;
+        const float cg = -kLog2e * (float)(0.25 / c.alpha);
+        const float g12 = (float)c.g12;
+        const float rhof = (float)rho;
+        float av = 0.f, ad = 0.f;
+        for (int n = 0; n < c.Nt; ++n) {
+            const float k = fmaf(h, glxf[n], h);
+            const float w = h * glwf[n];
+            const float E = __builtin_amdgcn_exp2f(k * c2);
+            const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+            float num, fd;
+            if (kind == 0) { num = __builtin_amdgcn_exp2f(k * ca) * E; fd = rD; }
+            else if (kind == 1) { num = __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca)); fd = E * rD; }
+            else { num = __builtin_amdgcn_exp2f(k * k * cg) * E; fd = rD; }
+            float J = 1.f;
+            if (!SELF) {
+                float J1;
+                bessel01_tab(k * rhof, tab, c.jt_n, c.jt_dx, c.jt_inv_dx, J, J1);
+            }
+            const float wt = w * num * rD * J;
+            av += wt;
+            ad = fmaf(wt, fd, ad);
+        }
+        v = av;
+        d = ad;
+    }
+}
+
+// d b_l / d gamma_w and d g12 / d gamma_w
+__device__ __forceinline__ double db_dg(const ShortConsts &c, int l, int w)
+{
+    if (l == 1) return w == 0 ? 1.0 : 0.0;
+    if (l == 2) return w == 1 ? 1.0 : 0.0;
+    if (l == 3) return w == 0 ? c.g2 : c.g1;
+    return 0.0;
+}
+
+// Pair potential u_ij (without q_i q_j) and du/dgamma_1, du/dgamma_2.
+template <int PREC>
+__device__ double4 pair_energy(const ShortConsts &c, const double *__restrict__ glx,
+                               const double *__restrict__ glw, const float *__restrict__ glxf,
+                               const float *__restrict__ glwf, const float4 *__restrict__ tab,
+                               double xi, double yi, double zi, double xj, double yj, double zj)
+{
+    double mx, my;
+    const double d2 = dev::min_image_d2(c, xi - xj, yi - yj, mx, my);
+    if (d2 > c.rc2) return make_double4(0.0, 0.0, 0.0, 0.0);
+    const double rho = sqrt(d2);
+    const double zn = fabs(zi - zj), zp = zi + zj, sg = jl_sign(zi - zj);
+    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
+    double u = 0.0, du0 = 0.0, du1 = 0.0;
+    for (int l = 0; l < 4; ++l) {
+        double a, b, s;
+        dev::image(c, l, zn, zp, sg, a, b, s);
+        const dev::Limits L = dev::limits(c, a);
+        const double Kp = swap ? L.kf1 : L.kf2, Kg = swap ? L.kf2 : L.kf1;
+        const double db0 = db_dg(c, l, 0), db1 = db_dg(c, l, 1);
+        const double inv_r = 1.0 / sqrt(a * a + rho * rho);
+        double Pv = 0.0, Pd = 0.0, Gv = 0.0, Gd = 0.0;
+        if (b != 0.0 || db0 != 0.0 || db1 != 0.0) {
+            nodes_energy<PREC, false>(Kp, a, rho, 0, c, glx, glw, glxf, glwf, tab, Pv, Pd);
+            nodes_energy<PREC, false>(Kg, a, rho, 1, c, glx, glw, glxf, glwf, tab, Gv, Gd);
+        }
+        const double hb = 0.5 * b;
+        u += hb * inv_r - hb * c.g12 * Pv + hb * Gv;
+        du0 += 0.5 * db0 * (inv_r - c.g12 * Pv + Gv) + hb * c.g2 * (Pd - Gd);
+        du1 += 0.5 * db1 * (inv_r - c.g12 * Pv + Gv) + hb * c.g1 * (Pd - Gd);
+    }
+    return make_double4(c.inv2pieps * u, c.inv2pieps * du0, c.inv2pieps * du1, 0.0);
+}
+
+// Self energy factor s_i (E_self = q_i^2 s_i) and its gamma derivatives:
+// the images of the charge itself (image 1 keeps only its multiple
+// reflections), split by exp(-k^2/4a) like the pair terms.
+template <int PREC>
+__device__ double3 self_energy(const ShortConsts &c, const double *__restrict__ glx,
+                               const double *__restrict__ glw, const float *__restrict__ glxf,
+                               const float *__restrict__ glwf, double zi)
+{
+    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
+    double s = 0.0, ds0 = 0.0, ds1 = 0.0;
+    for (int l = 0; l < 4; ++l) {
+        const double a = l == 0 ? 0.0 : (l == 1 ? 2.0 * zi : (l == 2 ? 2.0 * c.Lz - 2.0 * zi : 2.0 * c.Lz));
+        const double b = l == 0 ? 1.0 : (l == 1 ? c.g1 : (l == 2 ? c.g2 : c.g12));
+        const dev::Limits L = dev::limits(c, a);
+        const double Kp = swap ? L.kf1 : L.kf2, Kg = swap ? L.kf2 : L.kf1;
+        const double db0 = db_dg(c, l, 0), db1 = db_dg(c, l, 1);
+        if (b == 0.0 && db0 == 0.0 && db1 == 0.0) continue;
+        const double hb = 0.5 * b;
+        double Pv, Pd, Gv, Gd;
+        nodes_energy<PREC, true>(Kp, a, 0.0, 0, c, glx, glw, glxf, glwf, nullptr, Pv, Pd);
+        s += -hb * c.g12 * Pv;
+        ds0 += -0.5 * db0 * c.g12 * Pv + hb * c.g2 * Pd;
+        ds1 += -0.5 * db1 * c.g12 * Pv + hb * c.g1 * Pd;
+        if (l == 0) {
+            nodes_energy<PREC, true>(Kg, 0.0, 0.0, 2, c, glx, glw, glxf, glwf, nullptr, Gv, Gd);
+            s += hb * c.g12 * Gv;
+            ds0 += -hb * c.g2 * Gd;
+            ds1 += -hb * c.g1 * Gd;
+        } else {
+            nodes_energy<PREC, true>(Kg, a, 0.0, 1, c, glx, glw, glxf, glwf, nullptr, Gv, Gd);
+            s += hb * Gv + hb / a;
+            ds0 += 0.5 * db0 * (Gv + 1.0 / a) - hb * c.g2 * Gd;
+            ds1 += 0.5 * db1 * (Gv + 1.0 / a) - hb * c.g1 * Gd;
+        }
+    }
+    const double f = 0.5 * c.inv2pieps;
+    return make_double3(f * s, f * ds0, f * ds1);
+}
+
+// ---- tangent node sums --------------------------------------------------------
+// Value and first-order tangent of sum_n w k t(k) {J1, J0}(k rho), where the
+// limit K (so every node k_n = K (1 + x_n)/2 and weight) moves with dK, the
+// image distance a with da and rho with drho.
+struct Dual4 {
+    double sr, sz, dsr, dsz;
+};
+
+template <int PREC>
+__device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, double da,
+                                               double rho, double drho, bool gauss,
+                                               const ShortConsts &c,
+                                               const double *__restrict__ glx,
+                                               const double *__restrict__ glw,
+                                               const float *__restrict__ glxf,
+                                               const float *__restrict__ glwf,
+                                               const float4 *__restrict__ tab, bool self)
+{
+    Dual4 o{0.0, 0.0, 0.0, 0.0};
+    if (PREC == QEM_PREC_FP64) {
+        const double h = 0.5 * K, dh = 0.5 * dK, inv4a = 0.25 / c.alpha;
+        for (int n = 0; n < c.Nt; ++n) {
+            const double k = h + h * glx[n], dk = dh + dh * glx[n];
+            const double w = h * glw[n], dw = dh * glw[n];
+            const double E = exp(-2.0 * k * c.Lz);
+            const double rD = 1.0 / (c.g12 * E - 1.0);
+            const double dE = -2.0 * c.Lz * E * dk;
+            const double drD = -c.g12 * dE * rD * rD;
+            double num, dnum;
+            if (gauss) {
+                num = exp(-k * a - k * k * inv4a);
+                dnum = num * (-a * dk - k * da - 2.0 * k * inv4a * dk);
+            } else {
+                num = exp(-k * a) * E;
+                dnum = num * (-a * dk - k * da - 2.0 * c.Lz * dk);
+            }
+            const double t = num * rD, dt = dnum * rD + num * drD;
+            const double wk = w * k, dwk = dw * k + w * dk;
+            const double wkt = wk * t, dwkt = dwk * t + wk * dt;
+            double J0 = 1.0, J1 = 0.0, dJ0 = 0.0, dJ1 = 0.0;
+            if (!self) {
+                const double x = k * rho, dx = dk * rho + k * drho;
+                J0 = j0(x);
+                J1 = j1(x);
+                const double j1x = x > 1e-8 ? J1 / x : 0.5;
+                dJ0 = -J1 * dx;
+                dJ1 = (J0 - j1x) * dx;
+            }
+            o.sr += wkt * J1;
+            o.sz += wkt * J0;
+            o.dsr += dwkt * J1 + wkt * dJ1;
+            o.dsz += dwkt * J0 + wkt * dJ0;
+        }
+    } else {
+        const float h = 0.5f * (float)K, dh = 0.5f * (float)dK;
+        const float c2 = -2.0f * (float)c.Lz * kLog2e;
+        const float ca = -(float)a * kLog2e;
+        const float cg = -kLog2e * (float)(0.25 / c.alpha);
+        const float g12 = (float)c.g12;
+        const float Lz2 = 2.0f * (float)c.Lz;
+        const float af = (float)a, daf = (float)da, rhof = (float)rho, drhof = (float)drho;
+        const float i2a = (float)(0.5 / c.alpha);
+        float sr = 0.f, sz = 0.f, dsr = 0.f, dsz = 0.f;
+        for (int n = 0; n < c.Nt; ++n) {
+            const float k = fmaf(h, glxf[n], h), dk = fmaf(dh, glxf[n], dh);
+            const float w = h * glwf[n], dw = dh * glwf[n];
+            const float E = __builtin_amdgcn_exp2f(k * c2);
+            const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+            const float dE = -Lz2 * E * dk;
+            const float drD = -g12 * dE * rD * rD;
+            float num, dnum;
+            if (gauss) {
+                num = __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca));
+                dnum = num * (-af * dk - k * daf - k * i2a * dk);
+            } else {
+                num = __builtin_amdgcn_exp2f(k * ca) * E;
+                dnum = num * (-af * dk - k * daf - Lz2 * dk);
+            }
+            const float t = num * rD, dt = fmaf(dnum, rD, num * drD);
+            const float wk = w * k, dwk = fmaf(dw, k, w * dk);
+            const float wkt = wk * t, dwkt = fmaf(dwk, t, wk * dt);
+            float J0 = 1.f, J1 = 0.f, dJ0 = 0.f, dJ1 = 0.f;
+            if (!self) {
+                const float x = k * rhof, dx = fmaf(dk, rhof, k * drhof);
+                bessel01_tab(x, tab, c.jt_n, c.jt_dx, c.jt_inv_dx, J0, J1);
+                const float j1x = x > 1e-4f ? J1 * __builtin_amdgcn_rcpf(x) : 0.5f;
+                dJ0 = -J1 * dx;
+                dJ1 = (J0 - j1x) * dx;
+            }
+            sr = fmaf(wkt, J1, sr);
+            sz = fmaf(wkt, J0, sz);
+            dsr = fmaf(dwkt, J1, fmaf(wkt, dJ1, dsr));
+            dsz = fmaf(dwkt, J0, fmaf(wkt, dJ0, dsz));
+        }
+        o.sr = sr;
+        o.sz = sz;
+        o.dsr = dsr;
+        o.dsz = dsz;
+    }
+    return o;
+}
+
+// Tangent of F_short_ij (force on i; z force on j) along (vi, vj).
+template <int PREC>
+__device__ double4 pair_force_tangent(const ShortConsts &c, const double *__restrict__ glx,
+                                      const double *__restrict__ glw,
+                                      const float *__restrict__ glxf,
+                                      const float *__restrict__ glwf,
+                                      const float4 *__restrict__ tab, double qi, double qj,
+                                      const double *ri, const double *rj, const double *vi,
+                                      const double *vj)
+{
+    const double dx = ri[0] - rj[0], dy = ri[1] - rj[1];
+    double mx, my;
+    const double d2 = dev::min_image_d2(c, dx, dy, mx, my);
+    if (d2 > c.rc2) return make_double4(0.0, 0.0, 0.0, 0.0);
+    const double ddx = vi[0] - vj[0], ddy = vi[1] - vj[1];
+    const double rho = sqrt(d2);
+    const double drho = rho > 0.0 ? (mx * ddx + my * ddy) / rho : 0.0;
+    const double zn = fabs(ri[2] - rj[2]), zp = ri[2] + rj[2], sg = jl_sign(ri[2] - rj[2]);
+    const double dzn = sg * (vi[2] - vj[2]), dzp = vi[2] + vj[2];
+    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
+    double Fsr = 0.0, dFsr = 0.0, Fsz = 0.0, dFsz = 0.0;
+    double part[4], dpart[4];
+    for (int l = 0; l < 4; ++l) {
+        double a, b, s;
+        dev::image(c, l, zn, zp, sg, a, b, s);
+        const double da = l == 0 ? dzn : (l == 1 ? dzp : (l == 2 ? -dzp : -dzn));
+        const dev::Limits L = dev::limits(c, a);
+        const double Kp = swap ? L.kf1 : L.kf2, Kg = swap ? L.kf2 : L.kf1;
+        const double dKp = (swap ? L.dkf1 : L.dkf2) * da, dKg = (swap ? L.dkf2 : L.dkf1) * da;
+        const double r2 = a * a + rho * rho;
+        const double inv32 = 1.0 / (r2 * sqrt(r2));
+        const double dinv32 = -3.0 * inv32 * (a * da + rho * drho) / r2;
+        const double hb = 0.5 * b;
+        const double cr = hb * rho * inv32, dcr = hb * (drho * inv32 + rho * dinv32);
+        const double cz = hb * a * s * inv32, dcz = hb * s * (da * inv32 + a * dinv32);
+        Dual4 P{0.0, 0.0, 0.0, 0.0}, G{0.0, 0.0, 0.0, 0.0};
+        if (b != 0.0) {
+            P = nodes_tangent<PREC>(Kp, dKp, a, da, rho, drho, false, c, glx, glw, glxf, glwf,
+                                    tab, false);
+            if (!c.single)
+                G = nodes_tangent<PREC>(Kg, dKg, a, da, rho, drho, true, c, glx, glw, glxf,
+                                        glwf, tab, false);
+        }
+        Fsr += -(hb * c.g12) * P.sr + cr + hb * G.sr;
+        dFsr += -(hb * c.g12) * P.dsr + dcr + hb * G.dsr;
+        part[l] = (hb * c.g12 * s) * P.sz - cz - (hb * s) * G.sz;
+        dpart[l] = (hb * c.g12 * s) * P.dsz - dcz - (hb * s) * G.dsz;
+        Fsz += part[l];
+        dFsz += dpart[l];
+    }
+    const double pref = qi * qj * c.inv2pieps;
+    double4 o = make_double4(0.0, 0.0, pref * dFsz, 0.0);
+    if (rho != 0.0) {
+        const bool mi = (c.sem & QEM_SEM_MINIMAGE_DIR) != 0;
+        const double ux = mi ? mx : dx, uy = mi ? my : dy;
+        const double ir = 1.0 / rho;
+        o.x = pref * (dFsr * ux * ir + Fsr * (ddx * ir - ux * drho * ir * ir));
+        o.y = pref * (dFsr * uy * ir + Fsr * (ddy * ir - uy * drho * ir * ir));
+    }
+    if (c.sem & QEM_SEM_NEWTON_IMAGES)
+        o.w = pref * (-dpart[0] + dpart[1] + dpart[2] - dpart[3]);
+    else
+        o.w = -o.z;
+    (void)Fsz;
+    return o;
+}
+
+// Tangent of the self-image z force F_short_i along v_z.
+template <int PREC>
+__device__ double self_force_tangent(const ShortConsts &c, const double *__restrict__ glx,
+                                     const double *__restrict__ glw,
+                                     const float *__restrict__ glxf,
+                                     const float *__restrict__ glwf, double qi, double zi,
+                                     double vz)
+{
+    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
+    double dP1 = 0.0, dP2 = 0.0, dG = 0.0;
+    for (int l = 1; l <= 2; ++l) {
+        const double a = l == 1 ? 2.0 * zi : 2.0 * c.Lz - 2.0 * zi;
+        const double da = l == 1 ? 2.0 * vz : -2.0 * vz;
+        const double b = l == 1 ? c.g1 : c.g2;
+        const double s = l == 1 ? -1.0 : 1.0;
+        if (b == 0.0) continue;
+        const dev::Limits L = dev::limits(c, a);
+        const double Kp = swap ? L.kf1 : L.kf2, Kg = swap ? L.kf2 : L.kf1;
+        const double dKp = (swap ? L.dkf1 : L.dkf2) * da, dKg = (swap ? L.dkf2 : L.dkf1) * da;
+        const double hb = 0.5 * b;
+        const Dual4 P = nodes_tangent<PREC>(Kp, dKp, a, da, 0.0, 0.0, false, c, glx, glw, glxf,
+                                            glwf, nullptr, true);
+        dP1 += (hb * c.g12 * s) * P.dsz;
+        dP2 += b * s * (-2.0 * da / (a * a * a));
+        if (!c.single) {
+            const Dual4 G = nodes_tangent<PREC>(Kg, dKg, a, da, 0.0, 0.0, true, c, glx, glw,
+                                                glxf, glwf, nullptr, true);
+            dG += (hb * s) * G.dsz;
+        }
+    }
+    dP2 *= 0.5;
+    return -(qi * qi) * c.inv2pieps * (-dP1 + dP2 + dG);
+}
+
+// ---- kernels ------------------------------------------------------------------------
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pair_energy(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf,
+    const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
+    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
+    double4 *__restrict__ out)
+{
+    extern __shared__ float4 lds_tab[];
+    const float4 *tab = nullptr;
+    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+        const int64_t i = pi[p], j = pj[p];
+        out[p] = pair_energy<PREC>(c, glx, glw, glxf, glwf, tab, xyz[3 * i], xyz[3 * i + 1],
+                                   xyz[3 * i + 2], xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2]);
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pair_tangent(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf,
+    const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
+    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
+    const double *__restrict__ q, const double *__restrict__ v, double4 *__restrict__ out)
+{
+    extern __shared__ float4 lds_tab[];
+    const float4 *tab = nullptr;
+    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+        const int64_t i = pi[p], j = pj[p];
+        out[p] = pair_force_tangent<PREC>(c, glx, glw, glxf, glwf, tab, q[i], q[j], xyz + 3 * i,
+                                          xyz + 3 * j, v + 3 * i, v + 3 * j);
+    }
+}
+
+// per atom: phi (dE/dq) and psi_1,2 (d phi / d gamma) from the pair
+// potentials of both endpoints plus the self term; fixed order
+template <int PREC>
+__global__ __launch_bounds__(256) void k_gather_energy(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf, int64_t n,
+    const double *__restrict__ xyz, const double *__restrict__ q,
+    const double4 *__restrict__ pot, const int32_t *__restrict__ pi,
+    const int32_t *__restrict__ pj, const int32_t *__restrict__ i_start,
+    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
+    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
+    const int32_t *__restrict__ j_idx, double *__restrict__ phi, double *__restrict__ psi)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double f = 0.0, s0 = 0.0, s1 = 0.0;
+    for (int32_t t = i_start[a]; t < i_end[a]; ++t) {
+        const int32_t p = i_idx ? i_idx[t] : t;
+        const double4 u = pot[p];
+        const double qb = q[pj[p]];
+        f += qb * u.x;
+        s0 += qb * u.y;
+        s1 += qb * u.z;
+    }
+    for (int32_t t = j_start[a]; t < j_end[a]; ++t) {
+        const int32_t p = j_idx[t];
+        const double4 u = pot[p];
+        const double qb = q[pi[p]];
+        f += qb * u.x;
+        s0 += qb * u.y;
+        s1 += qb * u.z;
+    }
+    const double3 se = self_energy<PREC>(c, glx, glw, glxf, glwf, xyz[3 * a + 2]);
+    const double qa = q[a];
+    phi[a] = f + 2.0 * qa * se.x;
+    psi[2 * a] = s0 + 2.0 * qa * se.y;
+    psi[2 * a + 1] = s1 + 2.0 * qa * se.z;
+}
+
+// per atom: scale * (tangent of the short-range force), pair sides + self
+template <int PREC>
+__global__ __launch_bounds__(256) void k_gather_tangent(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf, int64_t n,
+    const double *__restrict__ xyz, const double *__restrict__ q, const double *__restrict__ v,
+    const double4 *__restrict__ tp, const int32_t *__restrict__ i_start,
+    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
+    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
+    const int32_t *__restrict__ j_idx, double scale, double *__restrict__ out)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int32_t t = i_start[a]; t < i_end[a]; ++t) {
+        const double4 d = tp[i_idx ? i_idx[t] : t];
+        fx += d.x;
+        fy += d.y;
+        fz += d.z;
+    }
+    for (int32_t t = j_start[a]; t < j_end[a]; ++t) {
+        const double4 d = tp[j_idx[t]];
+        fx -= d.x;
+        fy -= d.y;
+        fz += d.w;
+    }
+    fz += self_force_tangent<PREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], v[3 * a + 2]);
+    out[3 * a] = scale * fx;
+    out[3 * a + 1] = scale * fy;
+    out[3 * a + 2] = scale * fz;
+}
+
+// 1/2 sum_i q_i phi_i, 1/2 sum_i q_i psi_i,w (one block, fixed order)
+__global__ __launch_bounds__(256) void k_half_dots(int64_t n, const double *__restrict__ q,
+                                                   const double *__restrict__ phi,
+                                                   const double *__restrict__ psi,
+                                                   double *__restrict__ out3)
+{
+    __shared__ double s[3][256];
+    double a = 0.0, b = 0.0, d = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        a += q[i] * phi[i];
+        if (psi) {
+            b += q[i] * psi[2 * i];
+            d += q[i] * psi[2 * i + 1];
+        }
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = d;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int r = 0; r < 3; ++r) s[r][threadIdx.x] += s[r][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out3[0] = 0.5 * s[0][0];
+        out3[1] = 0.5 * s[1][0];
+        out3[2] = 0.5 * s[2][0];
+    }
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+inline unsigned pair_grid(int64_t m)
+{
+    const int64_t want = (m + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
+}
+
+}  // namespace
+
+hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
+                               const NeighborList &nl, int64_t n, const double *d_xyz,
+                               const double *d_q, double *d_pot, double *d_phi,
+                               double *d_psi, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t np = nl.npairs;
+    const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
+    double4 *pot = reinterpret_cast<double4 *>(d_pot);
+    const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+    if (precision == QEM_PREC_MIXED) {
+        if (np > 0)
+            k_pair_energy<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
+                nl.pj.as<int32_t>(), np, d_xyz, pot);
+        k_gather_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
+            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
+            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi, d_psi);
+    } else {
+        if (np > 0)
+            k_pair_energy<QEM_PREC_FP64><<<pair_grid(np), 256, 0, s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
+                d_xyz, pot);
+        k_gather_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
+            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
+            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi, d_psi);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
+                                const NeighborList &nl, int64_t n, const double *d_xyz,
+                                const double *d_q, const double *d_v, double *d_tp,
+                                double scale, double *d_out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t np = nl.npairs;
+    const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
+    double4 *tp = reinterpret_cast<double4 *>(d_tp);
+    const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+    if (precision == QEM_PREC_MIXED) {
+        if (np > 0)
+            k_pair_tangent<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
+                nl.pj.as<int32_t>(), np, d_xyz, d_q, d_v, tp);
+        k_gather_tangent<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_v, tp, nl.i_start.as<int32_t>(),
+            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+            nl.j_idx.as<int32_t>(), scale, d_out);
+    } else {
+        if (np > 0)
+            k_pair_tangent<QEM_PREC_FP64><<<pair_grid(np), 256, 0, s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
+                d_xyz, d_q, d_v, tp);
+        k_gather_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_v, tp, nl.i_start.as<int32_t>(),
+            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+            nl.j_idx.as<int32_t>(), scale, d_out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_half_dots(int64_t n, const double *d_q, const double *d_phi,
+                            const double *d_psi, double *d_out3, hipStream_t s)
+{
+    k_half_dots<<<1, 256, 0, s>>>(n, d_q, d_phi, d_psi, d_out3);
+    return hipGetLastError();
+}
+
+}  // namespace qem

@@ -19,7 +19,7 @@
 //          the structure factors minus the atom's own term give Fl_non_sort;
 //          Flz_self; weighted and summed over the tile's k in registers.
 //   R      per atom: sum over k tiles, scatter back to input order.
-// The k = 0 term rides along as an extra row with (q, 0, q, 0).
+// The k = 0 term rides along as an extra row with (q, q z, q, q z).
 // Nothing of size N x P is stored.
 #include <hipcub/hipcub.hpp>
 
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(CH) void k_long_A(const double4 *__restrict__ kv, i
         const double4 kk = kv[p];
         double4 v;
         if (kk.z == 0.0) {
-            v = make_double4(q, 0.0, q, 0.0);
+            v = make_double4(q, q * z, q, q * z);
         } else {
             double c, s, Ep, Em;
             atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(CH) void k_long_B(LongConsts lc, const double4 *__r
         double c = 1.0, s = 0.0, Ep = 1.0, Em = 1.0;
         double4 v;
         if (k0) {
-            v = make_double4(q, 0.0, q, 0.0);
+            v = make_double4(q, q * z, q, q * z);
         } else {
             atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
             v = live ? make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em)
@@ -349,6 +349,367 @@ __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint passes (no counterpart in the reference).
+//
+// Energy of one k (weight w_k including 1/(Lx Ly eps_0)):
+//   E_k = w_k/(2k) [ sum_{i<j} q_i q_j cos(k.rho_ij) e^{-k|z_ij|} + I_k ],
+//   I_k = [g1 |S-|^2 + g2 e2 |S+|^2 + 2 g1 g2 e2 Re(S+ conj S-)] / (2 (1 - g1 g2 e2))
+// with S+ = sum q e^{i k.rho} e^{kz} (= T.x + i T.y), S- likewise with
+// e^{-kz}, e2 = e^{-2kLz}.  phi_i = dE/dq_i = w_k/(2k) [direct_i + dI/dq_i]
+// with direct_i = c (e^- A + e^+ C) + s (e^- B + e^+ D) from the same
+// prefix/suffix sums as the force.  k = 0: phi_i = -1/(2 Lx Ly eps_0)
+// [z_i (Q_below - Q_above) - (QZ_below - QZ_above)].  E = 1/2 sum q_i phi_i.
+//
+// Tangents: the per-atom terms and all sums carry a first-order tangent
+// along v (8 channels per scan); the k = 0 force is piecewise constant in z
+// and has none.
+
+// in-wave exclusive prefix (x, y) / exclusive suffix (z, w); lane 63 / lane 0
+// leave the wave totals in wsum[wv]
+__device__ __forceinline__ double4 wave_scan_ex(double4 v, int lane, int wv, double4 *wsum)
+{
+    double4 inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double ux = __shfl_up(inc.x, d), uy = __shfl_up(inc.y, d);
+        const double dz = __shfl_down(inc.z, d), dw = __shfl_down(inc.w, d);
+        if (lane >= d) {
+            inc.x += ux;
+            inc.y += uy;
+        }
+        if (lane + d < 64) {
+            inc.z += dz;
+            inc.w += dw;
+        }
+    }
+    if (lane == 63) {
+        wsum[wv].x = inc.x;
+        wsum[wv].y = inc.y;
+    }
+    if (lane == 0) {
+        wsum[wv].z = inc.z;
+        wsum[wv].w = inc.w;
+    }
+    double4 ex;
+    ex.x = __shfl_up(inc.x, 1);
+    ex.y = __shfl_up(inc.y, 1);
+    ex.z = __shfl_down(inc.z, 1);
+    ex.w = __shfl_down(inc.w, 1);
+    if (lane == 0) ex.x = ex.y = 0.0;
+    if (lane == 63) ex.z = ex.w = 0.0;
+    return ex;
+}
+
+// after a barrier: add the totals of the waves below (prefix) / above (suffix)
+__device__ __forceinline__ double4 add_wave_offsets(double4 ex, int wv, const double4 *wsum)
+{
+    for (int w = 0; w < wv; ++w) {
+        ex.x += wsum[w].x;
+        ex.y += wsum[w].y;
+    }
+    for (int w = CH / 64 - 1; w > wv; --w) {
+        ex.z += wsum[w].z;
+        ex.w += wsum[w].w;
+    }
+    return ex;
+}
+
+// Energy pass B: per atom potential phi over the tile's k rows.
+template <int PREC>
+__global__ __launch_bounds__(CH) void k_long_B_energy(LongConsts lc, const double4 *__restrict__ kv,
+                                                      int64_t Ptot, int kt,
+                                                      const double *__restrict__ sx,
+                                                      const double *__restrict__ sy,
+                                                      const double *__restrict__ sz,
+                                                      const double *__restrict__ sq, int64_t n,
+                                                      int nchunk,
+                                                      const double4 *__restrict__ offA,
+                                                      const double4 *__restrict__ totA,
+                                                      double *__restrict__ partB)
+{
+    __shared__ double4 wsum[2][CH / 64];
+    const int m = blockIdx.x, tile = blockIdx.y;
+    const int64_t p0 = (int64_t)tile * kt, p1 = min(Ptot, p0 + kt);
+    const int64_t r = (int64_t)m * CH + threadIdx.x;
+    const bool live = r < n;
+    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0;
+    const double z = live ? sz[r] : 0.0, q = live ? sq[r] : 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double phi = 0.0;
+    for (int64_t p = p0; p < p1; ++p) {
+        const double4 kk = kv[p];
+        const bool k0 = kk.z == 0.0;
+        double c = 1.0, s = 0.0, Ep = 1.0, Em = 1.0;
+        double4 v;
+        if (k0) {
+            v = make_double4(q, q * z, q, q * z);
+        } else {
+            atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
+            v = live ? make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em)
+                     : make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        const int b = (int)(p & 1);
+        double4 ex = wave_scan_ex(v, lane, wv, wsum[b]);
+        __syncthreads();
+        ex = add_wave_offsets(ex, wv, wsum[b]);
+        const double4 off = offA[p * nchunk + m];
+        const double A = off.x + ex.x, B = off.y + ex.y, C = off.z + ex.z, D = off.w + ex.w;
+        if (k0) {
+            // A, C: charge below / above; B, D: charge * z below / above
+            phi -= kk.w * (z * (A - C) - (B - D));
+            continue;
+        }
+        const double k = kk.z;
+        const double4 T = totA[p];
+        const double e2 = exp(-2.0 * k * lc.Lz);
+        const double direct = c * (Em * A + Ep * C) + s * (Em * B + Ep * D);
+        const double ce = c * Em, se = s * Em, cp = c * Ep, sp = s * Ep;
+        const double dI = (lc.g1 * 2.0 * (T.z * ce + T.w * se) +
+                           lc.g2 * 2.0 * ((e2 * T.x) * cp + (e2 * T.y) * sp) +
+                           2.0 * lc.g12 * (cp * (e2 * T.z) + sp * (e2 * T.w) + (e2 * T.x) * ce +
+                                           (e2 * T.y) * se)) /
+                          (2.0 * (1.0 - lc.g12 * e2));
+        phi += kk.w / (2.0 * k) * (direct + dI);
+    }
+    if (live) partB[(int64_t)tile * n + r] = phi;
+}
+
+// per k row: w_k/(2k) dI_k/dgamma_1,2 from the structure factors
+__global__ void k_long_dgamma(LongConsts lc, const double4 *__restrict__ kv, int64_t Ptot,
+                              const double4 *__restrict__ totA, double2 *__restrict__ out)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Ptot) return;
+    const double4 kk = kv[p];
+    if (kk.z == 0.0) {
+        out[p] = make_double2(0.0, 0.0);
+        return;
+    }
+    const double k = kk.z;
+    const double4 S = totA[p];
+    const double e2 = exp(-2.0 * k * lc.Lz);
+    const double om = 1.0 - lc.g12 * e2;
+    const double A3 = S.z * S.z + S.w * S.w;
+    const double A1e = (e2 * S.x) * S.x + (e2 * S.y) * S.y;
+    const double Xe = (e2 * S.x) * S.z + (e2 * S.y) * S.w;
+    const double N = lc.g1 * A3 + lc.g2 * A1e + 2.0 * lc.g12 * Xe;
+    const double d1 = (A3 + 2.0 * lc.g2 * Xe) / (2.0 * om) + N * lc.g2 * e2 / (2.0 * om * om);
+    const double d2 = (A1e + 2.0 * lc.g1 * Xe) / (2.0 * om) + N * lc.g1 * e2 / (2.0 * om * om);
+    const double f = kk.w / (2.0 * k);
+    out[p] = make_double2(f * d1, f * d2);
+}
+
+__global__ void k_gather_sorted3(const double *__restrict__ v, const int32_t *__restrict__ perm,
+                                 int64_t n, double *__restrict__ svx, double *__restrict__ svy,
+                                 double *__restrict__ svz)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t a = perm[r];
+    svx[r] = v[3 * a];
+    svy[r] = v[3 * a + 1];
+    svz[r] = v[3 * a + 2];
+}
+
+// value and tangent of the per-atom terms of one k
+template <int PREC>
+__device__ __forceinline__ void atom_terms_t(double kx, double ky, double k, double x, double y,
+                                             double z, double vx, double vy, double vz,
+                                             double q, double4 &v, double4 &dv, double &c,
+                                             double &s, double &Ep, double &Em, double &dc,
+                                             double &ds, double &dEp, double &dEm)
+{
+    atom_terms<PREC>(kx, ky, k, x, y, z, c, s, Ep, Em);
+    const double dth = kx * vx + ky * vy;
+    dc = -s * dth;
+    ds = c * dth;
+    dEp = k * vz * Ep;
+    dEm = -k * vz * Em;
+    v = make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em);
+    dv = make_double4(q * (dc * Ep + c * dEp), q * (ds * Ep + s * dEp), q * (dc * Em + c * dEm),
+                      q * (ds * Em + s * dEm));
+}
+
+template <int PREC>
+__global__ __launch_bounds__(CH) void k_long_A_t(const double4 *__restrict__ kv, int64_t Ptot,
+                                                 int kt, const double *__restrict__ sx,
+                                                 const double *__restrict__ sy,
+                                                 const double *__restrict__ sz,
+                                                 const double *__restrict__ sq,
+                                                 const double *__restrict__ svx,
+                                                 const double *__restrict__ svy,
+                                                 const double *__restrict__ svz, int64_t n,
+                                                 int nchunk, double4 *__restrict__ partA,
+                                                 double4 *__restrict__ partT)
+{
+    __shared__ double4 red[2][2][CH / 64];
+    const int m = blockIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.y * kt, p1 = min(Ptot, p0 + kt);
+    const int64_t r = (int64_t)m * CH + threadIdx.x;
+    const bool live = r < n;
+    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0, z = live ? sz[r] : 0.0;
+    const double q = live ? sq[r] : 0.0;
+    const double vx = live ? svx[r] : 0.0, vy = live ? svy[r] : 0.0, vz = live ? svz[r] : 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t p = p0; p < p1; ++p) {
+        const double4 kk = kv[p];
+        double4 v, dv;
+        if (kk.z == 0.0) {
+            v = make_double4(q, q * z, q, q * z);
+            dv = make_double4(0.0, 0.0, 0.0, 0.0);
+        } else {
+            double c, s, Ep, Em, dc, ds, dEp, dEm;
+            atom_terms_t<PREC>(kk.x, kk.y, kk.z, x, y, z, vx, vy, vz, q, v, dv, c, s, Ep, Em, dc,
+                               ds, dEp, dEm);
+            if (!live) v = dv = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            v.x += __shfl_down(v.x, d);
+            v.y += __shfl_down(v.y, d);
+            v.z += __shfl_down(v.z, d);
+            v.w += __shfl_down(v.w, d);
+            dv.x += __shfl_down(dv.x, d);
+            dv.y += __shfl_down(dv.y, d);
+            dv.z += __shfl_down(dv.z, d);
+            dv.w += __shfl_down(dv.w, d);
+        }
+        const int b = (int)(p & 1);
+        if (lane == 0) {
+            red[b][0][wv] = v;
+            red[b][1][wv] = dv;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double4 t = red[b][0][0], u = red[b][1][0];
+            for (int w = 1; w < CH / 64; ++w) {
+                t = add4(t, red[b][0][w]);
+                u = add4(u, red[b][1][w]);
+            }
+            partA[p * nchunk + m] = t;
+            partT[p * nchunk + m] = u;
+        }
+    }
+}
+
+// Tangent pass B: tangent of the per-k force, summed over the tile's rows.
+template <int PREC>
+__global__ __launch_bounds__(CH) void k_long_B_t(LongConsts lc, const double4 *__restrict__ kv,
+                                                 int64_t Ptot, int kt,
+                                                 const double *__restrict__ sx,
+                                                 const double *__restrict__ sy,
+                                                 const double *__restrict__ sz,
+                                                 const double *__restrict__ sq,
+                                                 const double *__restrict__ svx,
+                                                 const double *__restrict__ svy,
+                                                 const double *__restrict__ svz, int64_t n,
+                                                 int nchunk, const double4 *__restrict__ offA,
+                                                 const double4 *__restrict__ totA,
+                                                 const double4 *__restrict__ offT,
+                                                 const double4 *__restrict__ totT,
+                                                 double *__restrict__ partB)
+{
+    __shared__ double4 wsum[2][2][CH / 64];
+    const int m = blockIdx.x, tile = blockIdx.y;
+    const int64_t p0 = (int64_t)tile * kt, p1 = min(Ptot, p0 + kt);
+    const int64_t r = (int64_t)m * CH + threadIdx.x;
+    const bool live = r < n;
+    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0, z = live ? sz[r] : 0.0;
+    const double q = live ? sq[r] : 0.0;
+    const double vx = live ? svx[r] : 0.0, vy = live ? svy[r] : 0.0, vz = live ? svz[r] : 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int64_t p = p0; p < p1; ++p) {
+        const double4 kk = kv[p];
+        if (kk.z == 0.0) continue;  // uniform per row: no barrier is skipped unevenly
+        double c, s, Ep, Em, dc, ds, dEp, dEm;
+        double4 v, dv;
+        atom_terms_t<PREC>(kk.x, kk.y, kk.z, x, y, z, vx, vy, vz, q, v, dv, c, s, Ep, Em, dc, ds,
+                           dEp, dEm);
+        if (!live) v = dv = make_double4(0.0, 0.0, 0.0, 0.0);
+        const int b = (int)(p & 1);
+        double4 ex = wave_scan_ex(v, lane, wv, wsum[b][0]);
+        double4 et = wave_scan_ex(dv, lane, wv, wsum[b][1]);
+        __syncthreads();
+        ex = add_wave_offsets(ex, wv, wsum[b][0]);
+        et = add_wave_offsets(et, wv, wsum[b][1]);
+        const double4 off = offA[p * nchunk + m], offt = offT[p * nchunk + m];
+        const double A = off.x + ex.x, B = off.y + ex.y, C = off.z + ex.z, D = off.w + ex.w;
+        const double dA = offt.x + et.x, dB = offt.y + et.y, dC = offt.z + et.z,
+                     dD = offt.w + et.w;
+        const double k = kk.z;
+        double gx = 0.0, gz = 0.0;
+        if (lc.parts & QEM_LONG_NONSORT) {
+            const double4 T = totA[p], dT = totT[p];
+            const double f1 = T.x - v.x, f2 = T.y - v.y, f3 = T.z - v.z, f4 = T.w - v.w;
+            const double df1 = dT.x - dv.x, df2 = dT.y - dv.y, df3 = dT.z - dv.z,
+                         df4 = dT.w - dv.w;
+            const double e2 = exp(-2.0 * k * lc.Lz);
+            const double g = 0.5 / (lc.g12 * e2 - 1.0);
+            const double u = lc.g2 * e2 * Ep + lc.g12 * e2 * Em;
+            const double du = lc.g2 * e2 * dEp + lc.g12 * e2 * dEm;
+            const double w_ = lc.g1 * Em + lc.g12 * e2 * Ep;
+            const double dw_ = lc.g1 * dEm + lc.g12 * e2 * dEp;
+            const double uz = lc.g2 * e2 * Ep - lc.g12 * e2 * Em;
+            const double duz = lc.g2 * e2 * dEp - lc.g12 * e2 * dEm;
+            const double wz = -lc.g1 * Em + lc.g12 * e2 * Ep;
+            const double dwz = -lc.g1 * dEm + lc.g12 * e2 * dEp;
+            const double Xa = u * f2 + w_ * f4, Xb = u * f1 + w_ * f3;
+            const double dXa = du * f2 + u * df2 + dw_ * f4 + w_ * df4;
+            const double dXb = du * f1 + u * df1 + dw_ * f3 + w_ * df3;
+            gx += q * g * (dc * Xa + c * dXa - ds * Xb - s * dXb);
+            const double Za = uz * f1 + wz * f3, Zb = uz * f2 + wz * f4;
+            const double dZa = duz * f1 + uz * df1 + dwz * f3 + wz * df3;
+            const double dZb = duz * f2 + uz * df2 + dwz * f4 + wz * df4;
+            gz += q * g * (dc * Za + c * dZa + ds * Zb + s * dZb);
+        }
+        if (lc.parts & QEM_LONG_SORT) {
+            const double dsxy = ds * Em * A + s * dEm * A + s * Em * dA
+                              - (dc * Em * B + c * dEm * B + c * Em * dB)
+                              + ds * Ep * C + s * dEp * C + s * Ep * dC
+                              - (dc * Ep * D + c * dEp * D + c * Ep * dD);
+            gx += 0.5 * q * dsxy;
+            const double dszz = dc * Em * A + c * dEm * A + c * Em * dA
+                              + ds * Em * B + s * dEm * B + s * Em * dB
+                              - (dc * Ep * C + c * dEp * C + c * Ep * dC)
+                              - (ds * Ep * D + s * dEp * D + s * Ep * dD);
+            gz += 0.5 * q * dszz;
+        }
+        if (lc.parts & QEM_LONG_SELF) {
+            const double e2 = exp(-2.0 * k * lc.Lz);
+            gz += q * q * (lc.g2 * e2 * 2.0 * Ep * dEp - lc.g1 * 2.0 * Em * dEm) /
+                  (2.0 * (lc.g12 * e2 - 1.0));
+        }
+        fx += kk.w * (kk.x / k) * gx;
+        fy += kk.w * (kk.y / k) * gx;
+        fz += kk.w * gz;
+    }
+    if (live) {
+        double *o = partB + ((int64_t)tile * n + r) * 3;
+        o[0] = fx;
+        o[1] = fy;
+        o[2] = fz;
+    }
+}
+
+// sum the k tiles of a width-w per-atom quantity, scale, scatter to input order
+__global__ void k_long_Rw(const double *__restrict__ partB, int ntile, int64_t n, int width,
+                          const int32_t *__restrict__ perm, int accumulate, double scale,
+                          double *__restrict__ f)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t a = perm[r];
+    for (int c = 0; c < width; ++c) {
+        double acc = 0.0;
+        for (int t = 0; t < ntile; ++t) acc += partB[((int64_t)t * n + r) * width + c];
+        acc *= scale;
+        if (accumulate) f[width * a + c] += acc;
+        else f[width * a + c] = acc;
+    }
+}
+
 #define LR_CHECK(expr)                                                        \
     do {                                                                      \
         hipError_t e_ = (expr);                                               \
@@ -448,6 +809,135 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Pto
     LR_CHECK(hipGetLastError());
     k_long_R<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ntile, n, lw.perm.as<int32_t>(),
                                           accumulate, d_f);
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+namespace {
+struct Tiling {
+    int nchunk, ntile, kt;
+};
+Tiling tiling(int64_t n, int64_t Ptot)
+{
+    Tiling t;
+    t.nchunk = (int)((n + CH - 1) / CH);
+    t.ntile = (int)std::min<int64_t>(Ptot, std::max<int64_t>(1, (2048 + t.nchunk - 1) / t.nchunk));
+    t.kt = (int)((Ptot + t.ntile - 1) / t.ntile);
+    t.ntile = (int)((Ptot + t.kt - 1) / t.kt);
+    return t;
+}
+}  // namespace
+
+hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                              double *d_phi, int accumulate, double *h_dgamma, hipStream_t s,
+                              std::string &err)
+{
+    h_dgamma[0] = h_dgamma[1] = 0.0;
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
+    const Tiling t = tiling(n, Ptot);
+    LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
+    LR_CHECK(ensure(lw.partB, sizeof(double) * (size_t)t.ntile * (size_t)n));
+    LR_CHECK(ensure(lw.q0part, sizeof(double2) * (size_t)Ptot));
+    const double4 *kv = lw.kv.as<double4>();
+    dim3 g(t.nchunk, t.ntile);
+    if (c.precision == QEM_PREC_MIXED)
+        k_long_A<QEM_PREC_MIXED><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
+                                                  lw.sz.as<double>(), lw.sq.as<double>(), n, t.nchunk,
+                                                  lw.partA.as<double4>());
+    else
+        k_long_A<QEM_PREC_FP64><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
+                                                 lw.sz.as<double>(), lw.sq.as<double>(), n, t.nchunk,
+                                                 lw.partA.as<double4>());
+    LR_CHECK(hipGetLastError());
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, t.nchunk,
+                                           lw.offA.as<double4>(), lw.totA.as<double4>());
+    LR_CHECK(hipGetLastError());
+    if (c.precision == QEM_PREC_MIXED)
+        k_long_B_energy<QEM_PREC_MIXED><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(),
+                                                         lw.sy.as<double>(), lw.sz.as<double>(),
+                                                         lw.sq.as<double>(), n, t.nchunk,
+                                                         lw.offA.as<double4>(), lw.totA.as<double4>(),
+                                                         lw.partB.as<double>());
+    else
+        k_long_B_energy<QEM_PREC_FP64><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(),
+                                                        lw.sy.as<double>(), lw.sz.as<double>(),
+                                                        lw.sq.as<double>(), n, t.nchunk,
+                                                        lw.offA.as<double4>(), lw.totA.as<double4>(),
+                                                        lw.partB.as<double>());
+    LR_CHECK(hipGetLastError());
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), t.ntile, n, 1,
+                                           lw.perm.as<int32_t>(), accumulate, 1.0, d_phi);
+    LR_CHECK(hipGetLastError());
+    k_long_dgamma<<<nblk(Ptot, 256), 256, 0, s>>>(c, kv, Ptot, lw.totA.as<double4>(),
+                                                  lw.q0part.as<double2>());
+    LR_CHECK(hipGetLastError());
+    std::vector<double> h(2 * (size_t)Ptot);
+    LR_CHECK(hipMemcpyAsync(h.data(), lw.q0part.p, sizeof(double2) * (size_t)Ptot,
+                            hipMemcpyDeviceToHost, s));
+    LR_CHECK(hipStreamSynchronize(s));
+    for (int64_t p = 0; p < Ptot; ++p) {
+        h_dgamma[0] += h[2 * (size_t)p];
+        h_dgamma[1] += h[2 * (size_t)p + 1];
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                               const double *d_v, double scale, double *d_out, int accumulate,
+                               hipStream_t s, std::string &err)
+{
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
+    const Tiling t = tiling(n, Ptot);
+    const size_t nd = sizeof(double) * (size_t)n;
+    LR_CHECK(ensure(lw.svx, nd));
+    LR_CHECK(ensure(lw.svy, nd));
+    LR_CHECK(ensure(lw.svz, nd));
+    LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
+    LR_CHECK(ensure(lw.partT, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.offT, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.totT, sizeof(double4) * (size_t)Ptot));
+    LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)t.ntile * (size_t)n));
+    k_gather_sorted3<<<nblk(n, 256), 256, 0, s>>>(d_v, lw.perm.as<int32_t>(), n, lw.svx.as<double>(),
+                                                  lw.svy.as<double>(), lw.svz.as<double>());
+    const double4 *kv = lw.kv.as<double4>();
+    dim3 g(t.nchunk, t.ntile);
+    if (c.precision == QEM_PREC_MIXED)
+        k_long_A_t<QEM_PREC_MIXED><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
+                                                    lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
+                                                    lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
+                                                    lw.partA.as<double4>(), lw.partT.as<double4>());
+    else
+        k_long_A_t<QEM_PREC_FP64><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
+                                                   lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
+                                                   lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
+                                                   lw.partA.as<double4>(), lw.partT.as<double4>());
+    LR_CHECK(hipGetLastError());
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, t.nchunk,
+                                           lw.offA.as<double4>(), lw.totA.as<double4>());
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partT.as<double4>(), Ptot, t.nchunk,
+                                           lw.offT.as<double4>(), lw.totT.as<double4>());
+    LR_CHECK(hipGetLastError());
+    if (c.precision == QEM_PREC_MIXED)
+        k_long_B_t<QEM_PREC_MIXED><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
+                                                    lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
+                                                    lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
+                                                    lw.offA.as<double4>(), lw.totA.as<double4>(),
+                                                    lw.offT.as<double4>(), lw.totT.as<double4>(),
+                                                    lw.partB.as<double>());
+    else
+        k_long_B_t<QEM_PREC_FP64><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
+                                                   lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
+                                                   lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
+                                                   lw.offA.as<double4>(), lw.totA.as<double4>(),
+                                                   lw.offT.as<double4>(), lw.totT.as<double4>(),
+                                                   lw.partB.as<double>());
+    LR_CHECK(hipGetLastError());
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), t.ntile, n, 3, lw.perm.as<int32_t>(),
+                                           accumulate, scale, d_out);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }

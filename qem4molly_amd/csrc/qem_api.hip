@@ -540,6 +540,110 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     return QEM_OK;
 }
 
+// the current neighbour list, built if the context has none for n atoms
+// (does not advance the step counter)
+int ensure_list(qem_ctx *ctx, int64_t n, const double *d_xyz, hipStream_t s)
+{
+    if (ctx->nl.n == n && !ctx->nl.has_i_idx) return QEM_OK;
+    PhaseScope ps(ctx, QEM_PHASE_NEIGHBOR, s);
+    hipError_t e = build_neighbor_list(ctx->nl, n, d_xyz, ctx->p.Lx, ctx->p.Ly, ctx->p.r_cutoff,
+                                       s, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, ctx->err.c_str());
+    return QEM_OK;
+}
+
+LongConsts long_consts(const qem_ctx *ctx, double weight, uint32_t parts)
+{
+    LongConsts c;
+    c.Lz = ctx->p.Lz;
+    c.g1 = ctx->p.gamma_1;
+    c.g2 = ctx->p.gamma_2;
+    c.g12 = c.g1 * c.g2;
+    c.inv_area_eps = 1.0 / (ctx->p.Lx * ctx->p.Ly * ctx->p.eps_0);
+    c.weight = weight;
+    c.inv4alpha = 0.25 / ctx->p.alpha;
+    c.parts = parts;
+    c.precision = ctx->p.precision;
+    return c;
+}
+
+// E, phi = dE/dq (device, N) and dE/dgamma for the current list + k batch
+int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+               const double *kvecs, int64_t P, double weight, double *d_phi, double *E,
+               double *dEdg, hipStream_t s)
+{
+    if (n == 0) {
+        if (E) *E = 0.0;
+        if (dEdg) dEdg[0] = dEdg[1] = 0.0;
+        return QEM_OK;
+    }
+    int st = ensure_list(ctx, n, d_xyz, s);
+    if (st) return st;
+    const ShortConsts c = short_consts(ctx, 0);
+    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (3 * (size_t)n + 4)));
+    double *psi = ctx->d_adj.as<double>();          // N x 2
+    double *out3 = psi + 2 * n;                      // 3 (+1 pad)
+    double *phi = d_phi;
+    if (!phi) {
+        API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
+        phi = ctx->d_f2.as<double>();
+    }
+    {
+        PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
+        API_HIP(ctx, launch_short_energy(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
+                                         ctx->sw.fpair.as<double>(), phi, psi, s));
+    }
+    double dg_long[2] = {0.0, 0.0};
+    if (P >= 0) {
+        int64_t Ptot = 0;
+        if ((st = stage_kvecs(ctx, kvecs, P, weight, true, s, &Ptot))) return st;
+        PhaseScope ps(ctx, QEM_PHASE_LONG, s);
+        hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "z sort");
+        e = launch_long_energy(long_consts(ctx, weight, QEM_LONG_ALL), ctx->lw, n, Ptot, phi, 1,
+                               dg_long, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
+    }
+    API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, s));
+    double h[3];
+    API_HIP(ctx, hipMemcpyAsync(h, out3, sizeof h, hipMemcpyDeviceToHost, s));
+    API_HIP(ctx, hipStreamSynchronize(s));
+    if (E) *E = h[0];
+    if (dEdg) {
+        dEdg[0] = h[1] + dg_long[0];
+        dEdg[1] = h[2] + dg_long[1];
+    }
+    return QEM_OK;
+}
+
+// hv = -(dF/dx) v for the current list + k batch
+int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, const double *kvecs,
+            int64_t P, double weight, const double *d_v, double *d_hv, hipStream_t s)
+{
+    if (n == 0) return QEM_OK;
+    int st = ensure_list(ctx, n, d_xyz, s);
+    if (st) return st;
+    const ShortConsts c = short_consts(ctx, 0);
+    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    {
+        PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
+        API_HIP(ctx, launch_short_tangent(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
+                                          d_v, ctx->sw.fpair.as<double>(), -1.0, d_hv, s));
+    }
+    if (P > 0) {
+        int64_t Ptot = 0;
+        if ((st = stage_kvecs(ctx, kvecs, P, weight, false, s, &Ptot))) return st;
+        PhaseScope ps(ctx, QEM_PHASE_LONG, s);
+        hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "z sort");
+        e = launch_long_tangent(long_consts(ctx, weight, QEM_LONG_NONSORT | QEM_LONG_SORT | QEM_LONG_SELF),
+                                ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "long-range tangent");
+    }
+    return QEM_OK;
+}
+
 #define CTX_GUARD(ctx)                                                        \
     do {                                                                      \
         if (!(ctx)) return QEM_ERR_INVALID_ARG;                               \
@@ -657,8 +761,9 @@ int qem_destroy(qem_ctx *ctx)
                       &ctx->lw.keys_sorted, &ctx->lw.idx, &ctx->lw.perm, &ctx->lw.tmp,
                       &ctx->lw.sx, &ctx->lw.sy, &ctx->lw.sz, &ctx->lw.sq, &ctx->lw.kv,
                       &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,
-                      &ctx->lw.q0off, &ctx->lw.partB, &ctx->d_xyz, &ctx->d_q, &ctx->d_f,
-                      &ctx->d_f2, &ctx->d_aux};
+                      &ctx->lw.q0off, &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,
+                      &ctx->lw.svz, &ctx->lw.partT, &ctx->lw.offT, &ctx->lw.totT, &ctx->d_xyz,
+                      &ctx->d_q, &ctx->d_f, &ctx->d_f2, &ctx->d_aux, &ctx->d_adj};
     for (DevBuf *b : bufs) release(*b);
     for (int t = 0; t < LongWork::kSlots; ++t) {
         if (ctx->lw.h_kv[t]) (void)hipHostFree(ctx->lw.h_kv[t]);
@@ -992,6 +1097,89 @@ int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds, int64_t *n
     if (n_builds) *n_builds = ctx->nl.builds;
     if (n_steps) *n_steps = ctx->n_steps;
     return QEM_OK;
+}
+
+int qem_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, const int64_t *k_idx,
+               int64_t P, double *E, double *dE_dq, double *dE_dgamma)
+{
+    CTX_GUARD(ctx);
+    if ((n > 0 && !q) || P < 0 || (P > 0 && !k_idx))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    std::vector<double> kv;
+    if ((st = kidx_to_vecs(ctx, k_idx, P, kv))) return st;
+    if (n == 0) return energy_dev(ctx, 0, nullptr, nullptr, nullptr, 0, 0.0, nullptr, E,
+                                  dE_dgamma, ctx->stream);
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    st = energy_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv.data(), P, w,
+                    ctx->d_f.as<double>(), E, dE_dgamma, ctx->stream);
+    if (st) return st;
+    if (dE_dq) {
+        API_HIP(ctx, hipMemcpyAsync(dE_dq, ctx->d_f.p, sizeof(double) * (size_t)n,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return QEM_OK;
+}
+
+int qem_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                   const int64_t *k_idx, int64_t P, double *d_dE_dq, double *E,
+                   double *dE_dgamma, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || P < 0 || (P > 0 && !k_idx) || (n > 0 && (!d_xyz || !d_q)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    std::vector<double> kv;
+    int st = kidx_to_vecs(ctx, k_idx, P, kv);
+    if (st) return st;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    return energy_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, d_dE_dq, E, dE_dgamma,
+                      pick_stream(ctx, stream));
+}
+
+int qem_hvp(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, const int64_t *k_idx,
+            int64_t P, const double *v, double *hv)
+{
+    CTX_GUARD(ctx);
+    if ((n > 0 && (!q || !v || !hv)) || P < 0 || (P > 0 && !k_idx))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    if (n > 0 && !finite_all(v, 3 * n)) return set_err(ctx, QEM_ERR_DOMAIN, "v must be finite");
+    std::vector<double> kv;
+    if ((st = kidx_to_vecs(ctx, k_idx, P, kv))) return st;
+    if (n == 0) return QEM_OK;
+    const size_t b3 = sizeof(double) * 3 * (size_t)n;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, b3)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_aux, v, b3)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, b3));
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    st = hvp_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv.data(), P, w,
+                 ctx->d_aux.as<double>(), ctx->d_f.as<double>(), ctx->stream);
+    if (st) return st;
+    API_HIP(ctx, hipMemcpyAsync(hv, ctx->d_f.p, b3, hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || P < 0 || (P > 0 && !k_idx) || (n > 0 && (!d_xyz || !d_q || !d_v || !d_hv)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    std::vector<double> kv;
+    int st = kidx_to_vecs(ctx, k_idx, P, kv);
+    if (st) return st;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    return hvp_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, d_v, d_hv, pick_stream(ctx, stream));
 }
 
 int qem_timing_enable(qem_ctx *ctx, int32_t on)

@@ -1,0 +1,84 @@
+// Device helpers shared by the short-range force, energy and tangent kernels.
+#pragma once
+
+#include "qem_internal.h"
+
+namespace qem {
+namespace dev {
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+__device__ __forceinline__ double jl_sign(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
+
+// FP32 J0/J1 from the LDS table of degree-7 Taylor polynomials of J0 about
+// the interval centres; J1 = -J0' from the same polynomial (p, p' together).
+__device__ __forceinline__ void bessel01_tab(float x, const float4 *__restrict__ tab, int ntab,
+                                             float dx, float inv_dx, float &J0, float &J1)
+{
+    int m = (int)(x * inv_dx);
+    m = m < ntab - 1 ? m : ntab - 1;
+    const float t = x - ((float)m + 0.5f) * dx;
+    const float4 c0 = tab[2 * m];
+    const float4 c1 = tab[2 * m + 1];
+    float p = c1.w;
+    float dp = 0.f;
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.z);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.y);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.x);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.w);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.z);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.y);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.x);
+    J0 = p;
+    J1 = -dp;
+}
+
+// Copies the Bessel table into this block's LDS.
+__device__ __forceinline__ const float4 *stage_table(int jt_n, const float4 *__restrict__ gtab,
+                                                     float4 *lds)
+{
+    const int n = 2 * jt_n;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) lds[t] = gtab[t];
+    __syncthreads();
+    return lds;
+}
+
+// Quadrature limits of one image (src/greens_function.jl:26-27) and their
+// derivatives with respect to the image distance a.
+struct Limits {
+    double kf1, kf2, dkf1, dkf2;
+};
+
+__device__ __forceinline__ Limits limits(const ShortConsts &c, double a)
+{
+    Limits L;
+    L.kf1 = -c.ln_acc / (2.0 * c.Lz + a);
+    const double root = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc);
+    L.kf2 = root - 2.0 * c.alpha * a;
+    L.dkf1 = -L.kf1 / (2.0 * c.Lz + a);
+    L.dkf2 = 4.0 * c.alpha * c.alpha * a / root - 2.0 * c.alpha;
+    return L;
+}
+
+// image l of a pair: distance a, factor b, sign s (src/greens_function.jl:17-24)
+__device__ __forceinline__ void image(const ShortConsts &c, int l, double zn, double zp, double sg,
+                                      double &a, double &b, double &s)
+{
+    a = l == 0 ? zn : (l == 1 ? zp : (l == 2 ? 2.0 * c.Lz - zp : 2.0 * c.Lz - zn));
+    b = l == 0 ? 1.0 : (l == 1 ? c.g1 : (l == 2 ? c.g2 : c.g12));
+    s = l == 0 ? -sg : (l == 1 ? -1.0 : (l == 2 ? 1.0 : sg));
+}
+
+// minimum-image xy separation (no contraction: the cutoff test must match
+// the neighbour-list builder bit for bit)
+__device__ __forceinline__ double min_image_d2(const ShortConsts &c, double dx, double dy,
+                                               double &mx, double &my)
+{
+#pragma clang fp contract(off)
+    mx = dx - c.Lx * rint(dx / c.Lx);
+    my = dy - c.Ly * rint(dy / c.Ly);
+    return mx * mx + my * my;
+}
+
+}  // namespace dev
+}  // namespace qem

@@ -124,6 +124,8 @@ struct LongWork {
     DevBuf totA;             // [P][4] totals
     DevBuf q0part, q0off;    // k = 0 chunk sums / offsets
     DevBuf partB;            // [ntile][n][3]
+    DevBuf svx, svy, svz;    // tangent directions in z order
+    DevBuf partT, offT, totT;  // tangent chunk sums / offsets / totals
     static constexpr int kSlots = 4;   // pinned staging ring for k vectors
     void *h_kv[kSlots] = {nullptr, nullptr, nullptr, nullptr};
     size_t h_kv_cap[kSlots] = {0, 0, 0, 0};
@@ -138,6 +140,31 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz,
 hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n,
                        int64_t P, double *d_f, int accumulate, hipStream_t s,
                        std::string &err);
+
+// Energy: phi (dE/dq) of the k rows in lw.kv into d_phi (overwrite or add);
+// sum over k of dE_k/dgamma_1,2 into h_dgamma (synchronises the stream).
+hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                              double *d_phi, int accumulate, double *h_dgamma, hipStream_t s,
+                              std::string &err);
+// Tangent of the long-range force along d_v (N x 3), times scale.
+hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                               const double *d_v, double scale, double *d_out, int accumulate,
+                               hipStream_t s, std::string &err);
+// Short-range energy: per-pair potentials in d_pot (double4 per pair), per
+// atom phi (N) and psi (N x 2: d phi / d gamma_1,2), self terms included.
+hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
+                               const NeighborList &nl, int64_t n, const double *d_xyz,
+                               const double *d_q, double *d_pot, double *d_phi,
+                               double *d_psi, hipStream_t s);
+// Short-range force tangent along d_v, times scale, into d_out (N x 3);
+// d_tp is per-pair scratch (double4 per pair).
+hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
+                                const NeighborList &nl, int64_t n, const double *d_xyz,
+                                const double *d_q, const double *d_v, double *d_tp,
+                                double scale, double *d_out, hipStream_t s);
+// out3 = (1/2 q.phi, 1/2 q.psi_1, 1/2 q.psi_2); psi may be null.
+hipError_t launch_half_dots(int64_t n, const double *d_q, const double *d_phi,
+                            const double *d_psi, double *d_out3, hipStream_t s);
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
@@ -161,7 +188,7 @@ struct qem_ctx {
     int64_t n_steps = 0;
     qem::LongWork lw;
     // host-pointer staging
-    qem::DevBuf d_xyz, d_q, d_f, d_f2, d_aux;
+    qem::DevBuf d_xyz, d_q, d_f, d_f2, d_aux, d_adj;
     // optional per-phase timing (HIP events on the launch stream)
     bool timing = false;
     struct Mark {

@@ -23,14 +23,14 @@
 //          (LDS resident; J1 = -J0' from the same polynomial, p and p' by one
 //          Horner pass).  Per-pair set-up, closed-form image terms and all
 //          sums over images, pairs and atoms stay FP64.
-#include "qem_internal.h"
+#include "qem_device.h"
 
 namespace qem {
 namespace {
 
-constexpr float kLog2e = 1.4426950408889634f;
-
-__device__ __forceinline__ double jl_sign(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
+using dev::bessel01_tab;
+using dev::jl_sign;
+using dev::kLog2e;
 
 // ---------------------------------------------------------------------------
 // Node sums for one image l and one integral type.
@@ -59,29 +59,6 @@ __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool 
     }
     sr = ar;
     sz = az;
-}
-
-// FP32 J0/J1 from the LDS table.
-__device__ __forceinline__ void bessel01_tab(float x, const float4 *__restrict__ tab, int ntab,
-                                             float dx, float inv_dx, float &J0, float &J1)
-{
-    int m = (int)(x * inv_dx);
-    m = m < ntab - 1 ? m : ntab - 1;
-    const float t = x - ((float)m + 0.5f) * dx;
-    const float4 c0 = tab[2 * m];
-    const float4 c1 = tab[2 * m + 1];
-    // p(t) and p'(t) together
-    float p = c1.w;
-    float dp = 0.f;
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.z);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.y);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.x);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.w);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.z);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.y);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.x);
-    J0 = p;
-    J1 = -dp;
 }
 
 __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool gauss,
@@ -266,17 +243,6 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
     return -(qi * qi) * c.inv2pieps * Fsz;
 }
 
-// LDS copy of the Bessel table (mixed precision only).
-__device__ __forceinline__ const float4 *stage_table(const ShortConsts &c,
-                                                     const float4 *__restrict__ gtab,
-                                                     float4 *lds)
-{
-    const int n = 2 * c.jt_n;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) lds[t] = gtab[t];
-    __syncthreads();
-    return lds;
-}
-
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_list(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
@@ -287,7 +253,7 @@ __global__ __launch_bounds__(256) void k_pair_list(
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = nullptr;
-    if (PREC == QEM_PREC_MIXED) tab = stage_table(c, gtab, lds_tab);
+    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const int64_t i = pi[p], j = pj[p];
@@ -308,7 +274,7 @@ __global__ __launch_bounds__(256) void k_pair_direct(
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = nullptr;
-    if (PREC == QEM_PREC_MIXED) tab = stage_table(c, gtab, lds_tab);
+    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += stride) {
         PairForce f = pair_force<PREC>(c, glx, glw, glxf, glwf, tab, qi[p], qj[p], ri[3 * p],

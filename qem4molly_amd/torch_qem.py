@@ -1,0 +1,89 @@
+"""PyTorch-ROCm binding: QEM energy and forces as one differentiable op.
+
+    from qem4molly_amd.torch_qem import qem_energy_forces
+    E, F = qem_energy_forces(ctx, xyz, q, k_idx, gamma=gamma)   # cuda float64
+
+forward  runs qem_forces_dev and qem_energy_dev on torch's current stream
+         (device pointers of the tensors; nothing is copied to the host except
+         the k indices and the scalar results).
+backward (no reference counterpart; the reference has forces only):
+  d/dxyz : -gE F  +  J_F^T gF, with J_F^T gF taken as -H gF from the
+           Hessian-vector product kernel (qem_hvp_dev): the force Jacobian is
+           symmetric for the conservative force (QEM_SEM_CORRECTED, up to the
+           motion of the quadrature limits, an effect at the accuracy level);
+  d/dq   : gE dE/dq (the per-atom potential); a force cotangent gF with
+           respect to q is not provided (raises);
+  d/dgamma: gE (dE/dgamma_1, dE/dgamma_2).  `gamma` must equal the
+           context's contrasts; it is only the handle autograd differentiates.
+The quadrature is recomputed in the backward pass: nothing per pair is kept.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+if L._lib is not None and not L._loaded_with_torch:
+    raise ImportError("libqem_hip.so was loaded before torch; import torch (or "
+                      "qem4molly_amd.torch_qem) first so that both use one HIP runtime")
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+class QEMEnergyForces(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, xyz, q, gamma, qctx, k_idx):
+        if xyz.dtype != torch.float64 or q.dtype != torch.float64 or not xyz.is_cuda:
+            raise TypeError("xyz and q must be float64 tensors on a ROCm device")
+        n = xyz.shape[0]
+        x = xyz.detach().contiguous()
+        qq = q.detach().contiguous()
+        F = torch.empty_like(x)
+        phi = torch.empty_like(qq)
+        s = _stream()
+        qctx.forces_dev(n, x.data_ptr(), qq.data_ptr(), k_idx, F.data_ptr(), s)
+        E, dg = qctx.energy_dev(n, x.data_ptr(), qq.data_ptr(), k_idx, phi.data_ptr(), s)
+        fctx.save_for_backward(x, qq, F, phi)
+        fctx.qctx = qctx
+        fctx.k_idx = np.array(k_idx, dtype=np.int64)
+        fctx.dg = dg
+        return x.new_tensor(E), F
+
+    @staticmethod
+    def backward(fctx, gE, gF):
+        x, qq, F, phi = fctx.saved_tensors
+        n = x.shape[0]
+        gx = gq = gg = None
+        if gE is None:
+            gE = x.new_zeros(())
+        if fctx.needs_input_grad[0]:
+            gx = -gE * F
+            if gF is not None and bool(torch.any(gF != 0)):
+                hv = torch.empty_like(x)
+                gFc = gF.contiguous()
+                fctx.qctx.hvp_dev(n, x.data_ptr(), qq.data_ptr(), fctx.k_idx, gFc.data_ptr(),
+                                  hv.data_ptr(), _stream())
+                gx = gx - hv
+        if fctx.needs_input_grad[1]:
+            if gF is not None and bool(torch.any(gF != 0)):
+                raise NotImplementedError(
+                    "gradient of the forces with respect to the charges is not provided")
+            gq = gE * phi
+        if fctx.needs_input_grad[2]:
+            gg = gE * x.new_tensor(fctx.dg)
+        return gx, gq, gg, None, None
+
+
+def qem_energy_forces(qctx: L.Context, xyz, q, k_idx, gamma=None):
+    """(E, F) for one electrostatic evaluation; differentiable (see module)."""
+    p = qctx.params
+    if gamma is None:
+        gamma = xyz.new_tensor([p.gamma_1, p.gamma_2])
+    else:
+        g = gamma.detach().cpu().numpy()
+        if not np.allclose(g, [p.gamma_1, p.gamma_2], rtol=0, atol=0):
+            raise ValueError("gamma must equal the context's gamma_1, gamma_2")
+    return QEMEnergyForces.apply(xyz, q, gamma, qctx, k_idx)

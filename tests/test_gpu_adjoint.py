@@ -1,0 +1,178 @@
+"""GPU adjoint: energy, dE/dq, dE/dgamma and Hessian-vector products.
+
+References: the FP64 oracle's energy (qo_energy: pair/self J0 quadrature and
+an O(N^2) k-space sum, independent of the device's structure factors) and
+central finite differences (h = 1e-5) of the forces and the energy.
+"""
+import numpy as np
+import pytest
+
+from helpers import CONFIGS, L, O, ctx_for, make_system, max_rel, oracle_params, rel_rms, \
+    scaled_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(name, n, seed=0, sem=0, precision="fp64"):
+    cfg = scaled_config(CONFIGS[name], n) if n != CONFIGS[name].n else CONFIGS[name]
+    xyz, q = make_system(cfg, seed=seed)
+    c = ctx_for(cfg, semantics=sem, precision=precision, iter_period=100000)
+    return cfg, xyz, q, c
+
+
+@pytest.mark.parametrize("name,n", [("slab_n100", 100), ("slab_n10k", 400)])
+@pytest.mark.parametrize("sem", [0, L.SEM_CORRECTED])
+def test_energy_matches_oracle(name, n, sem):
+    cfg, xyz, q, c = _setup(name, n, sem=sem)
+    with c:
+        K, P, sK = c.kset()
+        kidx = c.sample_k(3, cfg.P)
+        E, phi, dg = c.energy(xyz, q, kidx)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+    Eo, phio, dgo = O.energy(oracle_params(cfg, semantics=sem), xyz, q, pairs, K[kidx],
+                             weight=sK / cfg.P, nthreads=16)
+    assert E == pytest.approx(Eo, rel=1e-12)
+    assert rel_rms(phi, phio) < 1e-12
+    assert np.allclose(dg, dgo, rtol=1e-11, atol=1e-14 * abs(Eo))
+
+
+def test_energy_mixed_precision():
+    cfg, xyz, q, c = _setup("slab_n10k", 400, precision="mixed")
+    with c:
+        K, P, sK = c.kset()
+        kidx = c.sample_k(4, cfg.P)
+        E, phi, dg = c.energy(xyz, q, kidx)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+    Eo, phio, dgo = O.energy(oracle_params(cfg), xyz, q, pairs, K[kidx], weight=sK / cfg.P,
+                             nthreads=16)
+    assert E == pytest.approx(Eo, rel=1e-6)
+    assert rel_rms(phi, phio) < 1e-6
+    assert np.allclose(dg, dgo, rtol=1e-5)
+
+
+@pytest.mark.parametrize("sem", [0, L.SEM_CORRECTED])
+def test_hvp_matches_fd_of_oracle_forces(sem):
+    cfg, xyz, q, c = _setup("slab_n100", 100, sem=sem)
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=xyz.shape)
+    with c:
+        K, P, sK = c.kset()
+        kidx = c.sample_k(5, cfg.P)
+        hv = c.hvp(xyz, q, kidx, v)
+    p = oracle_params(cfg, semantics=sem)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+
+    def F(X):
+        return (O.short_forces(p, X, q, pairs, nthreads=16)
+                + O.long_forces(p, X, q, K[kidx], weight=sK / cfg.P, nthreads=16))
+    h = 1e-5
+    fd = -(F(xyz + h * v) - F(xyz - h * v)) / (2 * h)
+    assert rel_rms(hv, fd) < 1e-7
+
+
+def test_hvp_short_and_long_parts_separately():
+    # P = 0 isolates the short range; a huge cutoff-free check of the long
+    # range comes from the difference
+    cfg, xyz, q, c = _setup("slab_n100", 100, sem=L.SEM_CORRECTED)
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=xyz.shape)
+    p = oracle_params(cfg, semantics=L.SEM_CORRECTED)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+    h = 1e-5
+    with c:
+        hv_short = c.hvp(xyz, q, np.zeros(0, np.int64), v)
+        K, P, sK = c.kset()
+        kidx = c.sample_k(6, cfg.P)
+        hv_all = c.hvp(xyz, q, kidx, v)
+    fd_s = -(O.short_forces(p, xyz + h * v, q, pairs) - O.short_forces(p, xyz - h * v, q, pairs)) / (2 * h)
+    assert rel_rms(hv_short, fd_s) < 1e-7
+    fl = lambda X: O.long_forces(p, X, q, K[kidx], weight=sK / cfg.P)
+    fd_l = -(fl(xyz + h * v) - fl(xyz - h * v)) / (2 * h)
+    assert rel_rms(hv_all - hv_short, fd_l) < 1e-7
+
+
+def test_hvp_mixed_close_to_fp64():
+    cfg, xyz, q, c = _setup("slab_n10k", 1500, precision="mixed")
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=xyz.shape)
+    with c:
+        kidx = c.sample_k(7, cfg.P)
+        hv_m = c.hvp(xyz, q, kidx, v)
+    with ctx_for(cfg, iter_period=100000) as c64:
+        hv_d = c64.hvp(xyz, q, kidx, v)
+    assert rel_rms(hv_m, hv_d) < 1e-6
+
+
+def test_adjoint_slab_n10k_fd_and_symmetry():
+    # the N = 10,000 adjoint configuration: HVP for a seeded gaussian v
+    # (seed 1) against central differences (h = 1e-5) of the FP64 device
+    # forces (themselves pinned to the oracle), dE/dq and dE/dgamma against
+    # differences of the FP64 energy; corrected semantics (conservative force)
+    cfg = CONFIGS["slab_n10k"]
+    xyz, q = make_system(cfg)
+    v = np.random.default_rng(1).normal(size=xyz.shape)
+    h = 1e-5
+    with ctx_for(cfg, semantics=L.SEM_CORRECTED, iter_period=100000) as c:
+        # The inputs are unfiltered uniform positions: a few pairs nearly touch
+        # (F ~ 1/rho^2) or nearly share a z (the sign of z_i - z_j flips), and
+        # some pairs cross r_cutoff between x - h v and x + h v (the force is
+        # cut off there: a jump, not a derivative).  A central difference is
+        # not accurate across those, so their atoms are held fixed in v; their
+        # forces still respond to every other atom's motion.
+        # The long-range z forces jump where two atoms swap z order (the
+        # k = 0 term and the sorted direct term carry sgn(z_i - z_j)), so only
+        # atoms at least 1e-3 from their neighbours in z order move in z.
+        order = np.argsort(xyz[:, 2], kind="stable")
+        gaps = np.diff(xyz[order, 2])
+        lo = np.concatenate([[np.inf], gaps]); hi = np.concatenate([gaps, [np.inf]])
+        zfree = np.zeros(cfg.n, bool)
+        zfree[order] = (lo > 1e-3) & (hi > 1e-3)
+        v[~zfree, 2] = 0.0
+        assert zfree.sum() > 0.05 * cfg.n
+        pairs = c.neighbor_list(xyz)
+        d = xyz[pairs[:, 0]] - xyz[pairs[:, 1]]
+        d[:, :2] -= np.array([cfg.Lx, cfg.Ly]) * np.rint(d[:, :2] / [cfg.Lx, cfg.Ly])
+        bad = (np.linalg.norm(d, axis=1) < 0.2) | (np.abs(d[:, 2]) < 1e-4)
+        held = set(pairs[bad].ravel().tolist())
+        base = set(map(tuple, pairs.tolist()))
+        for _ in range(8):
+            v[sorted(held)] = 0.0
+            plus = set(map(tuple, c.neighbor_list(xyz + h * v).tolist()))
+            minus = set(map(tuple, c.neighbor_list(xyz - h * v).tolist()))
+            crossing = (plus ^ base) | (minus ^ base)
+            if not crossing:
+                break
+            held |= {a for pr in crossing for a in pr}
+        assert not crossing and len(held) < 0.02 * cfg.n
+        kidx = c.sample_k(11, cfg.P)
+        F = c.forces(xyz, q, kidx)          # builds the list at the unperturbed positions
+        hv = c.hvp(xyz, q, kidx, v)
+        fp = c.forces(xyz + h * v, q, kidx)
+        fm = c.forces(xyz - h * v, q, kidx)
+        E, phi, dg = c.energy(xyz, q, kidx)
+        # energy along v: dE/dh = -F.v
+        Ep, _, _ = c.energy(xyz + h * v, q, kidx)
+        Em, _, _ = c.energy(xyz - h * v, q, kidx)
+        u = np.random.default_rng(2).normal(size=xyz.shape)
+        hu = c.hvp(xyz, q, kidx, u)
+        # dE/dq along a random charge direction (E is quadratic in q)
+        dq = np.random.default_rng(3).normal(size=q.shape)
+        Eqp, _, _ = c.energy(xyz, q + 1e-3 * dq, kidx)
+        Eqm, _, _ = c.energy(xyz, q - 1e-3 * dq, kidx)
+    fd = -(fp - fm) / (2 * h)
+    assert rel_rms(hv, fd) < 1e-6
+    assert (Ep - Em) / (2 * h) == pytest.approx(-np.sum(F * v), rel=1e-6)
+    assert (Eqp - Eqm) / 2e-3 == pytest.approx(np.dot(phi, dq), rel=1e-9)
+    assert E == pytest.approx(0.5 * np.dot(q, phi), rel=1e-12)
+    # Hessian symmetry of the conservative force (node motion of the
+    # z-dependent limits breaks it only at the accuracy level)
+    assert np.dot(u.ravel(), hv.ravel()) == pytest.approx(np.dot(v.ravel(), hu.ravel()), rel=1e-5)
+    # dE/dgamma by differences
+    for w in range(2):
+        cfgp = cfg.__class__(**{**cfg.__dict__, "gamma_1": cfg.gamma_1 + (1e-5 if w == 0 else 0),
+                                "gamma_2": cfg.gamma_2 + (1e-5 if w == 1 else 0)})
+        cfgm = cfg.__class__(**{**cfg.__dict__, "gamma_1": cfg.gamma_1 - (1e-5 if w == 0 else 0),
+                                "gamma_2": cfg.gamma_2 - (1e-5 if w == 1 else 0)})
+        with ctx_for(cfgp, semantics=L.SEM_CORRECTED) as cp, ctx_for(cfgm, semantics=L.SEM_CORRECTED) as cm:
+            d = (cp.energy(xyz, q, kidx)[0] - cm.energy(xyz, q, kidx)[0]) / 2e-5
+        assert dg[w] == pytest.approx(d, rel=1e-6, abs=1e-9 * abs(E))

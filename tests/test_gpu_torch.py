@@ -1,0 +1,71 @@
+"""GPU: the torch.autograd binding over the device-pointer entry points.
+
+Runs in a child process that imports torch before the library, so torch and
+libqem_hip.so share torch's HIP runtime (the order the binding requires);
+the rest of the suite never imports torch.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+CHILD = r"""
+import json, sys
+import numpy as np
+import torch                                   # first: one HIP runtime
+sys.path[:0] = [ROOT, ROOT + "/tests", ROOT + "/oracle"]
+from qem4molly_amd.torch_qem import qem_energy_forces
+from helpers import CONFIGS, L, ctx_for, make_system, rel_rms, scaled_config
+out = {}
+cfg = scaled_config(CONFIGS["slab_n10k"], 800)
+xyz_h, q_h = make_system(cfg, seed=2)
+dev = torch.device("cuda:0")
+with ctx_for(cfg, semantics=L.SEM_CORRECTED, iter_period=100000) as c:
+    kidx = c.sample_k(3, cfg.P)
+    F_ref = c.forces(xyz_h, q_h, kidx)
+    E_ref, phi_ref, dg_ref = c.energy(xyz_h, q_h, kidx)
+    v = np.random.default_rng(0).normal(size=xyz_h.shape)
+    hv_ref = c.hvp(xyz_h, q_h, kidx, v)
+    xyz = torch.tensor(xyz_h, device=dev, requires_grad=True)
+    q = torch.tensor(q_h, device=dev, requires_grad=True)
+    gamma = torch.tensor([cfg.gamma_1, cfg.gamma_2], dtype=torch.float64, device=dev,
+                         requires_grad=True)
+    E, F = qem_energy_forces(c, xyz, q, kidx, gamma=gamma)
+    out["E"] = abs(float(E) - E_ref) / abs(E_ref)
+    out["F"] = rel_rms(F.detach().cpu().numpy(), F_ref)
+    E.backward()
+    out["gx"] = rel_rms(xyz.grad.cpu().numpy(), -F_ref)
+    out["gq"] = rel_rms(q.grad.cpu().numpy(), phi_ref)
+    out["gg"] = float(np.max(np.abs(gamma.grad.cpu().numpy() - dg_ref) / np.abs(dg_ref)))
+    xyz2 = torch.tensor(xyz_h, device=dev, requires_grad=True)
+    E2, F2 = qem_energy_forces(c, xyz2, torch.tensor(q_h, device=dev), kidx)
+    (F2 * torch.tensor(v, device=dev)).sum().backward()
+    out["hv"] = rel_rms(xyz2.grad.cpu().numpy(), -hv_ref)
+    # torch-allocated device memory through the plain device entry point
+    x = torch.tensor(xyz_h, device=dev); qq = torch.tensor(q_h, device=dev)
+    f = torch.empty_like(x)
+    c.forces_dev(cfg.n, x.data_ptr(), qq.data_ptr(), kidx, f.data_ptr(),
+                 torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    out["dev"] = rel_rms(f.cpu().numpy(), c.forces(xyz_h, q_h, kidx))
+print("RESULT " + json.dumps(out), flush=True)
+"""
+
+
+def test_torch_autograd_binding():
+    r = subprocess.run([sys.executable, "-u", "-c", CHILD.replace("ROOT", repr(ROOT))],
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout[-2000:], r.stderr[-4000:])
+    assert r.returncode == 0
+    line = [l for l in r.stdout.splitlines() if l.startswith("RESULT ")][-1]
+    res = json.loads(line[len("RESULT "):])
+    assert res["E"] < 1e-13 and res["F"] < 1e-13
+    assert res["gx"] < 1e-13 and res["gq"] < 1e-13 and res["gg"] < 1e-12
+    assert res["hv"] < 1e-13
+    assert res["dev"] < 1e-13
