@@ -131,6 +131,98 @@ def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, targe
     }
 
 
+def _params(cfg, precision, semantics=0, iter_period=100):
+    from qem4molly_amd import _lib as L
+    p = L.default_params()
+    p.Lx, p.Ly, p.Lz = cfg.L
+    p.r_cutoff, p.alpha = cfg.r_cutoff, cfg.alpha
+    p.gamma_1, p.gamma_2 = cfg.gamma_1, cfg.gamma_2
+    p.eps_0, p.accuracy = cfg.eps_0, cfg.accuracy
+    p.N_t, p.iter_period, p.rbe_p = cfg.N_t, iter_period, cfg.P
+    p.precision = L.PREC_MIXED if precision == "mixed" else L.PREC_FP64
+    p.semantics = semantics
+    return p
+
+
+def adjoint_cost(name, precision, reps=10):
+    """Forward (forces) vs backward (force HVP, quadrature recomputed) on the
+    same inputs and k batch; energy + dE/dq + dE/dgamma timed alongside."""
+    from qem4molly_amd import _lib as L
+    from qem4molly_amd.workloads import CONFIGS, make_system
+    cfg = CONFIGS[name]
+    xyz, q = make_system(cfg)
+    v = np.random.default_rng(1).normal(size=xyz.shape)
+    xd, qd, vd = L.DeviceArray(xyz), L.DeviceArray(q), L.DeviceArray(v)
+    fd, hd, pd = L.DeviceArray(np.zeros_like(xyz)), L.DeviceArray(np.zeros_like(xyz)), \
+        L.DeviceArray(np.zeros(cfg.n))
+    ctx = L.Context(_params(cfg, precision, semantics=7, iter_period=1000000))
+    kidx = ctx.sample_k(1, cfg.P)
+
+    def timed(fn):
+        fn()
+        L.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        L.synchronize()
+        return (time.perf_counter() - t) / reps * 1e3
+
+    t_f = timed(lambda: ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr))
+    t_h = timed(lambda: ctx.hvp_dev(cfg.n, xd.ptr, qd.ptr, kidx, vd.ptr, hd.ptr))
+    t_e = timed(lambda: ctx.energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, pd.ptr))
+    mem = ctx.device_bytes()
+    ctx.close()
+    return {"workload": name, "precision": precision, "forward_ms": t_f, "hvp_ms": t_h,
+            "energy_dE_dq_dE_dgamma_ms": t_e, "backward_over_forward": t_h / t_f,
+            "forward_plus_backward_over_forward": (t_f + t_h) / t_f,
+            "device_bytes": mem}
+
+
+def adjoint_md(name="slab_n100k", steps=100, sigma=0.01):
+    """100 steps of positions perturbed by N(0, sigma) per component
+    (x, y wrapped, z reflected into [0.5, Lz - 0.5]), fixed seeded k batch,
+    mixed precision; each step: forward (forces + energy) and backward (force
+    HVP along a fresh gaussian cotangent)."""
+    from qem4molly_amd import _lib as L
+    from qem4molly_amd.workloads import CONFIGS, make_system
+    cfg = CONFIGS[name]
+    xyz, q = make_system(cfg)
+    rng = np.random.default_rng(5)
+    xd, qd = L.DeviceArray(xyz), L.DeviceArray(q)
+    fd, hd, vd = (L.DeviceArray(np.zeros_like(xyz)) for _ in range(3))
+    pd = L.DeviceArray(np.zeros(cfg.n))
+    ctx = L.Context(_params(cfg, "mixed"))
+    ctx.timing_enable(True)
+    kidx = ctx.sample_k(0, cfg.P)
+    lo, hi = 0.5, cfg.Lz - 0.5
+    t_dev = 0.0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        xyz = xyz + rng.normal(scale=sigma, size=xyz.shape)
+        xyz[:, 0] %= cfg.Lx
+        xyz[:, 1] %= cfg.Ly
+        z = xyz[:, 2]
+        z = np.where(z < lo, 2 * lo - z, z)
+        xyz[:, 2] = np.where(z > hi, 2 * hi - z, z)
+        xd.upload(xyz)
+        vd.upload(rng.normal(size=xyz.shape))
+        ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr)
+        ctx.energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, pd.ptr)
+        ctx.hvp_dev(cfg.n, xd.ptr, qd.ptr, kidx, vd.ptr, hd.ptr)
+    L.synchronize()
+    wall = time.perf_counter() - t0
+    ph = ctx.timing_read()
+    f = fd.download()
+    mem = ctx.device_bytes()
+    npairs, builds, _ = ctx.short_state()
+    ctx.close()
+    return {"workload": name, "steps": steps, "sigma": sigma, "precision": "mixed",
+            "ms_per_step_wall": wall / steps * 1e3,
+            "gpu_ms_per_step": {k: v[0] / steps for k, v in ph.items()},
+            "neighbor_builds": builds, "n_pairs_last": npairs,
+            "finite": bool(np.isfinite(f).all()), "peak_device_bytes": mem}
+
+
 def traffic_from_profiles(workload):
     path = os.path.join(ROOT, "profiles", f"pair_kernel_traffic_{workload}.json")
     if not os.path.exists(path):
@@ -154,7 +246,11 @@ def main(argv=None):
                     help="steps for the FP64 plain comparison (default: --steps)")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy this many MiB three times with the library's "
+                         "plain copy kernel (a known-size reference for HBM byte counters)")
     a = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -244,6 +340,14 @@ def main(argv=None):
             "rel_rms_err_vs_fp64": e2,
         }
 
+    if not a.no_adjoint:
+        log("[bench] adjoint cost (slab_n10k fp64, mixed; slab_n100k mixed) and 100-step run")
+        out["adjoint"] = {
+            "cost": [adjoint_cost("slab_n10k", "fp64"), adjoint_cost("slab_n10k", "mixed"),
+                     adjoint_cost("slab_n100k", "mixed", reps=5)],
+            "md_slab_n100k": adjoint_md(),
+        }
+
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
             min(16, os.cpu_count() or 1)
@@ -258,6 +362,16 @@ def main(argv=None):
             kidx = c.sample_k(1000, cfg.P)
         log(f"[bench] cpu baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(cfg, xyz, q, pairs, K[kidx], sK / cfg.P, threads)
+
+    if a.copy_calibration_mib > 0:
+        nb = a.copy_calibration_mib << 20
+        src = L.DeviceArray(np.zeros(nb, np.uint8))
+        dst = L.DeviceArray(np.zeros(nb, np.uint8))
+        for _ in range(3):
+            L.dev_copy(dst, src, nb)
+        L.synchronize()
+        out["copy_calibration_bytes"] = nb
+        log(f"[bench] copy calibration: 3 x {a.copy_calibration_mib} MiB")
 
     print(json.dumps(out), flush=True)
     return 0

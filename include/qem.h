@@ -202,6 +202,9 @@ int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                 void *stream);
 
 /* ---- introspection ------------------------------------------------------ */
+/* device bytes held by the context's workspaces (they only grow: the
+ * current value is the peak) */
+int qem_device_bytes(qem_ctx *ctx, int64_t *bytes);
 /* pairs in the current neighbour list, its build count, and the step count */
 int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds,
                     int64_t *n_steps);

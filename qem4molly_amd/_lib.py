@@ -49,6 +49,7 @@ EXPORTS = (
     "qem_dev_malloc", "qem_dev_free", "qem_memcpy_htod", "qem_memcpy_dtoh",
     "qem_device_synchronize", "qem_dev_copy_kernel", "qem_timing_enable",
     "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
+    "qem_device_bytes",
 )
 
 PHASES = ("neighbor", "pair", "gather", "long")
@@ -116,6 +117,7 @@ _SIGS = {
     "qem_energy_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _pd, _pd, _vp]),
     "qem_hvp": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd]),
     "qem_hvp_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _vp]),
+    "qem_device_bytes": (C.c_int, [_vp, _pi64]),
 }
 
 _lib = None
@@ -272,6 +274,11 @@ class Context:
         self._check(lib().qem_short_self(self._h, xyz.shape[0], ptr(q), ptr(xyz),
                                          int(bool(single)), ptr(f)), "qem_short_self")
         return f
+
+    def device_bytes(self) -> int:
+        b = _i64()
+        self._check(lib().qem_device_bytes(self._h, C.byref(b)), "qem_device_bytes")
+        return b.value
 
     def short_state(self):
         a, b, c = _i64(), _i64(), _i64()
@@ -430,6 +437,15 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def dev_copy(dst: "DeviceArray", src: "DeviceArray", nbytes: int):
+    """Plain load/store copy kernel on the default stream (traffic calibration)."""
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("dev_copy: nbytes exceeds a buffer")
+    st = lib().qem_dev_copy_kernel(dst.ptr, src.ptr, nbytes, None)
+    if st != QEM_OK:
+        raise QEMError(st, "qem_dev_copy_kernel")
 
 
 def synchronize():

@@ -77,7 +77,7 @@ __device__ __forceinline__ void nodes_energy(double K, double a, double rho, int
             float J = 1.f;
             if (!SELF) {
                 float J1;
-                bessel01_tab(k * rhof, tab, c.jt_n, c.jt_dx, c.jt_inv_dx, J, J1);
+                bessel01_tab(k * rhof, tab, c.jt_n, c.jt_inv_dx, J, J1);
             }
             const float wt = w * num * rD * J;
             av += wt;
@@ -254,7 +254,7 @@ __device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, do
             float J0 = 1.f, J1 = 0.f, dJ0 = 0.f, dJ1 = 0.f;
             if (!self) {
                 const float x = k * rhof, dx = fmaf(dk, rhof, k * drhof);
-                bessel01_tab(x, tab, c.jt_n, c.jt_dx, c.jt_inv_dx, J0, J1);
+                bessel01_tab(x, tab, c.jt_n, c.jt_inv_dx, J0, J1);
                 const float j1x = x > 1e-4f ? J1 * __builtin_amdgcn_rcpf(x) : 0.5f;
                 dJ0 = -J1 * dx;
                 dJ1 = (J0 - j1x) * dx;
@@ -534,7 +534,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     double4 *pot = reinterpret_cast<double4 *>(d_pot);
-    const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+    const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     if (precision == QEM_PREC_MIXED) {
         if (np > 0)
             k_pair_energy<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
@@ -570,7 +570,7 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     double4 *tp = reinterpret_cast<double4 *>(d_tp);
-    const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+    const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     if (precision == QEM_PREC_MIXED) {
         if (np > 0)
             k_pair_tangent<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(

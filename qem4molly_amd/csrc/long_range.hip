@@ -52,15 +52,18 @@ __global__ void k_extract_z(const double *__restrict__ xyz, int64_t n, double *_
 __global__ void k_gather_sorted(const double *__restrict__ xyz, const double *__restrict__ q,
                                 const int32_t *__restrict__ perm, int64_t n,
                                 double *__restrict__ sx, double *__restrict__ sy,
-                                double *__restrict__ sz, double *__restrict__ sq)
+                                double *__restrict__ sz, double *__restrict__ sq,
+                                double4 *__restrict__ satom)
 {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int64_t a = perm[r];
-    sx[r] = xyz[3 * a];
-    sy[r] = xyz[3 * a + 1];
-    sz[r] = xyz[3 * a + 2];
-    sq[r] = q[a];
+    const double x = xyz[3 * a], y = xyz[3 * a + 1], z = xyz[3 * a + 2], qa = q[a];
+    sx[r] = x;
+    sy[r] = y;
+    sz[r] = z;
+    sq[r] = qa;
+    satom[r] = make_double4(x, y, z, qa);
 }
 
 // Per-atom terms of one k: cos, sin of k.rho and e^{+kz}, e^{-kz}.
@@ -204,122 +207,298 @@ __global__ __launch_bounds__(CH) void k_long_S(const double4 *__restrict__ partA
     }
 }
 
-// Pass B: forces for one chunk and one tile of k rows.
-template <int PREC>
-__global__ __launch_bounds__(CH) void k_long_B(LongConsts lc, const double4 *__restrict__ kv,
-                                               int64_t Ptot, int kt,
-                                               const double *__restrict__ sx,
-                                               const double *__restrict__ sy,
-                                               const double *__restrict__ sz,
-                                               const double *__restrict__ sq, int64_t n,
-                                               int nchunk, const double4 *__restrict__ offA,
-                                               const double4 *__restrict__ totA,
-                                               double *__restrict__ partB)
+// ---------------------------------------------------------------------------
+// Force passes, k-lane layout.  A wave owns one chunk of chk consecutive
+// z-ordered atoms and 64 k rows, one row per lane.  It walks the chunk's
+// atoms in ascending z with the atom read once per wave (uniform loads);
+// every lane keeps its row's constants, its running prefix sums and its
+// share of the force in registers.  The per-atom force is summed over the
+// wave's 64 rows with a fixed DPP pattern; the row groups are summed in
+// pass R.  Sums over atoms run in the reference's sequential order.
+
+// derived constants of one k row (k_make_krec)
+struct KRec {
+    double kx, ky, k, k0f;    // k vector; k0f = 1 on the k = 0 row
+    double kxt, kyt, kl, w;   // k_x / 2pi, k_y / 2pi, |k| log2 e; z weight
+    double wx, wy;            // w k_x / |k|, w k_y / |k| (xy weights)
+    double N1, N2, N3;        // non-sort: g g2 e2, g g12 e2, g g1 (g = 1 / 2(g12 e2 - 1))
+    double S1, S2;            // self: g g2 e2, g g1
+    double hs;                // sorted-sum factor: 1/2, or 1 on the k = 0 row
+};
+static_assert(sizeof(KRec) == 16 * sizeof(double), "KRec layout");
+
+__global__ void k_make_krec(LongConsts lc, const double4 *__restrict__ kv, int64_t Ptot,
+                            KRec *__restrict__ kr)
 {
-    __shared__ double4 wsum[2][CH / 64];
-    const int m = blockIdx.x;
-    const int tile = blockIdx.y;
-    const int64_t p0 = (int64_t)tile * kt;
-    const int64_t p1 = min(Ptot, p0 + kt);
-    const int64_t r = (int64_t)m * CH + threadIdx.x;
-    const bool live = r < n;
-    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0;
-    const double z = live ? sz[r] : 0.0, q = live ? sq[r] : 0.0;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int64_t p = p0; p < p1; ++p) {
-        const double4 kk = kv[p];
-        const bool k0 = kk.z == 0.0;
-        double c = 1.0, s = 0.0, Ep = 1.0, Em = 1.0;
-        double4 v;
-        if (k0) {
-            v = make_double4(q, q * z, q, q * z);
-        } else {
-            atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
-            v = live ? make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em)
-                     : make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-        // in-wave inclusive prefix (x, y) and inclusive suffix (z, w)
-        double4 inc = v;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const double ux = __shfl_up(inc.x, d), uy = __shfl_up(inc.y, d);
-            const double dz = __shfl_down(inc.z, d), dw = __shfl_down(inc.w, d);
-            if (lane >= d) {
-                inc.x += ux;
-                inc.y += uy;
-            }
-            if (lane + d < 64) {
-                inc.z += dz;
-                inc.w += dw;
-            }
-        }
-        const int b = (int)(p & 1);
-        if (lane == 63) {
-            wsum[b][wv].x = inc.x;
-            wsum[b][wv].y = inc.y;
-        }
-        if (lane == 0) {
-            wsum[b][wv].z = inc.z;
-            wsum[b][wv].w = inc.w;
-        }
-        // exclusive by shifting one lane
-        double ex = __shfl_up(inc.x, 1), ey = __shfl_up(inc.y, 1);
-        double ez = __shfl_down(inc.z, 1), ew = __shfl_down(inc.w, 1);
-        if (lane == 0) ex = ey = 0.0;
-        if (lane == 63) ez = ew = 0.0;
-        __syncthreads();
-        for (int w = 0; w < wv; ++w) {
-            ex += wsum[b][w].x;
-            ey += wsum[b][w].y;
-        }
-        for (int w = CH / 64 - 1; w > wv; --w) {
-            ez += wsum[b][w].z;
-            ew += wsum[b][w].w;
-        }
-        const double4 off = offA[p * nchunk + m];
-        const double A = off.x + ex, B = off.y + ey, C = off.z + ez, D = off.w + ew;
-        if (k0) {
-            // Flz_sort_k0 (src/Force_long.jl:130-161): q (Q1 - Q2) / (2 Lx Ly eps0)
-            if (lc.parts & QEM_LONG_K0) fz += kk.w * (q * (A - C));
-            continue;
-        }
-        const double k = kk.z;
-        double gx = 0.0, gz = 0.0;  // xy magnitude (times k_x/k, k_y/k) and z
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Ptot) return;
+    const double4 kk = kv[p];
+    KRec r;
+    r.kx = kk.x;
+    r.ky = kk.y;
+    r.k = kk.z;
+    r.kxt = kk.x * kInv2Pi;
+    r.kyt = kk.y * kInv2Pi;
+    r.kl = kk.z * kLog2eD;
+    r.w = kk.w;
+    r.wx = r.wy = r.N1 = r.N2 = r.N3 = r.S1 = r.S2 = 0.0;
+    if (kk.z == 0.0) {
+        // Flz_sort_k0: w q (Q_below - Q_above), carried by the sorted sums
+        r.k0f = 1.0;
+        r.hs = (lc.parts & QEM_LONG_K0) ? 1.0 : 0.0;
+    } else {
+        r.k0f = 0.0;
+        const double e2 = exp(-2.0 * kk.z * lc.Lz);
+        const double g = 0.5 / (lc.g12 * e2 - 1.0);
+        r.wx = kk.w * (kk.x / kk.z);
+        r.wy = kk.w * (kk.y / kk.z);
         if (lc.parts & QEM_LONG_NONSORT) {
-            // Fl_non_sort (src/Force_long.jl:5-70), own term removed
-            const double4 T = totA[p];
-            const double f1 = T.x - v.x, f2 = T.y - v.y, f3 = T.z - v.z, f4 = T.w - v.w;
-            const double e2 = exp(-2.0 * k * lc.Lz);
-            const double g = 0.5 / (lc.g12 * e2 - 1.0);
-            const double u = lc.g2 * e2 * Ep + lc.g12 * e2 * Em;
-            const double w_ = lc.g1 * Em + lc.g12 * e2 * Ep;
-            const double uz = lc.g2 * e2 * Ep - lc.g12 * e2 * Em;
-            const double wz = -lc.g1 * Em + lc.g12 * e2 * Ep;
-            gx += q * g * (c * (u * f2 + w_ * f4) - s * (u * f1 + w_ * f3));
-            gz += q * g * (c * (uz * f1 + wz * f3) + s * (uz * f2 + wz * f4));
-        }
-        if (lc.parts & QEM_LONG_SORT) {
-            // Fl_sort (src/Force_long.jl:72-127)
-            const double sxy = s * Em * A - c * Em * B + s * Ep * C - c * Ep * D;
-            gx += 0.5 * q * sxy;
-            gz += 0.5 * q * (c * Em * A + s * Em * B - c * Ep * C - s * Ep * D);
+            r.N1 = g * (lc.g2 * e2);
+            r.N2 = g * (lc.g12 * e2);
+            r.N3 = g * lc.g1;
         }
         if (lc.parts & QEM_LONG_SELF) {
-            // Flz_self (src/Force_long.jl:164-184)
-            const double e2 = exp(-2.0 * k * lc.Lz);
-            gz += q * q * (lc.g2 * (e2 * Ep * Ep) - lc.g1 * (Em * Em)) / (2.0 * (lc.g12 * e2 - 1.0));
+            r.S1 = g * (lc.g2 * e2);
+            r.S2 = g * lc.g1;
         }
-        const double wk = kk.w;
-        fx += wk * (kk.x / k) * gx;
-        fy += wk * (kk.y / k) * gx;
-        fz += wk * gz;
+        r.hs = (lc.parts & QEM_LONG_SORT) ? 0.5 : 0.0;
     }
-    if (live) {
-        double *o = partB + ((int64_t)tile * n + r) * 3;
-        o[0] = fx;
-        o[1] = fy;
-        o[2] = fz;
+    kr[p] = r;
+}
+
+// one lane's row constants
+struct LaneK {
+    double kx, ky, k, k0f, w, wx, wy, N1, N2, N3, S1, S2, hs;
+};
+
+__device__ __forceinline__ LaneK load_lane(const KRec &r, bool mixed)
+{
+    LaneK L;
+    L.kx = mixed ? r.kxt : r.kx;
+    L.ky = mixed ? r.kyt : r.ky;
+    L.k = mixed ? r.kl : r.k;
+    L.k0f = r.k0f;
+    L.w = r.w;
+    L.wx = r.wx;
+    L.wy = r.wy;
+    L.N1 = r.N1;
+    L.N2 = r.N2;
+    L.N3 = r.N3;
+    L.S1 = r.S1;
+    L.S2 = r.S2;
+    L.hs = r.hs;
+    return L;
+}
+
+// cos, sin of k.rho and e^{+kz}, e^{-kz} from the lane's row constants
+// (mixed: turns and log2 scaling precomputed; same arithmetic as atom_terms)
+template <int PREC>
+__device__ __forceinline__ void lane_terms(const LaneK &L, double x, double y, double z,
+                                           double &c, double &s, double &Ep, double &Em)
+{
+    if (PREC == QEM_PREC_FP64) {
+        sincos(L.kx * x + L.ky * y, &s, &c);
+        Ep = exp(L.k * z);
+        Em = exp(-L.k * z);
+    } else {
+        double r = fma(L.kx, x, L.ky * y);
+        r -= rint(r);
+        float sf, cf;
+        sincospif((float)(2.0 * r), &sf, &cf);
+        s = sf;
+        c = cf;
+        const double tz = L.k * z;
+        const double nf = floor(tz);
+        const float f = (float)(tz - nf);
+        const int ni = (int)nf;
+        Ep = ldexp((double)__builtin_amdgcn_exp2f(f), ni);
+        Em = ldexp((double)__builtin_amdgcn_exp2f(-f), -ni);
+    }
+}
+
+// lane permutation of a double (every lane of every row written)
+template <int CTRL>
+__device__ __forceinline__ double dpp_perm_f64(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// row broadcast of a double; rows outside ROW_MASK read 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_bcast_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave in a fixed order; lane 63 holds the sum.
+// (pairs, quads, half rows, rows of 16, then rows 0+1 into 1, 2+3 into 3,
+// and row 1 into rows 2, 3)
+__device__ __forceinline__ double wave_sum63(double v)
+{
+    v += dpp_perm_f64<0xB1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_perm_f64<0x4E>(v);        // quad_perm [2,3,0,1]
+    v += dpp_perm_f64<0x141>(v);       // row_half_mirror
+    v += dpp_perm_f64<0x140>(v);       // row_mirror
+    v += dpp_bcast_f64<0x142, 0xA>(v); // row_bcast:15 (rows 1, 3)
+    v += dpp_bcast_f64<0x143, 0xC>(v); // row_bcast:31 (rows 2, 3)
+    return v;
+}
+
+// a + b without contraction into a fused multiply-add, so that passes A and
+// B round their running sums identically
+__device__ __forceinline__ double add_nc(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+constexpr int kLaneWaves = 4;   // waves (= chunks) per block in the k-lane passes
+
+// Pass A (k lanes): per (k row, chunk) the four sums q cos e^{+kz},
+// q sin e^{+kz}, q cos e^{-kz}, q sin e^{-kz} (k = 0 row: q, q z, q, q z),
+// accumulated in ascending z.
+template <int PREC>
+__global__ __launch_bounds__(64 * kLaneWaves) void k_lane_A(const KRec *__restrict__ kr,
+                                                             int64_t Ptot,
+                                                             const double4 *__restrict__ satom,
+                                                             int64_t n, int chk, int nchunk,
+                                                             double4 *__restrict__ partA)
+{
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
+    if (chunk >= nchunk) return;
+    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
+    const bool live = p < Ptot;
+    const LaneK L = load_lane(kr[live ? p : Ptot - 1], PREC == QEM_PREC_MIXED);
+    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
+    for (int64_t a = a0; a < a1; ++a) {
+        const double4 at = satom[a];
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        const double qz = L.k0f * (at.w * at.z);
+        // the same rounded values and adds as pass B's running sums
+        sx = add_nc(sx, at.w * (c * Ep));
+        sy = add_nc(sy, fma(at.w, s * Ep, qz));
+        sz = add_nc(sz, at.w * (c * Em));
+        sw = add_nc(sw, fma(at.w, s * Em, qz));
+    }
+    if (live) partA[p * nchunk + chunk] = make_double4(sx, sy, sz, sw);
+}
+
+// Pass B (k lanes): forces of one chunk from one group of 64 k rows into
+// partB[group][atom][3].  Prefix sums (below) run with the walk; the suffix
+// sums (above) are the chunk offset plus the chunk total minus the running
+// inclusive sum when the chunk is flat in e^{-kz} (k_max dz <= 1 over the
+// chunk, so the subtraction loses at most a factor e in relative accuracy);
+// otherwise a second walk in descending z accumulates them from the top.
+template <int PREC>
+__global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restrict__ kr,
+                                                             int64_t Ptot,
+                                                             const double4 *__restrict__ satom,
+                                                             int64_t n, int chk, int nchunk,
+                                                             const double4 *__restrict__ partA,
+                                                             const double4 *__restrict__ offA,
+                                                             const double4 *__restrict__ totA,
+                                                             double *__restrict__ partB)
+{
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
+    if (chunk >= nchunk) return;
+    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
+    const bool live = p < Ptot;
+    const int64_t pl = live ? p : Ptot - 1;
+    const LaneK L = load_lane(kr[pl], PREC == QEM_PREC_MIXED);
+    const double kabs = live ? kr[pl].k : 0.0;
+    const double4 T = totA[pl];
+    const double4 off = offA[pl * nchunk + chunk];
+    const double4 own = partA[pl * nchunk + chunk];
+    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
+    double kmax = kabs;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, d));
+    const double dz = satom[a1 - 1].z - satom[a0].z;
+    const bool flat = kmax * dz <= 1.0;
+    double *out = partB + ((int64_t)blockIdx.y * n) * 3;
+
+    double A = off.x, B = off.y;   // exclusive prefix below the atom
+    double PZ = 0.0, PW = 0.0;     // inclusive in-chunk sums of the e^{-kz} pair
+    for (int64_t a = a0; a < a1; ++a) {
+        const double4 at = satom[a];
+        const double q = at.w;
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+        const double qz = L.k0f * (q * at.z);
+        const double vx = q * cp, vy = fma(q, sp, qz), vz = q * cm, vw = fma(q, sm, qz);
+        PZ = add_nc(PZ, vz);
+        PW = add_nc(PW, vw);
+        double C = 0.0, D = 0.0;
+        if (flat) {
+            C = off.z + (own.z - PZ);
+            D = off.w + (own.w - PW);
+        }
+        // Fl_non_sort (src/Force_long.jl:5-70), own term removed
+        const double f1 = T.x - vx, f2 = T.y - vy, f3 = T.z - vz, f4 = T.w - vw;
+        const double X1 = c * f2 - s * f1, X2 = c * f4 - s * f3;
+        const double Z1 = c * f1 + s * f2, Z2 = c * f3 + s * f4;
+        const double gu = L.N1 * Ep + L.N2 * Em, gw = L.N3 * Em + L.N2 * Ep;
+        const double guz = L.N1 * Ep - L.N2 * Em, gwz = L.N2 * Ep - L.N3 * Em;
+        const double gxn = gu * X1 + gw * X2, gzn = guz * Z1 + gwz * Z2;
+        // Fl_sort (src/Force_long.jl:72-127) and Flz_sort_k0 (130-161)
+        const double sxy = sm * A - cm * B + sp * C - cp * D;
+        const double szz = cm * A + sm * B - cp * C - sp * D;
+        // Flz_self (src/Force_long.jl:164-184)
+        const double self = L.S1 * (Ep * Ep) - L.S2 * (Em * Em);
+        const double gx = q * (gxn + L.hs * sxy);
+        const double gz = q * (gzn + L.hs * szz + q * self);
+        double fx = live ? L.wx * gx : 0.0;
+        double fy = live ? L.wy * gx : 0.0;
+        double fz = live ? L.w * gz : 0.0;
+        A += vx;
+        B += vy;
+        fx = wave_sum63(fx);
+        fy = wave_sum63(fy);
+        fz = wave_sum63(fz);
+        if (lane == 63) {
+            double *o = out + a * 3;
+            o[0] = fx;
+            o[1] = fy;
+            o[2] = fz;
+        }
+    }
+    if (flat) return;
+    // exact suffix walk: the sorted-sum terms with C, D (above the atom)
+    double C = off.z, D = off.w;
+    for (int64_t a = a1 - 1; a >= a0; --a) {
+        const double4 at = satom[a];
+        const double q = at.w;
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+        const double qz = L.k0f * (q * at.z);
+        const double gx = q * (L.hs * (sp * C - cp * D));
+        const double gz = q * (L.hs * (-cp * C - sp * D));
+        double fx = live ? L.wx * gx : 0.0;
+        double fy = live ? L.wy * gx : 0.0;
+        double fz = live ? L.w * gz : 0.0;
+        C += q * cm;
+        D += fma(q, sm, qz);
+        fx = wave_sum63(fx);
+        fy = wave_sum63(fy);
+        fz = wave_sum63(fz);
+        if (lane == 63) {
+            double *o = out + a * 3;
+            o[0] += fx;
+            o[1] += fy;
+            o[2] += fz;
+        }
     }
 }
 
@@ -739,6 +918,7 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const doub
     LR_CHECK(ensure(lw.sy, nd));
     LR_CHECK(ensure(lw.sz, nd));
     LR_CHECK(ensure(lw.sq, nd));
+    LR_CHECK(ensure(lw.satom, 4 * nd));
     k_extract_z<<<nblk(n, 256), 256, 0, s>>>(d_xyz, n, lw.keys.as<double>(), lw.idx.as<int32_t>());
     size_t tbytes = 0;
     LR_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, lw.keys.as<double>(),
@@ -752,7 +932,8 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const doub
                                                 (int)n, 0, 64, s));
     k_gather_sorted<<<nblk(n, 256), 256, 0, s>>>(d_xyz, d_q, lw.perm.as<int32_t>(), n,
                                                  lw.sx.as<double>(), lw.sy.as<double>(),
-                                                 lw.sz.as<double>(), lw.sq.as<double>());
+                                                 lw.sz.as<double>(), lw.sq.as<double>(),
+                                                 lw.satom.as<double4>());
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }
@@ -761,7 +942,6 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Pto
                        double *d_f, int accumulate, hipStream_t s, std::string &err)
 {
     if (n <= 0) return hipSuccess;
-    const int nchunk = (int)((n + CH - 1) / CH);
     if (Ptot <= 0) {
         // nothing to add; an overwrite still has to clear the output
         if (!accumulate) {
@@ -770,44 +950,41 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Pto
         }
         return hipSuccess;
     }
-    // k tiles: enough blocks to cover the chip, at most one tile per k row
-    int ntile = (int)std::min<int64_t>(Ptot, std::max<int64_t>(1, (2048 + nchunk - 1) / nchunk));
-    const int kt = (int)((Ptot + ntile - 1) / ntile);
-    ntile = (int)((Ptot + kt - 1) / kt);
+    // k rows in groups of 64 (one per lane); chunks sized so that the grid
+    // holds about 8192 waves
+    const int ngroup = (int)((Ptot + 63) / 64);
+    const int64_t want = (n * ngroup + 8191) / 8192;
+    const int chk = (int)std::min<int64_t>(256, std::max<int64_t>(16, want));
+    const int nchunk = (int)((n + chk - 1) / chk);
+    LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
     LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * nchunk));
     LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * nchunk));
     LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
-    LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)ntile * (size_t)n));
-    const double4 *kv = lw.kv.as<double4>();
-    dim3 gA(nchunk, ntile), gB(nchunk, ntile);
+    LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)ngroup * (size_t)n));
+    KRec *kr = lw.kr.as<KRec>();
+    k_make_krec<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot, kr);
+    LR_CHECK(hipGetLastError());
+    dim3 g((unsigned)((nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)ngroup);
     if (c.precision == QEM_PREC_MIXED)
-        k_long_A<QEM_PREC_MIXED><<<gA, CH, 0, s>>>(kv, Ptot, kt, lw.sx.as<double>(),
-                                                   lw.sy.as<double>(), lw.sz.as<double>(),
-                                                   lw.sq.as<double>(), n, nchunk,
-                                                   lw.partA.as<double4>());
+        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, lw.satom.as<double4>(), n,
+                                                               chk, nchunk, lw.partA.as<double4>());
     else
-        k_long_A<QEM_PREC_FP64><<<gA, CH, 0, s>>>(kv, Ptot, kt, lw.sx.as<double>(),
-                                                  lw.sy.as<double>(), lw.sz.as<double>(),
-                                                  lw.sq.as<double>(), n, nchunk,
-                                                  lw.partA.as<double4>());
+        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, lw.satom.as<double4>(), n,
+                                                              chk, nchunk, lw.partA.as<double4>());
     LR_CHECK(hipGetLastError());
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
                                            lw.offA.as<double4>(), lw.totA.as<double4>());
     LR_CHECK(hipGetLastError());
     if (c.precision == QEM_PREC_MIXED)
-        k_long_B<QEM_PREC_MIXED><<<gB, CH, 0, s>>>(c, kv, Ptot, kt, lw.sx.as<double>(),
-                                                   lw.sy.as<double>(), lw.sz.as<double>(),
-                                                   lw.sq.as<double>(), n, nchunk,
-                                                   lw.offA.as<double4>(), lw.totA.as<double4>(),
-                                                   lw.partB.as<double>());
+        k_lane_B<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>(),
+            lw.offA.as<double4>(), lw.totA.as<double4>(), lw.partB.as<double>());
     else
-        k_long_B<QEM_PREC_FP64><<<gB, CH, 0, s>>>(c, kv, Ptot, kt, lw.sx.as<double>(),
-                                                  lw.sy.as<double>(), lw.sz.as<double>(),
-                                                  lw.sq.as<double>(), n, nchunk,
-                                                  lw.offA.as<double4>(), lw.totA.as<double4>(),
-                                                  lw.partB.as<double>());
+        k_lane_B<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>(),
+            lw.offA.as<double4>(), lw.totA.as<double4>(), lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
-    k_long_R<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ntile, n, lw.perm.as<int32_t>(),
+    k_long_R<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ngroup, n, lw.perm.as<int32_t>(),
                                           accumulate, d_f);
     LR_CHECK(hipGetLastError());
     return hipSuccess;

@@ -7,7 +7,7 @@
 #include <limits>
 #include <new>
 
-#include "qem_internal.h"
+#include "qem_device.h"
 
 namespace qem {
 
@@ -89,27 +89,46 @@ GLRule make_gl_rule(int n)
     return r;
 }
 
-// J0 Taylor coefficients about c = (m + 1/2) dx, degree 7, from J0(c),
-// J0'(c) = -J1(c) and the Bessel equation x y'' + y' + x y = 0, which gives
-// c_{n+2} = -((n+1)^2 c_{n+1} + c c_n + c_{n-1}) / (c (n+1)(n+2)).
+// J0 on [m dx, (m+1) dx) as p(t) = c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1):
+// least squares at 20 Chebyshev points of [0, 1] (normal equations in long
+// double; J0 from the C library), one float4 per interval.  Two padding
+// intervals past x_max.
 std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals)
 {
-    const int nm = (int)std::ceil(x_max / dx) + 1;
-    std::vector<float> t((size_t)nm * 8);
+    constexpr int D = 3, S = 20;
+    const int nm = (int)std::ceil(x_max / dx) + 2;
+    std::vector<float> tab((size_t)nm * 4, 0.0f);
+    long double ts[S];
+    for (int i = 0; i < S; ++i)
+        ts[i] = 0.5L - 0.5L * cosl(3.141592653589793238462643383279502884L * (i + 0.5L) / S);
     for (int m = 0; m < nm; ++m) {
-        const long double c = ((long double)m + 0.5L) * (long double)dx;
-        long double co[9];
-        co[0] = (long double)::j0((double)c);
-        co[1] = -(long double)::j1((double)c);
-        for (int k = 0; k + 2 <= 7; ++k) {
-            const long double cm1 = k >= 1 ? co[k - 1] : 0.0L;
-            co[k + 2] = -(((long double)(k + 1) * (k + 1)) * co[k + 1] + c * co[k] + cm1) /
-                        (c * (long double)(k + 1) * (long double)(k + 2));
+        long double A[D + 1][D + 2] = {};
+        for (int i = 0; i < S; ++i) {
+            const long double f = (long double)::j0((double)((m + (double)ts[i]) * dx));
+            long double pw[D + 1];
+            pw[0] = 1.0L;
+            for (int k = 1; k <= D; ++k) pw[k] = pw[k - 1] * ts[i];
+            for (int r = 0; r <= D; ++r) {
+                for (int k = 0; k <= D; ++k) A[r][k] += pw[r] * pw[k];
+                A[r][D + 1] += pw[r] * f;
+            }
         }
-        for (int k = 0; k < 8; ++k) t[(size_t)m * 8 + k] = (float)co[k];
+        // Gauss-Jordan with partial pivoting
+        for (int col = 0; col <= D; ++col) {
+            int piv = col;
+            for (int r = col + 1; r <= D; ++r)
+                if (fabsl(A[r][col]) > fabsl(A[piv][col])) piv = r;
+            for (int k = 0; k <= D + 1; ++k) std::swap(A[col][k], A[piv][k]);
+            for (int r = 0; r <= D; ++r) {
+                if (r == col) continue;
+                const long double f = A[r][col] / A[col][col];
+                for (int k = col; k <= D + 1; ++k) A[r][k] -= f * A[col][k];
+            }
+        }
+        for (int k = 0; k <= D; ++k) tab[(size_t)m * 4 + k] = (float)(A[k][D + 1] / A[k][k]);
     }
     *n_intervals = nm;
-    return t;
+    return tab;
 }
 
 __global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
@@ -142,26 +161,15 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
 
 // evaluates the FP32 table Bessel pair used by the mixed-precision kernels
 __global__ void k_bessel_tab(const double *__restrict__ x, int64_t n,
-                             const float4 *__restrict__ tab, int ntab, float dx, float inv_dx,
+                             const float4 *__restrict__ tab, int ntab, float inv_dx,
                              double *__restrict__ j0o, double *__restrict__ j1o)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float xf = (float)x[i];
-    int m = (int)(xf * inv_dx);
-    m = m < ntab - 1 ? m : ntab - 1;
-    const float t = xf - ((float)m + 0.5f) * dx;
-    const float4 c0 = tab[2 * m], c1 = tab[2 * m + 1];
-    float p = c1.w, dp = 0.f;
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.z);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.y);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.x);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.w);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.z);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.y);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.x);
-    j0o[i] = p;
-    j1o[i] = -dp;
+    float J0, J1;
+    dev::bessel01_tab((float)x[i], tab, ntab, inv_dx, J0, J1);
+    j0o[i] = J0;
+    j1o[i] = J1;
 }
 
 }  // namespace qem
@@ -171,8 +179,11 @@ using namespace qem;
 namespace {
 
 constexpr double kTwoPi = 6.283185307179586;  // Float64(2pi)
-constexpr double kBesselDx = 0.5;
-constexpr int kMaxTabIntervals = 1536;        // 48 KiB of LDS
+// Bessel table spacing: 1/32 (cubic error ~1e-7 in J1), widened to 1/16
+// only when the argument range needs more than kMaxTabIntervals
+constexpr double kBesselDx = 1.0 / 32.0;
+constexpr double kBesselDxMax = 1.0 / 16.0;
+constexpr int kMaxTabIntervals = 4096;        // 64 KiB of LDS (16 B per interval)
 
 int set_err(qem_ctx *ctx, int code, const std::string &msg)
 {
@@ -720,6 +731,15 @@ int qem_create(const qem_params *p, qem_ctx **out)
         return fail(QEM_ERR_HIP);
     ctx->gl = make_gl_rule(p->N_t);
     std::vector<float> xf(ctx->gl.x.begin(), ctx->gl.x.end()), wf(ctx->gl.w.begin(), ctx->gl.w.end());
+    std::vector<float> nc(4 * (size_t)p->N_t);
+    for (int n = 0; n < p->N_t; ++n) {
+        const double u = 1.0 + ctx->gl.x[(size_t)n];
+        nc[4 * (size_t)n] = (float)u;
+        nc[4 * (size_t)n + 1] = (float)(u * u);
+        nc[4 * (size_t)n + 2] = (float)(ctx->gl.w[(size_t)n] * u);
+        nc[4 * (size_t)n + 3] = (float)std::log2(ctx->gl.w[(size_t)n] * u);
+    }
+    if (upload(ctx, ctx->sw.ncon, nc.data(), sizeof(float) * nc.size())) return fail(QEM_ERR_HIP);
     if (upload(ctx, ctx->sw.glx, ctx->gl.x.data(), sizeof(double) * p->N_t) ||
         upload(ctx, ctx->sw.glw, ctx->gl.w.data(), sizeof(double) * p->N_t) ||
         upload(ctx, ctx->sw.glxf, xf.data(), sizeof(float) * p->N_t) ||
@@ -729,15 +749,18 @@ int qem_create(const qem_params *p, qem_ctx **out)
         // largest Bessel argument: r_cutoff times the largest limit
         const double la = -std::log(p->accuracy);
         const double kmax = std::max(std::sqrt(4.0 * p->alpha * la), la / (2.0 * p->Lz));
-        const double x_max = p->r_cutoff * kmax * (1.0 + 1e-6) + kBesselDx;
+        double dx = kBesselDx;
+        while (p->r_cutoff * kmax * (1.0 + 1e-6) / dx + 4 > kMaxTabIntervals && dx < kBesselDxMax)
+            dx *= 2.0;
+        const double x_max = p->r_cutoff * kmax * (1.0 + 1e-6) + dx;
         int nm = 0;
-        if (x_max / kBesselDx + 2 > kMaxTabIntervals) {
+        if (x_max / dx + 2 > kMaxTabIntervals) {
             qem_destroy(ctx);
             return QEM_ERR_UNSUPPORTED;
         }
-        std::vector<float> tab = make_j0_table(x_max, kBesselDx, &nm);
+        std::vector<float> tab = make_j0_table(x_max, dx, &nm);
         ctx->sw.jt_n = nm;
-        ctx->sw.jt_dx = (float)kBesselDx;
+        ctx->sw.jt_dx = (float)dx;
         if (upload(ctx, ctx->sw.jtab, tab.data(), sizeof(float) * tab.size()))
             return fail(QEM_ERR_HIP);
     }
@@ -747,24 +770,38 @@ int qem_create(const qem_params *p, qem_ctx **out)
     return QEM_OK;
 }
 
-int qem_destroy(qem_ctx *ctx)
+static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
 {
-    if (!ctx) return QEM_OK;
-    (void)hipSetDevice(ctx->p.device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab,
+    return {&ctx->sw.ncon, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
                       &ctx->nl.cell_key, &ctx->nl.cell_key_sorted, &ctx->nl.atom_idx,
                       &ctx->nl.atom_idx_sorted, &ctx->nl.cell_start, &ctx->nl.cell_end,
                       &ctx->nl.counts, &ctx->nl.row_ptr, &ctx->nl.tmp, &ctx->lw.keys,
                       &ctx->lw.keys_sorted, &ctx->lw.idx, &ctx->lw.perm, &ctx->lw.tmp,
-                      &ctx->lw.sx, &ctx->lw.sy, &ctx->lw.sz, &ctx->lw.sq, &ctx->lw.kv,
+                      &ctx->lw.sx, &ctx->lw.sy, &ctx->lw.sz, &ctx->lw.sq, &ctx->lw.satom,
+                      &ctx->lw.kr, &ctx->lw.kv,
                       &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,
                       &ctx->lw.q0off, &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,
                       &ctx->lw.svz, &ctx->lw.partT, &ctx->lw.offT, &ctx->lw.totT, &ctx->d_xyz,
                       &ctx->d_q, &ctx->d_f, &ctx->d_f2, &ctx->d_aux, &ctx->d_adj};
-    for (DevBuf *b : bufs) release(*b);
+}
+
+int qem_device_bytes(qem_ctx *ctx, int64_t *bytes)
+{
+    if (!ctx || !bytes) return QEM_ERR_INVALID_ARG;
+    size_t t = 0;
+    for (DevBuf *b : all_buffers(ctx)) t += b->cap;
+    *bytes = (int64_t)t;
+    return QEM_OK;
+}
+
+int qem_destroy(qem_ctx *ctx)
+{
+    if (!ctx) return QEM_OK;
+    (void)hipSetDevice(ctx->p.device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf *b : all_buffers(ctx)) release(*b);
     for (int t = 0; t < LongWork::kSlots; ++t) {
         if (ctx->lw.h_kv[t]) (void)hipHostFree(ctx->lw.h_kv[t]);
         if (ctx->lw.kv_ev[t]) (void)hipEventDestroy(ctx->lw.kv_ev[t]);
@@ -1240,7 +1277,7 @@ int qem_bessel01_mixed(qem_ctx *ctx, int64_t n, const double *x, double *j0o, do
     API_HIP(ctx, hipMemcpyAsync(base, x, b, hipMemcpyHostToDevice, ctx->stream));
     k_bessel_tab<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
         reinterpret_cast<const double *>(base), n, ctx->sw.jtab.as<float4>(), ctx->sw.jt_n,
-        ctx->sw.jt_dx, 1.0f / ctx->sw.jt_dx, reinterpret_cast<double *>(base + b),
+        1.0f / ctx->sw.jt_dx, reinterpret_cast<double *>(base + b),
         reinterpret_cast<double *>(base + 2 * b));
     API_HIP(ctx, hipGetLastError());
     API_HIP(ctx, hipMemcpyAsync(j0o, base + b, b, hipMemcpyDeviceToHost, ctx->stream));

@@ -10,35 +10,31 @@ constexpr float kLog2e = 1.4426950408889634f;
 
 __device__ __forceinline__ double jl_sign(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
 
-// FP32 J0/J1 from the LDS table of degree-7 Taylor polynomials of J0 about
-// the interval centres; J1 = -J0' from the same polynomial (p, p' together).
+// FP32 J0/J1 from the LDS table: per interval of width dx (= 1/inv_dx) a
+// cubic p(t) ~ J0(x0 + t dx), t in [0, 1), fitted at Chebyshev points and
+// stored as one float4 (c0..c3).  J1 = -J0' = -p'(t) / dx from the same
+// polynomial.  The index is clamped to the last interval.
 __device__ __forceinline__ void bessel01_tab(float x, const float4 *__restrict__ tab, int ntab,
-                                             float dx, float inv_dx, float &J0, float &J1)
+                                             float inv_dx, float &J0, float &J1)
 {
-    int m = (int)(x * inv_dx);
+    const float xb = x * inv_dx;
+    int m = (int)xb;
     m = m < ntab - 1 ? m : ntab - 1;
-    const float t = x - ((float)m + 0.5f) * dx;
-    const float4 c0 = tab[2 * m];
-    const float4 c1 = tab[2 * m + 1];
-    float p = c1.w;
-    float dp = 0.f;
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.z);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.y);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c1.x);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.w);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.z);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.y);
-    dp = fmaf(dp, t, p); p = fmaf(p, t, c0.x);
+    const float t = __builtin_amdgcn_fractf(xb);
+    const float4 c = tab[m];
+    float p = fmaf(c.w, t, c.z);
+    float dp = c.w;
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c.y);
+    dp = fmaf(dp, t, p); p = fmaf(p, t, c.x);
     J0 = p;
-    J1 = -dp;
+    J1 = -dp * inv_dx;
 }
 
 // Copies the Bessel table into this block's LDS.
 __device__ __forceinline__ const float4 *stage_table(int jt_n, const float4 *__restrict__ gtab,
                                                      float4 *lds)
 {
-    const int n = 2 * jt_n;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) lds[t] = gtab[t];
+    for (int t = threadIdx.x; t < jt_n; t += blockDim.x) lds[t] = gtab[t];
     __syncthreads();
     return lds;
 }

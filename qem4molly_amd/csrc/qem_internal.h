@@ -34,7 +34,7 @@ struct ShortConsts {
     int32_t Nt;
     int32_t single;      // point-charge part only (F_short_ij single=true)
     int32_t jt_n;        // Bessel table intervals (mixed precision)
-    float jt_dx, jt_inv_dx;
+    float jt_dx, jt_inv_dx;  // interval width and its inverse
 };
 
 // Gauss-Legendre rule on [-1,1] (ascending) plus its FP32 copy.
@@ -44,8 +44,9 @@ struct GLRule {
 };
 GLRule make_gl_rule(int n);
 
-// Taylor coefficients of J0 about interval centres: 8 floats per interval,
-// J0(c + t) = sum_k C[k] t^k, J1 = -d/dt of the same polynomial.
+// Per interval [m dx, (m+1) dx): coefficients of a degree-4 polynomial in
+// t in [0, 1) fitted to J0(m dx + t dx) at Chebyshev points (least squares),
+// 8 floats per interval (c0..c4, padding).
 std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals);
 
 // ---- neighbour list (neighbor.hip) ------------------------------------
@@ -77,6 +78,7 @@ hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs,
 struct ShortWork {
     DevBuf glx, glw;      // double nodes / weights
     DevBuf glxf, glwf;    // float nodes / weights
+    DevBuf ncon;          // float4 per node: (1 + x_n, (1 + x_n)^2, w_n (1 + x_n), 0)
     DevBuf jtab;          // float4 pairs per interval
     int jt_n = 0;
     float jt_dx = 0.f;
@@ -118,6 +120,8 @@ struct LongConsts {
 struct LongWork {
     DevBuf keys, keys_sorted, idx, perm, tmp;
     DevBuf sx, sy, sz, sq;   // atoms in z order
+    DevBuf satom;            // atoms in z order, (x, y, z, q) per atom
+    DevBuf kr;               // per k row derived constants of the force passes
     DevBuf kv;               // P x 4 doubles (kx, ky, k, weight)
     DevBuf partA;            // [P][nchunk][4] chunk sums
     DevBuf offA;             // [P][nchunk][4] chunk offsets

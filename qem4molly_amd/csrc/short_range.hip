@@ -18,10 +18,9 @@
 // Precision modes
 //   FP64:  everything double; J0/J1 from the device math library.
 //   MIXED: node arithmetic, exponentials (v_exp_f32), reciprocal
-//          (v_rcp_f32) and J0/J1 in FP32; J0/J1 come from a table of degree-7
-//          Taylor polynomials of J0 about the centres of 0.5-wide intervals
-//          (LDS resident; J1 = -J0' from the same polynomial, p and p' by one
-//          Horner pass).  Per-pair set-up, closed-form image terms and all
+//          (v_rcp_f32) and J0/J1 in FP32; J0/J1 come from a table of degree-4
+//          polynomials of J0 fitted on 0.125-wide intervals (LDS resident;
+//          J1 = -J0' from the same polynomial, p and p' by one Horner pass).  Per-pair set-up, closed-form image terms and all
 //          sums over images, pairs and atoms stay FP64.
 #include "qem_device.h"
 
@@ -61,32 +60,53 @@ __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool 
     sz = az;
 }
 
+// Mixed-precision node sums with the node constants folded in: for the
+// rule's node n, u = (1 + x_n, (1 + x_n)^2, w_n (1 + x_n), log2(w_n (1 + x_n)));
+// with h = K/2 the node is k = h u.x and w k = h^2 u.z.  Every exponent is
+// written through e = -2 Lz k log2(e) (E = 2^e):
+//   point:  e^{-k (a + 2 Lz)} w k / h^2 = 2^{e (1 + a/2Lz) + u.w}
+//   gauss:  e^{-k a - k^2/4 alpha} w k / h^2 = 2^{e (a/2Lz + e cg/c2^2) + u.w}
+// so each VALU op reads at most one wave-uniform (scalar) operand.
+// Returns sum w k t J1 and sum w k t J0 as above.
+//
+// Table bound: k < K and rho <= r_cutoff, and the table spans
+// r_cutoff * max(K) plus two intervals, so x / dx stays below the last
+// interval; hR is clamped once per set so that holds for any input.
 __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool gauss,
-                                            const ShortConsts &c, const float *__restrict__ glx,
-                                            const float *__restrict__ glw,
+                                            const ShortConsts &c, const float4 *__restrict__ ncon,
                                             const float4 *__restrict__ tab, float &sr, float &sz)
 {
     const float h = 0.5f * K;
     const float c2 = -2.0f * (float)c.Lz * kLog2e;
-    const float ca = -a * kLog2e;
     const float cg = -kLog2e * (float)(0.25 / c.alpha);
     const float g12 = (float)c.g12;
+    const float hE = h * c2;                       // e = hE u.x
+    const float rA = a * (float)(0.5 / c.Lz);      // a / 2Lz
+    const float rP = 1.0f + rA;
+    const float rG = cg / (c2 * c2);
+    // Bessel argument in table units; u.x < 2 keeps m <= jt_n - 1
+    const float hR = fminf(h * rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
     float ar = 0.f, az = 0.f;
     for (int n = 0; n < c.Nt; ++n) {
-        const float k = fmaf(h, glx[n], h);
-        const float w = h * glw[n];
-        const float E = __builtin_amdgcn_exp2f(k * c2);
-        const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
-        const float num = gauss ? __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca))
-                                : __builtin_amdgcn_exp2f(k * ca) * E;
-        const float wkt = w * k * (num * rD);
-        float J0, J1;
-        bessel01_tab(k * rho, tab, c.jt_n, c.jt_dx, c.jt_inv_dx, J0, J1);
-        ar = fmaf(wkt, J1, ar);
-        az = fmaf(wkt, J0, az);
+        const float4 u = ncon[n];
+        const float e = hE * u.x;
+        const float E = __builtin_amdgcn_exp2f(e);
+        const float num = gauss ? __builtin_amdgcn_exp2f(fmaf(e, fmaf(e, rG, rA), u.w))
+                                : __builtin_amdgcn_exp2f(fmaf(e, rP, u.w));
+        const float t = num * __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+        const float xb = hR * u.x;
+        const float tt = __builtin_amdgcn_fractf(xb);
+        const float4 cc = tab[(int)xb];
+        float p = fmaf(cc.w, tt, cc.z);
+        float dp = cc.w;
+        dp = fmaf(dp, tt, p); p = fmaf(p, tt, cc.y);
+        dp = fmaf(dp, tt, p); p = fmaf(p, tt, cc.x);
+        ar = fmaf(t, dp, ar);   // J1 = -dp / dx
+        az = fmaf(t, p, az);
     }
-    sr = ar;
-    sz = az;
+    const float h2 = h * h;
+    sr = -h2 * c.jt_inv_dx * ar;
+    sz = h2 * az;
 }
 
 struct PairForce {
@@ -98,8 +118,8 @@ struct PairForce {
 // 2-point neighbour list is empty then, line 36-41).
 template <int PREC>
 __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__ glx,
-                                const double *__restrict__ glw, const float *__restrict__ glxf,
-                                const float *__restrict__ glwf, const float4 *__restrict__ tab,
+                                const double *__restrict__ glw, const float4 *__restrict__ ncon,
+                                const float4 *__restrict__ tab,
                                 double qi, double qj, double xi, double yi, double zi, double xj,
                                 double yj, double zj)
 {
@@ -143,11 +163,11 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
                 if (!c.single) nodes_fp64(Kg, a, rho, true, c, glx, glw, gr, gz);
             } else {
                 float fr, fz;
-                nodes_mixed((float)Kp, (float)a, (float)rho, false, c, glxf, glwf, tab, fr, fz);
+                nodes_mixed((float)Kp, (float)a, (float)rho, false, c, ncon, tab, fr, fz);
                 pr = fr;
                 pz = fz;
                 if (!c.single) {
-                    nodes_mixed((float)Kg, (float)a, (float)rho, true, c, glxf, glwf, tab, fr, fz);
+                    nodes_mixed((float)Kg, (float)a, (float)rho, true, c, ncon, tab, fr, fz);
                     gr = fr;
                     gz = fz;
                 }
@@ -246,8 +266,7 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_list(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
-    const float *__restrict__ glxf, const float *__restrict__ glwf,
-    const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
+    const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
     const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
     const double *__restrict__ q, double4 *__restrict__ fpair)
 {
@@ -257,7 +276,7 @@ __global__ __launch_bounds__(256) void k_pair_list(
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const int64_t i = pi[p], j = pj[p];
-        PairForce f = pair_force<PREC>(c, glx, glw, glxf, glwf, tab, q[i], q[j], xyz[3 * i],
+        PairForce f = pair_force<PREC>(c, glx, glw, ncon, tab, q[i], q[j], xyz[3 * i],
                                        xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
                                        xyz[3 * j + 1], xyz[3 * j + 2]);
         fpair[p] = make_double4(f.fx, f.fy, f.fz, f.fzj);
@@ -267,8 +286,7 @@ __global__ __launch_bounds__(256) void k_pair_list(
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_direct(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
-    const float *__restrict__ glxf, const float *__restrict__ glwf,
-    const float4 *__restrict__ gtab, int64_t m, const double *__restrict__ ri,
+    const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, int64_t m, const double *__restrict__ ri,
     const double *__restrict__ rj, const double *__restrict__ qi,
     const double *__restrict__ qj, double4 *__restrict__ fpair)
 {
@@ -277,7 +295,7 @@ __global__ __launch_bounds__(256) void k_pair_direct(
     if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += stride) {
-        PairForce f = pair_force<PREC>(c, glx, glw, glxf, glwf, tab, qi[p], qj[p], ri[3 * p],
+        PairForce f = pair_force<PREC>(c, glx, glw, ncon, tab, qi[p], qj[p], ri[3 * p],
                                        ri[3 * p + 1], ri[3 * p + 2], rj[3 * p], rj[3 * p + 1],
                                        rj[3 * p + 2]);
         fpair[p] = make_double4(f.fx, f.fy, f.fz, f.fzj);
@@ -341,15 +359,15 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
 {
     if (npairs <= 0) return hipSuccess;
     if (precision == QEM_PREC_MIXED) {
-        const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+        const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
         k_pair_list<QEM_PREC_MIXED><<<pair_grid(npairs), 256, lds, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), ws.jtab.as<float4>(), d_pi, d_pj, npairs, d_xyz, d_q,
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(),
+            ws.jtab.as<float4>(), d_pi, d_pj, npairs, d_xyz, d_q,
             reinterpret_cast<double4 *>(d_fpair));
     } else {
         k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), nullptr, d_pi, d_pj, npairs, d_xyz, d_q,
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, d_pi,
+            d_pj, npairs, d_xyz, d_q,
             reinterpret_cast<double4 *>(d_fpair));
     }
     return hipGetLastError();
@@ -362,15 +380,15 @@ hipError_t launch_pair_direct(const ShortConsts &c, int precision, const ShortWo
 {
     if (m <= 0) return hipSuccess;
     if (precision == QEM_PREC_MIXED) {
-        const size_t lds = sizeof(float4) * 2 * (size_t)ws.jt_n;
+        const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
         k_pair_direct<QEM_PREC_MIXED><<<pair_grid(m), 256, lds, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), ws.jtab.as<float4>(), m, d_ri, d_rj, d_qi, d_qj,
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(),
+            ws.jtab.as<float4>(), m, d_ri, d_rj, d_qi, d_qj,
             reinterpret_cast<double4 *>(d_fpair));
     } else {
         k_pair_direct<QEM_PREC_FP64><<<pair_grid(m), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), nullptr, m, d_ri, d_rj, d_qi, d_qj,
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, m, d_ri,
+            d_rj, d_qi, d_qj,
             reinterpret_cast<double4 *>(d_fpair));
     }
     return hipGetLastError();
