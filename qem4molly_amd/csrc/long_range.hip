@@ -23,7 +23,7 @@
 // Nothing of size N x P is stored.
 #include <hipcub/hipcub.hpp>
 
-#include "qem_internal.h"
+#include "qem_device.h"
 
 namespace qem {
 namespace {
@@ -316,35 +316,14 @@ __device__ __forceinline__ void lane_terms(const LaneK &L, double x, double y, d
     }
 }
 
-// lane permutation of a double (every lane of every row written)
-template <int CTRL>
-__device__ __forceinline__ double dpp_perm_f64(double v)
-{
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-// row broadcast of a double; rows outside ROW_MASK read 0
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_bcast_f64(double v)
-{
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
 // Sum over the 64 lanes of a wave in a fixed order; lane 63 holds the sum.
 // (pairs, quads, half rows, rows of 16, then rows 0+1 into 1, 2+3 into 3,
 // and row 1 into rows 2, 3)
 __device__ __forceinline__ double wave_sum63(double v)
 {
-    v += dpp_perm_f64<0xB1>(v);        // quad_perm [1,0,3,2]
-    v += dpp_perm_f64<0x4E>(v);        // quad_perm [2,3,0,1]
-    v += dpp_perm_f64<0x141>(v);       // row_half_mirror
-    v += dpp_perm_f64<0x140>(v);       // row_mirror
-    v += dpp_bcast_f64<0x142, 0xA>(v); // row_bcast:15 (rows 1, 3)
-    v += dpp_bcast_f64<0x143, 0xC>(v); // row_bcast:31 (rows 2, 3)
+    v = dev::row_sum16(v);
+    v += dev::dpp_bcast_f64<0x142, 0xA>(v); // row_bcast:15 (rows 1, 3)
+    v += dev::dpp_bcast_f64<0x143, 0xC>(v); // row_bcast:31 (rows 2, 3)
     return v;
 }
 

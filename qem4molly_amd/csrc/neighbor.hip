@@ -6,12 +6,17 @@
 //
 // Layout: atoms are bucketed into an ncx x ncy grid of cells at least
 // r_cutoff wide (stable radix sort of cell ids, so atoms inside a cell stay
-// in index order); one thread per atom walks the 3 x 3 block of cells
-// around its own, first counting and then writing its partners j > i.
-// Pairs are therefore grouped by i, in a deterministic order.  A second
-// stable sort of the pair list by j gives, per atom, the pairs in which it
-// is the second endpoint; the force gather uses both groupings so that the
-// per-atom reduction order is fixed (no atomics).
+// in index order).  The sorted order numbers the atoms by slot: slot s is
+// atom atom_idx_sorted[s], and a cell's atoms are a run of slots.  One
+// thread per slot walks the 3 x 3 block of cells around its own, first
+// counting and then writing its partners with a larger atom index (the
+// pair keeps the orientation i < j of the atom indices).  Pairs are stored
+// as slot pairs grouped by first slot, in a deterministic order.  A second
+// stable sort of the pair list by second slot gives, per slot, the pairs in
+// which it is the second endpoint; the force gather uses both groupings so
+// that the per-atom reduction order is fixed (no atomics).  Since pair
+// neighbours are near each other in slot order, the per-pair position reads
+// and the gather's reads of per-pair results stay within nearby cache lines.
 #include <hipcub/hipcub.hpp>
 
 #include "qem_internal.h"
@@ -79,19 +84,21 @@ __global__ void k_run_bounds(const uint32_t *__restrict__ skey, int64_t n,
 }
 
 template <bool FILL>
-__global__ void k_nl_pass(const double *__restrict__ xyz, int64_t n, CellGrid g, double rc2,
-                          const int32_t *__restrict__ cstart, const int32_t *__restrict__ cend,
-                          const int32_t *__restrict__ sidx, int64_t *__restrict__ counts,
-                          const int64_t *__restrict__ row_ptr, int32_t *__restrict__ pi,
-                          int32_t *__restrict__ pj)
+__global__ void k_nl_pass(const double *__restrict__ sxyz, int64_t n, CellGrid g, double rc2,
+                          const uint32_t *__restrict__ skey, const int32_t *__restrict__ cstart,
+                          const int32_t *__restrict__ cend, const int32_t *__restrict__ sidx,
+                          int64_t *__restrict__ counts, const int64_t *__restrict__ row_ptr,
+                          int32_t *__restrict__ pi, int32_t *__restrict__ pj)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double xi = xyz[3 * i], yi = xyz[3 * i + 1];
-    const int cx = cell_of(xi, g.Lx, g.ncx), cy = cell_of(yi, g.Ly, g.ncy);
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double xi = sxyz[3 * s], yi = sxyz[3 * s + 1];
+    const int32_t ai = sidx[s];
+    const int key = (int)skey[s];
+    const int cx = key % g.ncx, cy = key / g.ncx;
     const int nxi = g.ncx >= 3 ? 3 : g.ncx, nyi = g.ncy >= 3 ? 3 : g.ncy;
     int64_t cnt = 0;
-    const int64_t base = FILL ? row_ptr[i] : 0;
+    const int64_t base = FILL ? row_ptr[s] : 0;
     for (int oy = 0; oy < nyi; ++oy) {
         const int cyy = g.ncy >= 3 ? (cy + oy - 1 + g.ncy) % g.ncy : oy;
         for (int ox = 0; ox < nxi; ++ox) {
@@ -99,21 +106,39 @@ __global__ void k_nl_pass(const double *__restrict__ xyz, int64_t n, CellGrid g,
             const int c = cyy * g.ncx + cxx;
             const int t1 = cend[c];
             for (int t = cstart[c]; t < t1; ++t) {
-                const int32_t j = sidx[t];
-                if (j <= i) continue;
-                const double d2 = pair_d2(xi, yi, xyz[3 * (int64_t)j], xyz[3 * (int64_t)j + 1],
+                if (sidx[t] <= ai) continue;
+                const double d2 = pair_d2(xi, yi, sxyz[3 * (int64_t)t], sxyz[3 * (int64_t)t + 1],
                                           g.Lx, g.Ly);
                 if (d2 <= rc2) {
                     if (FILL) {
-                        pi[base + cnt] = (int32_t)i;
-                        pj[base + cnt] = j;
+                        pi[base + cnt] = (int32_t)s;
+                        pj[base + cnt] = t;
                     }
                     ++cnt;
                 }
             }
         }
     }
-    if (!FILL) counts[i] = cnt;
+    if (!FILL) counts[s] = cnt;
+}
+
+__global__ void k_rows_to_slots(const double *__restrict__ src, const int32_t *__restrict__ perm,
+                                int64_t n, int w, double *__restrict__ dst)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * w) return;
+    const int64_t s = e / w, c = e - s * w;
+    dst[e] = src[(int64_t)perm[s] * w + c];
+}
+
+__global__ void k_rows_from_slots(const double *__restrict__ src,
+                                  const int32_t *__restrict__ perm, int64_t n, int w,
+                                  double *__restrict__ dst)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * w) return;
+    const int64_t s = e / w, c = e - s * w;
+    dst[(int64_t)perm[s] * w + c] = src[e];
 }
 
 __global__ void k_row_bounds(const int64_t *__restrict__ row_ptr, int64_t n,
@@ -196,6 +221,7 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
     nl.n = n;
     nl.npairs = 0;
     nl.has_i_idx = false;
+    nl.slot_order = true;
     ++nl.builds;
     NL_CHECK(ensure(nl.i_start, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
     NL_CHECK(ensure(nl.i_end, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
@@ -246,11 +272,15 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
                                               nl.cell_start.as<int32_t>(),
                                               nl.cell_end.as<int32_t>());
     const double rc2 = rc * rc;
+    NL_CHECK(ensure(nl.sxyz, sizeof(double) * 3 * (size_t)n));
+    k_rows_to_slots<<<nblk(3 * n, 256), 256, 0, s>>>(d_xyz, nl.atom_idx_sorted.as<int32_t>(), n, 3,
+                                                      nl.sxyz.as<double>());
     // counts[n] = 0 so the exclusive scan's last entry is the total
     k_fill_i64<<<1, 64, 0, s>>>(nl.counts.as<int64_t>() + n, 1, 0);
     k_nl_pass<false><<<nblk(n, 128), 128, 0, s>>>(
-        d_xyz, n, g, rc2, nl.cell_start.as<int32_t>(), nl.cell_end.as<int32_t>(),
-        nl.atom_idx_sorted.as<int32_t>(), nl.counts.as<int64_t>(), nullptr, nullptr, nullptr);
+        nl.sxyz.as<double>(), n, g, rc2, nl.cell_key_sorted.as<uint32_t>(),
+        nl.cell_start.as<int32_t>(), nl.cell_end.as<int32_t>(), nl.atom_idx_sorted.as<int32_t>(),
+        nl.counts.as<int64_t>(), nullptr, nullptr, nullptr);
     NL_CHECK(hipGetLastError());
     tbytes = 0;
     NL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, nl.counts.as<int64_t>(),
@@ -271,7 +301,8 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
     NL_CHECK(ensure(nl.pj, sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
     if (total > 0) {
         k_nl_pass<true><<<nblk(n, 128), 128, 0, s>>>(
-            d_xyz, n, g, rc2, nl.cell_start.as<int32_t>(), nl.cell_end.as<int32_t>(),
+            nl.sxyz.as<double>(), n, g, rc2, nl.cell_key_sorted.as<uint32_t>(),
+            nl.cell_start.as<int32_t>(), nl.cell_end.as<int32_t>(),
             nl.atom_idx_sorted.as<int32_t>(), nullptr, nl.row_ptr.as<int64_t>(),
             nl.pi.as<int32_t>(), nl.pj.as<int32_t>());
         NL_CHECK(hipGetLastError());
@@ -290,6 +321,7 @@ hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs, in
     nl.n = n;
     nl.npairs = npairs;
     nl.has_i_idx = true;
+    nl.slot_order = false;
     ++nl.builds;
     std::vector<int32_t> a((size_t)npairs), b((size_t)npairs);
     for (int64_t t = 0; t < npairs; ++t) {
@@ -311,6 +343,24 @@ hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs, in
     // the host vectors must outlive the asynchronous copies
     NL_CHECK(hipStreamSynchronize(s));
     return hipSuccess;
+}
+
+hipError_t launch_rows_to_slots(const NeighborList &nl, int64_t n, int w, const double *src,
+                                double *dst, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_rows_to_slots<<<nblk(n * w, 256), 256, 0, s>>>(src, nl.atom_idx_sorted.as<int32_t>(), n, w,
+                                                      dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_from_slots(const NeighborList &nl, int64_t n, int w, const double *src,
+                                  double *dst, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_rows_from_slots<<<nblk(n * w, 256), 256, 0, s>>>(src, nl.atom_idx_sorted.as<int32_t>(), n,
+                                                        w, dst);
+    return hipGetLastError();
 }
 
 }  // namespace qem

@@ -511,17 +511,50 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
     }
     const ShortConsts c = short_consts(ctx, 0);
     API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    NeighborList &nl = ctx->nl;
+    const double *px = d_xyz, *pq = d_q;
+    double *pf = d_f;
     {
         PhaseScope ps_pair(ctx, QEM_PHASE_PAIR, s);
-        API_HIP(ctx, launch_pair_forces(c, ctx->p.precision, ctx->sw, ctx->nl.pi.as<int32_t>(),
-                                        ctx->nl.pj.as<int32_t>(), ctx->nl.npairs, d_xyz, d_q,
+        if (nl.slot_order) {
+            API_HIP(ctx, ensure(nl.sxyz, sizeof(double) * 3 * (size_t)n));
+            API_HIP(ctx, ensure(nl.sq, sizeof(double) * (size_t)n));
+            API_HIP(ctx, ensure(nl.sout, sizeof(double) * 3 * (size_t)n));
+            API_HIP(ctx, launch_rows_to_slots(nl, n, 3, d_xyz, nl.sxyz.as<double>(), s));
+            API_HIP(ctx, launch_rows_to_slots(nl, n, 1, d_q, nl.sq.as<double>(), s));
+            px = nl.sxyz.as<double>();
+            pq = nl.sq.as<double>();
+            pf = nl.sout.as<double>();
+        }
+        API_HIP(ctx, launch_pair_forces(c, ctx->p.precision, ctx->sw, nl.pi.as<int32_t>(),
+                                        nl.pj.as<int32_t>(), nl.npairs, px, pq,
                                         ctx->sw.fpair.as<double>(), s));
     }
     {
         PhaseScope ps_g(ctx, QEM_PHASE_GATHER, s);
-        API_HIP(ctx, launch_gather_self(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
-                                        ctx->sw.fpair.as<double>(), d_f, s));
+        API_HIP(ctx, launch_gather_self(c, ctx->p.precision, ctx->sw, nl, n, px, pq,
+                                        ctx->sw.fpair.as<double>(), pf, s));
+        if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, pf, d_f, s));
     }
+    return QEM_OK;
+}
+
+// Slot-ordered copies of positions and charges for the current list (the
+// inputs themselves for an installed list).
+int slot_inputs(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                const double **px, const double **pq, hipStream_t s)
+{
+    NeighborList &nl = ctx->nl;
+    *px = d_xyz;
+    *pq = d_q;
+    if (!nl.slot_order) return QEM_OK;
+    API_HIP(ctx, ensure(nl.sxyz, sizeof(double) * 3 * (size_t)n));
+    API_HIP(ctx, ensure(nl.sq, sizeof(double) * (size_t)n));
+    API_HIP(ctx, ensure(nl.sout, sizeof(double) * 3 * (size_t)n));
+    API_HIP(ctx, launch_rows_to_slots(nl, n, 3, d_xyz, nl.sxyz.as<double>(), s));
+    API_HIP(ctx, launch_rows_to_slots(nl, n, 1, d_q, nl.sq.as<double>(), s));
+    *px = nl.sxyz.as<double>();
+    *pq = nl.sq.as<double>();
     return QEM_OK;
 }
 
@@ -602,8 +635,18 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     }
     {
         PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
-        API_HIP(ctx, launch_short_energy(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
-                                         ctx->sw.fpair.as<double>(), phi, psi, s));
+        const double *px, *pq;
+        if ((st = slot_inputs(ctx, n, d_xyz, d_q, &px, &pq, s))) return st;
+        NeighborList &nl = ctx->nl;
+        // slot order: phi and psi rows land in sout first
+        double *phs = nl.slot_order ? nl.sout.as<double>() : phi;
+        double *pss = nl.slot_order ? nl.sout.as<double>() + n : psi;
+        API_HIP(ctx, launch_short_energy(c, ctx->p.precision, ctx->sw, nl, n, px, pq,
+                                         ctx->sw.fpair.as<double>(), phs, pss, s));
+        if (nl.slot_order) {
+            API_HIP(ctx, launch_rows_from_slots(nl, n, 1, phs, phi, s));
+            API_HIP(ctx, launch_rows_from_slots(nl, n, 2, pss, psi, s));
+        }
     }
     double dg_long[2] = {0.0, 0.0};
     if (P >= 0) {
@@ -639,8 +682,20 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
     {
         PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
-        API_HIP(ctx, launch_short_tangent(c, ctx->p.precision, ctx->sw, ctx->nl, n, d_xyz, d_q,
-                                          d_v, ctx->sw.fpair.as<double>(), -1.0, d_hv, s));
+        const double *px, *pq;
+        if ((st = slot_inputs(ctx, n, d_xyz, d_q, &px, &pq, s))) return st;
+        NeighborList &nl = ctx->nl;
+        const double *pv = d_v;
+        double *po = d_hv;
+        if (nl.slot_order) {
+            API_HIP(ctx, ensure(nl.sv, sizeof(double) * 3 * (size_t)n));
+            API_HIP(ctx, launch_rows_to_slots(nl, n, 3, d_v, nl.sv.as<double>(), s));
+            pv = nl.sv.as<double>();
+            po = nl.sout.as<double>();
+        }
+        API_HIP(ctx, launch_short_tangent(c, ctx->p.precision, ctx->sw, nl, n, px, pq, pv,
+                                          ctx->sw.fpair.as<double>(), -1.0, po, s));
+        if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, po, d_hv, s));
     }
     if (P > 0) {
         int64_t Ptot = 0;
@@ -775,6 +830,7 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
     return {&ctx->sw.ncon, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
+                      &ctx->nl.sxyz, &ctx->nl.sq, &ctx->nl.sv, &ctx->nl.sout,
                       &ctx->nl.cell_key, &ctx->nl.cell_key_sorted, &ctx->nl.atom_idx,
                       &ctx->nl.atom_idx_sorted, &ctx->nl.cell_start, &ctx->nl.cell_end,
                       &ctx->nl.counts, &ctx->nl.row_ptr, &ctx->nl.tmp, &ctx->lw.keys,
@@ -884,15 +940,19 @@ int qem_neighbor_list(qem_ctx *ctx, int64_t n, const double *xyz, int64_t cap, i
     *npairs = np;
     const int64_t m = std::min(np, cap);
     if (m > 0) {
-        std::vector<int32_t> a((size_t)m), b((size_t)m);
+        // slot pairs to atom pairs (orientation i < j is kept by the builder)
+        std::vector<int32_t> a((size_t)m), b((size_t)m), perm((size_t)n);
         API_HIP(ctx, hipMemcpyAsync(a.data(), ctx->nl.pi.p, sizeof(int32_t) * (size_t)m,
                                     hipMemcpyDeviceToHost, ctx->stream));
         API_HIP(ctx, hipMemcpyAsync(b.data(), ctx->nl.pj.p, sizeof(int32_t) * (size_t)m,
                                     hipMemcpyDeviceToHost, ctx->stream));
+        API_HIP(ctx, hipMemcpyAsync(perm.data(), ctx->nl.atom_idx_sorted.p,
+                                    sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                                    ctx->stream));
         API_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (int64_t t = 0; t < m; ++t) {
-            pairs[2 * t] = a[(size_t)t];
-            pairs[2 * t + 1] = b[(size_t)t];
+            pairs[2 * t] = perm[(size_t)a[(size_t)t]];
+            pairs[2 * t + 1] = perm[(size_t)b[(size_t)t]];
         }
     }
     return np > cap ? set_err(ctx, QEM_ERR_CAPACITY, "pair buffer too small") : QEM_OK;

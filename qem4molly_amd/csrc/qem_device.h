@@ -76,5 +76,34 @@ __device__ __forceinline__ double min_image_d2(const ShortConsts &c, double dx, 
     return mx * mx + my * my;
 }
 
+// lane permutation of a double (every lane of every row written)
+template <int CTRL>
+__device__ __forceinline__ double dpp_perm_f64(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// row broadcast of a double; rows outside ROW_MASK read 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_bcast_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over each row of 16 lanes in a fixed order (pairs, quads, half rows,
+// rows); every lane of the row receives its row's sum.
+__device__ __forceinline__ double row_sum16(double v)
+{
+    v += dpp_perm_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_perm_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_perm_f64<0x141>(v);  // row_half_mirror
+    v += dpp_perm_f64<0x140>(v);  // row_mirror
+    return v;
+}
+
 }  // namespace dev
 }  // namespace qem

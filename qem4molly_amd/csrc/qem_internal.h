@@ -61,10 +61,23 @@ struct NeighborList {
     bool has_i_idx = false;
     DevBuf i_start, i_end, i_idx;
     DevBuf j_start, j_end, j_idx;
+    // Endpoints, i_*/j_* and per-atom kernel rows are in slot numbering:
+    // for a built list the slots are the atoms in cell order (slot s is
+    // atom atom_idx_sorted[s]), so pair neighbours are near in memory; for
+    // an installed list slot = atom (slot_order false).
+    bool slot_order = false;
+    DevBuf sxyz, sq, sv;  // per call: positions, charges, directions in slot order
+    DevBuf sout;          // per call: per-slot results before the scatter
     // scratch
     DevBuf cell_key, cell_key_sorted, atom_idx, atom_idx_sorted;
     DevBuf cell_start, cell_end, counts, row_ptr, tmp;
 };
+// Slot-ordered copies: rows of w doubles, dst[s] = src[atom_idx_sorted[s]]
+// (to slots) and dst[atom_idx_sorted[s]] = src[s] (from slots).
+hipError_t launch_rows_to_slots(const NeighborList &nl, int64_t n, int w, const double *src,
+                                double *dst, hipStream_t s);
+hipError_t launch_rows_from_slots(const NeighborList &nl, int64_t n, int w, const double *src,
+                                  double *dst, hipStream_t s);
 // Builds the list for device coordinates; synchronises the stream once
 // (to learn the pair count).  Returns hipSuccess or the first error.
 hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,

@@ -302,42 +302,65 @@ __global__ __launch_bounds__(256) void k_pair_direct(
     }
 }
 
-// Per atom: sum of pair forces with the atom as first endpoint, minus the
-// xy (and, as written, z) forces of pairs with it as second endpoint, plus
-// the self-image term (src/QEM_short.jl:41-56).  Fixed order, no atomics.
+// Self-image z force per atom (src/QEM_short.jl:41-56, F_short_i) into
+// f[3a + 2]; with zero_xy the x, y entries are cleared as well.
 template <int PREC>
-__global__ __launch_bounds__(256) void k_gather_self(
+__global__ __launch_bounds__(256) void k_self_z(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float *__restrict__ glxf, const float *__restrict__ glwf, int64_t n,
-    const double *__restrict__ xyz, const double *__restrict__ q,
-    const double4 *__restrict__ fpair, const int32_t *__restrict__ i_start,
-    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
-    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
-    const int32_t *__restrict__ j_idx, int with_pairs, double *__restrict__ f)
+    const double *__restrict__ xyz, const double *__restrict__ q, int zero_xy,
+    double *__restrict__ f)
 {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
+    f[3 * a + 2] = self_force_z<PREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2]);
+    if (zero_xy) {
+        f[3 * a] = 0.0;
+        f[3 * a + 1] = 0.0;
+    }
+}
+
+// Per atom: sum of pair forces with the atom as first endpoint, minus the
+// xy (and, as written, z) forces of pairs with it as second endpoint
+// (src/QEM_short.jl:41-56), plus the self term already in f[3a + 2].
+// Sixteen lanes per atom read the atom's runs of pair results in
+// contiguous strides; each lane keeps a strided partial sum and the row of
+// 16 lanes is summed in a fixed order.  No atomics.
+__global__ __launch_bounds__(256) void k_gather_pairs(
+    int64_t n, const double4 *__restrict__ fpair, const int32_t *__restrict__ i_start,
+    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
+    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
+    const int32_t *__restrict__ j_idx, double *__restrict__ f)
+{
+    const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    const bool live = a < n;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (with_pairs) {
-        const int32_t t0 = i_start[a], t1 = i_end[a];
-        for (int32_t t = t0; t < t1; ++t) {
+    if (live) {
+        const int32_t t1 = i_end[a];
+        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
             const double4 v = fpair[i_idx ? i_idx[t] : t];
             fx += v.x;
             fy += v.y;
             fz += v.z;
         }
-        const int32_t u0 = j_start[a], u1 = j_end[a];
-        for (int32_t u = u0; u < u1; ++u) {
+        const int32_t u1 = j_end[a];
+        for (int32_t u = j_start[a] + l; u < u1; u += 16) {
             const double4 v = fpair[j_idx[u]];
             fx -= v.x;
             fy -= v.y;
             fz += v.w;
         }
     }
-    fz += self_force_z<PREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2]);
-    f[3 * a] = fx;
-    f[3 * a + 1] = fy;
-    f[3 * a + 2] = fz;
+    // every lane takes part in the row sums
+    fx = dev::row_sum16(fx);
+    fy = dev::row_sum16(fy);
+    fz = dev::row_sum16(fz);
+    if (live && l == 0) {
+        f[3 * a] = fx;
+        f[3 * a + 1] = fy;
+        f[3 * a + 2] = fz + f[3 * a + 2];
+    }
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -394,25 +417,35 @@ hipError_t launch_pair_direct(const ShortConsts &c, int precision, const ShortWo
     return hipGetLastError();
 }
 
+static hipError_t launch_self_z(const ShortConsts &c, int precision, const ShortWork &ws,
+                                int64_t n, const double *d_xyz, const double *d_q, int zero_xy,
+                                double *d_f, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (precision == QEM_PREC_MIXED)
+        k_self_z<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, zero_xy, d_f);
+    else
+        k_self_z<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, zero_xy, d_f);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWork &ws,
                               const NeighborList &nl, int64_t n, const double *d_xyz,
                               const double *d_q, const double *d_fpair, double *d_f,
                               hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
+    hipError_t e = launch_self_z(c, precision, ws, n, d_xyz, d_q, 0, d_f, s);
+    if (e != hipSuccess) return e;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
-    if (precision == QEM_PREC_MIXED)
-        k_gather_self<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, reinterpret_cast<const double4 *>(d_fpair),
-            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
-            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), 1, d_f);
-    else
-        k_gather_self<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, reinterpret_cast<const double4 *>(d_fpair),
-            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
-            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), 1, d_f);
+    k_gather_pairs<<<nblk(16 * n, 256), 256, 0, s>>>(
+        n, reinterpret_cast<const double4 *>(d_fpair), nl.i_start.as<int32_t>(),
+        nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+        nl.j_idx.as<int32_t>(), d_f);
     return hipGetLastError();
 }
 
@@ -420,18 +453,7 @@ hipError_t launch_self_only(const ShortConsts &c, int precision, const ShortWork
                             int64_t n, const double *d_xyz, const double *d_q, double *d_f,
                             hipStream_t s)
 {
-    if (n <= 0) return hipSuccess;
-    if (precision == QEM_PREC_MIXED)
-        k_gather_self<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, nullptr, nullptr, nullptr, nullptr, nullptr,
-            nullptr, nullptr, 0, d_f);
-    else
-        k_gather_self<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, nullptr, nullptr, nullptr, nullptr, nullptr,
-            nullptr, nullptr, 0, d_f);
-    return hipGetLastError();
+    return launch_self_z(c, precision, ws, n, d_xyz, d_q, 1, d_f, s);
 }
 
 }  // namespace qem
