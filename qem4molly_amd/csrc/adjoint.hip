@@ -178,6 +178,91 @@ struct Dual4 {
     double sr, sz, dsr, dsz;
 };
 
+// Mixed-precision value and tangent node sums.  With the node constants
+// u = (1 + x_n, (1 + x_n)^2, w_n (1 + x_n), log2(w_n (1 + x_n))) the node is
+// k = h u.x (h = K/2) and the summand t J is taken as in the force kernel
+// (t = w k e^{..}/D / h^2, J0 and J1 = -J0' from the cubic table).  Along the
+// tangent the nodes move with K (dk = r k, r = dK/K), the image distance
+// with da and rho with drho, so
+//   dt = t k (A + B E/D [+ C k]),   A = -r (a [+ 2 Lz]) - da,
+//   B = 2 Lz g12 r,  C = -r / (2 alpha) (gaussian set),
+//   dJ0 = -J1 x s,  dJ1 = (x J0 - J1) s,  s = r + drho/rho,  x = k rho,
+// and d(h^2 S) = h^2 (2 r S + dS).  Since k and the table argument are both
+// multiples of u.x, the tangent is a fixed combination of six (eight for the
+// gaussian set) node sums: S = sum t J, U = sum u.x t J, Q = sum u.x (E/D) t J,
+// R = sum u.x^2 t J, each for J0 and J1; the per-set factors are applied once.
+__device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a, float da,
+                                                     float rho, float drho, bool gauss,
+                                                     bool self, const ShortConsts &c,
+                                                     const float4 *__restrict__ tab)
+{
+    const float4 *__restrict__ ncon = c.ncon;
+    const float h = 0.5f * K;
+    const float r = K != 0.0f ? dK / K : 0.0f;
+    const float Lz2 = 2.0f * (float)c.Lz;
+    const float c2 = -Lz2 * kLog2e;
+    const float cg = -kLog2e * (float)(0.25 / c.alpha);
+    const float g12 = (float)c.g12;
+    const float hE = h * c2;
+    const float rA = a * (float)(0.5 / c.Lz);
+    const float rP = 1.0f + rA;
+    const float rG = cg / (c2 * c2);
+    const float hR = self ? 0.0f : fminf(h * rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
+    // S0, S1 = sum t p, t dp; U = u.x-weighted; Q = u.x E/D-weighted; R = u.x^2-weighted
+    float S0 = 0.f, S1 = 0.f, U0 = 0.f, U1 = 0.f, Q0 = 0.f, Q1 = 0.f, R0 = 0.f, R1 = 0.f;
+    for (int n = 0; n < c.Nt; ++n) {
+        const float4 u = ncon[n];
+        const float e = hE * u.x;
+        const float E = __builtin_amdgcn_exp2f(e);
+        const float num = gauss ? __builtin_amdgcn_exp2f(fmaf(e, fmaf(e, rG, rA), u.w))
+                                : __builtin_amdgcn_exp2f(fmaf(e, rP, u.w));
+        const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+        const float t = num * rD;
+        const float wq = (E * rD) * u.x;
+        if (self) {  // J0 = 1, J1 = 0
+            S0 += t;
+            U0 = fmaf(u.x, t, U0);
+            Q0 = fmaf(wq, t, Q0);
+            if (gauss) R0 = fmaf(u.y, t, R0);
+            continue;
+        }
+        const float xb = hR * u.x;
+        const float tt = __builtin_amdgcn_fractf(xb);
+        const float4 cc = tab[(int)xb];
+        float p = fmaf(cc.w, tt, cc.z);
+        float dp = cc.w;
+        dp = fmaf(dp, tt, p); p = fmaf(p, tt, cc.y);
+        dp = fmaf(dp, tt, p); p = fmaf(p, tt, cc.x);
+        const float A0 = t * p, A1 = t * dp;   // t J0, -t J1 dx
+        S0 += A0;
+        S1 += A1;
+        U0 = fmaf(u.x, A0, U0);
+        U1 = fmaf(u.x, A1, U1);
+        Q0 = fmaf(wq, A0, Q0);
+        Q1 = fmaf(wq, A1, Q1);
+        if (gauss) {
+            R0 = fmaf(u.y, A0, R0);
+            R1 = fmaf(u.y, A1, R1);
+        }
+    }
+    const float A = gauss ? -r * a - da : -r * (a + Lz2) - da;
+    const float B = Lz2 * g12 * r;
+    const float C = gauss ? -r * (float)(0.5 / c.alpha) : 0.0f;
+    const float sg = (!self && rho > 0.0f) ? r + drho / rho : 0.0f;
+    const float sg2 = sg * c.jt_dx * c.jt_dx;
+    const float hA = h * A, hB = h * B, h2C = h * h * C;
+    // tangents of the sums over nodes, before the h^2 factor
+    const float dS0 = hA * U0 + hB * Q0 + h2C * R0 + sg * hR * U1;
+    const float dS1 = hA * U1 + hB * Q1 + h2C * R1 - sg * S1 - sg2 * hR * U0;
+    const float h2 = h * h;
+    Dual4 o;
+    o.sr = -h2 * c.jt_inv_dx * S1;
+    o.sz = h2 * S0;
+    o.dsr = -h2 * c.jt_inv_dx * fmaf(2.0f * r, S1, dS1);
+    o.dsz = h2 * fmaf(2.0f * r, S0, dS0);
+    return o;
+}
+
 template <int PREC>
 __device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, double da,
                                                double rho, double drho, bool gauss,
@@ -224,50 +309,8 @@ __device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, do
             o.dsz += dwkt * J0 + wkt * dJ0;
         }
     } else {
-        const float h = 0.5f * (float)K, dh = 0.5f * (float)dK;
-        const float c2 = -2.0f * (float)c.Lz * kLog2e;
-        const float ca = -(float)a * kLog2e;
-        const float cg = -kLog2e * (float)(0.25 / c.alpha);
-        const float g12 = (float)c.g12;
-        const float Lz2 = 2.0f * (float)c.Lz;
-        const float af = (float)a, daf = (float)da, rhof = (float)rho, drhof = (float)drho;
-        const float i2a = (float)(0.5 / c.alpha);
-        float sr = 0.f, sz = 0.f, dsr = 0.f, dsz = 0.f;
-        for (int n = 0; n < c.Nt; ++n) {
-            const float k = fmaf(h, glxf[n], h), dk = fmaf(dh, glxf[n], dh);
-            const float w = h * glwf[n], dw = dh * glwf[n];
-            const float E = __builtin_amdgcn_exp2f(k * c2);
-            const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
-            const float dE = -Lz2 * E * dk;
-            const float drD = -g12 * dE * rD * rD;
-            float num, dnum;
-            if (gauss) {
-                num = __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca));
-                dnum = num * (-af * dk - k * daf - k * i2a * dk);
-            } else {
-                num = __builtin_amdgcn_exp2f(k * ca) * E;
-                dnum = num * (-af * dk - k * daf - Lz2 * dk);
-            }
-            const float t = num * rD, dt = fmaf(dnum, rD, num * drD);
-            const float wk = w * k, dwk = fmaf(dw, k, w * dk);
-            const float wkt = wk * t, dwkt = fmaf(dwk, t, wk * dt);
-            float J0 = 1.f, J1 = 0.f, dJ0 = 0.f, dJ1 = 0.f;
-            if (!self) {
-                const float x = k * rhof, dx = fmaf(dk, rhof, k * drhof);
-                bessel01_tab(x, tab, c.jt_n, c.jt_inv_dx, J0, J1);
-                const float j1x = x > 1e-4f ? J1 * __builtin_amdgcn_rcpf(x) : 0.5f;
-                dJ0 = -J1 * dx;
-                dJ1 = (J0 - j1x) * dx;
-            }
-            sr = fmaf(wkt, J1, sr);
-            sz = fmaf(wkt, J0, sz);
-            dsr = fmaf(dwkt, J1, fmaf(wkt, dJ1, dsr));
-            dsz = fmaf(dwkt, J0, fmaf(wkt, dJ0, dsz));
-        }
-        o.sr = sr;
-        o.sz = sz;
-        o.dsr = dsr;
-        o.dsz = dsz;
+        return nodes_tangent_mixed((float)K, (float)dK, (float)a, (float)da, (float)rho,
+                                   (float)drho, gauss, self, c, tab);
     }
     return o;
 }

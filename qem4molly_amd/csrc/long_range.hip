@@ -481,6 +481,208 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tangent passes in the k-lane layout (same walk as k_lane_A / k_lane_B).
+// Along the direction v the per-atom terms move with
+//   d(c e^{+kz}) = -s e^{+kz} dth + c e^{+kz} k v_z,   dth = k_x v_x + k_y v_y,
+//   d(s e^{+kz}) =  c e^{+kz} dth + s e^{+kz} k v_z,
+//   d(c e^{-kz}) = -s e^{-kz} dth - c e^{-kz} k v_z,
+//   d(s e^{-kz}) =  c e^{-kz} dth - s e^{-kz} k v_z,
+// and every sum carries its tangent.  The k = 0 row has no tangent (its z
+// force is piecewise constant in z); its lane contributes nothing.
+
+// values and tangents of the four per-atom channels (without the charge)
+struct LaneTerms {
+    double c, s, Ep, Em, cp, sp, cm, sm, dcp, dsp, dcm, dsm;
+};
+
+template <int PREC>
+__device__ __forceinline__ LaneTerms lane_terms_t(const LaneK &L, double kx, double ky,
+                                                  double kabs, const double4 &at,
+                                                  const double3 &dv)
+{
+    LaneTerms T;
+    lane_terms<PREC>(L, at.x, at.y, at.z, T.c, T.s, T.Ep, T.Em);
+    T.cp = T.c * T.Ep;
+    T.sp = T.s * T.Ep;
+    T.cm = T.c * T.Em;
+    T.sm = T.s * T.Em;
+    const double dth = kx * dv.x + ky * dv.y, kvz = kabs * dv.z;
+    T.dcp = T.cp * kvz - T.sp * dth;
+    T.dsp = T.sp * kvz + T.cp * dth;
+    T.dcm = -T.cm * kvz - T.sm * dth;
+    T.dsm = T.cm * dth - T.sm * kvz;
+    return T;
+}
+
+// Pass A (tangent): chunk sums of the four channels and of their tangents.
+template <int PREC>
+__global__ __launch_bounds__(64 * kLaneWaves) void k_lane_A_t(
+    const KRec *__restrict__ kr, int64_t Ptot, const double4 *__restrict__ satom,
+    const double *__restrict__ svx, const double *__restrict__ svy,
+    const double *__restrict__ svz, int64_t n, int chk, int nchunk,
+    double4 *__restrict__ partA, double4 *__restrict__ partT)
+{
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
+    if (chunk >= nchunk) return;
+    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
+    const bool live = p < Ptot;
+    const KRec &R = kr[live ? p : Ptot - 1];
+    const LaneK L = load_lane(R, PREC == QEM_PREC_MIXED);
+    const double kx = R.kx, ky = R.ky, kabs = R.k;
+    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0, tx = 0.0, ty = 0.0, tz = 0.0, tw = 0.0;
+    for (int64_t a = a0; a < a1; ++a) {
+        const double4 at = satom[a];
+        const double3 dv = make_double3(svx[a], svy[a], svz[a]);
+        const LaneTerms T = lane_terms_t<PREC>(L, kx, ky, kabs, at, dv);
+        const double q = at.w, qz = L.k0f * (q * at.z);
+        sx = add_nc(sx, q * T.cp);
+        sy = add_nc(sy, fma(q, T.sp, qz));
+        sz = add_nc(sz, q * T.cm);
+        sw = add_nc(sw, fma(q, T.sm, qz));
+        tx = add_nc(tx, q * T.dcp);
+        ty = add_nc(ty, q * T.dsp);
+        tz = add_nc(tz, q * T.dcm);
+        tw = add_nc(tw, q * T.dsm);
+    }
+    if (live) {
+        partA[p * nchunk + chunk] = make_double4(sx, sy, sz, sw);
+        partT[p * nchunk + chunk] = make_double4(tx, ty, tz, tw);
+    }
+}
+
+// Pass B (tangent): tangent of the per-k force of each atom, summed over
+// the wave's 64 rows, into partB[group][atom][3].
+template <int PREC>
+__global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
+    const KRec *__restrict__ kr, int64_t Ptot, const double4 *__restrict__ satom,
+    const double *__restrict__ svx, const double *__restrict__ svy,
+    const double *__restrict__ svz, int64_t n, int chk, int nchunk,
+    const double4 *__restrict__ partA, const double4 *__restrict__ offA,
+    const double4 *__restrict__ totA, const double4 *__restrict__ partT,
+    const double4 *__restrict__ offT, const double4 *__restrict__ totT,
+    double *__restrict__ partB)
+{
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
+    if (chunk >= nchunk) return;
+    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
+    const bool live = p < Ptot;
+    const int64_t pl = live ? p : Ptot - 1;
+    const KRec &R = kr[pl];
+    const LaneK L = load_lane(R, PREC == QEM_PREC_MIXED);
+    const double kx = R.kx, ky = R.ky, kabs = R.k;
+    // the k = 0 row and idle lanes add nothing
+    const bool act = live && R.k0f == 0.0;
+    const double4 T = totA[pl], dT = totT[pl];
+    const double4 off = offA[pl * nchunk + chunk], offt = offT[pl * nchunk + chunk];
+    const double4 own = partA[pl * nchunk + chunk], ownt = partT[pl * nchunk + chunk];
+    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
+    double kmax = live ? kabs : 0.0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, d));
+    const bool flat = kmax * (satom[a1 - 1].z - satom[a0].z) <= 1.0;
+    double *out = partB + ((int64_t)blockIdx.y * n) * 3;
+
+    double A = off.x, B = off.y, dA = offt.x, dB = offt.y;
+    double PZ = 0.0, PW = 0.0, dPZ = 0.0, dPW = 0.0;
+    for (int64_t a = a0; a < a1; ++a) {
+        const double4 at = satom[a];
+        const double3 dv = make_double3(svx[a], svy[a], svz[a]);
+        const LaneTerms X = lane_terms_t<PREC>(L, kx, ky, kabs, at, dv);
+        const double q = at.w, qz = L.k0f * (q * at.z);
+        const double vx = q * X.cp, vy = fma(q, X.sp, qz), vz = q * X.cm, vw = fma(q, X.sm, qz);
+        const double tvx = q * X.dcp, tvy = q * X.dsp, tvz = q * X.dcm, tvw = q * X.dsm;
+        PZ = add_nc(PZ, vz);
+        PW = add_nc(PW, vw);
+        dPZ = add_nc(dPZ, tvz);
+        dPW = add_nc(dPW, tvw);
+        double C = 0.0, D = 0.0, dC = 0.0, dD = 0.0;
+        if (flat) {
+            C = off.z + (own.z - PZ);
+            D = off.w + (own.w - PW);
+            dC = offt.z + (ownt.z - dPZ);
+            dD = offt.w + (ownt.w - dPW);
+        }
+        const double c = X.c, s = X.s, Ep = X.Ep, Em = X.Em;
+        const double dth = kx * dv.x + ky * dv.y, kvz = kabs * dv.z;
+        const double dc = -s * dth, ds = c * dth, dEp = kvz * Ep, dEm = -kvz * Em;
+        // Fl_non_sort (src/Force_long.jl:5-70), own term removed
+        const double f1 = T.x - vx, f2 = T.y - vy, f3 = T.z - vz, f4 = T.w - vw;
+        const double df1 = dT.x - tvx, df2 = dT.y - tvy, df3 = dT.z - tvz, df4 = dT.w - tvw;
+        const double X1 = c * f2 - s * f1, X2 = c * f4 - s * f3;
+        const double Z1 = c * f1 + s * f2, Z2 = c * f3 + s * f4;
+        const double dX1 = dc * f2 + c * df2 - ds * f1 - s * df1;
+        const double dX2 = dc * f4 + c * df4 - ds * f3 - s * df3;
+        const double dZ1 = dc * f1 + c * df1 + ds * f2 + s * df2;
+        const double dZ2 = dc * f3 + c * df3 + ds * f4 + s * df4;
+        const double gu = L.N1 * Ep + L.N2 * Em, gw = L.N3 * Em + L.N2 * Ep;
+        const double guz = L.N1 * Ep - L.N2 * Em, gwz = L.N2 * Ep - L.N3 * Em;
+        const double dgu = L.N1 * dEp + L.N2 * dEm, dgw = L.N3 * dEm + L.N2 * dEp;
+        const double dguz = L.N1 * dEp - L.N2 * dEm, dgwz = L.N2 * dEp - L.N3 * dEm;
+        const double dgxn = dgu * X1 + gu * dX1 + dgw * X2 + gw * dX2;
+        const double dgzn = dguz * Z1 + guz * dZ1 + dgwz * Z2 + gwz * dZ2;
+        // Fl_sort (src/Force_long.jl:72-127)
+        const double dsxy = X.dsm * A + X.sm * dA - X.dcm * B - X.cm * dB + X.dsp * C +
+                            X.sp * dC - X.dcp * D - X.cp * dD;
+        const double dszz = X.dcm * A + X.cm * dA + X.dsm * B + X.sm * dB - X.dcp * C -
+                            X.cp * dC - X.dsp * D - X.sp * dD;
+        // Flz_self (src/Force_long.jl:164-184)
+        const double dself = 2.0 * kvz * (L.S1 * (Ep * Ep) + L.S2 * (Em * Em));
+        const double gx = q * (dgxn + L.hs * dsxy);
+        const double gz = q * (dgzn + L.hs * dszz + q * dself);
+        double fx = act ? L.wx * gx : 0.0;
+        double fy = act ? L.wy * gx : 0.0;
+        double fz = act ? L.w * gz : 0.0;
+        A += vx;
+        B += vy;
+        dA += tvx;
+        dB += tvy;
+        fx = wave_sum63(fx);
+        fy = wave_sum63(fy);
+        fz = wave_sum63(fz);
+        if (lane == 63) {
+            double *o = out + a * 3;
+            o[0] = fx;
+            o[1] = fy;
+            o[2] = fz;
+        }
+    }
+    if (flat) return;
+    // exact suffix walk: the sorted-sum tangent terms with C, D (above)
+    double C = off.z, D = off.w, dC = offt.z, dD = offt.w;
+    for (int64_t a = a1 - 1; a >= a0; --a) {
+        const double4 at = satom[a];
+        const double3 dv = make_double3(svx[a], svy[a], svz[a]);
+        const LaneTerms X = lane_terms_t<PREC>(L, kx, ky, kabs, at, dv);
+        const double q = at.w, qz = L.k0f * (q * at.z);
+        const double dsxy = X.dsp * C + X.sp * dC - X.dcp * D - X.cp * dD;
+        const double dszz = -X.dcp * C - X.cp * dC - X.dsp * D - X.sp * dD;
+        const double gx = q * (L.hs * dsxy);
+        const double gz = q * (L.hs * dszz);
+        double fx = act ? L.wx * gx : 0.0;
+        double fy = act ? L.wy * gx : 0.0;
+        double fz = act ? L.w * gz : 0.0;
+        C += q * X.cm;
+        D += fma(q, X.sm, qz);
+        dC += q * X.dcm;
+        dD += q * X.dsm;
+        fx = wave_sum63(fx);
+        fy = wave_sum63(fy);
+        fz = wave_sum63(fz);
+        if (lane == 63) {
+            double *o = out + a * 3;
+            o[0] += fx;
+            o[1] += fy;
+            o[2] += fz;
+        }
+    }
+}
+
 // Pass R: sum the k tiles and scatter to input order.
 __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
                          const int32_t *__restrict__ perm, int accumulate,
@@ -670,187 +872,6 @@ __global__ void k_gather_sorted3(const double *__restrict__ v, const int32_t *__
     svz[r] = v[3 * a + 2];
 }
 
-// value and tangent of the per-atom terms of one k
-template <int PREC>
-__device__ __forceinline__ void atom_terms_t(double kx, double ky, double k, double x, double y,
-                                             double z, double vx, double vy, double vz,
-                                             double q, double4 &v, double4 &dv, double &c,
-                                             double &s, double &Ep, double &Em, double &dc,
-                                             double &ds, double &dEp, double &dEm)
-{
-    atom_terms<PREC>(kx, ky, k, x, y, z, c, s, Ep, Em);
-    const double dth = kx * vx + ky * vy;
-    dc = -s * dth;
-    ds = c * dth;
-    dEp = k * vz * Ep;
-    dEm = -k * vz * Em;
-    v = make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em);
-    dv = make_double4(q * (dc * Ep + c * dEp), q * (ds * Ep + s * dEp), q * (dc * Em + c * dEm),
-                      q * (ds * Em + s * dEm));
-}
-
-template <int PREC>
-__global__ __launch_bounds__(CH) void k_long_A_t(const double4 *__restrict__ kv, int64_t Ptot,
-                                                 int kt, const double *__restrict__ sx,
-                                                 const double *__restrict__ sy,
-                                                 const double *__restrict__ sz,
-                                                 const double *__restrict__ sq,
-                                                 const double *__restrict__ svx,
-                                                 const double *__restrict__ svy,
-                                                 const double *__restrict__ svz, int64_t n,
-                                                 int nchunk, double4 *__restrict__ partA,
-                                                 double4 *__restrict__ partT)
-{
-    __shared__ double4 red[2][2][CH / 64];
-    const int m = blockIdx.x;
-    const int64_t p0 = (int64_t)blockIdx.y * kt, p1 = min(Ptot, p0 + kt);
-    const int64_t r = (int64_t)m * CH + threadIdx.x;
-    const bool live = r < n;
-    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0, z = live ? sz[r] : 0.0;
-    const double q = live ? sq[r] : 0.0;
-    const double vx = live ? svx[r] : 0.0, vy = live ? svy[r] : 0.0, vz = live ? svz[r] : 0.0;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int64_t p = p0; p < p1; ++p) {
-        const double4 kk = kv[p];
-        double4 v, dv;
-        if (kk.z == 0.0) {
-            v = make_double4(q, q * z, q, q * z);
-            dv = make_double4(0.0, 0.0, 0.0, 0.0);
-        } else {
-            double c, s, Ep, Em, dc, ds, dEp, dEm;
-            atom_terms_t<PREC>(kk.x, kk.y, kk.z, x, y, z, vx, vy, vz, q, v, dv, c, s, Ep, Em, dc,
-                               ds, dEp, dEm);
-            if (!live) v = dv = make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            v.x += __shfl_down(v.x, d);
-            v.y += __shfl_down(v.y, d);
-            v.z += __shfl_down(v.z, d);
-            v.w += __shfl_down(v.w, d);
-            dv.x += __shfl_down(dv.x, d);
-            dv.y += __shfl_down(dv.y, d);
-            dv.z += __shfl_down(dv.z, d);
-            dv.w += __shfl_down(dv.w, d);
-        }
-        const int b = (int)(p & 1);
-        if (lane == 0) {
-            red[b][0][wv] = v;
-            red[b][1][wv] = dv;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double4 t = red[b][0][0], u = red[b][1][0];
-            for (int w = 1; w < CH / 64; ++w) {
-                t = add4(t, red[b][0][w]);
-                u = add4(u, red[b][1][w]);
-            }
-            partA[p * nchunk + m] = t;
-            partT[p * nchunk + m] = u;
-        }
-    }
-}
-
-// Tangent pass B: tangent of the per-k force, summed over the tile's rows.
-template <int PREC>
-__global__ __launch_bounds__(CH) void k_long_B_t(LongConsts lc, const double4 *__restrict__ kv,
-                                                 int64_t Ptot, int kt,
-                                                 const double *__restrict__ sx,
-                                                 const double *__restrict__ sy,
-                                                 const double *__restrict__ sz,
-                                                 const double *__restrict__ sq,
-                                                 const double *__restrict__ svx,
-                                                 const double *__restrict__ svy,
-                                                 const double *__restrict__ svz, int64_t n,
-                                                 int nchunk, const double4 *__restrict__ offA,
-                                                 const double4 *__restrict__ totA,
-                                                 const double4 *__restrict__ offT,
-                                                 const double4 *__restrict__ totT,
-                                                 double *__restrict__ partB)
-{
-    __shared__ double4 wsum[2][2][CH / 64];
-    const int m = blockIdx.x, tile = blockIdx.y;
-    const int64_t p0 = (int64_t)tile * kt, p1 = min(Ptot, p0 + kt);
-    const int64_t r = (int64_t)m * CH + threadIdx.x;
-    const bool live = r < n;
-    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0, z = live ? sz[r] : 0.0;
-    const double q = live ? sq[r] : 0.0;
-    const double vx = live ? svx[r] : 0.0, vy = live ? svy[r] : 0.0, vz = live ? svz[r] : 0.0;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int64_t p = p0; p < p1; ++p) {
-        const double4 kk = kv[p];
-        if (kk.z == 0.0) continue;  // uniform per row: no barrier is skipped unevenly
-        double c, s, Ep, Em, dc, ds, dEp, dEm;
-        double4 v, dv;
-        atom_terms_t<PREC>(kk.x, kk.y, kk.z, x, y, z, vx, vy, vz, q, v, dv, c, s, Ep, Em, dc, ds,
-                           dEp, dEm);
-        if (!live) v = dv = make_double4(0.0, 0.0, 0.0, 0.0);
-        const int b = (int)(p & 1);
-        double4 ex = wave_scan_ex(v, lane, wv, wsum[b][0]);
-        double4 et = wave_scan_ex(dv, lane, wv, wsum[b][1]);
-        __syncthreads();
-        ex = add_wave_offsets(ex, wv, wsum[b][0]);
-        et = add_wave_offsets(et, wv, wsum[b][1]);
-        const double4 off = offA[p * nchunk + m], offt = offT[p * nchunk + m];
-        const double A = off.x + ex.x, B = off.y + ex.y, C = off.z + ex.z, D = off.w + ex.w;
-        const double dA = offt.x + et.x, dB = offt.y + et.y, dC = offt.z + et.z,
-                     dD = offt.w + et.w;
-        const double k = kk.z;
-        double gx = 0.0, gz = 0.0;
-        if (lc.parts & QEM_LONG_NONSORT) {
-            const double4 T = totA[p], dT = totT[p];
-            const double f1 = T.x - v.x, f2 = T.y - v.y, f3 = T.z - v.z, f4 = T.w - v.w;
-            const double df1 = dT.x - dv.x, df2 = dT.y - dv.y, df3 = dT.z - dv.z,
-                         df4 = dT.w - dv.w;
-            const double e2 = exp(-2.0 * k * lc.Lz);
-            const double g = 0.5 / (lc.g12 * e2 - 1.0);
-            const double u = lc.g2 * e2 * Ep + lc.g12 * e2 * Em;
-            const double du = lc.g2 * e2 * dEp + lc.g12 * e2 * dEm;
-            const double w_ = lc.g1 * Em + lc.g12 * e2 * Ep;
-            const double dw_ = lc.g1 * dEm + lc.g12 * e2 * dEp;
-            const double uz = lc.g2 * e2 * Ep - lc.g12 * e2 * Em;
-            const double duz = lc.g2 * e2 * dEp - lc.g12 * e2 * dEm;
-            const double wz = -lc.g1 * Em + lc.g12 * e2 * Ep;
-            const double dwz = -lc.g1 * dEm + lc.g12 * e2 * dEp;
-            const double Xa = u * f2 + w_ * f4, Xb = u * f1 + w_ * f3;
-            const double dXa = du * f2 + u * df2 + dw_ * f4 + w_ * df4;
-            const double dXb = du * f1 + u * df1 + dw_ * f3 + w_ * df3;
-            gx += q * g * (dc * Xa + c * dXa - ds * Xb - s * dXb);
-            const double Za = uz * f1 + wz * f3, Zb = uz * f2 + wz * f4;
-            const double dZa = duz * f1 + uz * df1 + dwz * f3 + wz * df3;
-            const double dZb = duz * f2 + uz * df2 + dwz * f4 + wz * df4;
-            gz += q * g * (dc * Za + c * dZa + ds * Zb + s * dZb);
-        }
-        if (lc.parts & QEM_LONG_SORT) {
-            const double dsxy = ds * Em * A + s * dEm * A + s * Em * dA
-                              - (dc * Em * B + c * dEm * B + c * Em * dB)
-                              + ds * Ep * C + s * dEp * C + s * Ep * dC
-                              - (dc * Ep * D + c * dEp * D + c * Ep * dD);
-            gx += 0.5 * q * dsxy;
-            const double dszz = dc * Em * A + c * dEm * A + c * Em * dA
-                              + ds * Em * B + s * dEm * B + s * Em * dB
-                              - (dc * Ep * C + c * dEp * C + c * Ep * dC)
-                              - (ds * Ep * D + s * dEp * D + s * Ep * dD);
-            gz += 0.5 * q * dszz;
-        }
-        if (lc.parts & QEM_LONG_SELF) {
-            const double e2 = exp(-2.0 * k * lc.Lz);
-            gz += q * q * (lc.g2 * e2 * 2.0 * Ep * dEp - lc.g1 * 2.0 * Em * dEm) /
-                  (2.0 * (lc.g12 * e2 - 1.0));
-        }
-        fx += kk.w * (kk.x / k) * gx;
-        fy += kk.w * (kk.y / k) * gx;
-        fz += kk.w * gz;
-    }
-    if (live) {
-        double *o = partB + ((int64_t)tile * n + r) * 3;
-        o[0] = fx;
-        o[1] = fy;
-        o[2] = fz;
-    }
-}
-
 // sum the k tiles of a width-w per-atom quantity, scale, scatter to input order
 __global__ void k_long_Rw(const double *__restrict__ partB, int ntile, int64_t n, int width,
                           const int32_t *__restrict__ perm, int accumulate, double scale,
@@ -917,6 +938,23 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const doub
     return hipSuccess;
 }
 
+namespace {
+// k-lane passes: k rows in groups of 64 (one per lane); chunks of chk atoms
+// sized so that the grid holds about 8192 waves
+struct LaneGrid {
+    int ngroup, chk, nchunk;
+};
+LaneGrid lane_grid(int64_t n, int64_t Ptot)
+{
+    LaneGrid g;
+    g.ngroup = (int)((Ptot + 63) / 64);
+    const int64_t want = (n * g.ngroup + 8191) / 8192;
+    g.chk = (int)std::min<int64_t>(256, std::max<int64_t>(16, want));
+    g.nchunk = (int)((n + g.chk - 1) / g.chk);
+    return g;
+}
+}  // namespace
+
 hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                        double *d_f, int accumulate, hipStream_t s, std::string &err)
 {
@@ -929,12 +967,8 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Pto
         }
         return hipSuccess;
     }
-    // k rows in groups of 64 (one per lane); chunks sized so that the grid
-    // holds about 8192 waves
-    const int ngroup = (int)((Ptot + 63) / 64);
-    const int64_t want = (n * ngroup + 8191) / 8192;
-    const int chk = (int)std::min<int64_t>(256, std::max<int64_t>(16, want));
-    const int nchunk = (int)((n + chk - 1) / chk);
+    const LaneGrid lg = lane_grid(n, Ptot);
+    const int ngroup = lg.ngroup, chk = lg.chk, nchunk = lg.nchunk;
     LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
     LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * nchunk));
     LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * nchunk));
@@ -1045,55 +1079,56 @@ hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int
                                hipStream_t s, std::string &err)
 {
     if (n <= 0 || Ptot <= 0) return hipSuccess;
-    const Tiling t = tiling(n, Ptot);
+    const LaneGrid lg = lane_grid(n, Ptot);
     const size_t nd = sizeof(double) * (size_t)n;
     LR_CHECK(ensure(lw.svx, nd));
     LR_CHECK(ensure(lw.svy, nd));
     LR_CHECK(ensure(lw.svz, nd));
-    LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * t.nchunk));
-    LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
+    const size_t pc = sizeof(double4) * (size_t)Ptot * lg.nchunk;
+    LR_CHECK(ensure(lw.partA, pc));
+    LR_CHECK(ensure(lw.offA, pc));
     LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
-    LR_CHECK(ensure(lw.partT, sizeof(double4) * (size_t)Ptot * t.nchunk));
-    LR_CHECK(ensure(lw.offT, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    LR_CHECK(ensure(lw.partT, pc));
+    LR_CHECK(ensure(lw.offT, pc));
     LR_CHECK(ensure(lw.totT, sizeof(double4) * (size_t)Ptot));
-    LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)t.ntile * (size_t)n));
+    LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)lg.ngroup * (size_t)n));
     k_gather_sorted3<<<nblk(n, 256), 256, 0, s>>>(d_v, lw.perm.as<int32_t>(), n, lw.svx.as<double>(),
                                                   lw.svy.as<double>(), lw.svz.as<double>());
-    const double4 *kv = lw.kv.as<double4>();
-    dim3 g(t.nchunk, t.ntile);
-    if (c.precision == QEM_PREC_MIXED)
-        k_long_A_t<QEM_PREC_MIXED><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
-                                                    lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
-                                                    lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
-                                                    lw.partA.as<double4>(), lw.partT.as<double4>());
-    else
-        k_long_A_t<QEM_PREC_FP64><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
-                                                   lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
-                                                   lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
-                                                   lw.partA.as<double4>(), lw.partT.as<double4>());
+    KRec *kr = lw.kr.as<KRec>();
+    k_make_krec<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot, kr);
     LR_CHECK(hipGetLastError());
-    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, t.nchunk,
+    const dim3 g((unsigned)((lg.nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)lg.ngroup);
+    const double4 *sa = lw.satom.as<double4>();
+    if (c.precision == QEM_PREC_MIXED)
+        k_lane_A_t<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, lw.svx.as<double>(), lw.svy.as<double>(), lw.svz.as<double>(), n,
+            lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.partT.as<double4>());
+    else
+        k_lane_A_t<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, lw.svx.as<double>(), lw.svy.as<double>(), lw.svz.as<double>(), n,
+            lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.partT.as<double4>());
+    LR_CHECK(hipGetLastError());
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, lg.nchunk,
                                            lw.offA.as<double4>(), lw.totA.as<double4>());
-    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partT.as<double4>(), Ptot, t.nchunk,
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partT.as<double4>(), Ptot, lg.nchunk,
                                            lw.offT.as<double4>(), lw.totT.as<double4>());
     LR_CHECK(hipGetLastError());
     if (c.precision == QEM_PREC_MIXED)
-        k_long_B_t<QEM_PREC_MIXED><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
-                                                    lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
-                                                    lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
-                                                    lw.offA.as<double4>(), lw.totA.as<double4>(),
-                                                    lw.offT.as<double4>(), lw.totT.as<double4>(),
-                                                    lw.partB.as<double>());
+        k_lane_B_t<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, lw.svx.as<double>(), lw.svy.as<double>(), lw.svz.as<double>(), n,
+            lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
+            lw.totA.as<double4>(), lw.partT.as<double4>(), lw.offT.as<double4>(),
+            lw.totT.as<double4>(), lw.partB.as<double>());
     else
-        k_long_B_t<QEM_PREC_FP64><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
-                                                   lw.sz.as<double>(), lw.sq.as<double>(), lw.svx.as<double>(),
-                                                   lw.svy.as<double>(), lw.svz.as<double>(), n, t.nchunk,
-                                                   lw.offA.as<double4>(), lw.totA.as<double4>(),
-                                                   lw.offT.as<double4>(), lw.totT.as<double4>(),
-                                                   lw.partB.as<double>());
+        k_lane_B_t<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, lw.svx.as<double>(), lw.svy.as<double>(), lw.svz.as<double>(), n,
+            lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
+            lw.totA.as<double4>(), lw.partT.as<double4>(), lw.offT.as<double4>(),
+            lw.totT.as<double4>(), lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
-    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), t.ntile, n, 3, lw.perm.as<int32_t>(),
-                                           accumulate, scale, d_out);
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 3,
+                                           lw.perm.as<int32_t>(), accumulate, scale, d_out);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }

@@ -291,6 +291,7 @@ ShortConsts short_consts(const qem_ctx *ctx, int single)
     c.jt_n = ctx->sw.jt_n;
     c.jt_dx = ctx->sw.jt_dx;
     c.jt_inv_dx = ctx->sw.jt_dx > 0 ? 1.0f / ctx->sw.jt_dx : 0.0f;
+    c.ncon = ctx->sw.ncon.as<float4>();
     return c;
 }
 

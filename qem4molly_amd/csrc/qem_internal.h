@@ -35,6 +35,7 @@ struct ShortConsts {
     int32_t single;      // point-charge part only (F_short_ij single=true)
     int32_t jt_n;        // Bessel table intervals (mixed precision)
     float jt_dx, jt_inv_dx;  // interval width and its inverse
+    const float4 *ncon;  // device node constants (mixed precision), or null
 };
 
 // Gauss-Legendre rule on [-1,1] (ascending) plus its FP32 copy.
@@ -44,9 +45,9 @@ struct GLRule {
 };
 GLRule make_gl_rule(int n);
 
-// Per interval [m dx, (m+1) dx): coefficients of a degree-4 polynomial in
-// t in [0, 1) fitted to J0(m dx + t dx) at Chebyshev points (least squares),
-// 8 floats per interval (c0..c4, padding).
+// Per interval [m dx, (m+1) dx): coefficients of a cubic in t in [0, 1)
+// fitted to J0(m dx + t dx) at Chebyshev points (least squares), 4 floats
+// per interval (c0..c3).
 std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals);
 
 // ---- neighbour list (neighbor.hip) ------------------------------------
@@ -91,8 +92,9 @@ hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs,
 struct ShortWork {
     DevBuf glx, glw;      // double nodes / weights
     DevBuf glxf, glwf;    // float nodes / weights
-    DevBuf ncon;          // float4 per node: (1 + x_n, (1 + x_n)^2, w_n (1 + x_n), 0)
-    DevBuf jtab;          // float4 pairs per interval
+    DevBuf ncon;          // float4 per node: (1 + x_n, (1 + x_n)^2, w_n (1 + x_n),
+                          //                   log2(w_n (1 + x_n)))
+    DevBuf jtab;          // one float4 (cubic coefficients) per interval
     int jt_n = 0;
     float jt_dx = 0.f;
     DevBuf fpair;         // double4 per pair

@@ -332,7 +332,13 @@ __global__ __launch_bounds__(256) void k_gather_pairs(
     const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
     const int32_t *__restrict__ j_idx, double *__restrict__ f)
 {
-    const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    // XCD-aware block order: the hardware deals blocks to the 8 XCDs round
+    // robin, so block b runs on XCD b % 8; give each XCD a contiguous range
+    // of atoms, whose pairs' results then share that XCD's L2 (gridDim.x is
+    // a multiple of 8)
+    const int64_t per_xcd = gridDim.x / 8;
+    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
     const bool live = a < n;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -442,7 +448,8 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWo
     hipError_t e = launch_self_z(c, precision, ws, n, d_xyz, d_q, 0, d_f, s);
     if (e != hipSuccess) return e;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
-    k_gather_pairs<<<nblk(16 * n, 256), 256, 0, s>>>(
+    const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
+    k_gather_pairs<<<nb, 256, 0, s>>>(
         n, reinterpret_cast<const double4 *>(d_fpair), nl.i_start.as<int32_t>(),
         nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
         nl.j_idx.as<int32_t>(), d_f);
