@@ -17,6 +17,15 @@
  *   - CellListMap.neighborlist     -> O(N^2) minimum-image enumeration
  *   - sortperm                     -> stable sort by z, ties by index
  *   - StatsBase.sample             -> not restated: k indices are inputs
+ *
+ * Every function is pinned on the CPU against checks that do not reuse its
+ * formulas (tests/test_oracle_pins.py, tests/test_oracle_energy_pins.py):
+ * explicit image-charge sums (point and in-plane-gaussian-smeared, by 2-D
+ * quadrature), the image-series per-k energy and its gradient, O(N^2)
+ * direct sums, closed-form K-set enumeration, 40-digit Gauss-Legendre rules
+ * and finite differences.  The energy functions (qo_pair_potential,
+ * qo_self_potential, qo_energy) have no reference counterpart: they are
+ * the functional whose gradient is the corrected-semantics force.
  */
 #ifndef QEM_ORACLE_H
 #define QEM_ORACLE_H

@@ -10,6 +10,10 @@ dielectric slab rather than from the reference's formulas.
                            form of the exp(-k^2/(4 alpha)) filter
   k_energy(...)            per-k energy of the slab from the image-series
                            Green's function (no structure factors, no sorting)
+  smeared_potential(...)   potential of the same in-plane gaussian charge
+  short_pair_potential_brute / self_potential_brute
+                           short-range energy terms: every image's point
+                           potential minus its smeared potential
 """
 from __future__ import annotations
 
@@ -74,6 +78,46 @@ def short_pair_brute(ri, rj, alpha, g1, g2, Lz, layers=12):
         r = np.array([ri[0] - rj[0], ri[1] - rj[1], ri[2] - zimg])
         F += w * (r / np.linalg.norm(r) ** 3 - smeared_force(r, alpha)) / (4.0 * np.pi)
     return F
+
+
+def smeared_potential(r, alpha):
+    """Potential at r = (x, y, z) of a unit charge spread over the plane
+    z = 0 with density (alpha/pi) exp(-alpha s^2) (no 1/(4 pi) factor)."""
+    rx, ry, rz = r
+
+    def f(t, u):
+        sx, sy = u * np.cos(t), u * np.sin(t)
+        d2 = (rx - sx) ** 2 + (ry - sy) ** 2 + rz * rz
+        return (alpha / np.pi) * np.exp(-alpha * u * u) * u / np.sqrt(d2)
+    v, _ = integrate.dblquad(f, 0.0, 8.0 / np.sqrt(alpha), 0.0, 2.0 * np.pi,
+                             epsabs=1e-13, epsrel=1e-11)
+    return v
+
+
+def short_pair_potential_brute(ri, rj, alpha, g1, g2, Lz, layers=12):
+    """Short-range pair energy of unit charges i, j: every image of j (the
+    charge itself included) contributes point minus smeared potential at i,
+    over 4 pi (eps_0 = 1)."""
+    u = 0.0
+    for w, zimg in images_of(rj[2], g1, g2, Lz, layers):
+        if abs(w) < 1e-15:
+            continue
+        r = np.array([ri[0] - rj[0], ri[1] - rj[1], ri[2] - zimg])
+        u += w * (1.0 / np.linalg.norm(r) - smeared_potential(r, alpha)) / (4.0 * np.pi)
+    return u
+
+
+def self_potential_brute(ri, alpha, g1, g2, Lz, layers=12):
+    """Short-range self energy of a unit charge with its own images (the
+    charge itself excluded): 1/2 sum of point minus smeared potentials."""
+    s = 0.0
+    lay = images_of(ri[2], g1, g2, Lz, layers)
+    for w, zimg in lay[1:]:          # lay[0] is the charge itself
+        if abs(w) < 1e-15:
+            continue
+        r = np.array([0.0, 0.0, ri[2] - zimg])
+        s += w * (1.0 / abs(r[2]) - smeared_potential(r, alpha)) / (4.0 * np.pi)
+    return 0.5 * s
 
 
 def k_energy(X, q, k3, g1, g2, Lz):
