@@ -235,6 +235,20 @@ def traffic_from_profiles(workload):
         return None
 
 
+def valu_issue_from_profiles(workload, kernel):
+    """Measured VALU issue fraction of a kernel (rocprofv3 SQ_INSTS_VALU over
+    GRBM_GUI_ACTIVE, committed under profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_issue_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return float(d["kernels"][kernel]["issue_frac"])
+    except Exception:
+        return None
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,12 +334,14 @@ def main(argv=None):
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic_from_profiles(cfg.name),
+            "valu_issue_frac": valu_issue_from_profiles(cfg.name, "k_pair_list<mixed>"),
             "kernel": "k_pair_list<mixed>", "avg_ms": pair_avg_ms,
             "flop_per_launch": fl,
         },
         "roofline_fp64": {
             "bound": "valu", "achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved64 / PEAK_FP64_TFLOPS,
+            "valu_issue_frac": valu_issue_from_profiles(cfg.name, "k_pair_list<fp64>"),
             "kernel": "k_pair_list<fp64>", "avg_ms": p64_avg,
         },
     }
