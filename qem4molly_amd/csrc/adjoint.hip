@@ -58,33 +58,41 @@ __device__ __forceinline__ void nodes_energy(double K, double a, double rho, int
         v = av;
         d = ad;
     } else {
+        // node constants as in the force kernel, with u.z = log2 w_n (the
+        // energy sums carry w, not w k): every exponent is one operation
+        // with a wave-uniform constant, e = -2 Lz k log2(e) = (h c2) u.x
+        const float4 *__restrict__ ncon = c.ncon;
         const float h = 0.5f * (float)K;
         const float c2 = -2.0f * (float)c.Lz * kLog2e;
-        const float ca = -(float)a * kLog2e;
         const float cg = -kLog2e * (float)(0.25 / c.alpha);
         const float g12 = (float)c.g12;
-        const float rhof = (float)rho;
+        const float hE = h * c2;
+        const float rA = kind == 2 ? 1.0f : (float)a * (float)(0.5 / c.Lz);
+        const float rP = 1.0f + (float)a * (float)(0.5 / c.Lz);
+        const float rG = cg / (c2 * c2);
+        const float hR = SELF ? 0.0f
+                              : fminf(h * (float)rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
         float av = 0.f, ad = 0.f;
         for (int n = 0; n < c.Nt; ++n) {
-            const float k = fmaf(h, glxf[n], h);
-            const float w = h * glwf[n];
-            const float E = __builtin_amdgcn_exp2f(k * c2);
+            const float4 u = ncon[n];
+            const float e = hE * u.x;
+            const float E = __builtin_amdgcn_exp2f(e);
+            const float num = kind == 0 ? __builtin_amdgcn_exp2f(fmaf(e, rP, u.z))
+                                        : __builtin_amdgcn_exp2f(fmaf(e, fmaf(e, rG, rA), u.z));
             const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
-            float num, fd;
-            if (kind == 0) { num = __builtin_amdgcn_exp2f(k * ca) * E; fd = rD; }
-            else if (kind == 1) { num = __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca)); fd = E * rD; }
-            else { num = __builtin_amdgcn_exp2f(k * k * cg) * E; fd = rD; }
-            float J = 1.f;
+            float wt = num * rD;
             if (!SELF) {
-                float J1;
-                bessel01_tab(k * rhof, tab, c.jt_n, c.jt_inv_dx, J, J1);
+                const float xb = hR * u.x;
+                const float tt = __builtin_amdgcn_fractf(xb);
+                const float4 cc = tab[(int)xb];
+                const float p = fmaf(fmaf(fmaf(cc.w, tt, cc.z), tt, cc.y), tt, cc.x);
+                wt *= p;
             }
-            const float wt = w * num * rD * J;
             av += wt;
-            ad = fmaf(wt, fd, ad);
+            ad = fmaf(wt, kind == 1 ? E * rD : rD, ad);
         }
-        v = av;
-        d = ad;
+        v = h * av;
+        d = h * ad;
     }
 }
 
@@ -179,7 +187,7 @@ struct Dual4 {
 };
 
 // Mixed-precision value and tangent node sums.  With the node constants
-// u = (1 + x_n, (1 + x_n)^2, w_n (1 + x_n), log2(w_n (1 + x_n))) the node is
+// u = (1 + x_n, (1 + x_n)^2, log2 w_n, log2(w_n (1 + x_n))) the node is
 // k = h u.x (h = K/2) and the summand t J is taken as in the force kernel
 // (t = w k e^{..}/D / h^2, J0 and J1 = -J0' from the cubic table).  Along the
 // tangent the nodes move with K (dk = r k, r = dK/K), the image distance

@@ -1,24 +1,27 @@
 // QEM long range on gfx950: random-batch k-space sum with z-sorted prefix
-// sums and the k = 0 term.
+// sums and the k = 0 term; energy and tangent passes for the adjoint.
 //
 // Reference: Molly.forces(::QEM_long) (src/QEM_long.jl:32-44: sortperm of z,
 // then F_long), F_long (src/Force_long.jl:201-242, random-batch branch),
 // F_l_total (186-199), Fl_non_sort (5-70), Fl_sort (72-127), Flz_sort_k0
 // (130-161), Flz_self (164-184).
 //
-// Layout and passes (atoms in ascending-z order, chunks of 256 = one block):
+// Passes (atoms in ascending-z order):
 //   sort   stable radix sort of z with the atom index as payload; gather
-//          x, y, z, q into z-ordered SoA arrays.
-//   A      per (k, chunk): the four chunk sums q cos e^{+kz}, q sin e^{+kz},
-//          q cos e^{-kz}, q sin e^{-kz} (block reduction, fixed order).
+//          (x, y, z, q) into z order.
+//   krec   per k row: derived constants (scaled k, weights, the non-sort and
+//          self factors with their exponentials and divisions).
+//   A      per (chunk, group of 64 k rows; one row per lane): the four chunk
+//          sums q cos e^{+kz}, q sin e^{+kz}, q cos e^{-kz}, q sin e^{-kz},
+//          accumulated along the chunk in ascending z.
 //   S      per k: exclusive prefix over chunks of the e^{+kz} pair, exclusive
 //          suffix of the e^{-kz} pair (accumulated from the top, so no
 //          subtraction from a total), and the four structure factors.
-//   B      per (chunk, k tile): recompute the per-atom terms, in-block
-//          prefix/suffix scans plus chunk offsets give A, B, C, D of Fl_sort;
-//          the structure factors minus the atom's own term give Fl_non_sort;
-//          Flz_self; weighted and summed over the tile's k in registers.
-//   R      per atom: sum over k tiles, scatter back to input order.
+//   B      per (chunk, group): walk the chunk again; running sums plus chunk
+//          offsets give A, B, C, D of Fl_sort; the structure factors minus
+//          the atom's own term give Fl_non_sort; Flz_self; each atom's force
+//          is summed over the group's rows (fixed DPP order).
+//   R      per atom: sum over groups, scatter back to input order.
 // The k = 0 term rides along as an extra row with (q, q z, q, q z).
 // Nothing of size N x P is stored.
 #include <hipcub/hipcub.hpp>
@@ -28,7 +31,7 @@
 namespace qem {
 namespace {
 
-constexpr int CH = 256;  // atoms per chunk (= block size)
+constexpr int CH = 256;  // threads of the chunk-scan block (k_long_S)
 constexpr double kInv2Pi = 0.15915494309189535;
 constexpr double kLog2eD = 1.4426950408889634;
 
@@ -51,101 +54,12 @@ __global__ void k_extract_z(const double *__restrict__ xyz, int64_t n, double *_
 
 __global__ void k_gather_sorted(const double *__restrict__ xyz, const double *__restrict__ q,
                                 const int32_t *__restrict__ perm, int64_t n,
-                                double *__restrict__ sx, double *__restrict__ sy,
-                                double *__restrict__ sz, double *__restrict__ sq,
                                 double4 *__restrict__ satom)
 {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int64_t a = perm[r];
-    const double x = xyz[3 * a], y = xyz[3 * a + 1], z = xyz[3 * a + 2], qa = q[a];
-    sx[r] = x;
-    sy[r] = y;
-    sz[r] = z;
-    sq[r] = qa;
-    satom[r] = make_double4(x, y, z, qa);
-}
-
-// Per-atom terms of one k: cos, sin of k.rho and e^{+kz}, e^{-kz}.
-template <int PREC>
-__device__ __forceinline__ void atom_terms(double kx, double ky, double k, double x, double y,
-                                           double z, double &c, double &s, double &Ep,
-                                           double &Em)
-{
-    const double th = kx * x + ky * y;
-    if (PREC == QEM_PREC_FP64) {
-        sincos(th, &s, &c);
-        Ep = exp(k * z);
-        Em = exp(-k * z);
-    } else {
-        // phase reduced in FP64 to [-1/2, 1/2] turns, then FP32 sin/cos;
-        // e^{+-kz} = 2^{+-(n+f)} with the fraction in FP32, exponent exact
-        double r = th * kInv2Pi;
-        r -= rint(r);
-        float sf, cf;
-        sincospif((float)(2.0 * r), &sf, &cf);
-        s = sf;
-        c = cf;
-        const double tz = k * z * kLog2eD;
-        const double nf = floor(tz);
-        const float f = (float)(tz - nf);
-        const int ni = (int)nf;
-        Ep = ldexp((double)__builtin_amdgcn_exp2f(f), ni);
-        Em = ldexp((double)__builtin_amdgcn_exp2f(-f), -ni);
-    }
-}
-
-__device__ __forceinline__ double4 add4(double4 a, double4 b)
-{
-    return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-}
-
-// Pass A: chunk sums for the k rows [p0, p1).
-template <int PREC>
-__global__ __launch_bounds__(CH) void k_long_A(const double4 *__restrict__ kv, int64_t Ptot,
-                                               int kt, const double *__restrict__ sx,
-                                               const double *__restrict__ sy,
-                                               const double *__restrict__ sz,
-                                               const double *__restrict__ sq, int64_t n,
-                                               int nchunk, double4 *__restrict__ partA)
-{
-    __shared__ double4 red[2][CH / 64];
-    const int m = blockIdx.x;
-    const int64_t p0 = (int64_t)blockIdx.y * kt;
-    const int64_t p1 = min(Ptot, p0 + kt);
-    const int64_t r = (int64_t)m * CH + threadIdx.x;
-    const bool live = r < n;
-    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0;
-    const double z = live ? sz[r] : 0.0, q = live ? sq[r] : 0.0;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int64_t p = p0; p < p1; ++p) {
-        const double4 kk = kv[p];
-        double4 v;
-        if (kk.z == 0.0) {
-            v = make_double4(q, q * z, q, q * z);
-        } else {
-            double c, s, Ep, Em;
-            atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
-            v = make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em);
-            if (!live) v = make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            v.x += __shfl_down(v.x, d);
-            v.y += __shfl_down(v.y, d);
-            v.z += __shfl_down(v.z, d);
-            v.w += __shfl_down(v.w, d);
-        }
-        const int b = (int)(p & 1);
-        if (lane == 0) red[b][wv] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double4 t = red[b][0];
-#pragma unroll
-            for (int w = 1; w < CH / 64; ++w) t = add4(t, red[b][w]);
-            partA[p * nchunk + m] = t;
-        }
-    }
+    satom[r] = make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], q[a]);
 }
 
 // Pass S: per k row, chunk offsets and totals.  One block per row; thread t
@@ -291,7 +205,8 @@ __device__ __forceinline__ LaneK load_lane(const KRec &r, bool mixed)
 }
 
 // cos, sin of k.rho and e^{+kz}, e^{-kz} from the lane's row constants
-// (mixed: turns and log2 scaling precomputed; same arithmetic as atom_terms)
+// (mixed: phase in turns reduced in FP64, FP32 sin/cos; e^{+-kz} = 2^{+-(n+f)} with
+// the fraction f in FP32 and the integer n in the FP64 exponent, so no overflow)
 template <int PREC>
 __device__ __forceinline__ void lane_terms(const LaneK &L, double x, double y, double z,
                                            double &c, double &s, double &Ep, double &Em)
@@ -683,6 +598,96 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
     }
 }
 
+// Energy pass B (k lanes): per atom the potential phi_i = dE/dq_i of the
+// wave's 64 k rows into partB[group][atom].  For k != 0,
+//   phi_k(i) = w/(2k) [c e^{-kz} (A + P1) + s e^{-kz} (B + P2)
+//                      + c e^{+kz} (C + P3) + s e^{+kz} (D + P4)]
+// with the prefix/suffix sums A..D (the direct pair term) and P1..P4 the
+// per-k derivative of the image term I_k = [g1 |S-|^2 + g2 e2 |S+|^2
+// + 2 g1 g2 e2 Re(S+ conj S-)] / (2 (1 - g1 g2 e2)) with respect to q_i,
+// divided by the atom's own terms; for k = 0,
+//   phi_0(i) = -w0 [z_i (Q_below - Q_above) - (QZ_below - QZ_above)].
+template <int PREC>
+__global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_energy(
+    LongConsts lc, const KRec *__restrict__ kr, int64_t Ptot, const double4 *__restrict__ satom,
+    int64_t n, int chk, int nchunk, const double4 *__restrict__ partA,
+    const double4 *__restrict__ offA, const double4 *__restrict__ totA,
+    double *__restrict__ partB)
+{
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
+    if (chunk >= nchunk) return;
+    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
+    const bool live = p < Ptot;
+    const int64_t pl = live ? p : Ptot - 1;
+    const KRec &R = kr[pl];
+    const LaneK L = load_lane(R, PREC == QEM_PREC_MIXED);
+    const bool k0 = R.k0f != 0.0;
+    const double4 T = totA[pl];
+    const double4 off = offA[pl * nchunk + chunk];
+    const double4 own = partA[pl * nchunk + chunk];
+    // per-row factors: w/(2k) and the image-term derivative coefficients
+    double we = 0.0, P1 = 0.0, P2 = 0.0, P3 = 0.0, P4 = 0.0;
+    if (!k0) {
+        const double e2 = exp(-2.0 * R.k * lc.Lz);
+        const double F = 1.0 / (1.0 - lc.g12 * e2);   // 2 / (2 (1 - g12 e2))
+        we = R.w / (2.0 * R.k);
+        P1 = F * (lc.g1 * T.z + lc.g12 * e2 * T.x);
+        P2 = F * (lc.g1 * T.w + lc.g12 * e2 * T.y);
+        P3 = F * e2 * (lc.g2 * T.x + lc.g12 * T.z);
+        P4 = F * e2 * (lc.g2 * T.y + lc.g12 * T.w);
+    }
+    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
+    double kmax = live ? R.k : 0.0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, d));
+    const bool flat = kmax * (satom[a1 - 1].z - satom[a0].z) <= 1.0;
+    double *out = partB + (int64_t)blockIdx.y * n;
+
+    double A = off.x, B = off.y, PZ = 0.0, PW = 0.0;
+    for (int64_t a = a0; a < a1; ++a) {
+        const double4 at = satom[a];
+        const double q = at.w;
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+        const double qz = L.k0f * (q * at.z);
+        const double vx = q * cp, vy = fma(q, sp, qz), vz = q * cm, vw = fma(q, sm, qz);
+        PZ = add_nc(PZ, vz);
+        PW = add_nc(PW, vw);
+        double C = 0.0, D = 0.0;
+        if (flat) {
+            C = off.z + (own.z - PZ);
+            D = off.w + (own.w - PW);
+        }
+        double ph = k0 ? -L.w * (at.z * (A - C) - (B - D))
+                       : we * (cm * (A + P1) + sm * (B + P2) + cp * (C + P3) + sp * (D + P4));
+        if (!live) ph = 0.0;
+        A += vx;
+        B += vy;
+        ph = wave_sum63(ph);
+        if (lane == 63) out[a] = ph;
+    }
+    if (flat) return;
+    // exact suffix walk: the terms with C, D (above the atom)
+    double C = off.z, D = off.w;
+    for (int64_t a = a1 - 1; a >= a0; --a) {
+        const double4 at = satom[a];
+        const double q = at.w;
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+        const double qz = L.k0f * (q * at.z);
+        double ph = k0 ? L.w * (at.z * C - D) : we * (cp * C + sp * D);
+        if (!live) ph = 0.0;
+        C += q * cm;
+        D += fma(q, sm, qz);
+        ph = wave_sum63(ph);
+        if (lane == 63) out[a] += ph;
+    }
+}
+
 // Pass R: sum the k tiles and scatter to input order.
 __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
                          const int32_t *__restrict__ perm, int accumulate,
@@ -710,130 +715,7 @@ __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint passes (no counterpart in the reference).
-//
-// Energy of one k (weight w_k including 1/(Lx Ly eps_0)):
-//   E_k = w_k/(2k) [ sum_{i<j} q_i q_j cos(k.rho_ij) e^{-k|z_ij|} + I_k ],
-//   I_k = [g1 |S-|^2 + g2 e2 |S+|^2 + 2 g1 g2 e2 Re(S+ conj S-)] / (2 (1 - g1 g2 e2))
-// with S+ = sum q e^{i k.rho} e^{kz} (= T.x + i T.y), S- likewise with
-// e^{-kz}, e2 = e^{-2kLz}.  phi_i = dE/dq_i = w_k/(2k) [direct_i + dI/dq_i]
-// with direct_i = c (e^- A + e^+ C) + s (e^- B + e^+ D) from the same
-// prefix/suffix sums as the force.  k = 0: phi_i = -1/(2 Lx Ly eps_0)
-// [z_i (Q_below - Q_above) - (QZ_below - QZ_above)].  E = 1/2 sum q_i phi_i.
-//
-// Tangents: the per-atom terms and all sums carry a first-order tangent
-// along v (8 channels per scan); the k = 0 force is piecewise constant in z
-// and has none.
-
-// in-wave exclusive prefix (x, y) / exclusive suffix (z, w); lane 63 / lane 0
-// leave the wave totals in wsum[wv]
-__device__ __forceinline__ double4 wave_scan_ex(double4 v, int lane, int wv, double4 *wsum)
-{
-    double4 inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double ux = __shfl_up(inc.x, d), uy = __shfl_up(inc.y, d);
-        const double dz = __shfl_down(inc.z, d), dw = __shfl_down(inc.w, d);
-        if (lane >= d) {
-            inc.x += ux;
-            inc.y += uy;
-        }
-        if (lane + d < 64) {
-            inc.z += dz;
-            inc.w += dw;
-        }
-    }
-    if (lane == 63) {
-        wsum[wv].x = inc.x;
-        wsum[wv].y = inc.y;
-    }
-    if (lane == 0) {
-        wsum[wv].z = inc.z;
-        wsum[wv].w = inc.w;
-    }
-    double4 ex;
-    ex.x = __shfl_up(inc.x, 1);
-    ex.y = __shfl_up(inc.y, 1);
-    ex.z = __shfl_down(inc.z, 1);
-    ex.w = __shfl_down(inc.w, 1);
-    if (lane == 0) ex.x = ex.y = 0.0;
-    if (lane == 63) ex.z = ex.w = 0.0;
-    return ex;
-}
-
-// after a barrier: add the totals of the waves below (prefix) / above (suffix)
-__device__ __forceinline__ double4 add_wave_offsets(double4 ex, int wv, const double4 *wsum)
-{
-    for (int w = 0; w < wv; ++w) {
-        ex.x += wsum[w].x;
-        ex.y += wsum[w].y;
-    }
-    for (int w = CH / 64 - 1; w > wv; --w) {
-        ex.z += wsum[w].z;
-        ex.w += wsum[w].w;
-    }
-    return ex;
-}
-
-// Energy pass B: per atom potential phi over the tile's k rows.
-template <int PREC>
-__global__ __launch_bounds__(CH) void k_long_B_energy(LongConsts lc, const double4 *__restrict__ kv,
-                                                      int64_t Ptot, int kt,
-                                                      const double *__restrict__ sx,
-                                                      const double *__restrict__ sy,
-                                                      const double *__restrict__ sz,
-                                                      const double *__restrict__ sq, int64_t n,
-                                                      int nchunk,
-                                                      const double4 *__restrict__ offA,
-                                                      const double4 *__restrict__ totA,
-                                                      double *__restrict__ partB)
-{
-    __shared__ double4 wsum[2][CH / 64];
-    const int m = blockIdx.x, tile = blockIdx.y;
-    const int64_t p0 = (int64_t)tile * kt, p1 = min(Ptot, p0 + kt);
-    const int64_t r = (int64_t)m * CH + threadIdx.x;
-    const bool live = r < n;
-    const double x = live ? sx[r] : 0.0, y = live ? sy[r] : 0.0;
-    const double z = live ? sz[r] : 0.0, q = live ? sq[r] : 0.0;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    double phi = 0.0;
-    for (int64_t p = p0; p < p1; ++p) {
-        const double4 kk = kv[p];
-        const bool k0 = kk.z == 0.0;
-        double c = 1.0, s = 0.0, Ep = 1.0, Em = 1.0;
-        double4 v;
-        if (k0) {
-            v = make_double4(q, q * z, q, q * z);
-        } else {
-            atom_terms<PREC>(kk.x, kk.y, kk.z, x, y, z, c, s, Ep, Em);
-            v = live ? make_double4(q * c * Ep, q * s * Ep, q * c * Em, q * s * Em)
-                     : make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-        const int b = (int)(p & 1);
-        double4 ex = wave_scan_ex(v, lane, wv, wsum[b]);
-        __syncthreads();
-        ex = add_wave_offsets(ex, wv, wsum[b]);
-        const double4 off = offA[p * nchunk + m];
-        const double A = off.x + ex.x, B = off.y + ex.y, C = off.z + ex.z, D = off.w + ex.w;
-        if (k0) {
-            // A, C: charge below / above; B, D: charge * z below / above
-            phi -= kk.w * (z * (A - C) - (B - D));
-            continue;
-        }
-        const double k = kk.z;
-        const double4 T = totA[p];
-        const double e2 = exp(-2.0 * k * lc.Lz);
-        const double direct = c * (Em * A + Ep * C) + s * (Em * B + Ep * D);
-        const double ce = c * Em, se = s * Em, cp = c * Ep, sp = s * Ep;
-        const double dI = (lc.g1 * 2.0 * (T.z * ce + T.w * se) +
-                           lc.g2 * 2.0 * ((e2 * T.x) * cp + (e2 * T.y) * sp) +
-                           2.0 * lc.g12 * (cp * (e2 * T.z) + sp * (e2 * T.w) + (e2 * T.x) * ce +
-                                           (e2 * T.y) * se)) /
-                          (2.0 * (1.0 - lc.g12 * e2));
-        phi += kk.w / (2.0 * k) * (direct + dI);
-    }
-    if (live) partB[(int64_t)tile * n + r] = phi;
-}
+// Adjoint helpers (no counterpart in the reference).
 
 // per k row: w_k/(2k) dI_k/dgamma_1,2 from the structure factors
 __global__ void k_long_dgamma(LongConsts lc, const double4 *__restrict__ kv, int64_t Ptot,
@@ -914,10 +796,6 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const doub
     LR_CHECK(ensure(lw.keys_sorted, nd));
     LR_CHECK(ensure(lw.idx, ni));
     LR_CHECK(ensure(lw.perm, ni));
-    LR_CHECK(ensure(lw.sx, nd));
-    LR_CHECK(ensure(lw.sy, nd));
-    LR_CHECK(ensure(lw.sz, nd));
-    LR_CHECK(ensure(lw.sq, nd));
     LR_CHECK(ensure(lw.satom, 4 * nd));
     k_extract_z<<<nblk(n, 256), 256, 0, s>>>(d_xyz, n, lw.keys.as<double>(), lw.idx.as<int32_t>());
     size_t tbytes = 0;
@@ -931,8 +809,6 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const doub
                                                 lw.idx.as<int32_t>(), lw.perm.as<int32_t>(),
                                                 (int)n, 0, 64, s));
     k_gather_sorted<<<nblk(n, 256), 256, 0, s>>>(d_xyz, d_q, lw.perm.as<int32_t>(), n,
-                                                 lw.sx.as<double>(), lw.sy.as<double>(),
-                                                 lw.sz.as<double>(), lw.sq.as<double>(),
                                                  lw.satom.as<double4>());
     LR_CHECK(hipGetLastError());
     return hipSuccess;
@@ -1003,65 +879,49 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Pto
     return hipSuccess;
 }
 
-namespace {
-struct Tiling {
-    int nchunk, ntile, kt;
-};
-Tiling tiling(int64_t n, int64_t Ptot)
-{
-    Tiling t;
-    t.nchunk = (int)((n + CH - 1) / CH);
-    t.ntile = (int)std::min<int64_t>(Ptot, std::max<int64_t>(1, (2048 + t.nchunk - 1) / t.nchunk));
-    t.kt = (int)((Ptot + t.ntile - 1) / t.ntile);
-    t.ntile = (int)((Ptot + t.kt - 1) / t.kt);
-    return t;
-}
-}  // namespace
-
 hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                               double *d_phi, int accumulate, double *h_dgamma, hipStream_t s,
                               std::string &err)
 {
     h_dgamma[0] = h_dgamma[1] = 0.0;
     if (n <= 0 || Ptot <= 0) return hipSuccess;
-    const Tiling t = tiling(n, Ptot);
-    LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * t.nchunk));
-    LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * t.nchunk));
+    const LaneGrid lg = lane_grid(n, Ptot);
+    const size_t pc = sizeof(double4) * (size_t)Ptot * lg.nchunk;
+    LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
+    LR_CHECK(ensure(lw.partA, pc));
+    LR_CHECK(ensure(lw.offA, pc));
     LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
-    LR_CHECK(ensure(lw.partB, sizeof(double) * (size_t)t.ntile * (size_t)n));
+    LR_CHECK(ensure(lw.partB, sizeof(double) * (size_t)lg.ngroup * (size_t)n));
     LR_CHECK(ensure(lw.q0part, sizeof(double2) * (size_t)Ptot));
-    const double4 *kv = lw.kv.as<double4>();
-    dim3 g(t.nchunk, t.ntile);
-    if (c.precision == QEM_PREC_MIXED)
-        k_long_A<QEM_PREC_MIXED><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
-                                                  lw.sz.as<double>(), lw.sq.as<double>(), n, t.nchunk,
-                                                  lw.partA.as<double4>());
-    else
-        k_long_A<QEM_PREC_FP64><<<g, CH, 0, s>>>(kv, Ptot, t.kt, lw.sx.as<double>(), lw.sy.as<double>(),
-                                                 lw.sz.as<double>(), lw.sq.as<double>(), n, t.nchunk,
-                                                 lw.partA.as<double4>());
+    KRec *kr = lw.kr.as<KRec>();
+    k_make_krec<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot, kr);
     LR_CHECK(hipGetLastError());
-    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, t.nchunk,
+    const dim3 g((unsigned)((lg.nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)lg.ngroup);
+    const double4 *sa = lw.satom.as<double4>();
+    if (c.precision == QEM_PREC_MIXED)
+        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, lg.chk,
+                                                               lg.nchunk, lw.partA.as<double4>());
+    else
+        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, lg.chk,
+                                                              lg.nchunk, lw.partA.as<double4>());
+    LR_CHECK(hipGetLastError());
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, lg.nchunk,
                                            lw.offA.as<double4>(), lw.totA.as<double4>());
     LR_CHECK(hipGetLastError());
     if (c.precision == QEM_PREC_MIXED)
-        k_long_B_energy<QEM_PREC_MIXED><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(),
-                                                         lw.sy.as<double>(), lw.sz.as<double>(),
-                                                         lw.sq.as<double>(), n, t.nchunk,
-                                                         lw.offA.as<double4>(), lw.totA.as<double4>(),
-                                                         lw.partB.as<double>());
+        k_lane_B_energy<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
+            c, kr, Ptot, sa, n, lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
+            lw.totA.as<double4>(), lw.partB.as<double>());
     else
-        k_long_B_energy<QEM_PREC_FP64><<<g, CH, 0, s>>>(c, kv, Ptot, t.kt, lw.sx.as<double>(),
-                                                        lw.sy.as<double>(), lw.sz.as<double>(),
-                                                        lw.sq.as<double>(), n, t.nchunk,
-                                                        lw.offA.as<double4>(), lw.totA.as<double4>(),
-                                                        lw.partB.as<double>());
+        k_lane_B_energy<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
+            c, kr, Ptot, sa, n, lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
+            lw.totA.as<double4>(), lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
-    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), t.ntile, n, 1,
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 1,
                                            lw.perm.as<int32_t>(), accumulate, 1.0, d_phi);
     LR_CHECK(hipGetLastError());
-    k_long_dgamma<<<nblk(Ptot, 256), 256, 0, s>>>(c, kv, Ptot, lw.totA.as<double4>(),
-                                                  lw.q0part.as<double2>());
+    k_long_dgamma<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot,
+                                                  lw.totA.as<double4>(), lw.q0part.as<double2>());
     LR_CHECK(hipGetLastError());
     std::vector<double> h(2 * (size_t)Ptot);
     LR_CHECK(hipMemcpyAsync(h.data(), lw.q0part.p, sizeof(double2) * (size_t)Ptot,

@@ -792,7 +792,7 @@ int qem_create(const qem_params *p, qem_ctx **out)
         const double u = 1.0 + ctx->gl.x[(size_t)n];
         nc[4 * (size_t)n] = (float)u;
         nc[4 * (size_t)n + 1] = (float)(u * u);
-        nc[4 * (size_t)n + 2] = (float)(ctx->gl.w[(size_t)n] * u);
+        nc[4 * (size_t)n + 2] = (float)std::log2(ctx->gl.w[(size_t)n]);
         nc[4 * (size_t)n + 3] = (float)std::log2(ctx->gl.w[(size_t)n] * u);
     }
     if (upload(ctx, ctx->sw.ncon, nc.data(), sizeof(float) * nc.size())) return fail(QEM_ERR_HIP);
@@ -836,10 +836,10 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
                       &ctx->nl.atom_idx_sorted, &ctx->nl.cell_start, &ctx->nl.cell_end,
                       &ctx->nl.counts, &ctx->nl.row_ptr, &ctx->nl.tmp, &ctx->lw.keys,
                       &ctx->lw.keys_sorted, &ctx->lw.idx, &ctx->lw.perm, &ctx->lw.tmp,
-                      &ctx->lw.sx, &ctx->lw.sy, &ctx->lw.sz, &ctx->lw.sq, &ctx->lw.satom,
+                      &ctx->lw.satom,
                       &ctx->lw.kr, &ctx->lw.kv,
                       &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,
-                      &ctx->lw.q0off, &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,
+                      &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,
                       &ctx->lw.svz, &ctx->lw.partT, &ctx->lw.offT, &ctx->lw.totT, &ctx->d_xyz,
                       &ctx->d_q, &ctx->d_f, &ctx->d_f2, &ctx->d_aux, &ctx->d_adj};
 }

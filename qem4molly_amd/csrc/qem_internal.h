@@ -92,7 +92,7 @@ hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs,
 struct ShortWork {
     DevBuf glx, glw;      // double nodes / weights
     DevBuf glxf, glwf;    // float nodes / weights
-    DevBuf ncon;          // float4 per node: (1 + x_n, (1 + x_n)^2, w_n (1 + x_n),
+    DevBuf ncon;          // float4 per node: (1 + x_n, (1 + x_n)^2, log2 w_n,
                           //                   log2(w_n (1 + x_n)))
     DevBuf jtab;          // one float4 (cubic coefficients) per interval
     int jt_n = 0;
@@ -134,14 +134,13 @@ struct LongConsts {
 };
 struct LongWork {
     DevBuf keys, keys_sorted, idx, perm, tmp;
-    DevBuf sx, sy, sz, sq;   // atoms in z order
     DevBuf satom;            // atoms in z order, (x, y, z, q) per atom
     DevBuf kr;               // per k row derived constants of the force passes
     DevBuf kv;               // P x 4 doubles (kx, ky, k, weight)
     DevBuf partA;            // [P][nchunk][4] chunk sums
     DevBuf offA;             // [P][nchunk][4] chunk offsets
     DevBuf totA;             // [P][4] totals
-    DevBuf q0part, q0off;    // k = 0 chunk sums / offsets
+    DevBuf q0part;           // per k row: dE_k/dgamma_1,2 (energy)
     DevBuf partB;            // [ntile][n][3]
     DevBuf svx, svy, svz;    // tangent directions in z order
     DevBuf partT, offT, totT;  // tangent chunk sums / offsets / totals
@@ -151,7 +150,7 @@ struct LongWork {
     hipEvent_t kv_ev[kSlots] = {nullptr, nullptr, nullptr, nullptr};
     int kv_slot = 0;
 };
-// z-sort; fills lw.perm (int32, rank -> atom) and the sorted SoA arrays.
+// z-sort; fills lw.perm (int32, rank -> atom) and lw.satom (atoms in z order).
 hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz,
                         const double *d_q, hipStream_t s, std::string &err);
 // Full long-range force for the k vectors already in lw.kv (P rows).

@@ -61,8 +61,8 @@ __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool 
 }
 
 // Mixed-precision node sums with the node constants folded in: for the
-// rule's node n, u = (1 + x_n, (1 + x_n)^2, w_n (1 + x_n), log2(w_n (1 + x_n)));
-// with h = K/2 the node is k = h u.x and w k = h^2 u.z.  Every exponent is
+// rule's node n, u = (1 + x_n, (1 + x_n)^2, log2 w_n, log2(w_n (1 + x_n)));
+// with h = K/2 the node is k = h u.x and w k = h^2 w_n u.x.  Every exponent is
 // written through e = -2 Lz k log2(e) (E = 2^e):
 //   point:  e^{-k (a + 2 Lz)} w k / h^2 = 2^{e (1 + a/2Lz) + u.w}
 //   gauss:  e^{-k a - k^2/4 alpha} w k / h^2 = 2^{e (a/2Lz + e cg/c2^2) + u.w}
