@@ -464,86 +464,136 @@ __global__ __launch_bounds__(256) void k_pair_tangent(
     }
 }
 
-// per atom: phi (dE/dq) and psi_1,2 (d phi / d gamma) from the pair
-// potentials of both endpoints plus the self term; fixed order
+// self terms per atom: phi = 2 q s_i, psi_w = 2 q ds_i/dgamma_w (overwrite)
 template <int PREC>
-__global__ __launch_bounds__(256) void k_gather_energy(
+__global__ __launch_bounds__(256) void k_self_energy(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float *__restrict__ glxf, const float *__restrict__ glwf, int64_t n,
-    const double *__restrict__ xyz, const double *__restrict__ q,
-    const double4 *__restrict__ pot, const int32_t *__restrict__ pi,
-    const int32_t *__restrict__ pj, const int32_t *__restrict__ i_start,
-    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
-    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
-    const int32_t *__restrict__ j_idx, double *__restrict__ phi, double *__restrict__ psi)
+    const double *__restrict__ xyz, const double *__restrict__ q, double *__restrict__ phi,
+    double *__restrict__ psi)
 {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
-    double f = 0.0, s0 = 0.0, s1 = 0.0;
-    for (int32_t t = i_start[a]; t < i_end[a]; ++t) {
-        const int32_t p = i_idx ? i_idx[t] : t;
-        const double4 u = pot[p];
-        const double qb = q[pj[p]];
-        f += qb * u.x;
-        s0 += qb * u.y;
-        s1 += qb * u.z;
-    }
-    for (int32_t t = j_start[a]; t < j_end[a]; ++t) {
-        const int32_t p = j_idx[t];
-        const double4 u = pot[p];
-        const double qb = q[pi[p]];
-        f += qb * u.x;
-        s0 += qb * u.y;
-        s1 += qb * u.z;
-    }
     const double3 se = self_energy<PREC>(c, glx, glw, glxf, glwf, xyz[3 * a + 2]);
     const double qa = q[a];
-    phi[a] = f + 2.0 * qa * se.x;
-    psi[2 * a] = s0 + 2.0 * qa * se.y;
-    psi[2 * a + 1] = s1 + 2.0 * qa * se.z;
+    phi[a] = 2.0 * qa * se.x;
+    psi[2 * a] = 2.0 * qa * se.y;
+    psi[2 * a + 1] = 2.0 * qa * se.z;
 }
 
-// per atom: scale * (tangent of the short-range force), pair sides + self
+// per atom: phi (dE/dq) and psi_1,2 (d phi / d gamma) from the pair
+// potentials of both endpoints, added to the self terms already there.
+// Sixteen lanes per atom, strided contiguous reads, fixed-order row sums.
+__global__ __launch_bounds__(256) void k_gather_energy(
+    int64_t n, const double *__restrict__ q, const double4 *__restrict__ pot,
+    const int32_t *__restrict__ pi, const int32_t *__restrict__ pj,
+    const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
+    const int32_t *__restrict__ i_idx, const int32_t *__restrict__ j_start,
+    const int32_t *__restrict__ j_end, const int32_t *__restrict__ j_idx,
+    double *__restrict__ phi, double *__restrict__ psi)
+{
+    const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    const bool live = a < n;
+    double f = 0.0, s0 = 0.0, s1 = 0.0;
+    if (live) {
+        const int32_t t1 = i_end[a];
+        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
+            const int32_t p = i_idx ? i_idx[t] : t;
+            const double4 u = pot[p];
+            const double qb = q[pj[p]];
+            f += qb * u.x;
+            s0 += qb * u.y;
+            s1 += qb * u.z;
+        }
+        const int32_t u1 = j_end[a];
+        for (int32_t t = j_start[a] + l; t < u1; t += 16) {
+            const int32_t p = j_idx[t];
+            const double4 u = pot[p];
+            const double qb = q[pi[p]];
+            f += qb * u.x;
+            s0 += qb * u.y;
+            s1 += qb * u.z;
+        }
+    }
+    f = dev::row_sum16(f);
+    s0 = dev::row_sum16(s0);
+    s1 = dev::row_sum16(s1);
+    if (live && l == 0) {
+        phi[a] += f;
+        psi[2 * a] += s0;
+        psi[2 * a + 1] += s1;
+    }
+}
+
+// self term of the force tangent per atom: out = scale (0, 0, dF_self,z)
 template <int PREC>
-__global__ __launch_bounds__(256) void k_gather_tangent(
+__global__ __launch_bounds__(256) void k_self_tangent(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float *__restrict__ glxf, const float *__restrict__ glwf, int64_t n,
     const double *__restrict__ xyz, const double *__restrict__ q, const double *__restrict__ v,
-    const double4 *__restrict__ tp, const int32_t *__restrict__ i_start,
+    double scale, double *__restrict__ out)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    out[3 * a] = 0.0;
+    out[3 * a + 1] = 0.0;
+    out[3 * a + 2] =
+        scale * self_force_tangent<PREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], v[3 * a + 2]);
+}
+
+// per atom: scale * (tangent of the pair forces of both endpoints), added to
+// the self term already there; sixteen lanes per atom as above
+__global__ __launch_bounds__(256) void k_gather_tangent(
+    int64_t n, const double4 *__restrict__ tp, const int32_t *__restrict__ i_start,
     const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
     const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
     const int32_t *__restrict__ j_idx, double scale, double *__restrict__ out)
 {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
+    const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    const bool live = a < n;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int32_t t = i_start[a]; t < i_end[a]; ++t) {
-        const double4 d = tp[i_idx ? i_idx[t] : t];
-        fx += d.x;
-        fy += d.y;
-        fz += d.z;
+    if (live) {
+        const int32_t t1 = i_end[a];
+        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
+            const double4 d = tp[i_idx ? i_idx[t] : t];
+            fx += d.x;
+            fy += d.y;
+            fz += d.z;
+        }
+        const int32_t u1 = j_end[a];
+        for (int32_t t = j_start[a] + l; t < u1; t += 16) {
+            const double4 d = tp[j_idx[t]];
+            fx -= d.x;
+            fy -= d.y;
+            fz += d.w;
+        }
     }
-    for (int32_t t = j_start[a]; t < j_end[a]; ++t) {
-        const double4 d = tp[j_idx[t]];
-        fx -= d.x;
-        fy -= d.y;
-        fz += d.w;
+    fx = dev::row_sum16(fx);
+    fy = dev::row_sum16(fy);
+    fz = dev::row_sum16(fz);
+    if (live && l == 0) {
+        out[3 * a] = scale * fx;
+        out[3 * a + 1] = scale * fy;
+        out[3 * a + 2] += scale * fz;
     }
-    fz += self_force_tangent<PREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], v[3 * a + 2]);
-    out[3 * a] = scale * fx;
-    out[3 * a + 1] = scale * fy;
-    out[3 * a + 2] = scale * fz;
 }
 
-// 1/2 sum_i q_i phi_i, 1/2 sum_i q_i psi_i,w (one block, fixed order)
-__global__ __launch_bounds__(256) void k_half_dots(int64_t n, const double *__restrict__ q,
-                                                   const double *__restrict__ phi,
-                                                   const double *__restrict__ psi,
-                                                   double *__restrict__ out3)
+// 1/2 sum_i q_i phi_i and 1/2 sum_i q_i psi_i,w in two fixed-order passes:
+// block b sums atoms [b*chunk, (b+1)*chunk) into part[b], one block sums the
+// parts in block order
+constexpr int kDotBlocks = 128;
+__global__ __launch_bounds__(256) void k_half_dots_part(int64_t n, const double *__restrict__ q,
+                                                        const double *__restrict__ phi,
+                                                        const double *__restrict__ psi,
+                                                        double *__restrict__ part)
 {
     __shared__ double s[3][256];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     double a = 0.0, b = 0.0, d = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         a += q[i] * phi[i];
         if (psi) {
             b += q[i] * psi[2 * i];
@@ -559,11 +609,23 @@ __global__ __launch_bounds__(256) void k_half_dots(int64_t n, const double *__re
             for (int r = 0; r < 3; ++r) s[r][threadIdx.x] += s[r][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out3[0] = 0.5 * s[0][0];
-        out3[1] = 0.5 * s[1][0];
-        out3[2] = 0.5 * s[2][0];
+    if (threadIdx.x == 0)
+        for (int r = 0; r < 3; ++r) part[3 * blockIdx.x + r] = s[r][0];
+}
+
+__global__ void k_half_dots_final(int nb, const double *__restrict__ part,
+                                  double *__restrict__ out3)
+{
+    if (threadIdx.x != 0) return;
+    double a = 0.0, b = 0.0, d = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        a += part[3 * i];
+        b += part[3 * i + 1];
+        d += part[3 * i + 2];
     }
+    out3[0] = 0.5 * a;
+    out3[1] = 0.5 * b;
+    out3[2] = 0.5 * d;
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -592,23 +654,23 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
                 nl.pj.as<int32_t>(), np, d_xyz, pot);
-        k_gather_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
-            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
-            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi, d_psi);
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
     } else {
         if (np > 0)
             k_pair_energy<QEM_PREC_FP64><<<pair_grid(np), 256, 0, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
                 d_xyz, pot);
-        k_gather_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
-            nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
-            nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi, d_psi);
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
     }
+    k_gather_energy<<<nblk(16 * n, 256), 256, 0, s>>>(
+        n, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(),
+        nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+        nl.j_idx.as<int32_t>(), d_phi, d_psi);
     return hipGetLastError();
 }
 
@@ -628,31 +690,33 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
                 nl.pj.as<int32_t>(), np, d_xyz, d_q, d_v, tp);
-        k_gather_tangent<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_tangent<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, d_v, tp, nl.i_start.as<int32_t>(),
-            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-            nl.j_idx.as<int32_t>(), scale, d_out);
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);
     } else {
         if (np > 0)
             k_pair_tangent<QEM_PREC_FP64><<<pair_grid(np), 256, 0, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
                 d_xyz, d_q, d_v, tp);
-        k_gather_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-            ws.glwf.as<float>(), n, d_xyz, d_q, d_v, tp, nl.i_start.as<int32_t>(),
-            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-            nl.j_idx.as<int32_t>(), scale, d_out);
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);
     }
+    k_gather_tangent<<<nblk(16 * n, 256), 256, 0, s>>>(
+        n, tp, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
+        nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), scale, d_out);
     return hipGetLastError();
 }
 
 hipError_t launch_half_dots(int64_t n, const double *d_q, const double *d_phi,
-                            const double *d_psi, double *d_out3, hipStream_t s)
+                            const double *d_psi, double *d_out3, double *d_part, hipStream_t s)
 {
-    k_half_dots<<<1, 256, 0, s>>>(n, d_q, d_phi, d_psi, d_out3);
+    k_half_dots_part<<<kDotBlocks, 256, 0, s>>>(n, d_q, d_phi, d_psi, d_part);
+    k_half_dots_final<<<1, 64, 0, s>>>(kDotBlocks, d_part, d_out3);
     return hipGetLastError();
 }
+
+int half_dots_scratch() { return 3 * kDotBlocks; }
 
 }  // namespace qem

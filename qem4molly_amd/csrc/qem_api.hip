@@ -626,9 +626,10 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     if (st) return st;
     const ShortConsts c = short_consts(ctx, 0);
     API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
-    API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (3 * (size_t)n + 4)));
+    API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (2 * (size_t)n + 4 + half_dots_scratch())));
     double *psi = ctx->d_adj.as<double>();          // N x 2
     double *out3 = psi + 2 * n;                      // 3 (+1 pad)
+    double *part = out3 + 4;                         // reduction scratch
     double *phi = d_phi;
     if (!phi) {
         API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
@@ -660,7 +661,7 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                                dg_long, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
     }
-    API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, s));
+    API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s));
     double h[3];
     API_HIP(ctx, hipMemcpyAsync(h, out3, sizeof h, hipMemcpyDeviceToHost, s));
     API_HIP(ctx, hipStreamSynchronize(s));

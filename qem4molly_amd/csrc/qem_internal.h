@@ -180,9 +180,11 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,
                                 double scale, double *d_out, hipStream_t s);
-// out3 = (1/2 q.phi, 1/2 q.psi_1, 1/2 q.psi_2); psi may be null.
+// out3 = (1/2 q.phi, 1/2 q.psi_1, 1/2 q.psi_2); psi may be null; d_part is
+// scratch of half_dots_scratch() doubles.
 hipError_t launch_half_dots(int64_t n, const double *d_q, const double *d_phi,
-                            const double *d_psi, double *d_out3, hipStream_t s);
+                            const double *d_psi, double *d_out3, double *d_part, hipStream_t s);
+int half_dots_scratch();
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
