@@ -224,10 +224,13 @@ int qem_timing_read(qem_ctx *ctx, double *ms, int64_t *calls, int32_t reset);
 /* Gauss-Legendre rule on [-1, 1], ascending nodes (stands in for
  * GaussQuadrature.legendre(n), used at src/Gaussian_int.jl:11); 1 <= n <= 256. */
 int qem_gauss_legendre(int32_t n, double *x, double *w);
-/* The FP32 J0/J1 pair used by QEM_PREC_MIXED kernels, evaluated on the GPU
- * for n arguments in [0, table range] (no counterpart in the reference: it
- * replaces SpecialFunctions.besselj0/besselj1 at src/Force_short.jl:5-21). */
-int qem_bessel01_mixed(qem_ctx *ctx, int64_t n, const double *x, double *j0,
+/* The tabulated J0/J1 pair of the context's kernels, evaluated on the GPU
+ * for n arguments in [0, table range]: the FP32 cubic table in
+ * QEM_PREC_MIXED contexts, the FP64 degree-7 table in QEM_PREC_FP64 ones
+ * (no counterpart in the reference: it replaces SpecialFunctions.besselj0/
+ * besselj1 at src/Force_short.jl:5-21).  QEM_ERR_UNSUPPORTED when the
+ * context has no table. */
+int qem_bessel01_table(qem_ctx *ctx, int64_t n, const double *x, double *j0,
                        double *j1);
 
 /* ---- device memory utilities (no counterpart in the reference) ----------

@@ -45,7 +45,7 @@ EXPORTS = (
     "qem_short_forces_pairs", "qem_short_pair", "qem_short_self",
     "qem_sortperm_z", "qem_long_forces", "qem_long_terms", "qem_forces",
     "qem_short_forces_dev", "qem_long_forces_dev", "qem_forces_dev",
-    "qem_short_state", "qem_gauss_legendre", "qem_bessel01_mixed",
+    "qem_short_state", "qem_gauss_legendre", "qem_bessel01_table",
     "qem_dev_malloc", "qem_dev_free", "qem_memcpy_htod", "qem_memcpy_dtoh",
     "qem_device_synchronize", "qem_dev_copy_kernel", "qem_timing_enable",
     "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
@@ -104,7 +104,7 @@ _SIGS = {
     "qem_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp]),
     "qem_short_state": (C.c_int, [_vp, _pi64, _pi64, _pi64]),
     "qem_gauss_legendre": (C.c_int, [_i32, _pd, _pd]),
-    "qem_bessel01_mixed": (C.c_int, [_vp, _i64, _pd, _pd, _pd]),
+    "qem_bessel01_table": (C.c_int, [_vp, _i64, _pd, _pd, _pd]),
     "qem_dev_malloc": (C.c_int, [_P(_vp), C.c_size_t]),
     "qem_dev_free": (C.c_int, [_vp]),
     "qem_memcpy_htod": (C.c_int, [_vp, _vp, C.c_size_t]),
@@ -375,11 +375,12 @@ class Context:
                     "qem_timing_read")
         return {ph: (float(ms[i]), int(calls[i])) for i, ph in enumerate(PHASES)}
 
-    def bessel01_mixed(self, x):
+    def bessel01_table(self, x):
+        """J0, J1 from the context's table (FP32 cubic or FP64 degree 7)."""
         x = f64(x)
         j0 = np.zeros_like(x); j1 = np.zeros_like(x)
-        self._check(lib().qem_bessel01_mixed(self._h, x.shape[0], ptr(x), ptr(j0), ptr(j1)),
-                    "qem_bessel01_mixed")
+        self._check(lib().qem_bessel01_table(self._h, x.shape[0], ptr(x), ptr(j0), ptr(j1)),
+                    "qem_bessel01_table")
         return j0, j1
 
 

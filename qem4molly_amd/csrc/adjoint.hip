@@ -38,7 +38,7 @@ __device__ __forceinline__ void nodes_energy(double K, double a, double rho, int
                                              const float *__restrict__ glwf,
                                              const float4 *__restrict__ tab, double &v, double &d)
 {
-    if (PREC == QEM_PREC_FP64) {
+    if (PREC != QEM_PREC_MIXED) {
         const double h = 0.5 * K, inv4a = 0.25 / c.alpha;
         double av = 0.0, ad = 0.0;
         for (int n = 0; n < c.Nt; ++n) {
@@ -50,7 +50,16 @@ __device__ __forceinline__ void nodes_energy(double K, double a, double rho, int
             if (kind == 0) { num = exp(-k * a) * E; fd = rD; }
             else if (kind == 1) { num = exp(-k * a - k * k * inv4a); fd = E * rD; }
             else { num = exp(-k * k * inv4a) * E; fd = rD; }
-            const double J = SELF ? 1.0 : j0(k * rho);
+            double J = 1.0;
+            if (!SELF) {
+                if (PREC == QEM_PREC_FP64) {
+                    double J1;
+                    dev::bessel01_tab64(k * rho, reinterpret_cast<const double *>(tab), c.jd_n,
+                                        c.jd_inv_dx, J, J1);
+                } else {
+                    J = j0(k * rho);
+                }
+            }
             const double wt = w * num * rD * J;
             av += wt;
             ad += wt * fd;
@@ -282,7 +291,7 @@ __device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, do
                                                const float4 *__restrict__ tab, bool self)
 {
     Dual4 o{0.0, 0.0, 0.0, 0.0};
-    if (PREC == QEM_PREC_FP64) {
+    if (PREC != QEM_PREC_MIXED) {
         const double h = 0.5 * K, dh = 0.5 * dK, inv4a = 0.25 / c.alpha;
         for (int n = 0; n < c.Nt; ++n) {
             const double k = h + h * glx[n], dk = dh + dh * glx[n];
@@ -305,8 +314,13 @@ __device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, do
             double J0 = 1.0, J1 = 0.0, dJ0 = 0.0, dJ1 = 0.0;
             if (!self) {
                 const double x = k * rho, dx = dk * rho + k * drho;
-                J0 = j0(x);
-                J1 = j1(x);
+                if (PREC == QEM_PREC_FP64) {
+                    dev::bessel01_tab64(x, reinterpret_cast<const double *>(tab), c.jd_n,
+                                        c.jd_inv_dx, J0, J1);
+                } else {
+                    J0 = j0(x);
+                    J1 = j1(x);
+                }
                 const double j1x = x > 1e-8 ? J1 / x : 0.5;
                 dJ0 = -J1 * dx;
                 dJ1 = (J0 - j1x) * dx;
@@ -435,8 +449,7 @@ __global__ __launch_bounds__(256) void k_pair_energy(
     double4 *__restrict__ out)
 {
     extern __shared__ float4 lds_tab[];
-    const float4 *tab = nullptr;
-    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
+    const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const int64_t i = pi[p], j = pj[p];
@@ -454,8 +467,7 @@ __global__ __launch_bounds__(256) void k_pair_tangent(
     const double *__restrict__ q, const double *__restrict__ v, double4 *__restrict__ out)
 {
     extern __shared__ float4 lds_tab[];
-    const float4 *tab = nullptr;
-    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
+    const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const int64_t i = pi[p], j = pj[p];
@@ -658,11 +670,18 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
     } else {
-        if (np > 0)
-            k_pair_energy<QEM_PREC_FP64><<<pair_grid(np), 256, 0, s>>>(
-                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-                ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                d_xyz, pot);
+        if (np > 0) {
+            if (c.jd_n > 0)
+                k_pair_energy<QEM_PREC_FP64><<<pair_grid(np), 256, dev::lds_bytes64(c), s>>>(
+                    c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
+                    d_xyz, pot);
+            else
+                k_pair_energy<kPrecFP64Lib><<<pair_grid(np), 256, 0, s>>>(
+                    c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
+                    d_xyz, pot);
+        }
         k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
@@ -694,11 +713,18 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);
     } else {
-        if (np > 0)
-            k_pair_tangent<QEM_PREC_FP64><<<pair_grid(np), 256, 0, s>>>(
-                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-                ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                d_xyz, d_q, d_v, tp);
+        if (np > 0) {
+            if (c.jd_n > 0)
+                k_pair_tangent<QEM_PREC_FP64><<<pair_grid(np), 256, dev::lds_bytes64(c), s>>>(
+                    c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
+                    d_xyz, d_q, d_v, tp);
+            else
+                k_pair_tangent<kPrecFP64Lib><<<pair_grid(np), 256, 0, s>>>(
+                    c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
+                    d_xyz, d_q, d_v, tp);
+        }
         k_self_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);

@@ -89,23 +89,27 @@ GLRule make_gl_rule(int n)
     return r;
 }
 
-// J0 on [m dx, (m+1) dx) as p(t) = c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1):
-// least squares at 20 Chebyshev points of [0, 1] (normal equations in long
-// double; J0 from the C library), one float4 per interval.  Two padding
-// intervals past x_max.
-std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals)
+// J_order (order 0 or 1) on [m dx, (m+1) dx) as p(t) = c0 + c1 t + ... +
+// cD t^D, t in [0, 1): least squares at 20 Chebyshev points of [0, 1]
+// (normal equations in long double; J0/J1 from the C library's long double
+// j0l/j1l).  Used at low degree (the mixed table), where the normal
+// equations are well conditioned.  Two padding intervals past x_max.
+std::vector<long double> fit_bessel_table(double x_max, double dx, int D, int order,
+                                          int *n_intervals)
 {
-    constexpr int D = 3, S = 20;
+    constexpr int S = 20, DM = 8;
+    if (D < 1 || D >= DM) return {};
     const int nm = (int)std::ceil(x_max / dx) + 2;
-    std::vector<float> tab((size_t)nm * 4, 0.0f);
+    std::vector<long double> tab((size_t)nm * (D + 1), 0.0L);
     long double ts[S];
     for (int i = 0; i < S; ++i)
         ts[i] = 0.5L - 0.5L * cosl(3.141592653589793238462643383279502884L * (i + 0.5L) / S);
     for (int m = 0; m < nm; ++m) {
-        long double A[D + 1][D + 2] = {};
+        long double A[DM][DM + 1] = {};
         for (int i = 0; i < S; ++i) {
-            const long double f = (long double)::j0((double)((m + (double)ts[i]) * dx));
-            long double pw[D + 1];
+            const long double xi = ((long double)m + ts[i]) * (long double)dx;
+            const long double f = order == 0 ? ::j0l(xi) : ::j1l(xi);
+            long double pw[DM];
             pw[0] = 1.0L;
             for (int k = 1; k <= D; ++k) pw[k] = pw[k - 1] * ts[i];
             for (int r = 0; r <= D; ++r) {
@@ -125,7 +129,54 @@ std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals)
                 for (int k = col; k <= D + 1; ++k) A[r][k] -= f * A[col][k];
             }
         }
-        for (int k = 0; k <= D; ++k) tab[(size_t)m * 4 + k] = (float)(A[k][D + 1] / A[k][k]);
+        for (int k = 0; k <= D; ++k) tab[(size_t)m * (D + 1) + k] = A[k][D + 1] / A[k][k];
+    }
+    *n_intervals = nm;
+    return tab;
+}
+
+// J0 on [m dx, (m+1) dx) as p(t) = c0 + c1 t + ... + cD t^D, t in [0, 1),
+// by interpolation at the D + 1 Chebyshev points of the interval (J0 from
+// the C library's long double j0l): the Chebyshev coefficients by the
+// discrete cosine sum, then re-expanded in powers of t, all in long double.
+// Unlike the least-squares fit this stays well conditioned at degree 7, so
+// the derivative (J1) keeps FP64 accuracy too.  Two padding intervals past
+// x_max.
+std::vector<long double> interp_j0_table(double x_max, double dx, int D, int *n_intervals)
+{
+    constexpr int DM = 12;
+    if (D < 1 || D >= DM) return {};
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const int N = D + 1;
+    const int nm = (int)std::ceil(x_max / dx) + 2;
+    // T_j(2t - 1) as coefficient rows in powers of t
+    long double T[DM][DM] = {};
+    T[0][0] = 1.0L;
+    T[1][0] = -1.0L;
+    T[1][1] = 2.0L;
+    for (int j = 1; j + 1 < N; ++j)
+        for (int k = 0; k < N; ++k) {
+            long double v = -2.0L * T[j][k] - T[j - 1][k];
+            if (k > 0) v += 4.0L * T[j][k - 1];
+            T[j + 1][k] = v;
+        }
+    std::vector<long double> tab((size_t)nm * N, 0.0L);
+    for (int m = 0; m < nm; ++m) {
+        long double f[DM], a[DM];
+        for (int i = 0; i < N; ++i) {
+            const long double u = cosl(pi * (i + 0.5L) / N);
+            f[i] = ::j0l(((long double)m + 0.5L * (u + 1.0L)) * (long double)dx);
+        }
+        for (int j = 0; j < N; ++j) {
+            long double acc = 0.0L;
+            for (int i = 0; i < N; ++i) acc += f[i] * cosl(pi * j * (i + 0.5L) / N);
+            a[j] = (j == 0 ? 1.0L : 2.0L) * acc / N;
+        }
+        for (int k = 0; k < N; ++k) {
+            long double c = 0.0L;
+            for (int j = 0; j < N; ++j) c += a[j] * T[j][k];
+            tab[(size_t)m * N + k] = c;
+        }
     }
     *n_intervals = nm;
     return tab;
@@ -159,6 +210,16 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
     return hipGetLastError();
 }
 
+// evaluates the FP64 table Bessel pair used by the FP64 kernels
+__global__ void k_bessel_tab64(const double *__restrict__ x, int64_t n,
+                               const double *__restrict__ tab, int ntab, double inv_dx,
+                               double *__restrict__ j0o, double *__restrict__ j1o)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dev::bessel01_tab64(x[i], tab, ntab, inv_dx, j0o[i], j1o[i]);
+}
+
 // evaluates the FP32 table Bessel pair used by the mixed-precision kernels
 __global__ void k_bessel_tab(const double *__restrict__ x, int64_t n,
                              const float4 *__restrict__ tab, int ntab, float inv_dx,
@@ -184,6 +245,8 @@ constexpr double kTwoPi = 6.283185307179586;  // Float64(2pi)
 constexpr double kBesselDx = 1.0 / 32.0;
 constexpr double kBesselDxMax = 1.0 / 16.0;
 constexpr int kMaxTabIntervals = 4096;        // 64 KiB of LDS (16 B per interval)
+constexpr double kBesselDx64 = 1.0 / 16.0;    // FP64 table spacing
+constexpr int kMaxTab64Intervals = 1024;      // 64 KiB of LDS (64 B per interval)
 
 int set_err(qem_ctx *ctx, int code, const std::string &msg)
 {
@@ -292,6 +355,9 @@ ShortConsts short_consts(const qem_ctx *ctx, int single)
     c.jt_dx = ctx->sw.jt_dx;
     c.jt_inv_dx = ctx->sw.jt_dx > 0 ? 1.0f / ctx->sw.jt_dx : 0.0f;
     c.ncon = ctx->sw.ncon.as<float4>();
+    c.jd_n = ctx->sw.jd_n;
+    c.jd_inv_dx = ctx->sw.jd_dx > 0 ? 1.0 / ctx->sw.jd_dx : 0.0;
+    c.jdtab = ctx->sw.jdtab.as<double>();
     return c;
 }
 
@@ -815,11 +881,29 @@ int qem_create(const qem_params *p, qem_ctx **out)
             qem_destroy(ctx);
             return QEM_ERR_UNSUPPORTED;
         }
-        std::vector<float> tab = make_j0_table(x_max, dx, &nm);
+        const std::vector<long double> fit = fit_bessel_table(x_max, dx, 3, 0, &nm);
+        const std::vector<float> tab(fit.begin(), fit.end());
         ctx->sw.jt_n = nm;
         ctx->sw.jt_dx = (float)dx;
         if (upload(ctx, ctx->sw.jtab, tab.data(), sizeof(float) * tab.size()))
             return fail(QEM_ERR_HIP);
+    } else {
+        // FP64: J0 as a degree-7 interpolant per 1/16-wide interval, J1 from
+        // its derivative (both ~5e-16 absolute); when the argument range needs
+        // more than kMaxTab64Intervals the kernels call the device library's
+        // j0/j1
+        const double la = -std::log(p->accuracy);
+        const double kmax = std::max(std::sqrt(4.0 * p->alpha * la), la / (2.0 * p->Lz));
+        const double x_max = p->r_cutoff * kmax * (1.0 + 1e-6) + kBesselDx64;
+        if (x_max / kBesselDx64 + 2 <= kMaxTab64Intervals) {
+            int nm = 0;
+            const std::vector<long double> fit = interp_j0_table(x_max, kBesselDx64, 7, &nm);
+            const std::vector<double> tab(fit.begin(), fit.end());
+            ctx->sw.jd_n = nm;
+            ctx->sw.jd_dx = kBesselDx64;
+            if (upload(ctx, ctx->sw.jdtab, tab.data(), sizeof(double) * tab.size()))
+                return fail(QEM_ERR_HIP);
+        }
     }
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(QEM_ERR_HIP);
     if (p->rbe_p > 0 && ensure_kset(ctx)) return fail(QEM_ERR_NOMEM);
@@ -829,7 +913,7 @@ int qem_create(const qem_params *p, qem_ctx **out)
 
 static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
 {
-    return {&ctx->sw.ncon, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab,
+    return {&ctx->sw.ncon, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab, &ctx->sw.jdtab,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
                       &ctx->nl.sxyz, &ctx->nl.sq, &ctx->nl.sv, &ctx->nl.sout,
@@ -1321,14 +1405,16 @@ int qem_gauss_legendre(int32_t n, double *x, double *w)
     return QEM_OK;
 }
 
-int qem_bessel01_mixed(qem_ctx *ctx, int64_t n, const double *x, double *j0o, double *j1o)
+int qem_bessel01_table(qem_ctx *ctx, int64_t n, const double *x, double *j0o, double *j1o)
 {
     CTX_GUARD(ctx);
     if (n < 0 || (n > 0 && (!x || !j0o || !j1o)))
         return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
-    if (ctx->sw.jt_n == 0)
-        return set_err(ctx, QEM_ERR_UNSUPPORTED, "context is not in mixed precision");
-    const double xmax = ctx->sw.jt_n * (double)ctx->sw.jt_dx;
+    const bool mixed = ctx->p.precision == QEM_PREC_MIXED;
+    const int ntab = mixed ? ctx->sw.jt_n : ctx->sw.jd_n;
+    if (ntab == 0)
+        return set_err(ctx, QEM_ERR_UNSUPPORTED, "context has no Bessel table");
+    const double xmax = ntab * (mixed ? (double)ctx->sw.jt_dx : ctx->sw.jd_dx);
     for (int64_t i = 0; i < n; ++i)
         if (!(x[i] >= 0.0 && x[i] <= xmax))
             return set_err(ctx, QEM_ERR_DOMAIN, "argument outside the table range");
@@ -1337,10 +1423,17 @@ int qem_bessel01_mixed(qem_ctx *ctx, int64_t n, const double *x, double *j0o, do
     API_HIP(ctx, ensure(ctx->d_aux, 3 * b));
     char *base = ctx->d_aux.as<char>();
     API_HIP(ctx, hipMemcpyAsync(base, x, b, hipMemcpyHostToDevice, ctx->stream));
-    k_bessel_tab<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
-        reinterpret_cast<const double *>(base), n, ctx->sw.jtab.as<float4>(), ctx->sw.jt_n,
-        1.0f / ctx->sw.jt_dx, reinterpret_cast<double *>(base + b),
-        reinterpret_cast<double *>(base + 2 * b));
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (mixed)
+        k_bessel_tab<<<g, 256, 0, ctx->stream>>>(
+            reinterpret_cast<const double *>(base), n, ctx->sw.jtab.as<float4>(), ntab,
+            1.0f / ctx->sw.jt_dx, reinterpret_cast<double *>(base + b),
+            reinterpret_cast<double *>(base + 2 * b));
+    else
+        k_bessel_tab64<<<g, 256, 0, ctx->stream>>>(
+            reinterpret_cast<const double *>(base), n, ctx->sw.jdtab.as<double>(), ntab,
+            1.0 / ctx->sw.jd_dx, reinterpret_cast<double *>(base + b),
+            reinterpret_cast<double *>(base + 2 * b));
     API_HIP(ctx, hipGetLastError());
     API_HIP(ctx, hipMemcpyAsync(j0o, base + b, b, hipMemcpyDeviceToHost, ctx->stream));
     API_HIP(ctx, hipMemcpyAsync(j1o, base + 2 * b, b, hipMemcpyDeviceToHost, ctx->stream));

@@ -30,6 +30,49 @@ __device__ __forceinline__ void bessel01_tab(float x, const float4 *__restrict__
     J1 = -dp * inv_dx;
 }
 
+// FP64 J0/J1 from the FP64 table: per interval of width dx a degree-7
+// polynomial p(t) ~ J0(x0 + t dx) (8 doubles), J1 = -p'(t) / dx from the
+// same polynomial.  The index is clamped to the last interval.
+__device__ __forceinline__ void bessel01_tab64(double x, const double *__restrict__ tab, int ntab,
+                                               double inv_dx, double &J0, double &J1)
+{
+    const double xb = x * inv_dx;
+    int m = (int)xb;
+    m = m < ntab - 1 ? m : ntab - 1;
+    const double t = xb - (double)m;
+    const double *__restrict__ c = tab + 8 * m;
+    double p = c[7], dp = 0.0;
+#pragma unroll
+    for (int k = 6; k >= 0; --k) {
+        dp = fma(dp, t, p);
+        p = fma(p, t, c[k]);
+    }
+    J0 = p;
+    J1 = -dp * inv_dx;
+}
+
+// Copies the FP64 Bessel table into this block's LDS (8 doubles per interval).
+__device__ __forceinline__ const double *stage_table64(int jd_n, const double *__restrict__ gtab,
+                                                       void *lds)
+{
+    double2 *d = static_cast<double2 *>(lds);
+    const double2 *g = reinterpret_cast<const double2 *>(gtab);
+    for (int t = threadIdx.x; t < 4 * jd_n; t += blockDim.x) d[t] = g[t];
+    __syncthreads();
+    return static_cast<const double *>(lds);
+}
+
+// Stages the kernel's Bessel table in LDS: the FP32 cubic table in mixed
+// precision, the FP64 table in FP64 (returned through the same pointer,
+// to be read as doubles), or null when an FP64 context has no table.
+template <int PREC>
+__device__ __forceinline__ const float4 *stage_tables(const ShortConsts &c,
+                                                      const float4 *__restrict__ gtab,
+                                                      float4 *lds);
+
+// LDS bytes of the FP64 table (0 without one)
+inline size_t lds_bytes64(const ShortConsts &c) { return (size_t)c.jd_n * 8 * sizeof(double); }
+
 // Copies the Bessel table into this block's LDS.
 __device__ __forceinline__ const float4 *stage_table(int jt_n, const float4 *__restrict__ gtab,
                                                      float4 *lds)
@@ -37,6 +80,16 @@ __device__ __forceinline__ const float4 *stage_table(int jt_n, const float4 *__r
     for (int t = threadIdx.x; t < jt_n; t += blockDim.x) lds[t] = gtab[t];
     __syncthreads();
     return lds;
+}
+
+template <int PREC>
+__device__ __forceinline__ const float4 *stage_tables(const ShortConsts &c,
+                                                      const float4 *__restrict__ gtab,
+                                                      float4 *lds)
+{
+    if (PREC == QEM_PREC_MIXED) return stage_table(c.jt_n, gtab, lds);
+    if (PREC == kPrecFP64Lib || c.jd_n == 0) return nullptr;
+    return reinterpret_cast<const float4 *>(stage_table64(c.jd_n, c.jdtab, lds));
 }
 
 // Quadrature limits of one image (src/greens_function.jl:26-27) and their

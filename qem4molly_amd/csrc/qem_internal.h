@@ -22,6 +22,10 @@ struct DevBuf {
 hipError_t ensure(DevBuf &b, size_t bytes);
 void release(DevBuf &b);
 
+// Kernel template precision for FP64 contexts whose Bessel argument range
+// exceeds the FP64 table: FP64 arithmetic with the device library's j0/j1.
+constexpr int kPrecFP64Lib = 2;
+
 // Constants of one short-range launch (kernel argument, passed by value).
 struct ShortConsts {
     double Lx, Ly, Lz;
@@ -36,6 +40,9 @@ struct ShortConsts {
     int32_t jt_n;        // Bessel table intervals (mixed precision)
     float jt_dx, jt_inv_dx;  // interval width and its inverse
     const float4 *ncon;  // device node constants (mixed precision), or null
+    int32_t jd_n;        // FP64 Bessel table intervals (0: device-library j0/j1)
+    double jd_inv_dx;    // its inverse interval width
+    const double *jdtab; // FP64 table in device memory (staged in LDS by the kernels)
 };
 
 // Gauss-Legendre rule on [-1,1] (ascending) plus its FP32 copy.
@@ -45,10 +52,15 @@ struct GLRule {
 };
 GLRule make_gl_rule(int n);
 
-// Per interval [m dx, (m+1) dx): coefficients of a cubic in t in [0, 1)
-// fitted to J0(m dx + t dx) at Chebyshev points (least squares), 4 floats
-// per interval (c0..c3).
-std::vector<float> make_j0_table(double x_max, double dx, int *n_intervals);
+// Per interval [m dx, (m+1) dx): coefficients c0..cD of a polynomial in
+// t in [0, 1) fitted to J_order(m dx + t dx) (order 0 or 1) at Chebyshev
+// points (least squares in long double), D + 1 values per interval; two
+// padding intervals past x_max.  Both paths tabulate J0 and take J1 from the
+// derivative: the mixed path with D = 3 (one float4), the FP64 path D = 7.
+std::vector<long double> fit_bessel_table(double x_max, double dx, int D, int order,
+                                          int *n_intervals);
+// The FP64 table: J0 interpolated at D + 1 Chebyshev points per interval.
+std::vector<long double> interp_j0_table(double x_max, double dx, int D, int *n_intervals);
 
 // ---- neighbour list (neighbor.hip) ------------------------------------
 struct NeighborList {
@@ -97,6 +109,9 @@ struct ShortWork {
     DevBuf jtab;          // one float4 (cubic coefficients) per interval
     int jt_n = 0;
     float jt_dx = 0.f;
+    DevBuf jdtab;         // FP64: 8 doubles (degree-7 J0 coefficients) per interval
+    int jd_n = 0;
+    double jd_dx = 0.0;
     DevBuf fpair;         // double4 per pair
 };
 // Evaluates every pair of the list into ws.fpair (fx, fy, fz_i, fz_j).

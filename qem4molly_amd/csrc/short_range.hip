@@ -38,9 +38,14 @@ using dev::kLog2e;
 // with D(k) = g12 e^{-2kLz} - 1 and the nodes mapped to [0, K].
 // The constant factors 0.5 b_l (g12 for point) and s_l are applied by the
 // caller.
+// TAB: J0/J1 from the FP64 Bessel table staged in LDS (tab64), else from
+// the device library (argument ranges beyond the table's capacity).
+template <bool TAB>
 __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool gauss,
                                            const ShortConsts &c, const double *__restrict__ glx,
-                                           const double *__restrict__ glw, double &sr, double &sz)
+                                           const double *__restrict__ glw,
+                                           const double *__restrict__ tab64, double &sr,
+                                           double &sz)
 {
     const double h = 0.5 * K;
     const double inv4a = 0.25 / c.alpha;
@@ -53,8 +58,15 @@ __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool 
         const double num = gauss ? exp(-k * a - k * k * inv4a) : exp(-k * a) * E;
         const double wkt = w * k * (num / D);
         const double x = k * rho;
-        ar += wkt * j1(x);
-        az += wkt * j0(x);
+        double J0, J1;
+        if (TAB) {
+            dev::bessel01_tab64(x, tab64, c.jd_n, c.jd_inv_dx, J0, J1);
+        } else {
+            J0 = j0(x);
+            J1 = j1(x);
+        }
+        ar += wkt * J1;
+        az += wkt * J0;
     }
     sr = ar;
     sz = az;
@@ -158,9 +170,11 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
         const double cz = 0.5 * b * a * s * inv32;
         double pr = 0.0, pz = 0.0, gr = 0.0, gz = 0.0;
         if (b != 0.0) {
-            if (PREC == QEM_PREC_FP64) {
-                nodes_fp64(Kp, a, rho, false, c, glx, glw, pr, pz);
-                if (!c.single) nodes_fp64(Kg, a, rho, true, c, glx, glw, gr, gz);
+            if (PREC != QEM_PREC_MIXED) {
+                constexpr bool TAB = PREC == QEM_PREC_FP64;
+                const double *tab64 = reinterpret_cast<const double *>(tab);
+                nodes_fp64<TAB>(Kp, a, rho, false, c, glx, glw, tab64, pr, pz);
+                if (!c.single) nodes_fp64<TAB>(Kg, a, rho, true, c, glx, glw, tab64, gr, gz);
             } else {
                 float fr, fz;
                 nodes_mixed((float)Kp, (float)a, (float)rho, false, c, ncon, tab, fr, fz);
@@ -221,7 +235,7 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
                 const bool gauss = pass == 1;
                 const double K = gauss ? Kg : Kp;
                 double acc = 0.0;
-                if (PREC == QEM_PREC_FP64) {
+                if (PREC != QEM_PREC_MIXED) {
                     const double h = 0.5 * K;
                     const double inv4a = 0.25 / c.alpha;
                     for (int n = 0; n < c.Nt; ++n) {
@@ -271,8 +285,7 @@ __global__ __launch_bounds__(256) void k_pair_list(
     const double *__restrict__ q, double4 *__restrict__ fpair)
 {
     extern __shared__ float4 lds_tab[];
-    const float4 *tab = nullptr;
-    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
+    const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const int64_t i = pi[p], j = pj[p];
@@ -291,8 +304,7 @@ __global__ __launch_bounds__(256) void k_pair_direct(
     const double *__restrict__ qj, double4 *__restrict__ fpair)
 {
     extern __shared__ float4 lds_tab[];
-    const float4 *tab = nullptr;
-    if (PREC == QEM_PREC_MIXED) tab = dev::stage_table(c.jt_n, gtab, lds_tab);
+    const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += stride) {
         PairForce f = pair_force<PREC>(c, glx, glw, ncon, tab, qi[p], qj[p], ri[3 * p],
@@ -394,10 +406,16 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
             ws.jtab.as<float4>(), d_pi, d_pj, npairs, d_xyz, d_q,
             reinterpret_cast<double4 *>(d_fpair));
     } else {
-        k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, d_pi,
-            d_pj, npairs, d_xyz, d_q,
-            reinterpret_cast<double4 *>(d_fpair));
+        if (c.jd_n > 0)
+            k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs), 256, dev::lds_bytes64(c), s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, d_pi,
+                d_pj, npairs, d_xyz, d_q,
+                reinterpret_cast<double4 *>(d_fpair));
+        else
+            k_pair_list<kPrecFP64Lib><<<pair_grid(npairs), 256, 0, s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, d_pi,
+                d_pj, npairs, d_xyz, d_q,
+                reinterpret_cast<double4 *>(d_fpair));
     }
     return hipGetLastError();
 }
@@ -415,10 +433,16 @@ hipError_t launch_pair_direct(const ShortConsts &c, int precision, const ShortWo
             ws.jtab.as<float4>(), m, d_ri, d_rj, d_qi, d_qj,
             reinterpret_cast<double4 *>(d_fpair));
     } else {
-        k_pair_direct<QEM_PREC_FP64><<<pair_grid(m), 256, 0, s>>>(
-            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, m, d_ri,
-            d_rj, d_qi, d_qj,
-            reinterpret_cast<double4 *>(d_fpair));
+        if (c.jd_n > 0)
+            k_pair_direct<QEM_PREC_FP64><<<pair_grid(m), 256, dev::lds_bytes64(c), s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, m, d_ri,
+                d_rj, d_qi, d_qj,
+                reinterpret_cast<double4 *>(d_fpair));
+        else
+            k_pair_direct<kPrecFP64Lib><<<pair_grid(m), 256, 0, s>>>(
+                c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, m, d_ri,
+                d_rj, d_qi, d_qj,
+                reinterpret_cast<double4 *>(d_fpair));
     }
     return hipGetLastError();
 }

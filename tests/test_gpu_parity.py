@@ -243,7 +243,36 @@ def test_bessel_table_accuracy():
     cfg = CONFIGS["slab_n10k"]
     with ctx_for(cfg, precision="mixed") as c:
         x = np.linspace(0.0, 42.0, 20001)
-        j0, j1 = c.bessel01_mixed(x)
+        j0, j1 = c.bessel01_table(x)
     xf = x.astype(np.float32).astype(np.float64)   # the kernel sees FP32 arguments
     assert np.max(np.abs(j0 - special.j0(xf))) < 4e-7
     assert np.max(np.abs(j1 - special.j1(xf))) < 4e-7
+
+
+def test_bessel_table_fp64_accuracy():
+    # FP64 contexts tabulate J0 with degree-7 polynomials per 1/16 (J1 from
+    # the derivative)
+    from scipy import special
+    cfg = CONFIGS["slab_n10k"]
+    with ctx_for(cfg, precision="fp64") as c:
+        x = np.linspace(0.0, 42.0, 40001)
+        j0, j1 = c.bessel01_table(x)
+    assert np.max(np.abs(j0 - special.j0(x))) < 5e-15
+    assert np.max(np.abs(j1 - special.j1(x))) < 5e-15
+
+
+def test_fp64_without_table_uses_device_library():
+    # a Bessel argument range beyond the FP64 table (r_c sqrt(4 alpha ln(1/acc))
+    # > 64): the FP64 kernels call the device library's j0/j1 instead
+    from qem4molly_amd.workloads import SlabConfig
+    cfg = SlabConfig("wide", 60, 20.0, 20.0, 4.0, 0.7, -0.3, 1.0, 9.5, 20, 10)
+    xyz, q = make_system(cfg, seed=4)
+    p = oracle_params(cfg)
+    with ctx_for(cfg, precision="fp64") as c:
+        with pytest.raises(L.QEMError):
+            c.bessel01_table(np.array([1.0]))
+        got = c.short_forces(xyz, q)
+        K, P, sK = c.kset()
+        pairs = c.neighbor_list(xyz)
+    ref = O.short_forces(p, xyz, q, pairs)
+    assert rel_rms(got, ref) < 1e-12
