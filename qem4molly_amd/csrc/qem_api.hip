@@ -246,7 +246,7 @@ constexpr double kBesselDx = 1.0 / 32.0;
 constexpr double kBesselDxMax = 1.0 / 16.0;
 constexpr int kMaxTabIntervals = 4096;        // 64 KiB of LDS (16 B per interval)
 constexpr double kBesselDx64 = 1.0 / 16.0;    // FP64 table spacing
-constexpr int kMaxTab64Intervals = 1024;      // 64 KiB of LDS (64 B per interval)
+constexpr int kMaxTab64Intervals = 910;       // 64 KiB of LDS (72 B per interval)
 
 int set_err(qem_ctx *ctx, int code, const std::string &msg)
 {
@@ -898,7 +898,12 @@ int qem_create(const qem_params *p, qem_ctx **out)
         if (x_max / kBesselDx64 + 2 <= kMaxTab64Intervals) {
             int nm = 0;
             const std::vector<long double> fit = interp_j0_table(x_max, kBesselDx64, 7, &nm);
-            const std::vector<double> tab(fit.begin(), fit.end());
+            // rows of 9 doubles (8 coefficients + padding): consecutive rows
+            // start 18 LDS banks apart, which spreads random lookups over the
+            // banks (a 16-dword stride would put every row on 4 banks)
+            std::vector<double> tab(9 * (size_t)nm, 0.0);
+            for (size_t m = 0; m < (size_t)nm; ++m)
+                for (int k = 0; k < 8; ++k) tab[9 * m + k] = (double)fit[8 * m + k];
             ctx->sw.jd_n = nm;
             ctx->sw.jd_dx = kBesselDx64;
             if (upload(ctx, ctx->sw.jdtab, tab.data(), sizeof(double) * tab.size()))

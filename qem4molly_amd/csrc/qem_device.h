@@ -31,8 +31,8 @@ __device__ __forceinline__ void bessel01_tab(float x, const float4 *__restrict__
 }
 
 // FP64 J0/J1 from the FP64 table: per interval of width dx a degree-7
-// polynomial p(t) ~ J0(x0 + t dx) (8 doubles), J1 = -p'(t) / dx from the
-// same polynomial.  The index is clamped to the last interval.
+// polynomial p(t) ~ J0(x0 + t dx) (8 doubles in a row of 9), J1 = -p'(t) / dx
+// from the same polynomial.  The index is clamped to the last interval.
 __device__ __forceinline__ void bessel01_tab64(double x, const double *__restrict__ tab, int ntab,
                                                double inv_dx, double &J0, double &J1)
 {
@@ -40,7 +40,7 @@ __device__ __forceinline__ void bessel01_tab64(double x, const double *__restric
     int m = (int)xb;
     m = m < ntab - 1 ? m : ntab - 1;
     const double t = xb - (double)m;
-    const double *__restrict__ c = tab + 8 * m;
+    const double *__restrict__ c = tab + 9 * m;
     double p = c[7], dp = 0.0;
 #pragma unroll
     for (int k = 6; k >= 0; --k) {
@@ -51,15 +51,14 @@ __device__ __forceinline__ void bessel01_tab64(double x, const double *__restric
     J1 = -dp * inv_dx;
 }
 
-// Copies the FP64 Bessel table into this block's LDS (8 doubles per interval).
+// Copies the FP64 Bessel table into this block's LDS (9 doubles per interval).
 __device__ __forceinline__ const double *stage_table64(int jd_n, const double *__restrict__ gtab,
                                                        void *lds)
 {
-    double2 *d = static_cast<double2 *>(lds);
-    const double2 *g = reinterpret_cast<const double2 *>(gtab);
-    for (int t = threadIdx.x; t < 4 * jd_n; t += blockDim.x) d[t] = g[t];
+    double *d = static_cast<double *>(lds);
+    for (int t = threadIdx.x; t < 9 * jd_n; t += blockDim.x) d[t] = gtab[t];
     __syncthreads();
-    return static_cast<const double *>(lds);
+    return d;
 }
 
 // Stages the kernel's Bessel table in LDS: the FP32 cubic table in mixed
@@ -71,7 +70,7 @@ __device__ __forceinline__ const float4 *stage_tables(const ShortConsts &c,
                                                       float4 *lds);
 
 // LDS bytes of the FP64 table (0 without one)
-inline size_t lds_bytes64(const ShortConsts &c) { return (size_t)c.jd_n * 8 * sizeof(double); }
+inline size_t lds_bytes64(const ShortConsts &c) { return (size_t)c.jd_n * 9 * sizeof(double); }
 
 // Copies the Bessel table into this block's LDS.
 __device__ __forceinline__ const float4 *stage_table(int jt_n, const float4 *__restrict__ gtab,

@@ -109,7 +109,7 @@ struct ShortWork {
     DevBuf jtab;          // one float4 (cubic coefficients) per interval
     int jt_n = 0;
     float jt_dx = 0.f;
-    DevBuf jdtab;         // FP64: 8 doubles (degree-7 J0 coefficients) per interval
+    DevBuf jdtab;         // FP64: 9 doubles per interval (degree-7 J0 coefficients, pad)
     int jd_n = 0;
     double jd_dx = 0.0;
     DevBuf fpair;         // double4 per pair
