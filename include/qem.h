@@ -173,6 +173,41 @@ int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
                    const double *d_q, const int64_t *k_idx, int64_t P,
                    double *d_f, void *stream);
 
+/* ---- Substrate_LJ walls (src/Substrate_LJ.jl:7-46) ----------------------
+ * A 12-6 Lennard-Jones wall at z = sub_pos acting on every atom's z
+ * coordinate, the other general interaction of the reference's MD setup
+ * (test/QEM_test.jl:20-28).  As written (src/Substrate_LJ.jl:24-46), with
+ * dz = z_i - sub_pos: atoms with dz < cutoff (signed: a wall with
+ * direction -1 acts at every distance) get
+ *   F_z = direction * 24 eps / dz^2 * (2 (sigma/dz)^12 - (sigma/dz)^6),
+ * which is -dU/d(dz) / dz for U = 4 eps ((sigma/dz)^12 - (sigma/dz)^6);
+ * direction * dz <= 0 for any atom fails the reference's @assert. */
+#define QEM_WALL_SEM_ABS_CUTOFF 1u  /* cutoff on |dz| instead of signed dz       */
+#define QEM_WALL_SEM_TRUE_FORCE 2u  /* F_z = -dU/dz (as written: -dU/dz / |dz|) */
+
+typedef struct qem_wall {
+    double cutoff, sigma, eps, sub_pos;
+    int32_t direction;   /* +1: wall below the atoms, -1: above */
+    uint32_t semantics;  /* QEM_WALL_SEM_* bits, 0 = as written  */
+} qem_wall;
+
+/* One wall's forces (host pointers, n x 3); accumulate != 0 adds them to f.
+ * QEM_ERR_DOMAIN if direction * dz <= 0 for some atom (f is untouched). */
+int qem_substrate_lj(qem_ctx *ctx, const qem_wall *w, int64_t n, const double *xyz,
+                     double *f, int32_t accumulate);
+/* Device pointers.  The domain check reads a device flag, so the call
+ * synchronises the stream; on QEM_ERR_DOMAIN d_f holds partial results. */
+int qem_substrate_lj_dev(qem_ctx *ctx, const qem_wall *w, int64_t n, const double *d_xyz,
+                         double *d_f, int32_t accumulate, void *stream);
+/* All general interactions of a Molly system in one device-resident call
+ * (no counterpart in the reference, whose Molly loop moves coordinates and
+ * forces across per interaction): d_f = sum of the walls + QEM_short (step
+ * counter and list rebuilds as qem_short_forces) + QEM_long for the k batch.
+ * nwalls may be 0.  Synchronises the stream when nwalls > 0. */
+int qem_general_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                           const int64_t *k_idx, int64_t P, const qem_wall *walls,
+                           int32_t nwalls, double *d_f, void *stream);
+
 /* ---- adjoint (no counterpart in the reference, which has forces only) ----
  * Energy E whose negative gradient at frozen quadrature limits is the force
  * (with QEM_SEM_CORRECTED the physical slab energy, split by

@@ -272,3 +272,20 @@ def energy(p: Params, xyz, q, pairs, kvecs=None, weight=1.0, full_sum=False,
                 int(bool(full_sum)), int(bool(short)), int(bool(long)), C.byref(E), _pd(phi),
                 _pd(dg), int(nthreads))
     return E.value, phi, dg
+
+
+def substrate_lj(xyz, cutoff=1.0, sigma=0.5, eps=1.0, sub_pos=0.0, direction=1, sem=0):
+    """(F, status): qo_substrate_lj with accumulate = 0; status 1 = the
+    reference's @assert failed (F is zeros then)."""
+    L = lib()
+    if not hasattr(L.qo_substrate_lj, "_sig"):
+        d = C.c_double
+        L.qo_substrate_lj.restype = C.c_int
+        L.qo_substrate_lj.argtypes = [d, d, d, d, C.c_int, C.c_uint, C.c_int64, _P(d), _P(d),
+                                      C.c_int]
+        L.qo_substrate_lj._sig = True
+    xyz = _c(xyz, np.float64).reshape(-1, 3)
+    f = np.zeros_like(xyz)
+    st = L.qo_substrate_lj(float(cutoff), float(sigma), float(eps), float(sub_pos),
+                           int(direction), int(sem), xyz.shape[0], _pd(xyz), _pd(f), 0)
+    return f, st

@@ -880,3 +880,37 @@ void qo_energy(const qo_params *p, int64_t n, const double *xyz, const double *q
     free(phi);
     free(psi);
 }
+
+/* ---- Substrate_LJ wall ---------------------------------------------------- */
+
+/* substrate_LJ_force (src/Substrate_LJ.jl:19-22) and Molly.forces
+ * (src/Substrate_LJ.jl:24-46): per atom dz = z - sub_pos; the @assert on
+ * direction * dz > 0; F_z = direction * substrate_LJ_force(1/dz^2, sigma^2,
+ * eps) when dz < cutoff (signed dz, as written). */
+int qo_substrate_lj(double cutoff, double sigma, double eps, double sub_pos, int direction,
+                    unsigned sem, int64_t n, const double *xyz, double *f, int accumulate)
+{
+    for (int64_t i = 0; i < n; ++i)
+        if (!(direction * (xyz[3 * i + 2] - sub_pos) > 0.0)) return 1;
+    const double sigma2 = sigma * sigma;
+    for (int64_t i = 0; i < n; ++i) {
+        const double dz = xyz[3 * i + 2] - sub_pos;
+        const double dcut = (sem & 1u) ? fabs(dz) : dz;
+        double fz = 0.0;
+        if (dcut < cutoff) {
+            const double invr2 = 1.0 / (dz * dz);
+            const double six = pow(sigma2 * invr2, 3.0);
+            fz = (24.0 * eps * invr2) * (2.0 * six * six - six);
+            if (sem & 2u) fz *= fabs(dz);
+            fz *= (double)direction;
+        }
+        if (accumulate) {
+            f[3 * i + 2] += fz;
+        } else {
+            f[3 * i] = 0.0;
+            f[3 * i + 1] = 0.0;
+            f[3 * i + 2] = fz;
+        }
+    }
+    return 0;
+}

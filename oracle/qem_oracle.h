@@ -142,6 +142,14 @@ void qo_energy(const qo_params *p, int64_t n, const double *xyz, const double *q
                double weight, int full_sum, int with_short, int with_long,
                double *E, double *phi, double *dEdg, int nthreads);
 
+/* Substrate_LJ wall (src/Substrate_LJ.jl:7-46): writes (accumulate == 0)
+ * or adds the z forces of one wall at z = sub_pos.  Returns 0, or 1 (and
+ * writes nothing) when direction * (z_i - sub_pos) <= 0 for some atom (the
+ * reference's @assert, src/Substrate_LJ.jl:37).  sem bit 1: cutoff on |dz|;
+ * bit 2: the true force -dU/dz instead of the as-written -dU/dz / |dz|. */
+int qo_substrate_lj(double cutoff, double sigma, double eps, double sub_pos, int direction,
+                    unsigned sem, int64_t n, const double *xyz, double *f, int accumulate);
+
 #ifdef __cplusplus
 }
 #endif

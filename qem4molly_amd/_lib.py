@@ -49,7 +49,7 @@ EXPORTS = (
     "qem_dev_malloc", "qem_dev_free", "qem_memcpy_htod", "qem_memcpy_dtoh",
     "qem_device_synchronize", "qem_dev_copy_kernel", "qem_timing_enable",
     "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
-    "qem_device_bytes",
+    "qem_device_bytes", "qem_substrate_lj", "qem_substrate_lj_dev", "qem_general_forces_dev",
 )
 
 PHASES = ("neighbor", "pair", "gather", "long")
@@ -71,6 +71,24 @@ class Params(C.Structure):
         ("N_t", C.c_int32), ("iter_period", C.c_int32), ("rbe_p", C.c_int32),
         ("precision", C.c_int32), ("semantics", C.c_uint32), ("device", C.c_int32),
     ]
+
+
+class Wall(C.Structure):
+    """struct qem_wall (include/qem.h): one Substrate_LJ wall."""
+    _fields_ = [
+        ("cutoff", C.c_double), ("sigma", C.c_double), ("eps", C.c_double),
+        ("sub_pos", C.c_double), ("direction", C.c_int32), ("semantics", C.c_uint32),
+    ]
+
+
+WALL_SEM_ABS_CUTOFF = 1
+WALL_SEM_TRUE_FORCE = 2
+
+
+def wall(cutoff=1.0, sigma=0.5, eps=1.0, sub_pos=0.0, direction=1, semantics=0) -> Wall:
+    """A Wall with the reference constructor's defaults (src/Substrate_LJ.jl:15-17)."""
+    return Wall(float(cutoff), float(sigma), float(eps), float(sub_pos), int(direction),
+                int(semantics))
 
 
 _P = C.POINTER
@@ -118,6 +136,10 @@ _SIGS = {
     "qem_hvp": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd]),
     "qem_hvp_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _vp]),
     "qem_device_bytes": (C.c_int, [_vp, _pi64]),
+    "qem_substrate_lj": (C.c_int, [_vp, _P(Wall), _i64, _pd, _pd, _i32]),
+    "qem_substrate_lj_dev": (C.c_int, [_vp, _P(Wall), _i64, _vp, _vp, _i32, _vp]),
+    "qem_general_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _P(Wall), _i32,
+                                         _vp, _vp]),
 }
 
 _lib = None
@@ -319,7 +341,28 @@ class Context:
                                      k_idx.shape[0], ptr(f)), "qem_forces")
         return f
 
+    def substrate_lj(self, xyz, w: "Wall", f=None) -> np.ndarray:
+        """One wall's forces (N x 3); added to f when f is given."""
+        xyz = f64(xyz, (-1, 3))
+        acc = f is not None
+        out = np.ascontiguousarray(f, dtype=np.float64).copy() if acc else np.zeros_like(xyz)
+        self._check(lib().qem_substrate_lj(self._h, C.byref(w), xyz.shape[0], ptr(xyz),
+                                           ptr(out), int(acc)), "qem_substrate_lj")
+        return out
+
     # ---- device-pointer variants -----------------------------------------
+    def substrate_lj_dev(self, n, w: "Wall", d_xyz, d_f, accumulate=False, stream=None):
+        self._check(lib().qem_substrate_lj_dev(self._h, C.byref(w), int(n), d_xyz, d_f,
+                                               int(bool(accumulate)), stream),
+                    "qem_substrate_lj_dev")
+
+    def general_forces_dev(self, n, d_xyz, d_q, k_idx, walls, d_f, stream=None):
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        arr = (Wall * max(1, len(walls)))(*walls)
+        self._check(lib().qem_general_forces_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
+                                                 k_idx.shape[0], arr, len(walls), d_f, stream),
+                    "qem_general_forces_dev")
+
     def forces_dev(self, n, d_xyz, d_q, k_idx, d_f, stream=None):
         k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
         self._check(lib().qem_forces_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),

@@ -931,7 +931,8 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
                       &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,
                       &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,
                       &ctx->lw.svz, &ctx->lw.partT, &ctx->lw.offT, &ctx->lw.totT, &ctx->d_xyz,
-                      &ctx->d_q, &ctx->d_f, &ctx->d_f2, &ctx->d_aux, &ctx->d_adj};
+                      &ctx->d_q, &ctx->d_f, &ctx->d_f2, &ctx->d_aux, &ctx->d_adj,
+                      &ctx->d_flag};
 }
 
 int qem_device_bytes(qem_ctx *ctx, int64_t *bytes)
@@ -1472,6 +1473,98 @@ int qem_dev_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
     return launch_copy(dst, src, bytes, static_cast<hipStream_t>(stream)) == hipSuccess
                ? QEM_OK
                : QEM_ERR_HIP;
+}
+
+// ---- Substrate_LJ walls -------------------------------------------------------
+
+static int check_wall(qem_ctx *ctx, const qem_wall *w)
+{
+    if (!w) return set_err(ctx, QEM_ERR_INVALID_ARG, "null wall");
+    if (w->direction != 1 && w->direction != -1)
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "wall direction must be +1 or -1");
+    if (w->semantics & ~(QEM_WALL_SEM_ABS_CUTOFF | QEM_WALL_SEM_TRUE_FORCE))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "unknown wall semantics bits");
+    if (!(std::isfinite(w->cutoff) && std::isfinite(w->sigma) && std::isfinite(w->eps) &&
+          std::isfinite(w->sub_pos)))
+        return set_err(ctx, QEM_ERR_DOMAIN, "wall parameters must be finite");
+    return QEM_OK;
+}
+
+// one wall on device data; reads the device domain flag (synchronises s)
+static int wall_dev(qem_ctx *ctx, const qem_wall *w, int64_t n, const double *d_xyz, double *d_f,
+                    int accumulate, hipStream_t s)
+{
+    if (n == 0) return QEM_OK;
+    API_HIP(ctx, ensure(ctx->d_flag, sizeof(int) * 2));
+    int *flag = ctx->d_flag.as<int>();
+    API_HIP(ctx, launch_substrate_lj(*w, n, d_xyz, d_f, accumulate, flag, s));
+    int bad = 0;
+    API_HIP(ctx, hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    API_HIP(ctx, hipStreamSynchronize(s));
+    if (bad)
+        return set_err(ctx, QEM_ERR_DOMAIN,
+                       "Substrate_LJ: an atom is on the wrong side of the wall "
+                       "(direction * (z - sub_pos) <= 0)");
+    return QEM_OK;
+}
+
+int qem_substrate_lj(qem_ctx *ctx, const qem_wall *w, int64_t n, const double *xyz, double *f,
+                     int32_t accumulate)
+{
+    CTX_GUARD(ctx);
+    int st = check_wall(ctx, w);
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!xyz || !f))) return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    // the assertion is checked on the host first, so an error writes nothing
+    for (int64_t i = 0; i < n; ++i) {
+        if (!(std::isfinite(xyz[3 * i]) && std::isfinite(xyz[3 * i + 1]) &&
+              std::isfinite(xyz[3 * i + 2])))
+            return set_err(ctx, QEM_ERR_DOMAIN, "coordinates must be finite");
+        if (!((double)w->direction * (xyz[3 * i + 2] - w->sub_pos) > 0.0))
+            return set_err(ctx, QEM_ERR_DOMAIN,
+                           "Substrate_LJ: an atom is on the wrong side of the wall "
+                           "(direction * (z - sub_pos) <= 0)");
+    }
+    if (n == 0) return QEM_OK;
+    const size_t b3 = sizeof(double) * 3 * (size_t)n;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, b3))) return st;
+    API_HIP(ctx, ensure(ctx->d_f, b3));
+    if (accumulate) API_HIP(ctx, hipMemcpyAsync(ctx->d_f.p, f, b3, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = wall_dev(ctx, w, n, ctx->d_xyz.as<double>(), ctx->d_f.as<double>(), accumulate,
+                       ctx->stream)))
+        return st;
+    API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, b3, hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_substrate_lj_dev(qem_ctx *ctx, const qem_wall *w, int64_t n, const double *d_xyz,
+                         double *d_f, int32_t accumulate, void *stream)
+{
+    CTX_GUARD(ctx);
+    int st = check_wall(ctx, w);
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!d_xyz || !d_f)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    return wall_dev(ctx, w, n, d_xyz, d_f, accumulate, pick_stream(ctx, stream));
+}
+
+int qem_general_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                           const int64_t *k_idx, int64_t P, const qem_wall *walls,
+                           int32_t nwalls, double *d_f, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (nwalls < 0 || (nwalls > 0 && !walls))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad wall list");
+    for (int32_t i = 0; i < nwalls; ++i) {
+        const int st = check_wall(ctx, walls + i);
+        if (st) return st;
+    }
+    int st = qem_forces_dev(ctx, n, d_xyz, d_q, k_idx, P, d_f, stream);
+    if (st) return st;
+    for (int32_t i = 0; i < nwalls; ++i)
+        if ((st = wall_dev(ctx, walls + i, n, d_xyz, d_f, 1, pick_stream(ctx, stream)))) return st;
+    return QEM_OK;
 }
 
 }  // extern "C"

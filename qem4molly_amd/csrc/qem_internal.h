@@ -203,6 +203,12 @@ int half_dots_scratch();
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
+// ---- Substrate_LJ walls (wall.hip) --------------------------------------
+// One wall's z forces into d_f (overwrite or add); *d_bad becomes nonzero if
+// some atom is on the wrong side of the wall.
+hipError_t launch_substrate_lj(const qem_wall &w, int64_t n, const double *d_xyz, double *d_f,
+                               int accumulate, int *d_bad, hipStream_t s);
+
 }  // namespace qem
 
 // The context behind the opaque qem_ctx handle.
@@ -224,6 +230,7 @@ struct qem_ctx {
     qem::LongWork lw;
     // host-pointer staging
     qem::DevBuf d_xyz, d_q, d_f, d_f2, d_aux, d_adj;
+    qem::DevBuf d_flag;   // device status word (wall domain check)
     // optional per-phase timing (HIP events on the launch stream)
     bool timing = false;
     struct Mark {

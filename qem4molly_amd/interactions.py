@@ -16,6 +16,8 @@ entry point as the reference's Molly plugin (QEM4Molly, Julia):
     K_set_generator                               src/K_set_generator.jl:5-45
     Fl_non_sort, Fl_sort, Flz_sort_k0, Flz_self,
     F_l_total, F_long                             src/Force_long.jl:5-242
+    Substrate_LJ(; cutoff, sigma, eps, sub_pos, direction)
+                                                  src/Substrate_LJ.jl:7-46
 
 Every numeric result comes from the GPU library; a missing library raises.
 Indices here are 0-based (Python), as are `z_list` values.
@@ -146,6 +148,40 @@ class QEM_long:
         return idx
 
 
+_wall_ctx = None
+
+
+def _utility_context() -> L.Context:
+    """A small context for interactions that need only the device (walls)."""
+    global _wall_ctx
+    if _wall_ctx is None:
+        p = L.default_params()
+        p.rbe_p = 0
+        _wall_ctx = L.Context(p)
+    return _wall_ctx
+
+
+class Substrate_LJ:
+    """12-6 Lennard-Jones wall on the z coordinate (src/Substrate_LJ.jl:7-17).
+
+    `semantics` (WALL_SEM_* bits of _lib) is an addition: 0 reproduces the
+    reference as written (signed-dz cutoff, force -dU/dz / |dz|)."""
+
+    def __init__(self, *, cutoff=1.0, sigma=0.5, eps=1.0, sub_pos=0.0, direction=+1,
+                 semantics=0, ctx=None):
+        self.cutoff = float(cutoff)
+        self.sigma = float(sigma)
+        self.eps = float(eps)
+        self.sub_pos = float(sub_pos)
+        self.direction = int(direction)
+        self.semantics = int(semantics)
+        self.ctx = ctx
+
+    def wall(self) -> L.Wall:
+        return L.wall(self.cutoff, self.sigma, self.eps, self.sub_pos, self.direction,
+                      self.semantics)
+
+
 def QEM_inter(n_atoms, L_, *, s=1.0, accuracy=1e-6, rbe_mode=True, rbe_p=30, eps_0=1.0,
               gamma_1=0.0, gamma_2=0.0, N_t=30, iter_period=100, precision="fp64", seed=0,
               device=0):
@@ -167,7 +203,8 @@ def forces(inter, sys, neighbors=None, *, k_idx=None) -> np.ndarray:
     QEM_short: step counter, neighbour list every iter_period calls
     (src/QEM_short.jl:27-59).  QEM_long: z order, then F_long with a batch
     from the interaction's sampler, or the given `k_idx`
-    (src/QEM_long.jl:32-44)."""
+    (src/QEM_long.jl:32-44).  Substrate_LJ: the wall's z forces
+    (src/Substrate_LJ.jl:24-46)."""
     coords = np.ascontiguousarray(sys.coords, dtype=np.float64).reshape(-1, 3)
     q = _charges(sys)
     if isinstance(inter, QEM_short):
@@ -181,6 +218,11 @@ def forces(inter, sys, neighbors=None, *, k_idx=None) -> np.ndarray:
         if k_idx is None:
             k_idx = inter.sample()
         return inter.ctx.long_forces(coords, q, k_idx)
+    if isinstance(inter, Substrate_LJ):
+        # Molly.forces(::Substrate_LJ) (src/Substrate_LJ.jl:24-46); an atom on
+        # the wrong side of the wall raises (the reference's @assert)
+        ctx = inter.ctx if inter.ctx is not None else _utility_context()
+        return ctx.substrate_lj(coords, inter.wall())
     raise TypeError(f"forces: unsupported interaction {type(inter).__name__}")
 
 
