@@ -178,6 +178,35 @@ def adjoint_cost(name, precision, reps=10):
             "device_bytes": mem}
 
 
+def general_step(cfg, xyz_d, q_d, f_d, steps, warmup, seed0=1000):
+    """The device-resident general-interaction step of test/QEM_test.jl:20-28
+    (two Substrate_LJ walls, at z = 0 facing up and z = Lz facing down, plus
+    QEM short + long) through qem_general_forces_dev, mixed precision; the
+    wall pass and its domain-flag readback are inside the timed region."""
+    from qem4molly_amd import _lib as L
+    ctx = L.Context(_params(cfg, "mixed"))
+    walls = [L.wall(sub_pos=0.0, direction=1), L.wall(sub_pos=cfg.Lz, direction=-1)]
+    step = 0
+
+    def one():
+        nonlocal step
+        kidx = ctx.sample_k(seed0 + step, cfg.P)
+        ctx.general_forces_dev(cfg.n, xyz_d.ptr, q_d.ptr, kidx, walls, f_d.ptr)
+        step += 1
+
+    for _ in range(warmup):
+        one()
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    L.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.close()
+    return {"workload": cfg.name, "walls": 2, "ms_per_step": dt * 1e3,
+            "value": cfg.n / dt, "unit": "atom-force evals/s"}
+
+
 def adjoint_md(name="slab_n100k", steps=100, sigma=0.01):
     """100 steps of positions perturbed by N(0, sigma) per component
     (x, y wrapped, z reflected into [0.5, Lz - 0.5]), fixed seeded k batch,
@@ -345,6 +374,12 @@ def main(argv=None):
             "kernel": "k_pair_list<fp64>", "avg_ms": p64_avg,
         },
     }
+
+    xd_g, qd_g, fd_g = (L.DeviceArray(xyz), L.DeviceArray(q),
+                        L.DeviceArray(np.zeros_like(xyz)))
+    log(f"[bench] {cfg.name}: general step (2 walls + QEM) x {a.steps}")
+    out["general_step"] = general_step(cfg, xd_g, qd_g, fd_g, a.steps, a.warmup)
+    del xd_g, qd_g, fd_g
 
     if not a.no_secondary and a.workload != "slab_n10k":
         c2, _, _, m2, d2, e2 = measure("slab_n10k", 50, 3, 50)

@@ -202,8 +202,12 @@ int qem_substrate_lj_dev(qem_ctx *ctx, const qem_wall *w, int64_t n, const doubl
 /* All general interactions of a Molly system in one device-resident call
  * (no counterpart in the reference, whose Molly loop moves coordinates and
  * forces across per interaction): d_f = sum of the walls + QEM_short (step
- * counter and list rebuilds as qem_short_forces) + QEM_long for the k batch.
- * nwalls may be 0.  Synchronises the stream when nwalls > 0. */
+ * counter and list rebuilds as qem_short_forces) + QEM_long for the k batch,
+ * summed in the reference's order (walls, QEM_short, QEM_long;
+ * test/QEM_test.jl:28).  nwalls may be 0.  With walls, the call waits for
+ * the wall pass's domain flag only, after the QEM kernels are queued behind
+ * it, so consecutive calls keep the stream busy; on QEM_ERR_DOMAIN d_f holds
+ * the full sum all the same. */
 int qem_general_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                            const int64_t *k_idx, int64_t P, const qem_wall *walls,
                            int32_t nwalls, double *d_f, void *stream);

@@ -133,12 +133,15 @@ __global__ void k_rows_to_slots(const double *__restrict__ src, const int32_t *_
 
 __global__ void k_rows_from_slots(const double *__restrict__ src,
                                   const int32_t *__restrict__ perm, int64_t n, int w,
-                                  double *__restrict__ dst)
+                                  double *__restrict__ dst, const double *__restrict__ addz)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * w) return;
     const int64_t s = e / w, c = e - s * w;
-    dst[(int64_t)perm[s] * w + c] = src[e];
+    const int64_t r = perm[s];
+    double v = src[e];
+    if (addz && c == 2) v = __dadd_rn(v, addz[r]);
+    dst[r * w + c] = v;
 }
 
 __global__ void k_row_bounds(const int64_t *__restrict__ row_ptr, int64_t n,
@@ -355,11 +358,12 @@ hipError_t launch_rows_to_slots(const NeighborList &nl, int64_t n, int w, const 
 }
 
 hipError_t launch_rows_from_slots(const NeighborList &nl, int64_t n, int w, const double *src,
-                                  double *dst, hipStream_t s)
+                                  double *dst, hipStream_t s, const double *addz)
 {
     if (n <= 0) return hipSuccess;
+    if (addz && w != 3) return hipErrorInvalidValue;
     k_rows_from_slots<<<nblk(n * w, 256), 256, 0, s>>>(src, nl.atom_idx_sorted.as<int32_t>(), n,
-                                                        w, dst);
+                                                        w, dst, addz);
     return hipGetLastError();
 }
 

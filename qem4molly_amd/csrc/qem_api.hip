@@ -562,8 +562,10 @@ struct PhaseScope {
 };
 
 // device part of Molly.forces(::QEM_short) (src/QEM_short.jl:27-59)
+// addz (optional, n doubles in atom order): added to the z forces in the
+// final pass, after the short-range sum
 int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, double *d_f,
-              hipStream_t s, bool use_counter)
+              hipStream_t s, bool use_counter, const double *addz = nullptr)
 {
     if (n == 0) return QEM_OK;
     if (use_counter) {
@@ -601,7 +603,8 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
         PhaseScope ps_g(ctx, QEM_PHASE_GATHER, s);
         API_HIP(ctx, launch_gather_self(c, ctx->p.precision, ctx->sw, nl, n, px, pq,
                                         ctx->sw.fpair.as<double>(), pf, s));
-        if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, pf, d_f, s));
+        if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, pf, d_f, s, addz));
+        else if (addz) API_HIP(ctx, launch_add_z(n, addz, d_f, s));
     }
     return QEM_OK;
 }
@@ -932,7 +935,7 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
                       &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,
                       &ctx->lw.svz, &ctx->lw.partT, &ctx->lw.offT, &ctx->lw.totT, &ctx->d_xyz,
                       &ctx->d_q, &ctx->d_f, &ctx->d_f2, &ctx->d_aux, &ctx->d_adj,
-                      &ctx->d_flag};
+                      &ctx->d_flag, &ctx->d_wz};
 }
 
 int qem_device_bytes(qem_ctx *ctx, int64_t *bytes)
@@ -954,6 +957,8 @@ int qem_destroy(qem_ctx *ctx)
         if (ctx->lw.h_kv[t]) (void)hipHostFree(ctx->lw.h_kv[t]);
         if (ctx->lw.kv_ev[t]) (void)hipEventDestroy(ctx->lw.kv_ev[t]);
     }
+    if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+    if (ctx->wall_ev) (void)hipEventDestroy(ctx->wall_ev);
     for (auto &m : ctx->marks) {
         (void)hipEventDestroy(m.a);
         (void)hipEventDestroy(m.b);
@@ -1490,18 +1495,29 @@ static int check_wall(qem_ctx *ctx, const qem_wall *w)
     return QEM_OK;
 }
 
-// one wall on device data; reads the device domain flag (synchronises s)
-static int wall_dev(qem_ctx *ctx, const qem_wall *w, int64_t n, const double *d_xyz, double *d_f,
-                    int accumulate, hipStream_t s)
+// walls on device data, at most kMaxWallsPerLaunch per pass; reads the
+// device domain flag once (synchronises s)
+static int wall_dev(qem_ctx *ctx, const qem_wall *walls, int32_t nwalls, int64_t n,
+                    const double *d_xyz, double *d_f, int accumulate, hipStream_t s)
 {
-    if (n == 0) return QEM_OK;
-    API_HIP(ctx, ensure(ctx->d_flag, sizeof(int) * 2));
-    int *flag = ctx->d_flag.as<int>();
-    API_HIP(ctx, launch_substrate_lj(*w, n, d_xyz, d_f, accumulate, flag, s));
-    int bad = 0;
-    API_HIP(ctx, hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (n == 0 || nwalls == 0) return QEM_OK;
+    if (!ctx->d_flag.p) {
+        API_HIP(ctx, ensure(ctx->d_flag, sizeof(uint32_t) * 2));
+        API_HIP(ctx, launch_zero_flag(ctx->d_flag.as<uint32_t>(), s));
+    }
+    uint32_t *flag = ctx->d_flag.as<uint32_t>();
+    if (++ctx->wall_epoch == 0) ++ctx->wall_epoch;   // 0 is the cleared value
+    WallSet ws{};
+    ws.epoch = ctx->wall_epoch;
+    for (int32_t k0 = 0; k0 < nwalls; k0 += kMaxWallsPerLaunch) {
+        ws.nw = std::min<int32_t>(kMaxWallsPerLaunch, nwalls - k0);
+        for (int32_t k = 0; k < ws.nw; ++k) ws.w[k] = walls[k0 + k];
+        API_HIP(ctx, launch_substrate_lj(ws, n, d_xyz, d_f, 3, accumulate || k0 > 0, flag, s));
+    }
+    uint32_t seen = 0;
+    API_HIP(ctx, hipMemcpyAsync(&seen, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     API_HIP(ctx, hipStreamSynchronize(s));
-    if (bad)
+    if (seen == ws.epoch)
         return set_err(ctx, QEM_ERR_DOMAIN,
                        "Substrate_LJ: an atom is on the wrong side of the wall "
                        "(direction * (z - sub_pos) <= 0)");
@@ -1530,7 +1546,7 @@ int qem_substrate_lj(qem_ctx *ctx, const qem_wall *w, int64_t n, const double *x
     if ((st = upload(ctx, ctx->d_xyz, xyz, b3))) return st;
     API_HIP(ctx, ensure(ctx->d_f, b3));
     if (accumulate) API_HIP(ctx, hipMemcpyAsync(ctx->d_f.p, f, b3, hipMemcpyHostToDevice, ctx->stream));
-    if ((st = wall_dev(ctx, w, n, ctx->d_xyz.as<double>(), ctx->d_f.as<double>(), accumulate,
+    if ((st = wall_dev(ctx, w, 1, n, ctx->d_xyz.as<double>(), ctx->d_f.as<double>(), accumulate,
                        ctx->stream)))
         return st;
     API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, b3, hipMemcpyDeviceToHost, ctx->stream));
@@ -1546,7 +1562,7 @@ int qem_substrate_lj_dev(qem_ctx *ctx, const qem_wall *w, int64_t n, const doubl
     if (st) return st;
     if (n < 0 || (n > 0 && (!d_xyz || !d_f)))
         return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
-    return wall_dev(ctx, w, n, d_xyz, d_f, accumulate, pick_stream(ctx, stream));
+    return wall_dev(ctx, w, 1, n, d_xyz, d_f, accumulate, pick_stream(ctx, stream));
 }
 
 int qem_general_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
@@ -1560,10 +1576,47 @@ int qem_general_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const d
         const int st = check_wall(ctx, walls + i);
         if (st) return st;
     }
-    int st = qem_forces_dev(ctx, n, d_xyz, d_q, k_idx, P, d_f, stream);
+    if (nwalls == 0 || n == 0) return qem_forces_dev(ctx, n, d_xyz, d_q, k_idx, P, d_f, stream);
+    if (n < 0 || P < 0 || (P > 0 && !k_idx) || !d_xyz || !d_q || !d_f)
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    std::vector<double> kv;
+    int st = kidx_to_vecs(ctx, k_idx, P, kv);
     if (st) return st;
-    for (int32_t i = 0; i < nwalls; ++i)
-        if ((st = wall_dev(ctx, walls + i, n, d_xyz, d_f, 1, pick_stream(ctx, stream)))) return st;
+    hipStream_t s = pick_stream(ctx, stream);
+    // 1. the walls (the reference lists them first, test/QEM_test.jl:28) into
+    //    d_wz, and the domain flag copied to pinned memory behind an event
+    if (!ctx->d_flag.p) {
+        API_HIP(ctx, ensure(ctx->d_flag, sizeof(uint32_t) * 2));
+        API_HIP(ctx, launch_zero_flag(ctx->d_flag.as<uint32_t>(), s));
+    }
+    if (!ctx->h_flag) {
+        API_HIP(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_flag), sizeof(uint32_t),
+                                   hipHostMallocDefault));
+        API_HIP(ctx, hipEventCreateWithFlags(&ctx->wall_ev, hipEventDisableTiming));
+    }
+    API_HIP(ctx, ensure(ctx->d_wz, sizeof(double) * (size_t)n));
+    uint32_t *flag = ctx->d_flag.as<uint32_t>();
+    if (++ctx->wall_epoch == 0) ++ctx->wall_epoch;
+    WallSet ws{};
+    ws.epoch = ctx->wall_epoch;
+    for (int32_t k0 = 0; k0 < nwalls; k0 += kMaxWallsPerLaunch) {
+        ws.nw = std::min<int32_t>(kMaxWallsPerLaunch, nwalls - k0);
+        for (int32_t k = 0; k < ws.nw; ++k) ws.w[k] = walls[k0 + k];
+        API_HIP(ctx, launch_substrate_lj(ws, n, d_xyz, ctx->d_wz.as<double>(), 1, k0 > 0, flag, s));
+    }
+    API_HIP(ctx, hipMemcpyAsync(ctx->h_flag, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    API_HIP(ctx, hipEventRecord(ctx->wall_ev, s));
+    // 2. QEM_short (its final pass adds the wall z forces) and QEM_long
+    if ((st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, ctx->d_wz.as<double>()))) return st;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    if ((st = long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 1, s))) return st;
+    // 3. the domain check waits for the wall pass only: the QEM kernels of
+    //    this step are already queued behind it
+    API_HIP(ctx, hipEventSynchronize(ctx->wall_ev));
+    if (*ctx->h_flag == ws.epoch)
+        return set_err(ctx, QEM_ERR_DOMAIN,
+                       "Substrate_LJ: an atom is on the wrong side of the wall "
+                       "(direction * (z - sub_pos) <= 0)");
     return QEM_OK;
 }
 

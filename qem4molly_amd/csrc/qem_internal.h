@@ -86,11 +86,13 @@ struct NeighborList {
     DevBuf cell_start, cell_end, counts, row_ptr, tmp;
 };
 // Slot-ordered copies: rows of w doubles, dst[s] = src[atom_idx_sorted[s]]
-// (to slots) and dst[atom_idx_sorted[s]] = src[s] (from slots).
+// (to slots) and dst[atom_idx_sorted[s]] = src[s] (from slots).  With
+// addz (w == 3, one double per atom, atom order) the from-slots copy writes
+// src[s].z + addz[atom] as the z component.
 hipError_t launch_rows_to_slots(const NeighborList &nl, int64_t n, int w, const double *src,
                                 double *dst, hipStream_t s);
 hipError_t launch_rows_from_slots(const NeighborList &nl, int64_t n, int w, const double *src,
-                                  double *dst, hipStream_t s);
+                                  double *dst, hipStream_t s, const double *addz = nullptr);
 // Builds the list for device coordinates; synchronises the stream once
 // (to learn the pair count).  Returns hipSuccess or the first error.
 hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
@@ -204,10 +206,23 @@ int half_dots_scratch();
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
 // ---- Substrate_LJ walls (wall.hip) --------------------------------------
-// One wall's z forces into d_f (overwrite or add); *d_bad becomes nonzero if
-// some atom is on the wrong side of the wall.
-hipError_t launch_substrate_lj(const qem_wall &w, int64_t n, const double *d_xyz, double *d_f,
-                               int accumulate, int *d_bad, hipStream_t s);
+// Up to kMaxWallsPerLaunch walls in one pass (kernel argument, by value).
+constexpr int kMaxWallsPerLaunch = 8;
+struct WallSet {
+    qem_wall w[kMaxWallsPerLaunch];
+    int32_t nw;
+    uint32_t epoch;   // nonzero; written to the flag when an atom is on the wrong side
+};
+// The walls' z forces into d_f (overwrite or add, walls in order): rows of
+// fstride doubles with z at column fstride - 1 (3: an n x 3 force array,
+// whose x and y an overwrite zeroes; 1: z only).  *d_flag becomes ws.epoch
+// if some atom is on the wrong side of some wall; the launch never clears
+// the flag: each call brings a new epoch.
+hipError_t launch_substrate_lj(const WallSet &ws, int64_t n, const double *d_xyz, double *d_f,
+                               int fstride, int accumulate, uint32_t *d_flag, hipStream_t s);
+hipError_t launch_zero_flag(uint32_t *d_flag, hipStream_t s);
+// d_f[3 i + 2] += d_z[i]
+hipError_t launch_add_z(int64_t n, const double *d_z, double *d_f, hipStream_t s);
 
 }  // namespace qem
 
@@ -231,6 +246,10 @@ struct qem_ctx {
     // host-pointer staging
     qem::DevBuf d_xyz, d_q, d_f, d_f2, d_aux, d_adj;
     qem::DevBuf d_flag;   // device status word (wall domain check)
+    qem::DevBuf d_wz;     // wall z forces of a general-interaction step (atom order)
+    uint32_t wall_epoch = 0;
+    uint32_t *h_flag = nullptr;     // pinned copy of the status word
+    hipEvent_t wall_ev = nullptr;   // recorded after that copy
     // optional per-phase timing (HIP events on the launch stream)
     bool timing = false;
     struct Mark {

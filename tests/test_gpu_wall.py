@@ -77,6 +77,44 @@ def test_general_forces_dev_is_walls_plus_qem():
                 c2.substrate_lj_dev(cfg.n, w, d_x.ptr, d_f2.ptr, accumulate=True)
             L.synchronize()
             a, b = d_f1.download(), d_f2.download()
-            assert np.array_equal(a, b)
+            # same terms; the general step sums them in the reference's order
+            # (walls, short, long), the sequential calls in another
+            assert np.allclose(a, b, rtol=1e-14, atol=1e-15 * np.abs(b).max())
     wall_only = sum(O.substrate_lj(xyz, 1.0, 0.5, 1.0, w.sub_pos, w.direction)[0] for w in walls)
     assert np.count_nonzero(wall_only[:, 2]) > 0
+
+
+def test_general_forces_dev_many_walls_in_order():
+    # more walls than one pass takes (8): the second pass adds to the first
+    cfg = scaled_config(CONFIGS["slab_n10k"], 1500)
+    xyz, q = make_system(cfg, seed=6)
+    walls = [L.wall(cutoff=1.0 + 0.05 * i, sigma=0.4 + 0.01 * i, eps=1.0 + 0.1 * i,
+                    sub_pos=-0.03 * i, direction=1, semantics=i % 4) for i in range(6)] + \
+            [L.wall(cutoff=0.9 + 0.05 * i, sigma=0.45, eps=0.7 + 0.2 * i,
+                    sub_pos=cfg.Lz + 0.02 * i, direction=-1, semantics=(i + 1) % 4)
+             for i in range(5)]
+    with ctx_for(cfg, precision="mixed") as c1, ctx_for(cfg, precision="mixed") as c2:
+        kidx = c1.sample_k(3, cfg.P)
+        d_x, d_q = L.DeviceArray(xyz), L.DeviceArray(q)
+        d_f1, d_f2 = L.DeviceArray(np.zeros_like(xyz)), L.DeviceArray(np.zeros_like(xyz))
+        c1.general_forces_dev(cfg.n, d_x.ptr, d_q.ptr, kidx, walls, d_f1.ptr)
+        c2.forces_dev(cfg.n, d_x.ptr, d_q.ptr, kidx, d_f2.ptr)
+        L.synchronize()             # the context stream does not order the copy
+        qem_only = d_f2.download()
+        for w in walls:
+            c2.substrate_lj_dev(cfg.n, w, d_x.ptr, d_f2.ptr, accumulate=True)
+        L.synchronize()
+        a, b = d_f1.download(), d_f2.download()
+        assert np.allclose(a, b, rtol=1e-14, atol=1e-15 * np.abs(b).max())
+        ref = sum(O.substrate_lj(xyz, w.cutoff, w.sigma, w.eps, w.sub_pos, w.direction,
+                                 w.semantics)[0] for w in walls)
+        assert np.count_nonzero(ref[:, 2]) > 0
+        assert np.allclose(a, qem_only + ref, rtol=1e-13, atol=1e-14 * np.abs(a).max())
+        # atoms behind the ninth wall (second pass): the whole call reports it,
+        # and the next call with the original walls does not see the stale flag
+        assert np.any(xyz[:, 2] < 1.0)
+        bad = walls[:8] + [L.wall(sub_pos=1.0, direction=1)] + walls[8:]
+        with pytest.raises(L.QEMError) as e:
+            c1.general_forces_dev(cfg.n, d_x.ptr, d_q.ptr, kidx, bad, d_f1.ptr)
+        assert e.value.status == L.QEM_ERR_DOMAIN
+        c1.general_forces_dev(cfg.n, d_x.ptr, d_q.ptr, kidx, walls, d_f1.ptr)
