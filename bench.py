@@ -207,6 +207,33 @@ def general_step(cfg, xyz_d, q_d, f_d, steps, warmup, seed0=1000):
             "value": cfg.n / dt, "unit": "atom-force evals/s"}
 
 
+def rbe_variance(cfg, xyz_d, q_d, reps=5):
+    """Cost of the random-batch variance diagnostic (qem_long_variance_dev,
+    mixed precision) next to the long-range force of the same batch, and the
+    batch's relative RMS force noise sqrt(sum_i var_i) / |F_long|."""
+    from qem4molly_amd import _lib as L
+    ctx = L.Context(_params(cfg, "mixed"))
+    kidx = ctx.sample_k(77, cfg.P)
+    vd = L.DeviceArray(np.zeros(cfg.n))
+    fd = L.DeviceArray(np.zeros((cfg.n, 3)))
+
+    def timed(fn):
+        fn()
+        L.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        L.synchronize()
+        return (time.perf_counter() - t) / reps * 1e3
+
+    t_v = timed(lambda: ctx.long_variance_dev(cfg.n, xyz_d.ptr, q_d.ptr, kidx, vd.ptr))
+    t_f = timed(lambda: ctx.long_forces_dev(cfg.n, xyz_d.ptr, q_d.ptr, kidx, fd.ptr))
+    var, f = vd.download(), fd.download()
+    ctx.close()
+    return {"workload": cfg.name, "P": cfg.P, "variance_ms": t_v, "long_forces_ms": t_f,
+            "relative_rms_noise": float(np.sqrt(var.sum() / (f ** 2).sum()))}
+
+
 def adjoint_md(name="slab_n100k", steps=100, sigma=0.01):
     """100 steps of positions perturbed by N(0, sigma) per component
     (x, y wrapped, z reflected into [0.5, Lz - 0.5]), fixed seeded k batch,
@@ -379,6 +406,8 @@ def main(argv=None):
                         L.DeviceArray(np.zeros_like(xyz)))
     log(f"[bench] {cfg.name}: general step (2 walls + QEM) x {a.steps}")
     out["general_step"] = general_step(cfg, xd_g, qd_g, fd_g, a.steps, a.warmup)
+    log(f"[bench] {cfg.name}: random-batch variance diagnostic")
+    out["rbe_variance"] = rbe_variance(cfg, xd_g, qd_g)
     del xd_g, qd_g, fd_g
 
     if not a.no_secondary and a.workload != "slab_n10k":

@@ -158,6 +158,21 @@ int qem_long_terms(qem_ctx *ctx, int64_t n, const double *xyz,
                    const double *q, const double *kvecs, int64_t P,
                    double weight, uint32_t parts, double *f);
 
+/* Random-batch diagnostics (no counterpart in the reference, whose batch
+ * estimator is src/Force_long.jl:201-218,234-242).  For the batch k_idx
+ * (P >= 2), with X_ip = (sum_K / P) F_l_total(k_p) on atom i, the k = 0 term
+ * left out (it is not sampled):
+ *   var_i = (P sum_p |X_ip|^2 - |sum_p X_ip|^2) / (P - 1),
+ * the unbiased estimate, from the batch itself, of E|F_i - E F_i|^2 for the
+ * batch force F_i (whose mean over batches is the full k sum weighted by
+ * exp(-k^2/(4 alpha))).  f (may be NULL) receives qem_long_forces for the
+ * same batch. */
+int qem_long_variance(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                      const int64_t *k_idx, int64_t P, double *f, double *var);
+int qem_long_variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                          const int64_t *k_idx, int64_t P, double *d_f, double *d_var,
+                          void *stream);
+
 /* ---- both halves (one MD step's electrostatics) ------------------------ */
 int qem_forces(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
                const int64_t *k_idx, int64_t P, double *f);

@@ -250,6 +250,24 @@ def long_forces(p: Params, xyz, q, kvecs, weight=1.0, full_sum=False, nthreads=1
     return f
 
 
+def long_variance(p: Params, xyz, q, kvecs, weight, nthreads=1):
+    """Per-atom batch variance of the random-batch long-range force
+    (qo_long_variance); kvecs is the batch (P x 3, P >= 2)."""
+    L = lib()
+    if not hasattr(L.qo_long_variance, "_sig"):
+        d, i64 = C.c_double, C.c_int64
+        L.qo_long_variance.restype = None
+        L.qo_long_variance.argtypes = [_P(Params), i64, _P(d), _P(d), _P(d), i64, d, _P(d),
+                                       C.c_int]
+        L.qo_long_variance._sig = True
+    xyz = _c(xyz, np.float64); q = _c(q, np.float64)
+    kv = _c(kvecs, np.float64).reshape(-1, 3)
+    var = np.zeros(xyz.shape[0])
+    L.qo_long_variance(C.byref(p), xyz.shape[0], _pd(xyz), _pd(q), _pd(kv), kv.shape[0],
+                       float(weight), _pd(var), int(nthreads))
+    return var
+
+
 def energy(p: Params, xyz, q, pairs, kvecs=None, weight=1.0, full_sum=False,
            short=True, long=True, nthreads=1):
     """E, phi = dE/dq, (dE/dgamma_1, dE/dgamma_2) -- see qo_energy."""

@@ -664,6 +664,59 @@ void qo_long_forces(const qo_params *p, int64_t n, const double *xyz,
     free(zlist);
 }
 
+/* Batch variance of the random-batch long-range force (no counterpart in
+ * the reference; the estimator is src/Force_long.jl:201-218,234-242):
+ * X_ip = weight * F_l_total(k_p) on atom i, var_i = (P sum_p |X_ip|^2 -
+ * |sum_p X_ip|^2) / (P - 1), the k = 0 term left out. */
+void qo_long_variance(const qo_params *p, int64_t n, const double *xyz, const double *q,
+                      const double *kvecs, int64_t P, double weight, double *var,
+                      int nthreads)
+{
+    int64_t *zlist = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    qo_sortperm_z(n, xyz, zlist);
+    int nt = nthreads > 0 ? nthreads : 1;
+    /* per thread: 3 n sums of X, n sums of |X|^2 */
+    double *part = (double *)calloc((size_t)nt * 4 * (size_t)n, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        int tid = 0, nth = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        int64_t lo = P * tid / nth, hi = P * (tid + 1) / nth;
+        double *fk = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+        double *s1 = part + (size_t)tid * 4 * (size_t)n, *s2 = s1 + 3 * (size_t)n;
+        for (int64_t t = lo; t < hi; ++t) {
+            qo_F_l_total(kvecs + 3 * t, n, xyz, q, zlist, p, fk);
+            for (int64_t i = 0; i < n; ++i) {
+                const double x = weight * fk[3 * i], y = weight * fk[3 * i + 1],
+                             z = weight * fk[3 * i + 2];
+                s1[3 * i] += x;
+                s1[3 * i + 1] += y;
+                s1[3 * i + 2] += z;
+                s2[i] += x * x + y * y + z * z;
+            }
+        }
+        free(fk);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        double x = 0.0, y = 0.0, z = 0.0, ss = 0.0;
+        for (int th = 0; th < nt; ++th) {
+            const double *s1 = part + (size_t)th * 4 * (size_t)n, *s2 = s1 + 3 * (size_t)n;
+            x += s1[3 * i];
+            y += s1[3 * i + 1];
+            z += s1[3 * i + 2];
+            ss += s2[i];
+        }
+        var[i] = ((double)P * ss - (x * x + y * y + z * z)) / ((double)P - 1.0);
+    }
+    free(part);
+    free(zlist);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Energy, dE/dq and dE/dgamma.  The reference has no energy; this is the
  * functional whose negative gradient at frozen quadrature limits is the

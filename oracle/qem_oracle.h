@@ -132,6 +132,13 @@ void qo_long_forces(const qo_params *p, int64_t n, const double *xyz,
                     const double *q, const double *kvecs, int64_t P,
                     double weight, int full_sum, double *f, int nthreads);
 
+/* Per-atom batch variance of the random-batch long-range force (no
+ * counterpart in the reference): X_ip = weight * F_l_total(k_p),
+ * var_i = (P sum_p |X_ip|^2 - |sum_p X_ip|^2) / (P - 1), P >= 2. */
+void qo_long_variance(const qo_params *p, int64_t n, const double *xyz, const double *q,
+                      const double *kvecs, int64_t P, double weight, double *var,
+                      int nthreads);
+
 /* Energy (no counterpart in the reference; see qem_oracle.c for the
  * definition), phi = dE/dq, dE/dgamma_1, dE/dgamma_2. */
 void qo_pair_potential(const qo_params *p, const double *ri, const double *rj,

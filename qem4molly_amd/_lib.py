@@ -50,6 +50,7 @@ EXPORTS = (
     "qem_device_synchronize", "qem_dev_copy_kernel", "qem_timing_enable",
     "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
     "qem_device_bytes", "qem_substrate_lj", "qem_substrate_lj_dev", "qem_general_forces_dev",
+    "qem_long_variance", "qem_long_variance_dev",
 )
 
 PHASES = ("neighbor", "pair", "gather", "long")
@@ -117,6 +118,8 @@ _SIGS = {
     "qem_long_forces": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd]),
     "qem_long_terms": (C.c_int, [_vp, _i64, _pd, _pd, _pd, _i64, _d, _u32, _pd]),
     "qem_forces": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd]),
+    "qem_long_variance": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd]),
+    "qem_long_variance_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _vp]),
     "qem_short_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "qem_long_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp]),
     "qem_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp]),
@@ -332,6 +335,24 @@ class Context:
                                          kv.shape[0], float(weight), int(parts), ptr(f)),
                     "qem_long_terms")
         return f
+
+    def long_variance(self, xyz, q, k_idx):
+        """(F, var): the batch's long-range force (N x 3) and the per-atom
+        batch estimate of its variance (N)."""
+        xyz = f64(xyz, (-1, 3)); q = f64(q)
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        f = np.zeros_like(xyz)
+        var = np.zeros(xyz.shape[0])
+        self._check(lib().qem_long_variance(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
+                                            k_idx.shape[0], ptr(f), ptr(var)),
+                    "qem_long_variance")
+        return f, var
+
+    def long_variance_dev(self, n, d_xyz, d_q, k_idx, d_var, d_f=None, stream=None):
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        self._check(lib().qem_long_variance_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
+                                                k_idx.shape[0], d_f, d_var, stream),
+                    "qem_long_variance_dev")
 
     def forces(self, xyz, q, k_idx) -> np.ndarray:
         xyz = f64(xyz, (-1, 3)); q = f64(q)

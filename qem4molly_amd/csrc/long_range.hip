@@ -397,6 +397,101 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restri
 }
 
 // ---------------------------------------------------------------------------
+// Batch variance (k lanes): per atom and group of 64 k rows, the sum over the
+// k != 0 rows of the row's force X = w F(k) and of |X|^2, into
+// partV[group][atom] = (X_x, X_y, X_z, |X|^2).  Each lane needs its whole
+// per-k force, so the suffix sums above each atom come from a descending
+// pre-walk kept in LDS (chunks of at most kVarChunk atoms) instead of the
+// subtractive form or a second walk.
+constexpr int kVarChunk = 16;
+
+template <int PREC>
+__global__ __launch_bounds__(64 * kLaneWaves) void k_lane_V(const KRec *__restrict__ kr,
+                                                             int64_t Ptot,
+                                                             const double4 *__restrict__ satom,
+                                                             int64_t n, int chk, int nchunk,
+                                                             const double4 *__restrict__ offA,
+                                                             const double4 *__restrict__ totA,
+                                                             double4 *__restrict__ partV)
+{
+    __shared__ double2 sCD[kLaneWaves][kVarChunk][64];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
+    if (chunk >= nchunk) return;
+    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
+    const bool live = p < Ptot;
+    const int64_t pl = live ? p : Ptot - 1;
+    const LaneK L = load_lane(kr[pl], PREC == QEM_PREC_MIXED);
+    const bool real = live && L.k0f == 0.0;   // the k = 0 row is not sampled
+    const double4 T = totA[pl];
+    const double4 off = offA[pl * nchunk + chunk];
+    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
+    // each lane reads back only what it wrote: no barrier
+    double C = off.z, D = off.w;
+    for (int64_t a = a1 - 1; a >= a0; --a) {
+        const double4 at = satom[a];
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        sCD[wv][a - a0][lane] = make_double2(C, D);
+        C += at.w * (c * Em);
+        D += fma(at.w, s * Em, L.k0f * (at.w * at.z));
+    }
+    double4 *out = partV + (int64_t)blockIdx.y * n;
+    double A = off.x, B = off.y;
+    for (int64_t a = a0; a < a1; ++a) {
+        const double4 at = satom[a];
+        const double q = at.w;
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+        const double qz = L.k0f * (q * at.z);
+        const double vx = q * cp, vy = fma(q, sp, qz), vz = q * cm, vw = fma(q, sm, qz);
+        const double2 cd = sCD[wv][a - a0][lane];
+        // the force terms of k_lane_B with exact suffix sums
+        const double f1 = T.x - vx, f2 = T.y - vy, f3 = T.z - vz, f4 = T.w - vw;
+        const double X1 = c * f2 - s * f1, X2 = c * f4 - s * f3;
+        const double Z1 = c * f1 + s * f2, Z2 = c * f3 + s * f4;
+        const double gu = L.N1 * Ep + L.N2 * Em, gw = L.N3 * Em + L.N2 * Ep;
+        const double guz = L.N1 * Ep - L.N2 * Em, gwz = L.N2 * Ep - L.N3 * Em;
+        const double gxn = gu * X1 + gw * X2, gzn = guz * Z1 + gwz * Z2;
+        const double sxy = sm * A - cm * B + sp * cd.x - cp * cd.y;
+        const double szz = cm * A + sm * B - cp * cd.x - sp * cd.y;
+        const double self = L.S1 * (Ep * Ep) - L.S2 * (Em * Em);
+        const double gx = q * (gxn + L.hs * sxy);
+        const double gz = q * (gzn + L.hs * szz + q * self);
+        double fx = real ? L.wx * gx : 0.0;
+        double fy = real ? L.wy * gx : 0.0;
+        double fz = real ? L.w * gz : 0.0;
+        double f2s = fx * fx + fy * fy + fz * fz;
+        A += vx;
+        B += vy;
+        fx = wave_sum63(fx);
+        fy = wave_sum63(fy);
+        fz = wave_sum63(fz);
+        f2s = wave_sum63(f2s);
+        if (lane == 63) out[a] = make_double4(fx, fy, fz, f2s);
+    }
+}
+
+// var[atom] = (P sum|X|^2 - |sum X|^2) / (P - 1) over the groups' partials
+__global__ void k_var_R(const double4 *__restrict__ partV, int ngroup, int64_t n,
+                        const int32_t *__restrict__ perm, double P, double *__restrict__ var)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0, f2 = 0.0;
+    for (int g = 0; g < ngroup; ++g) {
+        const double4 v = partV[(int64_t)g * n + r];
+        fx += v.x;
+        fy += v.y;
+        fz += v.z;
+        f2 += v.w;
+    }
+    var[perm[r]] = (P * f2 - (fx * fx + fy * fy + fz * fz)) / (P - 1.0);
+}
+
+// ---------------------------------------------------------------------------
 // Tangent passes in the k-lane layout (same walk as k_lane_A / k_lane_B).
 // Along the direction v the per-atom terms move with
 //   d(c e^{+kz}) = -s e^{+kz} dth + c e^{+kz} k v_z,   dth = k_x v_x + k_y v_y,
@@ -875,6 +970,49 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Pto
     LR_CHECK(hipGetLastError());
     k_long_R<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ngroup, n, lw.perm.as<int32_t>(),
                                           accumulate, d_f);
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                               int64_t P, double *d_var, hipStream_t s, std::string &err)
+{
+    if (n <= 0) return hipSuccess;
+    if (Ptot <= 0 || P < 2) {
+        err = "the batch variance needs at least two k vectors";
+        return hipErrorInvalidValue;
+    }
+    const int ngroup = (int)((Ptot + 63) / 64), chk = kVarChunk;
+    const int nchunk = (int)((n + chk - 1) / chk);
+    LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
+    LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * nchunk));
+    LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * nchunk));
+    LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
+    LR_CHECK(ensure(lw.partB, sizeof(double4) * (size_t)ngroup * (size_t)n));
+    KRec *kr = lw.kr.as<KRec>();
+    k_make_krec<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot, kr);
+    LR_CHECK(hipGetLastError());
+    const dim3 g((unsigned)((nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)ngroup);
+    const double4 *sa = lw.satom.as<double4>();
+    if (c.precision == QEM_PREC_MIXED)
+        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk,
+                                                               lw.partA.as<double4>());
+    else
+        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk,
+                                                              lw.partA.as<double4>());
+    LR_CHECK(hipGetLastError());
+    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
+                                           lw.offA.as<double4>(), lw.totA.as<double4>());
+    LR_CHECK(hipGetLastError());
+    double4 *pv = lw.partB.as<double4>();
+    if (c.precision == QEM_PREC_MIXED)
+        k_lane_V<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, n, chk, nchunk, lw.offA.as<double4>(), lw.totA.as<double4>(), pv);
+    else
+        k_lane_V<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, n, chk, nchunk, lw.offA.as<double4>(), lw.totA.as<double4>(), pv);
+    LR_CHECK(hipGetLastError());
+    k_var_R<<<nblk(n, 256), 256, 0, s>>>(pv, ngroup, n, lw.perm.as<int32_t>(), (double)P, d_var);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }

@@ -1268,6 +1268,70 @@ int qem_long_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const doub
                     pick_stream(ctx, stream));
 }
 
+// batch variance of the long-range force (and optionally the force itself)
+static int variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                        const std::vector<double> &kv, int64_t P, double *d_f, double *d_var,
+                        hipStream_t s)
+{
+    if (n == 0) return QEM_OK;
+    const double w = ctx->sum_K / (double)P;
+    int64_t Ptot = 0;
+    int st = stage_kvecs(ctx, kv.data(), P, w, false, s, &Ptot);
+    if (st) return st;
+    {
+        PhaseScope ps(ctx, QEM_PHASE_LONG, s);
+        hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "z sort");
+        e = launch_long_variance(long_consts(ctx, w, QEM_LONG_NONSORT | QEM_LONG_SORT | QEM_LONG_SELF),
+                                 ctx->lw, n, Ptot, P, d_var, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "long-range batch variance");
+    }
+    if (d_f) return long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 0, s);
+    return QEM_OK;
+}
+
+int qem_long_variance(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                      const int64_t *k_idx, int64_t P, double *f, double *var)
+{
+    CTX_GUARD(ctx);
+    if (P < 2 || !k_idx) return set_err(ctx, QEM_ERR_INVALID_ARG, "the batch variance needs P >= 2");
+    if (n > 0 && (!q || !var)) return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    std::vector<double> kv;
+    if ((st = kidx_to_vecs(ctx, k_idx, P, kv))) return st;
+    if (n == 0) return QEM_OK;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, sizeof(double) * 3 * (size_t)n)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
+    API_HIP(ctx, ensure(ctx->d_aux, sizeof(double) * (size_t)n));
+    st = variance_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv, P,
+                      f ? ctx->d_f.as<double>() : nullptr, ctx->d_aux.as<double>(), ctx->stream);
+    if (st) return st;
+    API_HIP(ctx, hipMemcpyAsync(var, ctx->d_aux.p, sizeof(double) * (size_t)n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    if (f)
+        API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, sizeof(double) * 3 * (size_t)n,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
+}
+
+int qem_long_variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                          const int64_t *k_idx, int64_t P, double *d_f, double *d_var,
+                          void *stream)
+{
+    CTX_GUARD(ctx);
+    if (P < 2 || !k_idx) return set_err(ctx, QEM_ERR_INVALID_ARG, "the batch variance needs P >= 2");
+    if (n < 0 || (n > 0 && (!d_xyz || !d_q || !d_var)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    std::vector<double> kv;
+    const int st = kidx_to_vecs(ctx, k_idx, P, kv);
+    if (st) return st;
+    return variance_dev(ctx, n, d_xyz, d_q, kv, P, d_f, d_var, pick_stream(ctx, stream));
+}
+
 int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                    const int64_t *k_idx, int64_t P, double *d_f, void *stream)
 {

@@ -176,6 +176,12 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n,
                        int64_t P, double *d_f, int accumulate, hipStream_t s,
                        std::string &err);
 
+// Batch variance of the random-batch long-range force for the k rows in
+// lw.kv (P sampled rows, plus the k = 0 row if staged, which is left out):
+// d_var[atom] = (P sum_p |X_p|^2 - |sum_p X_p|^2) / (P - 1), X_p the row's
+// weighted force on the atom.  P >= 2.
+hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                               int64_t P, double *d_var, hipStream_t s, std::string &err);
 // Energy: phi (dE/dq) of the k rows in lw.kv into d_phi (overwrite or add);
 // sum over k of dE_k/dgamma_1,2 into h_dgamma (synchronises the stream).
 hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
