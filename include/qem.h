@@ -52,11 +52,24 @@ extern "C" {
  * QEM_SEM_MINIMAGE_DIR  xy force direction from the minimum-image separation
  *                       (src/Force_short.jl:57-58 uses raw x_i - x_j)
  * QEM_SEM_NEWTON_IMAGES z force on j from d/dz_j: images 2 and 3 push i and j
- *                       the same way (src/QEM_short.jl:47 negates all of F_ij) */
+ *                       the same way (src/QEM_short.jl:47 negates all of F_ij)
+ * QEM_SEM_POLE_SPLIT    for gamma_1 gamma_2 > 1, where the image-series
+ *                       denominator gamma_1 gamma_2 e^{-2 k Lz} - 1 vanishes
+ *                       at k0 = ln(gamma_1 gamma_2)/(2 Lz): the short-range
+ *                       integrals (all images together, limits
+ *                       max(-ln(acc)/(2 Lz), 2.1 k0) for the point-charge and
+ *                       max(sqrt(-4 alpha ln acc), 2.1 k0) for the gaussian
+ *                       ones) as principal values by the segment rule of
+ *                       src/python_lib/Force_short_norm_njit.py:11-414 and
+ *                       green_norm_njit.py:13-32, middle half-width pole_eps.
+ *                       Without it (as written, src/Force_short.jl) the
+ *                       Gauss-Legendre nodes run across the pole; that is
+ *                       accepted in FP64 only.  No effect for g1 g2 <= 1. */
 #define QEM_SEM_SWAP_LIMITS    1u
 #define QEM_SEM_MINIMAGE_DIR   2u
 #define QEM_SEM_NEWTON_IMAGES  4u
 #define QEM_SEM_CORRECTED      7u
+#define QEM_SEM_POLE_SPLIT     8u
 
 /* ---- long-range parts (qem_long_terms) ------------------------------------ */
 #define QEM_LONG_NONSORT  1u  /* Fl_non_sort  src/Force_long.jl:5-70    */
@@ -81,13 +94,21 @@ typedef struct qem_params {
     int32_t precision;      /* QEM_PREC_FP64 | QEM_PREC_MIXED                */
     uint32_t semantics;     /* QEM_SEM_* bitmask                             */
     int32_t device;         /* HIP device ordinal                            */
+    double pole_eps;        /* QEM_SEM_POLE_SPLIT middle half-width eps
+                               (<= 0: k0 / 20); the reference's INCAR 'eps' */
 } qem_params;
 
 typedef struct qem_ctx qem_ctx;
 
 /* Reference keyword defaults: iter_period=100, gamma=0, eps_0=1,
  * accuracy=1e-6, N_t=30 (src/QEM_short.jl:19), rbe_p=30, alpha=1
- * (src/QEM_long.jl:19); Lx=Ly=Lz=1, r_cutoff=1, FP64, as written. */
+ * (src/QEM_long.jl:19); Lx=Ly=Lz=1, r_cutoff=1, FP64, as written,
+ * pole_eps=0 (k0/20).
+ * gamma_1 gamma_2 may take any finite value.  For gamma_1 gamma_2 > 1,
+ * mixed precision needs QEM_SEM_POLE_SPLIT, and with it qem_create returns
+ * QEM_ERR_UNSUPPORTED when the rule has no nodes for a segment
+ * (floor(N_t k0 / k_f) < 1), as the reference's leggauss(0) raises.  The
+ * energy and HVP entry points need |gamma_1 gamma_2| < 1. */
 void qem_default_params(qem_params *p);
 
 /* Stands in for the QEM_short / QEM_long constructors

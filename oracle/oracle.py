@@ -20,6 +20,7 @@ SEM_SWAP_LIMITS = 1
 SEM_MINIMAGE_DIR = 2
 SEM_NEWTON_IMAGES = 4
 SEM_CORRECTED = SEM_SWAP_LIMITS | SEM_MINIMAGE_DIR | SEM_NEWTON_IMAGES
+SEM_POLE_SPLIT = 8
 
 
 class Params(C.Structure):
@@ -29,6 +30,7 @@ class Params(C.Structure):
         ("gamma_1", C.c_double), ("gamma_2", C.c_double),
         ("eps_0", C.c_double), ("accuracy", C.c_double),
         ("N_t", C.c_int32), ("semantics", C.c_uint32),
+        ("pole_eps", C.c_double),
     ]
 
 
@@ -101,13 +103,51 @@ def _c(a, dtype):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
+def default_pole_eps(gamma_1, gamma_2, Lz):
+    """Middle-segment half-width when none is given: k0 / 20,
+    k0 = ln(gamma_1 gamma_2) / (2 Lz) (0 when there is no pole)."""
+    g12 = gamma_1 * gamma_2
+    return float(np.log(g12) / (2.0 * Lz) / 20.0) if g12 > 1.0 else 0.0
+
+
 def params(L, r_cutoff, alpha, gamma_1=0.0, gamma_2=0.0, eps_0=1.0,
-           accuracy=1e-6, N_t=30, semantics=0) -> Params:
+           accuracy=1e-6, N_t=30, semantics=0, pole_eps=None) -> Params:
     """Mirror of the QEM_short/QEM_long keyword set (src/QEM_short.jl:19)."""
     Lx, Ly, Lz = L
+    if pole_eps is None:
+        pole_eps = default_pole_eps(gamma_1, gamma_2, Lz)
     return Params(float(Lx), float(Ly), float(Lz), float(r_cutoff),
                   float(alpha), float(gamma_1), float(gamma_2), float(eps_0),
-                  float(accuracy), int(N_t), int(semantics))
+                  float(accuracy), int(N_t), int(semantics), float(pole_eps))
+
+
+def pole_rule(g12, Lz, kf_base, eps, N_t):
+    """(status, k0, kf, N1, N3) of the pole-splitting rule (qo_pole_rule)."""
+    L = lib()
+    if not hasattr(L.qo_pole_rule, "_sig"):
+        d = C.c_double
+        L.qo_pole_rule.restype = C.c_int
+        L.qo_pole_rule.argtypes = [d, d, d, d, C.c_int, _P(d), _P(d), _P(C.c_int), _P(C.c_int)]
+        L.qo_pole_rule._sig = True
+    k0, kf, n1, n3 = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+    st = L.qo_pole_rule(float(g12), float(Lz), float(kf_base), float(eps), int(N_t),
+                        C.byref(k0), C.byref(kf), C.byref(n1), C.byref(n3))
+    return st, k0.value, kf.value, n1.value, n3.value
+
+
+POLE_KINDS = {"rho_gauss": 0, "rho_point": 1, "z_gauss": 2, "z_point": 3}
+
+
+def pole_integral(kind, e, mask, kf_base, eps, N_t):
+    """qo_pole_integral: one short-range integral by the pole-splitting rule."""
+    L = lib()
+    if not hasattr(L.qo_pole_integral, "_sig"):
+        d = C.c_double
+        L.qo_pole_integral.restype = d
+        L.qo_pole_integral.argtypes = [C.c_int, _P(Elem), C.c_uint, d, d, C.c_int]
+        L.qo_pole_integral._sig = True
+    return L.qo_pole_integral(POLE_KINDS[kind], C.byref(e), int(mask), float(kf_base),
+                              float(eps), int(N_t))
 
 
 def legendre(n):

@@ -220,6 +220,111 @@ static double gauss_legendre(int kind, const qo_elem *e, double K, int l, const 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Pole splitting for gamma_1 gamma_2 > 1 -- src/python_lib/Force_short_norm_njit.py
+ * (gauss: 11-21, the four intcore functions: 23-73, 138-184, 298-344, segment
+ * sums: 75-93, 186-229; the node counts of Force_short_range_njit.py:258-286) and
+ * the regular part of 1/D, green_norm_njit.py:13-32.  The rule is applied to the
+ * Julia integrands (src/Force_short.jl:4-22) with the images summed. */
+
+/* the integrand times D(k): 1/D taken out */
+static double pole_num(int kind, double k, const qo_elem *e, unsigned mask)
+{
+    double gu = 0.0, gz = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        if (!(mask & (1u << i))) continue;
+        const double t = e->b[i] * exp(-k * e->a[i]);
+        gu += t;
+        gz += k * e->s[i] * t;
+    }
+    gu *= 0.5;
+    gz *= 0.5;
+    const double pt = e->gamma_1 * e->gamma_2 * exp(-2.0 * k * e->L_z);
+    const double ga = exp(-(k * k) / (4.0 * e->alpha));
+    switch (kind) {
+    case F_SR_GAUSS: return k * ga * gu * j1(k * e->rho);
+    case F_SR_POINT: return k * pt * gu * j1(k * e->rho);
+    case F_SZ_GAUSS: return ga * gz * j0(k * e->rho);
+    default:         return pt * gz * j0(k * e->rho);
+    }
+}
+
+/* 1/D + 1/x at x = 2 Lz (k - k0), where D = e^{-x} - 1: with
+ * u = sum_{m=0}^{10} (-x)^m / (m+2)!  (the series of (e^{-x} - 1 + x)/x^2)
+ * and d = -1 + x u (that of D/x), the value is u/d, truncated as the
+ * reference truncates it */
+static double pole_regular(double x)
+{
+    double c[11], f = 2.0;   /* c[m] = 1/(m+2)! */
+    for (int m = 0; m <= 10; ++m) {
+        c[m] = 1.0 / f;
+        f *= (double)(m + 3);
+    }
+    double u = 0.0;
+    for (int m = 10; m >= 0; --m) u = u * (-x) + c[m];
+    return u / (-1.0 + x * u);
+}
+
+int qo_pole_rule(double g12, double Lz, double kf_base, double eps, int N_t, double *k0,
+                 double *kf, int *n1, int *n3)
+{
+    const double kk0 = log(g12) / (2.0 * Lz);
+    const double kkf = fmax(kf_base, 2.1 * kk0);
+    const int m1 = (int)((double)N_t * kk0 / kkf);
+    if (k0) *k0 = kk0;
+    if (kf) *kf = kkf;
+    if (n1) *n1 = m1;
+    if (n3) *n3 = N_t - 2 * m1;
+    if (!(g12 > 1.0) || m1 < 1 || N_t - 2 * m1 < 1 || N_t > 256 || !(eps > 0.0 && eps < kk0))
+        return -1;
+    return 0;
+}
+
+/* the integrand at one node (the three branches of each intcore) */
+static double pole_core(int kind, double k, const qo_elem *e, unsigned mask, double k0,
+                        double eps)
+{
+    const double two_lz = 2.0 * e->L_z;
+    if (fabs(k - k0) <= eps) {
+        const double k1 = k0 - 0.01 * eps, k2 = k0 + 0.01 * eps;
+        const double df = (pole_num(kind, k2, e, mask) - pole_num(kind, k1, e, mask)) / (k2 - k1);
+        return pole_num(kind, k, e, mask) * pole_regular(two_lz * (k - k0)) - df / two_lz;
+    }
+    double f = pole_num(kind, k, e, mask) / green_d(k, e);
+    if (k <= 2.0 * k0) f += pole_num(kind, k0, e, mask) / (two_lz * (k - k0));
+    return f;
+}
+
+static double pole_segment(int kind, const qo_elem *e, unsigned mask, double k0, double eps,
+                           double a, double b, int n)
+{
+    const gl_rule *r = get_rule(n);
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double k = 0.5 * (b - a) * r->x[i] + 0.5 * (b + a);
+        s += 0.5 * (b - a) * (r->w[i] * pole_core(kind, k, e, mask, k0, eps));
+    }
+    return s;
+}
+
+double qo_pole_integral(int kind, const qo_elem *e, unsigned mask, double kf_base,
+                        double eps, int N_t)
+{
+    double k0, kf;
+    int n1, n3;
+    if (qo_pole_rule(e->gamma_1 * e->gamma_2, e->L_z, kf_base, eps, N_t, &k0, &kf, &n1, &n3))
+        return NAN;
+    return pole_segment(kind, e, mask, k0, eps, 0.0, k0 - eps, n1) +
+           pole_segment(kind, e, mask, k0, eps, k0 - eps, k0 + eps, 10) +
+           pole_segment(kind, e, mask, k0, eps, k0 + eps, 2.0 * k0, n1) +
+           pole_segment(kind, e, mask, k0, eps, 2.0 * k0, kf, n3);
+}
+
+static int pole_mode(const qo_params *p)
+{
+    return (p->semantics & QO_SEM_POLE_SPLIT) && p->gamma_1 * p->gamma_2 > 1.0;
+}
+
+/* ------------------------------------------------------------------------ */
 static void min_image(double dx, double dy, double Lx, double Ly, double *mx, double *my)
 {
     *mx = dx - Lx * rint(dx / Lx);
@@ -246,14 +351,22 @@ void qo_F_short_ij(const qo_params *p, double qi, double qj, const double *ri,
     const int swap = (p->semantics & QO_SEM_SWAP_LIMITS) != 0;
     const double *Kp = swap ? e.k_f1 : e.k_f2; /* point-charge integrals */
     const double *Kg = swap ? e.k_f2 : e.k_f1; /* gaussian integrals     */
+    /* pole splitting: common limits, the images summed (odd: 1 and 4, even:
+     * 2 and 3, kept apart for the j-side option) */
+    const int pole = pole_mode(p);
+    const double kfp = -log(p->accuracy) / (2.0 * p->Lz);
+    const double kfg = sqrt(-4.0 * p->alpha * log(p->accuracy));
 
     double Fsx = 0.0, Fsy = 0.0;
     if (rho != 0.0) {
         double p1 = 0.0, p2 = 0.0, g = 0.0;
-        for (int l = 1; l <= 4; ++l) p1 += gauss_legendre(F_SR_POINT, &e, Kp[l - 1], l, r);
+        if (pole) p1 = qo_pole_integral(F_SR_POINT, &e, 15u, kfp, p->pole_eps, p->N_t);
+        else
+            for (int l = 1; l <= 4; ++l) p1 += gauss_legendre(F_SR_POINT, &e, Kp[l - 1], l, r);
         for (int l = 0; l < 4; ++l) p2 += e.b[l] * rho / pow(e.a[l] * e.a[l] + rho * rho, 1.5);
         p2 *= 0.5;
-        if (!single)
+        if (!single && pole) g = qo_pole_integral(F_SR_GAUSS, &e, 15u, kfg, p->pole_eps, p->N_t);
+        else if (!single)
             for (int l = 1; l <= 4; ++l) g += gauss_legendre(F_SR_GAUSS, &e, Kg[l - 1], l, r);
         double Fsr = -p1 + p2 + g;
         double dirx, diry;
@@ -265,18 +378,33 @@ void qo_F_short_ij(const qo_params *p, double qi, double qj, const double *ri,
     /* z: keep the per-image parts for the j-side option */
     double zp1[4], zp2[4], zg[4];
     double P1 = 0.0, P2 = 0.0, G = 0.0;
-    for (int l = 1; l <= 4; ++l) {
-        zp1[l - 1] = gauss_legendre(F_SZ_POINT, &e, Kp[l - 1], l, r);
-        P1 += zp1[l - 1];
+    if (pole) {
+        /* the odd and even image groups' integrals, carried on images 1 and 2 */
+        zp1[0] = qo_pole_integral(F_SZ_POINT, &e, 9u, kfp, p->pole_eps, p->N_t);
+        zp1[1] = qo_pole_integral(F_SZ_POINT, &e, 6u, kfp, p->pole_eps, p->N_t);
+        zp1[2] = zp1[3] = 0.0;
+        P1 = zp1[0] + zp1[1];
+    } else {
+        for (int l = 1; l <= 4; ++l) {
+            zp1[l - 1] = gauss_legendre(F_SZ_POINT, &e, Kp[l - 1], l, r);
+            P1 += zp1[l - 1];
+        }
     }
     for (int l = 0; l < 4; ++l) {
         zp2[l] = e.b[l] * e.a[l] * e.s[l] / pow(e.a[l] * e.a[l] + rho * rho, 1.5);
         P2 += zp2[l];
     }
     P2 *= 0.5;
-    for (int l = 1; l <= 4; ++l) {
-        zg[l - 1] = single ? 0.0 : gauss_legendre(F_SZ_GAUSS, &e, Kg[l - 1], l, r);
-        G += zg[l - 1];
+    if (pole) {
+        zg[0] = single ? 0.0 : qo_pole_integral(F_SZ_GAUSS, &e, 9u, kfg, p->pole_eps, p->N_t);
+        zg[1] = single ? 0.0 : qo_pole_integral(F_SZ_GAUSS, &e, 6u, kfg, p->pole_eps, p->N_t);
+        zg[2] = zg[3] = 0.0;
+        G = zg[0] + zg[1];
+    } else {
+        for (int l = 1; l <= 4; ++l) {
+            zg[l - 1] = single ? 0.0 : gauss_legendre(F_SZ_GAUSS, &e, Kg[l - 1], l, r);
+            G += zg[l - 1];
+        }
     }
     double Fsz = P1 - P2 - G;
     double pref = qi * qj / (QO_TWO_PI * p->eps_0);
@@ -284,7 +412,12 @@ void qo_F_short_ij(const qo_params *p, double qi, double qj, const double *ri,
     out[1] = pref * Fsy;
     out[2] = pref * Fsz;
     if (fz_j) {
-        if (p->semantics & QO_SEM_NEWTON_IMAGES) {
+        if ((p->semantics & QO_SEM_NEWTON_IMAGES) && pole) {
+            /* odd images change sign under z_i <-> z_j */
+            const double odd = zp1[0] - 0.5 * (zp2[0] + zp2[3]) - zg[0];
+            const double even = zp1[1] - 0.5 * (zp2[1] + zp2[2]) - zg[1];
+            *fz_j = pref * (-odd + even);
+        } else if (p->semantics & QO_SEM_NEWTON_IMAGES) {
             double part[4];
             for (int l = 0; l < 4; ++l) part[l] = zp1[l] - 0.5 * zp2[l] - zg[l];
             *fz_j = pref * (-part[0] + part[1] + part[2] - part[3]);
@@ -304,13 +437,21 @@ void qo_F_short_i(const qo_params *p, double qi, const double *ri, int single, d
     const double *Kp = swap ? e.k_f1 : e.k_f2;
     const double *Kg = swap ? e.k_f2 : e.k_f1;
     double P1 = 0.0, P2 = 0.0, G = 0.0;
-    for (int l = 1; l <= 4; ++l) P1 += gauss_legendre(F_SZ_POINT, &e, Kp[l - 1], l, r);
+    const int pole = pole_mode(p);
+    if (pole)
+        P1 = qo_pole_integral(F_SZ_POINT, &e, 15u, -log(p->accuracy) / (2.0 * p->Lz),
+                              p->pole_eps, p->N_t);
+    else
+        for (int l = 1; l <= 4; ++l) P1 += gauss_legendre(F_SZ_POINT, &e, Kp[l - 1], l, r);
     for (int l = 1; l <= 2; ++l) { /* images 2 and 3 (0-based 1, 2) */
         double a = e.a[l];
         P2 += e.b[l] * a * e.s[l] / (a * a * a);
     }
     P2 *= 0.5;
-    if (!single)
+    if (!single && pole)
+        G = qo_pole_integral(F_SZ_GAUSS, &e, 15u, sqrt(-4.0 * p->alpha * log(p->accuracy)),
+                             p->pole_eps, p->N_t);
+    else if (!single)
         for (int l = 1; l <= 4; ++l) G += gauss_legendre(F_SZ_GAUSS, &e, Kg[l - 1], l, r);
     double Fsz = -P1 + P2 + G;
     out[0] = 0.0;

@@ -40,6 +40,8 @@ extern "C" {
 #define QO_SEM_SWAP_LIMITS   1u  /* gauss integrals on k_f2, point on k_f1   */
 #define QO_SEM_MINIMAGE_DIR  2u  /* xy direction from minimum-image dx/rho   */
 #define QO_SEM_NEWTON_IMAGES 4u  /* j-side z force uses d/dz_j (images 2,3)  */
+#define QO_SEM_POLE_SPLIT    8u  /* gamma_1 gamma_2 > 1: principal-value rule
+                                    of src/python_lib/Force_short_norm_njit.py */
 
 typedef struct qo_params {
     double Lx, Ly, Lz;
@@ -51,6 +53,7 @@ typedef struct qo_params {
     double accuracy;
     int32_t N_t;
     uint32_t semantics;
+    double pole_eps;  /* half-width of the middle segment (QO_SEM_POLE_SPLIT) */
 } qo_params;
 
 /* greens_element (src/greens_function.jl:4-15) */
@@ -82,6 +85,21 @@ void qo_F_short_ij(const qo_params *p, double qi, double qj, const double *ri,
                    const double *rj, int single, double *out, double *fz_j);
 void qo_F_short_i(const qo_params *p, double qi, const double *ri, int single,
                   double *out);
+
+/* Pole splitting for gamma_1 gamma_2 > 1 (src/python_lib/Force_short_norm_njit.py:
+ * 11-414 with green_norm_njit.py:13-32): the integrand N(k)/D(k) of one short-range
+ * integral, N summed over the images in mask (bit l-1 = image l), integrated over
+ * [0, k0-eps], [k0-eps, k0+eps], [k0+eps, 2k0], [2k0, kf] with N1, 10, N1,
+ * N_t - 2 N1 Gauss-Legendre nodes, k0 = ln(g1 g2)/(2 Lz), kf = max(kf_base, 2.1 k0),
+ * N1 = floor(N_t k0 / kf); N(k0)/(2 Lz (k - k0)) is added on [0, 2k0] and the
+ * middle segment uses N(k) (1/D + 1/x) - N'(k0)/(2 Lz).  kind: 0 rho gauss,
+ * 1 rho point, 2 z gauss, 3 z point.  kf_base: -ln(acc)/(2 Lz) for the point
+ * integrals, sqrt(-4 alpha ln acc) for the gaussian ones.  Returns NaN when
+ * the rule does not exist (N1 < 1, N_t - 2 N1 < 1, or eps outside (0, k0)). */
+int qo_pole_rule(double g12, double Lz, double kf_base, double eps, int N_t, double *k0,
+                 double *kf, int *n1, int *n3);
+double qo_pole_integral(int kind, const qo_elem *e, unsigned mask, double kf_base,
+                        double eps, int N_t);
 
 /* O(N^2) minimum-image 2-D pair enumeration, i < j, row-major order.
  * Returns the number of pairs; writes at most cap pairs (2 int32 each). */

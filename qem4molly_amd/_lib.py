@@ -30,6 +30,7 @@ SEM_SWAP_LIMITS = 1
 SEM_MINIMAGE_DIR = 2
 SEM_NEWTON_IMAGES = 4
 SEM_CORRECTED = 7
+SEM_POLE_SPLIT = 8
 
 LONG_NONSORT = 1
 LONG_SORT = 2
@@ -71,6 +72,7 @@ class Params(C.Structure):
         ("eps_0", C.c_double), ("accuracy", C.c_double),
         ("N_t", C.c_int32), ("iter_period", C.c_int32), ("rbe_p", C.c_int32),
         ("precision", C.c_int32), ("semantics", C.c_uint32), ("device", C.c_int32),
+        ("pole_eps", C.c_double),
     ]
 
 

@@ -237,6 +237,64 @@ __global__ void k_bessel_tab(const double *__restrict__ x, int64_t n,
 
 using namespace qem;
 
+double qem::pole_eps_of(const qem_params &p)
+{
+    const double k0 = std::log(p.gamma_1 * p.gamma_2) / (2.0 * p.Lz);
+    return p.pole_eps > 0.0 ? p.pole_eps : k0 / 20.0;
+}
+
+// The segment rule of src/python_lib/Force_short_norm_njit.py (node counts
+// Force_short_range_njit.py:258-286): per node k_n the constant
+// c_n = w_n k_n pf(k_n) R_n, R_n = 1/D(k_n) off the middle segment and the
+// regular part 1/D + 1/x (green_norm_njit.py:13-32, the series of
+// (e^{-x} - 1 + x)/x^2 to the 10th power over that of D/x) on it.
+bool qem::make_pole_set(const qem_params &p, double eps, bool gauss, PoleSet &out)
+{
+    const double g12 = p.gamma_1 * p.gamma_2;
+    if (!(g12 > 1.0) || p.N_t > 256) return false;
+    const double two_lz = 2.0 * p.Lz, la = std::log(p.accuracy);
+    const double k0 = std::log(g12) / two_lz;
+    const double base = gauss ? std::sqrt(-4.0 * p.alpha * la) : -la / two_lz;
+    const double kf = std::max(base, 2.1 * k0);
+    const int n1 = (int)((double)p.N_t * k0 / kf), n3 = p.N_t - 2 * n1;
+    if (n1 < 1 || n3 < 1 || !(eps > 0.0 && eps < k0)) return false;
+    out = PoleSet();
+    out.n1 = n1;
+    out.n3 = n3;
+    out.kf = kf;
+    auto pf = [&](double k) {
+        return gauss ? std::exp(-(k * k) / (4.0 * p.alpha)) : g12 * std::exp(-two_lz * k);
+    };
+    auto regular = [](double x) {
+        double c[11], f = 2.0;
+        for (int m = 0; m <= 10; ++m) {
+            c[m] = 1.0 / f;
+            f *= (double)(m + 3);
+        }
+        double u = 0.0;
+        for (int m = 10; m >= 0; --m) u = u * (-x) + c[m];
+        return u / (-1.0 + x * u);
+    };
+    const double seg[4][2] = {{0.0, k0 - eps}, {k0 - eps, k0 + eps}, {k0 + eps, 2.0 * k0},
+                              {2.0 * k0, kf}};
+    const int cnt[4] = {n1, 10, n1, n3};
+    for (int sgi = 0; sgi < 4; ++sgi) {
+        const GLRule r = make_gl_rule(cnt[sgi]);
+        const double a = seg[sgi][0], b = seg[sgi][1];
+        for (int i = 0; i < cnt[sgi]; ++i) {
+            const double k = 0.5 * (b - a) * r.x[(size_t)i] + 0.5 * (b + a);
+            const double w = 0.5 * (b - a) * r.w[(size_t)i];
+            const double R = sgi == 1 ? regular(two_lz * (k - k0))
+                                      : 1.0 / (g12 * std::exp(-two_lz * k) - 1.0);
+            out.k.push_back(k);
+            out.c.push_back(w * k * pf(k) * R);
+            if (sgi == 1) out.Wm += w;
+            if (sgi == 0 || sgi == 2) out.Wx += w / (two_lz * (k - k0));
+        }
+    }
+    return true;
+}
+
 namespace {
 
 constexpr double kTwoPi = 6.283185307179586;  // Float64(2pi)
@@ -304,14 +362,29 @@ int check_params(const qem_params *p, std::string &msg)
     if (p->rbe_p < 0) return bad("rbe_p must be >= 0");
     if (p->precision != QEM_PREC_FP64 && p->precision != QEM_PREC_MIXED)
         return bad("precision must be QEM_PREC_FP64 or QEM_PREC_MIXED");
-    if (p->semantics & ~QEM_SEM_CORRECTED) return bad("unknown semantics bits");
-    if (!(std::fabs(p->gamma_1 * p->gamma_2) < 1.0)) {
-        msg = "|gamma_1 gamma_2| must be < 1: the image-series denominator "
-              "gamma_1 gamma_2 exp(-2 k Lz) - 1 vanishes otherwise";
-        return QEM_ERR_UNSUPPORTED;
+    if (p->semantics & ~(QEM_SEM_CORRECTED | QEM_SEM_POLE_SPLIT))
+        return bad("unknown semantics bits");
+    if (!std::isfinite(p->pole_eps)) return bad("pole_eps must be finite");
+    const double g12 = p->gamma_1 * p->gamma_2;
+    if (g12 > 1.0) {
+        // the denominator gamma_1 gamma_2 exp(-2 k Lz) - 1 vanishes at k0 > 0
+        if (p->semantics & QEM_SEM_POLE_SPLIT) {
+            PoleSet ps;
+            if (!make_pole_set(*p, pole_eps_of(*p), false, ps) ||
+                !make_pole_set(*p, pole_eps_of(*p), true, ps)) {
+                msg = "the pole-splitting rule has no nodes for a segment: it needs "
+                      "floor(N_t k0 / k_f) >= 1 and 0 < pole_eps < k0";
+                return QEM_ERR_UNSUPPORTED;
+            }
+        } else if (p->precision == QEM_PREC_MIXED) {
+            msg = "gamma_1 gamma_2 > 1 in mixed precision needs QEM_SEM_POLE_SPLIT: "
+                  "the quadrature nodes run across the pole of 1/D";
+            return QEM_ERR_UNSUPPORTED;
+        }
     }
     return QEM_OK;
 }
+
 
 // positions finite, z strictly inside the slab, charges finite
 int check_atoms(qem_ctx *ctx, int64_t n, const double *xyz, const double *q)
@@ -358,6 +431,34 @@ ShortConsts short_consts(const qem_ctx *ctx, int single)
     c.jd_n = ctx->sw.jd_n;
     c.jd_inv_dx = ctx->sw.jd_dx > 0 ? 1.0 / ctx->sw.jd_dx : 0.0;
     c.jdtab = ctx->sw.jdtab.as<double>();
+    c.pole = 0;
+    c.pm[0] = c.pm[1] = 0;
+    c.pnode = nullptr;
+    c.pnodef = nullptr;
+    c.pk0 = c.pdel = 0.0;
+    for (int s = 0; s < 2; ++s) {
+        c.pWx[s] = c.pWm[s] = c.pdpf[s] = 0.0;
+        c.ppf[s][0] = c.ppf[s][1] = c.ppf[s][2] = 0.0;
+    }
+    if (pole_active(p)) {
+        const double g12 = c.g12, two_lz = 2.0 * p.Lz;
+        c.pole = 1;
+        c.pm[0] = ctx->pole_m[0];
+        c.pm[1] = ctx->pole_m[1];
+        c.pnode = ctx->sw.pnode.as<double2>();
+        c.pnodef = ctx->sw.pnodef.as<float2>();
+        c.pk0 = std::log(g12) / two_lz;
+        c.pdel = 0.01 * pole_eps_of(p);
+        for (int s = 0; s < 2; ++s) {
+            c.pWx[s] = ctx->pole_wx[s];
+            c.pWm[s] = ctx->pole_wm[s];
+            const double ks[3] = {c.pk0 - c.pdel, c.pk0, c.pk0 + c.pdel};
+            for (int t = 0; t < 3; ++t)
+                c.ppf[s][t] = s ? std::exp(-(ks[t] * ks[t]) / (4.0 * p.alpha))
+                                : g12 * std::exp(-two_lz * ks[t]);
+            c.pdpf[s] = s ? -(c.pk0 / (2.0 * p.alpha)) * c.ppf[s][1] : -two_lz * c.ppf[s][1];
+        }
+    }
     return c;
 }
 
@@ -682,10 +783,21 @@ LongConsts long_consts(const qem_ctx *ctx, double weight, uint32_t parts)
 }
 
 // E, phi = dE/dq (device, N) and dE/dgamma for the current list + k batch
+// the energy and tangent kernels implement the image-series forms for
+// |gamma_1 gamma_2| < 1 only
+static int check_adjoint(qem_ctx *ctx)
+{
+    if (!(std::fabs(ctx->p.gamma_1 * ctx->p.gamma_2) < 1.0))
+        return set_err(ctx, QEM_ERR_UNSUPPORTED,
+                       "energy and Hessian-vector products need |gamma_1 gamma_2| < 1");
+    return QEM_OK;
+}
+
 int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                const double *kvecs, int64_t P, double weight, double *d_phi, double *E,
                double *dEdg, hipStream_t s)
 {
+    if (int st0 = check_adjoint(ctx)) return st0;
     if (n == 0) {
         if (E) *E = 0.0;
         if (dEdg) dEdg[0] = dEdg[1] = 0.0;
@@ -746,6 +858,7 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
 int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, const double *kvecs,
             int64_t P, double weight, const double *d_v, double *d_hv, hipStream_t s)
 {
+    if (int st0 = check_adjoint(ctx)) return st0;
     if (n == 0) return QEM_OK;
     int st = ensure_list(ctx, n, d_xyz, s);
     if (st) return st;
@@ -871,10 +984,38 @@ int qem_create(const qem_params *p, qem_ctx **out)
         upload(ctx, ctx->sw.glxf, xf.data(), sizeof(float) * p->N_t) ||
         upload(ctx, ctx->sw.glwf, wf.data(), sizeof(float) * p->N_t))
         return fail(QEM_ERR_HIP);
+    // the pole rule's nodes (pair-independent), point set then gaussian set
+    double kmax_pole = 0.0;
+    if (pole_active(*p)) {
+        PoleSet ps[2];
+        const double eps = pole_eps_of(*p);
+        if (!make_pole_set(*p, eps, false, ps[0]) || !make_pole_set(*p, eps, true, ps[1])) {
+            qem_destroy(ctx);
+            return QEM_ERR_UNSUPPORTED;
+        }
+        std::vector<double> nd;
+        std::vector<float> ndf;
+        for (int s = 0; s < 2; ++s) {
+            ctx->pole_m[s] = (int)ps[s].k.size();
+            ctx->pole_wx[s] = ps[s].Wx;
+            ctx->pole_wm[s] = ps[s].Wm;
+            kmax_pole = std::max(kmax_pole, ps[s].kf);
+            for (size_t n = 0; n < ps[s].k.size(); ++n) {
+                nd.push_back(ps[s].k[n]);
+                nd.push_back(ps[s].c[n]);
+                ndf.push_back((float)ps[s].k[n]);
+                ndf.push_back((float)ps[s].c[n]);
+            }
+        }
+        if (upload(ctx, ctx->sw.pnode, nd.data(), sizeof(double) * nd.size()) ||
+            upload(ctx, ctx->sw.pnodef, ndf.data(), sizeof(float) * ndf.size()))
+            return fail(QEM_ERR_HIP);
+    }
     if (p->precision == QEM_PREC_MIXED) {
         // largest Bessel argument: r_cutoff times the largest limit
         const double la = -std::log(p->accuracy);
-        const double kmax = std::max(std::sqrt(4.0 * p->alpha * la), la / (2.0 * p->Lz));
+        const double kmax = std::max({std::sqrt(4.0 * p->alpha * la), la / (2.0 * p->Lz),
+                                      kmax_pole});
         double dx = kBesselDx;
         while (p->r_cutoff * kmax * (1.0 + 1e-6) / dx + 4 > kMaxTabIntervals && dx < kBesselDxMax)
             dx *= 2.0;
@@ -896,7 +1037,8 @@ int qem_create(const qem_params *p, qem_ctx **out)
         // more than kMaxTab64Intervals the kernels call the device library's
         // j0/j1
         const double la = -std::log(p->accuracy);
-        const double kmax = std::max(std::sqrt(4.0 * p->alpha * la), la / (2.0 * p->Lz));
+        const double kmax = std::max({std::sqrt(4.0 * p->alpha * la), la / (2.0 * p->Lz),
+                                      kmax_pole});
         const double x_max = p->r_cutoff * kmax * (1.0 + 1e-6) + kBesselDx64;
         if (x_max / kBesselDx64 + 2 <= kMaxTab64Intervals) {
             int nm = 0;
@@ -922,6 +1064,7 @@ int qem_create(const qem_params *p, qem_ctx **out)
 static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
 {
     return {&ctx->sw.ncon, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab, &ctx->sw.jdtab,
+                      &ctx->sw.pnode, &ctx->sw.pnodef,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
                       &ctx->nl.sxyz, &ctx->nl.sq, &ctx->nl.sv, &ctx->nl.sout,

@@ -43,7 +43,37 @@ struct ShortConsts {
     int32_t jd_n;        // FP64 Bessel table intervals (0: device-library j0/j1)
     double jd_inv_dx;    // its inverse interval width
     const double *jdtab; // FP64 table in device memory (staged in LDS by the kernels)
+    // pole splitting (gamma_1 gamma_2 > 1 with QEM_SEM_POLE_SPLIT)
+    int32_t pole;             // 1: the pair and self kernels use the pole rule
+    int32_t pm[2];            // nodes of the point and the gaussian set
+    const double2 *pnode;     // (k_n, c_n) per node, point set then gaussian set
+    const float2 *pnodef;     // the same in FP32
+    double pk0, pdel;         // k0 and the central-difference step 0.01 eps
+    double pWx[2], pWm[2];    // per set: sum of w/(2 Lz (k - k0)) over the side
+                              // segments, sum of w over the middle segment
+    double ppf[2][3];         // per set: its factor at k0 - del, k0, k0 + del
+    double pdpf[2];           // per set: the factor's k-derivative at k0
 };
+
+// The pole rule of one set: the node layout and its constants (host side).
+struct PoleSet {
+    std::vector<double> k, c;  // nodes and node constants
+    double Wx = 0.0, Wm = 0.0;
+    int n1 = 0, n3 = 0;
+    double kf = 0.0;
+};
+// Builds the rule for one set (gauss = the gaussian set): k0 = ln(g12)/(2 Lz),
+// kf = max(kf_base, 2.1 k0), N1 = floor(N_t k0 / kf) nodes on [0, k0 - eps]
+// and [k0 + eps, 2 k0], 10 on [k0 - eps, k0 + eps], N_t - 2 N1 on [2 k0, kf].
+// Returns false when the rule does not exist.
+bool make_pole_set(const qem_params &p, double eps, bool gauss, PoleSet &out);
+// The half-width used for p (p.pole_eps, or k0 / 20 when that is <= 0).
+double pole_eps_of(const qem_params &p);
+// Whether p's short range uses the pole rule.
+inline bool pole_active(const qem_params &p)
+{
+    return (p.semantics & QEM_SEM_POLE_SPLIT) && p.gamma_1 * p.gamma_2 > 1.0;
+}
 
 // Gauss-Legendre rule on [-1,1] (ascending) plus its FP32 copy.
 struct GLRule {
@@ -115,6 +145,7 @@ struct ShortWork {
     int jd_n = 0;
     double jd_dx = 0.0;
     DevBuf fpair;         // double4 per pair
+    DevBuf pnode, pnodef; // pole-splitting nodes (double2, float2)
 };
 // Evaluates every pair of the list into ws.fpair (fx, fy, fz_i, fz_j).
 hipError_t launch_pair_forces(const ShortConsts &c, int precision,
@@ -128,6 +159,18 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision,
                               int64_t n, const double *d_xyz,
                               const double *d_q, const double *d_fpair,
                               double *d_f, hipStream_t s);
+// The pole-splitting variants (pole.hip): same outputs as the kernels they
+// replace when ShortConsts::pole is set.
+hipError_t launch_pair_pole(const ShortConsts &c, int precision, const ShortWork &ws,
+                            const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
+                            const double *d_xyz, const double *d_q, double *d_fpair,
+                            hipStream_t s);
+hipError_t launch_pair_direct_pole(const ShortConsts &c, int precision, const ShortWork &ws,
+                                   int64_t m, const double *d_ri, const double *d_rj,
+                                   const double *d_qi, const double *d_qj, double *d_fpair,
+                                   hipStream_t s);
+hipError_t launch_self_pole(const ShortConsts &c, int precision, int64_t n, const double *d_xyz,
+                            const double *d_q, int zero_xy, double *d_f, hipStream_t s);
 // Self term only (F_short_i), n atoms.
 hipError_t launch_self_only(const ShortConsts &c, int precision,
                             const ShortWork &ws, int64_t n,
@@ -253,6 +296,9 @@ struct qem_ctx {
     qem::DevBuf d_xyz, d_q, d_f, d_f2, d_aux, d_adj;
     qem::DevBuf d_flag;   // device status word (wall domain check)
     qem::DevBuf d_wz;     // wall z forces of a general-interaction step (atom order)
+    // pole splitting: node counts and weights of the point and gaussian sets
+    int pole_m[2] = {0, 0};
+    double pole_wx[2] = {0.0, 0.0}, pole_wm[2] = {0.0, 0.0};
     uint32_t wall_epoch = 0;
     uint32_t *h_flag = nullptr;     // pinned copy of the status word
     hipEvent_t wall_ev = nullptr;   // recorded after that copy

@@ -399,6 +399,8 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
                               hipStream_t s)
 {
     if (npairs <= 0) return hipSuccess;
+    if (c.pole)
+        return launch_pair_pole(c, precision, ws, d_pi, d_pj, npairs, d_xyz, d_q, d_fpair, s);
     if (precision == QEM_PREC_MIXED) {
         const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
         k_pair_list<QEM_PREC_MIXED><<<pair_grid(npairs), 256, lds, s>>>(
@@ -426,6 +428,8 @@ hipError_t launch_pair_direct(const ShortConsts &c, int precision, const ShortWo
                               hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
+    if (c.pole)
+        return launch_pair_direct_pole(c, precision, ws, m, d_ri, d_rj, d_qi, d_qj, d_fpair, s);
     if (precision == QEM_PREC_MIXED) {
         const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
         k_pair_direct<QEM_PREC_MIXED><<<pair_grid(m), 256, lds, s>>>(
@@ -452,6 +456,7 @@ static hipError_t launch_self_z(const ShortConsts &c, int precision, const Short
                                 double *d_f, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
+    if (c.pole) return launch_self_pole(c, precision, n, d_xyz, d_q, zero_xy, d_f, s);
     if (precision == QEM_PREC_MIXED)
         k_self_z<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),

@@ -81,8 +81,8 @@ def test_inter_params_match_reference_factory():
     ("N_t", 300, L.QEM_ERR_INVALID_ARG),
     ("iter_period", 0, L.QEM_ERR_INVALID_ARG),
     ("precision", 7, L.QEM_ERR_INVALID_ARG),
-    ("semantics", 8, L.QEM_ERR_INVALID_ARG),
-    ("gamma_1", 1.0, L.QEM_ERR_UNSUPPORTED),        # gamma_1 gamma_2 = 1: pole
+    ("semantics", 16, L.QEM_ERR_INVALID_ARG),
+    ("pole_eps", float("nan"), L.QEM_ERR_INVALID_ARG),
     ("eps_0", 0.0, L.QEM_ERR_INVALID_ARG),
     ("Lx", float("nan"), L.QEM_ERR_INVALID_ARG),
 ])
@@ -92,11 +92,30 @@ def test_create_rejects_bad_parameters(field, value, status):
     p.Lz = 5.0
     p.r_cutoff = 5.0
     p.gamma_1 = 0.9
-    p.gamma_2 = 1.0 if field == "gamma_1" else 0.9
+    p.gamma_2 = 0.9
     setattr(p, field, value)
     with pytest.raises(L.QEMError) as e:
         L.Context(p)
     assert e.value.status == status
+
+
+@pytest.mark.parametrize("g1,g2,prec,sem,eps,N_t,alpha", [
+    (1.1, 1.1, L.PREC_MIXED, 0, 0.0, 30, 1.0),                  # pole, mixed, no rule
+    (1.005, 1.0, L.PREC_FP64, L.SEM_POLE_SPLIT, 0.0, 30, 1.0),  # floor(N_t k0 / kf) = 0
+    (2.0, 2.0, L.PREC_FP64, L.SEM_POLE_SPLIT, 0.2, 60, 0.1),    # eps >= k0 = ln 4 / 10
+    (2.0, 2.0, L.PREC_MIXED, L.SEM_POLE_SPLIT, 0.0, 4, 0.1),    # too few nodes
+])
+def test_create_rejects_pole_settings_without_a_rule(g1, g2, prec, sem, eps, N_t, alpha):
+    # gamma_1 gamma_2 > 1: D(k) vanishes at k0 = ln(g1 g2) / (2 Lz)
+    p = L.default_params()
+    p.Lx = p.Ly = 20.0
+    p.Lz = 5.0
+    p.r_cutoff = 5.0
+    p.gamma_1, p.gamma_2 = g1, g2
+    p.precision, p.semantics, p.pole_eps, p.N_t, p.alpha = prec, sem, eps, N_t, alpha
+    with pytest.raises(L.QEMError) as e:
+        L.Context(p)
+    assert e.value.status == L.QEM_ERR_UNSUPPORTED
 
 
 def test_default_params_are_reference_defaults():
@@ -104,7 +123,7 @@ def test_default_params_are_reference_defaults():
     # src/QEM_short.jl:19 and src/QEM_long.jl:19 keyword defaults
     assert (p.iter_period, p.N_t, p.rbe_p) == (100, 30, 30)
     assert (p.gamma_1, p.gamma_2, p.eps_0, p.accuracy, p.alpha) == (0.0, 0.0, 1.0, 1e-6, 1.0)
-    assert p.precision == L.PREC_FP64 and p.semantics == 0
+    assert p.precision == L.PREC_FP64 and p.semantics == 0 and p.pole_eps == 0.0
 
 
 def test_null_and_status_strings():
