@@ -17,8 +17,11 @@
  *     forces are N x 3 doubles, overwritten.
  *   - Host-pointer entry points are synchronous and validate their inputs.
  *     *_dev entry points take device pointers and a hipStream_t passed as
- *     void* (NULL = the context's own stream); they enqueue work and return
- *     without synchronising, and do not validate array contents.
+ *     void* (NULL = the context's own non-blocking stream; QEM_STREAM_NULL =
+ *     HIP's null stream, which a caller working on the default stream, such
+ *     as torch's default stream, needs for its own work to be ordered with
+ *     the library's); they enqueue work and return without synchronising,
+ *     and do not validate array contents.
  *   - A context is not thread-safe: use one per host thread or stream.
  *     Device workspaces belong to the context and grow on demand.
  */
@@ -31,6 +34,9 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* ---- streams ------------------------------------------------------------ */
+#define QEM_STREAM_NULL ((void *)1)   /* HIP's null (default) stream */
 
 /* ---- status codes ------------------------------------------------------ */
 #define QEM_OK                 0
@@ -208,6 +214,12 @@ int qem_long_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
 int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
                    const double *d_q, const int64_t *k_idx, int64_t P,
                    double *d_f, void *stream);
+/* qem_long_terms on device data (kvecs stays a host array, P x 3); with
+ * accumulate != 0 the terms are added to d_f.  With weight < 0 over the whole
+ * K set this is the deterministic full k sum (rbe_mode = false). */
+int qem_long_terms_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                       const double *kvecs, int64_t P, double weight, uint32_t parts,
+                       double *d_f, int32_t accumulate, void *stream);
 
 /* ---- Substrate_LJ walls (src/Substrate_LJ.jl:7-46) ----------------------
  * A 12-6 Lennard-Jones wall at z = sub_pos acting on every atom's z

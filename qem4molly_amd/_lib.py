@@ -31,6 +31,9 @@ SEM_MINIMAGE_DIR = 2
 SEM_NEWTON_IMAGES = 4
 SEM_CORRECTED = 7
 SEM_POLE_SPLIT = 8
+# stream argument of the *_dev calls selecting HIP's null (default) stream;
+# None selects the context's own stream
+STREAM_NULL = 1
 
 LONG_NONSORT = 1
 LONG_SORT = 2
@@ -51,7 +54,7 @@ EXPORTS = (
     "qem_device_synchronize", "qem_dev_copy_kernel", "qem_timing_enable",
     "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
     "qem_device_bytes", "qem_substrate_lj", "qem_substrate_lj_dev", "qem_general_forces_dev",
-    "qem_long_variance", "qem_long_variance_dev",
+    "qem_long_variance", "qem_long_variance_dev", "qem_long_terms_dev",
 )
 
 PHASES = ("neighbor", "pair", "gather", "long")
@@ -120,6 +123,7 @@ _SIGS = {
     "qem_long_forces": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd]),
     "qem_long_terms": (C.c_int, [_vp, _i64, _pd, _pd, _pd, _i64, _d, _u32, _pd]),
     "qem_forces": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd]),
+    "qem_long_terms_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pd, _i64, _d, _u32, _vp, _i32, _vp]),
     "qem_long_variance": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd]),
     "qem_long_variance_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _vp]),
     "qem_short_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp]),
@@ -337,6 +341,14 @@ class Context:
                                          kv.shape[0], float(weight), int(parts), ptr(f)),
                     "qem_long_terms")
         return f
+
+    def long_terms_dev(self, n, d_xyz, d_q, kvecs, weight, parts, d_f, accumulate=False,
+                       stream=None):
+        kv = f64(kvecs, (-1, 3))
+        self._check(lib().qem_long_terms_dev(self._h, int(n), d_xyz, d_q, ptr(kv), kv.shape[0],
+                                             float(weight), int(parts), d_f,
+                                             int(bool(accumulate)), stream),
+                    "qem_long_terms_dev")
 
     def long_variance(self, xyz, q, k_idx):
         """(F, var): the batch's long-range force (N x 3) and the per-atom

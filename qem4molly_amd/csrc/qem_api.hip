@@ -624,6 +624,7 @@ int kidx_to_vecs(qem_ctx *ctx, const int64_t *k_idx, int64_t P, std::vector<doub
 
 hipStream_t pick_stream(qem_ctx *ctx, void *stream)
 {
+    if (stream == QEM_STREAM_NULL) return nullptr;   // HIP's null stream
     return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
 }
 
@@ -1475,6 +1476,19 @@ int qem_long_variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const do
     return variance_dev(ctx, n, d_xyz, d_q, kv, P, d_f, d_var, pick_stream(ctx, stream));
 }
 
+int qem_long_terms_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                       const double *kvecs, int64_t P, double weight, uint32_t parts,
+                       double *d_f, int32_t accumulate, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || P < 0 || (P > 0 && !kvecs) || (n > 0 && (!d_xyz || !d_q || !d_f)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    if (parts & ~QEM_LONG_ALL) return set_err(ctx, QEM_ERR_INVALID_ARG, "unknown parts bits");
+    if (!std::isfinite(weight)) return set_err(ctx, QEM_ERR_DOMAIN, "weight must be finite");
+    return long_dev(ctx, n, d_xyz, d_q, kvecs, P, weight, parts, d_f, accumulate ? 1 : 0,
+                    pick_stream(ctx, stream));
+}
+
 int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                    const int64_t *k_idx, int64_t P, double *d_f, void *stream)
 {
@@ -1682,7 +1696,8 @@ int qem_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ?
 int qem_dev_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
 {
     if (!dst || !src) return QEM_ERR_INVALID_ARG;
-    return launch_copy(dst, src, bytes, static_cast<hipStream_t>(stream)) == hipSuccess
+    hipStream_t hs = stream == QEM_STREAM_NULL ? nullptr : static_cast<hipStream_t>(stream);
+    return launch_copy(dst, src, bytes, hs) == hipSuccess
                ? QEM_OK
                : QEM_ERR_HIP;
 }

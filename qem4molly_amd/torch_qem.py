@@ -30,7 +30,12 @@ if L._lib is not None and not L._loaded_with_torch:
 
 
 def _stream():
-    return torch.cuda.current_stream().cuda_stream
+    """torch's current stream for the library: its handle, or, for torch's
+    default stream (handle 0), the library's marker for HIP's null stream
+    (a NULL stream argument would select the context's own stream, whose
+    work torch's default stream does not wait for)."""
+    h = torch.cuda.current_stream().cuda_stream
+    return h if h else L.STREAM_NULL
 
 
 class QEMEnergyForces(torch.autograd.Function):
@@ -87,3 +92,31 @@ def qem_energy_forces(qctx: L.Context, xyz, q, k_idx, gamma=None):
         if not np.allclose(g, [p.gamma_1, p.gamma_2], rtol=0, atol=0):
             raise ValueError("gamma must equal the context's gamma_1, gamma_2")
     return QEMEnergyForces.apply(xyz, q, gamma, qctx, k_idx)
+
+
+def qem_forces(qctx: L.Context, xyz, q, k_idx=None, walls=(), full_sum=False, out=None):
+    """Forces only (not differentiable): QEM short + long for the batch k_idx
+    plus the given Substrate_LJ walls, on torch's current stream, or with
+    full_sum the deterministic sum over the whole K set (weights
+    exp(-k^2/(4 alpha))) in place of the batch.  xyz (N x 3) and q (N) are
+    float64 tensors on the device; out (N x 3) is reused when given."""
+    if xyz.dtype != torch.float64 or q.dtype != torch.float64 or not xyz.is_cuda:
+        raise TypeError("xyz and q must be float64 tensors on a ROCm device")
+    x = xyz.contiguous()
+    qq = q.contiguous()
+    n = x.shape[0]
+    F = torch.empty_like(x) if out is None else out
+    s = _stream()
+    if full_sum:
+        if not hasattr(qctx, "_kset_dev"):
+            K, _, _ = qctx.kset()
+            qctx._kset_dev = K
+        qctx.short_forces_dev(n, x.data_ptr(), qq.data_ptr(), F.data_ptr(), s)
+        qctx.long_terms_dev(n, x.data_ptr(), qq.data_ptr(), qctx._kset_dev, -1.0, L.LONG_ALL,
+                            F.data_ptr(), accumulate=True, stream=s)
+        for w in walls:
+            qctx.substrate_lj_dev(n, w, x.data_ptr(), F.data_ptr(), accumulate=True, stream=s)
+    else:
+        qctx.general_forces_dev(n, x.data_ptr(), qq.data_ptr(), k_idx, list(walls),
+                                F.data_ptr(), s)
+    return F
