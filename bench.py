@@ -234,6 +234,126 @@ def rbe_variance(cfg, xyz_d, q_d, reps=5):
             "relative_rms_noise": float(np.sqrt(var.sum() / (f ** 2).sum()))}
 
 
+def full_sum(cfg, xyz_d, q_d, reps=3):
+    """The deterministic full k sum (rbe_mode = false): QEM short plus the
+    long range over the whole K set with weights exp(-k^2/(4 alpha)),
+    device-resident, mixed precision."""
+    from qem4molly_amd import _lib as L
+    ctx = L.Context(_params(cfg, "mixed"))
+    K, _, _ = ctx.kset()
+    fd = L.DeviceArray(np.zeros((cfg.n, 3)))
+
+    def one():
+        ctx.short_forces_dev(cfg.n, xyz_d.ptr, q_d.ptr, fd.ptr)
+        ctx.long_terms_dev(cfg.n, xyz_d.ptr, q_d.ptr, K, -1.0, L.LONG_ALL, fd.ptr,
+                           accumulate=True)
+
+    one()
+    L.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        one()
+    L.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    mem = ctx.device_bytes()
+    ctx.close()
+    return {"workload": cfg.name, "n_k": int(K.shape[0]), "ms_per_step": dt * 1e3,
+            "value": cfg.n / dt, "unit": "atom-force evals/s", "device_bytes": mem}
+
+
+def pole_step(steps=20, warmup=2):
+    """gamma_1 = gamma_2 = -2 on the 100k slab (gamma_1 gamma_2 = 4 > 1): the
+    pole-splitting rule (QEM_SEM_POLE_SPLIT, eps = k0/20), N_t = 240 so that
+    every segment of the rule has nodes (k0 = ln 4 / 40), mixed precision,
+    device-resident QEM step."""
+    from dataclasses import replace
+    from qem4molly_amd import _lib as L
+    from qem4molly_amd.workloads import CONFIGS, make_system
+    cfg = replace(CONFIGS["slab_n100k"], name="slab_n100k_pole", gamma_1=-2.0, gamma_2=-2.0,
+                  N_t=240)
+    xyz, q = make_system(CONFIGS["slab_n100k"], seed=0)
+    xd, qd, fd = L.DeviceArray(xyz), L.DeviceArray(q), L.DeviceArray(np.zeros_like(xyz))
+    ctx = L.Context(_params(cfg, "mixed", semantics=L.SEM_CORRECTED | L.SEM_POLE_SPLIT))
+    ctx.timing_enable(True)
+    for s in range(warmup):
+        ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, ctx.sample_k(500 + s, cfg.P), fd.ptr)
+    L.synchronize()
+    ctx.timing_read(reset=True)
+    t = time.perf_counter()
+    for s in range(steps):
+        ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, ctx.sample_k(600 + s, cfg.P), fd.ptr)
+    L.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    ph = ctx.timing_read(reset=True)
+    npairs = ctx.short_state()[0]
+    f = fd.download()
+    ctx.close()
+    return {"workload": cfg.name, "gamma": [-2.0, -2.0], "N_t": 240, "ms_per_step": dt * 1e3,
+            "value": cfg.n / dt, "unit": "atom-force evals/s", "n_pairs": npairs,
+            "pair_ms": ph["pair"][0] / steps, "finite": bool(np.all(np.isfinite(f)))}
+
+
+LAMMPS_CHILD = r"""
+import json, sys, time
+import numpy as np
+import torch                                   # first: one HIP runtime
+sys.path.insert(0, ROOT)
+from qem4molly_amd import lammps_coupling as LC
+from qem4molly_amd.workloads import CONFIGS, make_system
+
+class Atoms:                                   # numpy per-atom arrays, local order
+    def __init__(self, xyz, types, rng):
+        order = rng.permutation(xyz.shape[0])
+        self.id = order + 1
+        self.type = types[order]
+        self.x = xyz[order].copy()
+        self.f = np.zeros_like(self.x)
+        self.v = np.zeros_like(self.x)
+    def get_natoms(self):
+        return self.id.shape[0]
+    def extract_atom(self, name):
+        return getattr(self, name)
+
+cfg = CONFIGS["slab_n10k"]
+xyz, q = make_system(cfg, seed=0)
+types = np.where(q > 0, 1, 2)
+parcar = {"1": [1.0, 1.0, int((q > 0).sum())], "2": [1.0, -1.0, int((q < 0).sum())]}
+incar = LC.load_incar(parcar, incar={"L_x": cfg.Lx, "L_y": cfg.Ly, "L_z": cfg.Lz,
+                                     "g_1": cfg.gamma_1, "g_2": cfg.gamma_2,
+                                     "accuracy": cfg.accuracy, "mode": "rbm",
+                                     "RBM_p": cfg.P, "N_t": cfg.N_t})
+lmp = Atoms(xyz, types, np.random.default_rng(1))
+with LC.QEMLammps(incar, parcar, precision="mixed") as drv:
+    for _ in range(3):
+        drv.update_force(lmp)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(STEPS):
+        drv.update_force(lmp)
+    dt = (time.perf_counter() - t) / STEPS
+print("RESULT " + json.dumps({"workload": "slab_n10k", "alpha": incar["alpha"],
+                              "r_c": incar["r_c"], "ms_per_step": dt * 1e3,
+                              "value": cfg.n / dt, "unit": "atom-force evals/s"}), flush=True)
+"""
+
+
+def lammps_step(steps=20):
+    """One LAMMPS-coupled force update per step (the legacy driver's
+    update_Force_njit through the torch binding), slab_n10k charges and
+    box with the driver's own alpha = N/(Lx Ly) and r_c: POSCAR from the
+    per-atom arrays by id, host-to-device copy, QEM step, forces back and
+    added in local order.  Runs in a child process that imports torch first."""
+    import subprocess
+    root = os.path.dirname(os.path.abspath(__file__))
+    code = LAMMPS_CHILD.replace("ROOT", repr(root)).replace("STEPS", str(int(steps)))
+    r = subprocess.run([sys.executable, "-u", "-c", code], capture_output=True, text=True,
+                       timeout=600)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("RESULT ")]
+    if r.returncode != 0 or not lines:
+        return {"error": (r.stderr or r.stdout)[-500:]}
+    return json.loads(lines[-1][len("RESULT "):])
+
+
 def adjoint_md(name="slab_n100k", steps=100, sigma=0.01):
     """100 steps of positions perturbed by N(0, sigma) per component
     (x, y wrapped, z reflected into [0.5, Lz - 0.5]), fixed seeded k batch,
@@ -317,6 +437,8 @@ def main(argv=None):
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the full-sum, pole-splitting and LAMMPS-coupling lines")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy this many MiB three times with the library's "
@@ -408,7 +530,15 @@ def main(argv=None):
     out["general_step"] = general_step(cfg, xd_g, qd_g, fd_g, a.steps, a.warmup)
     log(f"[bench] {cfg.name}: random-batch variance diagnostic")
     out["rbe_variance"] = rbe_variance(cfg, xd_g, qd_g)
+    if not a.no_extras:
+        log(f"[bench] {cfg.name}: full k sum")
+        out["full_sum"] = full_sum(cfg, xd_g, qd_g)
     del xd_g, qd_g, fd_g
+    if not a.no_extras:
+        log("[bench] slab_n100k_pole: pole-splitting step")
+        out["pole_step"] = pole_step()
+        log("[bench] slab_n10k: LAMMPS-coupled force update")
+        out["lammps_step"] = lammps_step()
 
     if not a.no_secondary and a.workload != "slab_n10k":
         c2, _, _, m2, d2, e2 = measure("slab_n10k", 50, 3, 50)

@@ -730,13 +730,31 @@ int slot_inputs(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     return QEM_OK;
 }
 
+// k rows per long-range pass: a longer k list (the full sum over a K set of
+// ~10^6 vectors) runs in passes that add into the output, so the per-group
+// buffers stay at (8192/64) x N x 3 doubles instead of growing with P
+constexpr int64_t kLongBatch = 8192;
+
 int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
              const double *kvecs, int64_t P, double weight, uint32_t parts, double *d_f,
              int accumulate, hipStream_t s)
 {
     if (n == 0) return QEM_OK;
+    const bool k0 = (parts & QEM_LONG_K0) != 0;
+    if (P > kLongBatch) {
+        // whole batches first, the k = 0 row with the last one
+        for (int64_t off = 0; off < P; off += kLongBatch) {
+            const int64_t Pb = std::min(kLongBatch, P - off);
+            const bool last = off + Pb >= P;
+            const uint32_t pb = last ? parts : (parts & ~QEM_LONG_K0);
+            int st = long_dev(ctx, n, d_xyz, d_q, kvecs + 3 * off, Pb, weight, pb, d_f,
+                              accumulate || off > 0, s);
+            if (st) return st;
+        }
+        return QEM_OK;
+    }
     int64_t Ptot = 0;
-    int st = stage_kvecs(ctx, kvecs, P, weight, (parts & QEM_LONG_K0) != 0, s, &Ptot);
+    int st = stage_kvecs(ctx, kvecs, P, weight, k0, s, &Ptot);
     if (st) return st;
     PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
     hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import json
 import math
+from collections.abc import Mapping
 from typing import Any
 
 import numpy as np
@@ -65,14 +66,27 @@ def load_incar(parcar: dict, path="./INPUT/INCAR", incar: dict | None = None) ->
     return incar
 
 
+def _extract(lmp, name):
+    """The per-atom array: numpy's view when the object offers one
+    (lammps.numpy.extract_atom, or an array already), else the ctypes one."""
+    num = getattr(lmp, "numpy", None)
+    if num is not None and hasattr(num, "extract_atom"):
+        return num.extract_atom(name)
+    return lmp.extract_atom(name)
+
+
 def _per_atom(lmp, name, n, width):
-    a = lmp.extract_atom(name)
+    a = _extract(lmp, name)
+    if isinstance(a, np.ndarray):
+        return np.array(a[:n, :width], dtype=np.float64)
     return np.array([[a[i][j] for j in range(width)] for i in range(n)], dtype=np.float64)
 
 
 def _ids_types(lmp, n):
-    ids = lmp.extract_atom("id")
-    types = lmp.extract_atom("type")
+    ids = _extract(lmp, "id")
+    types = _extract(lmp, "type")
+    if isinstance(ids, np.ndarray) and isinstance(types, np.ndarray):
+        return ids[:n].astype(np.int64), types[:n].astype(np.int64)
     return (np.array([int(ids[i]) for i in range(n)], dtype=np.int64),
             np.array([int(types[i]) for i in range(n)], dtype=np.int64))
 
@@ -83,7 +97,10 @@ def init_poscar(lmp, parcar: dict) -> np.ndarray:
     x = _per_atom(lmp, "x", n, 3)
     ids, types = _ids_types(lmp, n)
     poscar = np.zeros((n, 4))
-    poscar[ids - 1, 0] = [float(parcar[str(t)][1]) for t in types]
+    charge = np.zeros(int(types.max()) + 1 if n else 1)
+    for t in np.unique(types):
+        charge[t] = float(parcar[str(int(t))][1])
+    poscar[ids - 1, 0] = charge[types]
     poscar[ids - 1, 1:] = x
     return poscar
 
@@ -93,12 +110,39 @@ def add_forces(lmp, f_by_id: np.ndarray) -> dict:
     the reference's Force_dict (str(id) -> [Fx, Fy, Fz])."""
     n = int(lmp.get_natoms())
     ids, _ = _ids_types(lmp, n)
-    f = lmp.extract_atom("f")
-    for i in range(n):
-        row = f_by_id[ids[i] - 1]
-        for j in range(3):
-            f[i][j] += float(row[j])
-    return {str(k + 1): list(map(float, f_by_id[k])) for k in range(f_by_id.shape[0])}
+    f = _extract(lmp, "f")
+    if isinstance(f, np.ndarray):
+        f[:n, :3] += f_by_id[ids - 1]
+    else:
+        for i in range(n):
+            row = f_by_id[ids[i] - 1]
+            for j in range(3):
+                f[i][j] += float(row[j])
+    return _force_dict(f_by_id)
+
+
+class ForceDict(Mapping):
+    """The reference's Force_dict (str(id) -> [Fx, Fy, Fz]) as a read-only
+    view of the forces by id, built without a per-atom loop."""
+
+    def __init__(self, f_by_id: np.ndarray):
+        self._f = f_by_id
+
+    def __getitem__(self, key):
+        i = int(key) - 1
+        if not 0 <= i < self._f.shape[0]:
+            raise KeyError(key)
+        return self._f[i].tolist()
+
+    def __iter__(self):
+        return (str(i + 1) for i in range(self._f.shape[0]))
+
+    def __len__(self):
+        return self._f.shape[0]
+
+
+def _force_dict(f_by_id):
+    return ForceDict(f_by_id)
 
 
 def kick_velocities(lmp, f_by_id: np.ndarray, parcar: dict, E, dt) -> dict:
@@ -106,13 +150,23 @@ def kick_velocities(lmp, f_by_id: np.ndarray, parcar: dict, E, dt) -> dict:
     (update_lammps.py:94-107)."""
     n = int(lmp.get_natoms())
     ids, types = _ids_types(lmp, n)
-    v = lmp.extract_atom("v")
-    for i in range(n):
-        mass, q = float(parcar[str(types[i])][0]), float(parcar[str(types[i])][1])
-        row = f_by_id[ids[i] - 1]
-        for j in range(3):
-            v[i][j] += float(row[j]) * dt / mass + q * float(E[j]) * dt / mass
-    return {str(k + 1): list(map(float, f_by_id[k])) for k in range(f_by_id.shape[0])}
+    v = _extract(lmp, "v")
+    if isinstance(v, np.ndarray):
+        mt = np.zeros(int(types.max()) + 1 if n else 1)
+        qt = np.zeros_like(mt)
+        for t in np.unique(types):
+            mt[t], qt[t] = float(parcar[str(int(t))][0]), float(parcar[str(int(t))][1])
+        mass = mt[types][:, None]
+        q = qt[types][:, None]
+        Ev = np.asarray(E, dtype=np.float64)[None, :3]
+        v[:n, :3] += f_by_id[ids - 1] * dt / mass + q * Ev * dt / mass
+    else:
+        for i in range(n):
+            mass, q = float(parcar[str(types[i])][0]), float(parcar[str(types[i])][1])
+            row = f_by_id[ids[i] - 1]
+            for j in range(3):
+                v[i][j] += float(row[j]) * dt / mass + q * float(E[j]) * dt / mass
+    return _force_dict(f_by_id)
 
 
 class QEMLammps:

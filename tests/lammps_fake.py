@@ -5,7 +5,11 @@ import numpy as np
 
 
 class FakeLammps:
-    def __init__(self, xyz_by_id, types_by_id, seed=0):
+    """lists=True hands out per-atom arrays the way the plain module does
+    (indexable as a[i][j], not numpy arrays; the rows write through)."""
+
+    def __init__(self, xyz_by_id, types_by_id, seed=0, lists=False):
+        self.lists = lists
         n = xyz_by_id.shape[0]
         order = np.random.default_rng(seed).permutation(n)   # local index -> id - 1
         self.id = (order + 1).astype(np.int64)
@@ -18,4 +22,7 @@ class FakeLammps:
         return int(self.id.shape[0])
 
     def extract_atom(self, name):
-        return getattr(self, name)
+        a = getattr(self, name)
+        if not self.lists:
+            return a
+        return [row for row in a] if a.ndim == 2 else [int(v) for v in a]

@@ -5,7 +5,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import CONFIGS, L, O, ctx_for, make_system, max_rel, rel_rms
+from helpers import CONFIGS, L, O, ctx_for, make_system, max_rel, oracle_params, rel_rms, scaled_config
 
 pytestmark = pytest.mark.gpu
 
@@ -148,3 +148,19 @@ def test_neighbor_list_full_size_sampled():
         dx -= cfg.Lx * np.rint(dx / cfg.Lx); dy -= cfg.Ly * np.rint(dy / cfg.Ly)
         ref = set(np.nonzero(dx * dx + dy * dy <= cfg.r_cutoff ** 2)[0].tolist()) - {int(a)}
         assert have.get(int(a), set()) == ref
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_full_sum_over_a_large_k_set(prec):
+    # a K set of 23,408 vectors runs in passes of 8,192 that add into the
+    # output (the k = 0 row with the last); same result as one pass
+    cfg = scaled_config(CONFIGS["slab_n10k"], 2000)
+    xyz, q = make_system(cfg, seed=8)
+    sel = np.random.default_rng(0).choice(cfg.n, 300, replace=False)
+    xyz, q = xyz[sel], q[sel]
+    with ctx_for(cfg, precision=prec) as c:
+        K, _, _ = c.kset()
+        assert K.shape[0] > 2 * 8192
+        full = c.long_terms(xyz, q, K, weight=-1.0)
+    ref = O.long_forces(oracle_params(cfg), xyz, q, K, full_sum=True, nthreads=16)
+    assert rel_rms(full, ref) < (1e-12 if prec == "fp64" else 1e-6)

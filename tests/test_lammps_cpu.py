@@ -41,17 +41,19 @@ def _system(seed=0):
     return xyz, types
 
 
-def test_poscar_rows_follow_atom_ids():
+@pytest.mark.parametrize("lists", [False, True])
+def test_poscar_rows_follow_atom_ids(lists):
     xyz, types = _system()
-    lmp = FakeLammps(xyz, types)
+    lmp = FakeLammps(xyz, types, lists=lists)
     pos = LC.init_poscar(lmp, PARCAR)
     np.testing.assert_array_equal(pos[:, 1:], xyz)
     np.testing.assert_array_equal(pos[:, 0], np.where(types == 1, 1.0, -2.0))
 
 
-def test_force_and_velocity_updates_in_local_order():
+@pytest.mark.parametrize("lists", [False, True])
+def test_force_and_velocity_updates_in_local_order(lists):
     xyz, types = _system(1)
-    lmp = FakeLammps(xyz, types, seed=3)
+    lmp = FakeLammps(xyz, types, seed=3, lists=lists)
     f0, v0 = lmp.f.copy(), lmp.v.copy()
     F = np.random.default_rng(9).normal(size=(45, 3))      # rows by id
     d = LC.add_forces(lmp, F)
