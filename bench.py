@@ -131,6 +131,20 @@ def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, targe
     }
 
 
+def host_cpu():
+    """The host's CPU model and logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def _params(cfg, precision, semantics=0, iter_period=100):
     from qem4molly_amd import _lib as L
     p = L.default_params()
@@ -572,6 +586,22 @@ def main(argv=None):
             kidx = c.sample_k(1000, cfg.P)
         log(f"[bench] cpu baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(cfg, xyz, q, pairs, K[kidx], sK / cfg.P, threads)
+        out["cpu_baseline"]["host"] = host_cpu()
+        # the same restatement on one core, and on the smaller workloads
+        more = [dict(cpu_baseline(cfg, xyz, q, pairs, K[kidx], sK / cfg.P, 1, target_s=3.0),
+                     workload=cfg.name)]
+        for name in ("slab_n100", "slab_n10k"):
+            c2 = CONFIGS[name]
+            x2, q2 = make_system(c2, seed=0)
+            with L.Context(_params(c2, "fp64")) as c:
+                pr2 = c.neighbor_list(x2)
+                K2, _, sK2 = c.kset()
+                ki2 = c.sample_k(1000, c2.P)
+            for th in (1, threads):
+                log(f"[bench] cpu baseline {name} on {th} threads")
+                more.append(dict(cpu_baseline(c2, x2, q2, pr2, K2[ki2], sK2 / c2.P, th,
+                                              target_s=3.0), workload=name))
+        out["cpu_baseline_more"] = more
 
     if a.copy_calibration_mib > 0:
         nb = a.copy_calibration_mib << 20

@@ -737,18 +737,18 @@ constexpr int64_t kLongBatch = 8192;
 
 int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
              const double *kvecs, int64_t P, double weight, uint32_t parts, double *d_f,
-             int accumulate, hipStream_t s)
+             int accumulate, hipStream_t s, bool sorted = false)
 {
     if (n == 0) return QEM_OK;
     const bool k0 = (parts & QEM_LONG_K0) != 0;
     if (P > kLongBatch) {
-        // whole batches first, the k = 0 row with the last one
+        // whole batches first, the k = 0 row with the last one; one z sort
         for (int64_t off = 0; off < P; off += kLongBatch) {
             const int64_t Pb = std::min(kLongBatch, P - off);
             const bool last = off + Pb >= P;
             const uint32_t pb = last ? parts : (parts & ~QEM_LONG_K0);
             int st = long_dev(ctx, n, d_xyz, d_q, kvecs + 3 * off, Pb, weight, pb, d_f,
-                              accumulate || off > 0, s);
+                              accumulate || off > 0, s, sorted || off > 0);
             if (st) return st;
         }
         return QEM_OK;
@@ -757,8 +757,11 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     int st = stage_kvecs(ctx, kvecs, P, weight, k0, s, &Ptot);
     if (st) return st;
     PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
-    hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
-    if (e != hipSuccess) return hip_err(ctx, e, "z sort");
+    if (!sorted) {
+        hipError_t e0 = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
+        if (e0 != hipSuccess) return hip_err(ctx, e0, "z sort");
+    }
+    hipError_t e;
     LongConsts c;
     c.Lz = ctx->p.Lz;
     c.g1 = ctx->p.gamma_1;
@@ -1448,7 +1451,8 @@ static int variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const doub
                                  ctx->lw, n, Ptot, P, d_var, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "long-range batch variance");
     }
-    if (d_f) return long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 0, s);
+    // the z order of these positions is already in place
+    if (d_f) return long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 0, s, true);
     return QEM_OK;
 }
 
