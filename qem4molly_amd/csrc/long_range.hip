@@ -205,8 +205,11 @@ __device__ __forceinline__ LaneK load_lane(const KRec &r, bool mixed)
 }
 
 // cos, sin of k.rho and e^{+kz}, e^{-kz} from the lane's row constants
-// (mixed: phase in turns reduced in FP64, FP32 sin/cos; e^{+-kz} = 2^{+-(n+f)} with
-// the fraction f in FP32 and the integer n in the FP64 exponent, so no overflow)
+// (mixed: phase in turns reduced to [-1/2, 1/2] in FP64, then the hardware
+// sine and cosine, which take their argument in turns (v_sin_f32,
+// v_cos_f32; 1.2e-7 absolute over that range, measured on gfx950);
+// e^{+-kz} = 2^{+-(n+f)} with the fraction f in FP32 and the integer n in
+// the FP64 exponent, so no overflow)
 template <int PREC>
 __device__ __forceinline__ void lane_terms(const LaneK &L, double x, double y, double z,
                                            double &c, double &s, double &Ep, double &Em)
@@ -218,10 +221,9 @@ __device__ __forceinline__ void lane_terms(const LaneK &L, double x, double y, d
     } else {
         double r = fma(L.kx, x, L.ky * y);
         r -= rint(r);
-        float sf, cf;
-        sincospif((float)(2.0 * r), &sf, &cf);
-        s = sf;
-        c = cf;
+        const float t = (float)r;
+        s = __builtin_amdgcn_sinf(t);
+        c = __builtin_amdgcn_cosf(t);
         const double tz = L.k * z;
         const double nf = floor(tz);
         const float f = (float)(tz - nf);
@@ -251,6 +253,49 @@ __device__ __forceinline__ double add_nc(double a, double b)
 }
 
 constexpr int kLaneWaves = 4;   // waves (= chunks) per block in the k-lane passes
+
+// Lane sums of three per-lane values for up to kRedBatch atoms at a time,
+// through LDS: each lane stores its values of one atom in a row slot, and
+// at a flush lane 3 j + c adds up the 64 values of slot j, component c, in
+// lane order (one FP64 add per value instead of a DPP tree per atom).  Rows
+// are kRedPitch doubles apart, so the 24 summing lanes read 24 different
+// bank pairs.  A wave only reads what it wrote itself, and the LDS executes
+// a wave's accesses in issue order; the wavefront fences keep the compiler
+// from moving them across a flush.
+constexpr int kRedBatch = 8;
+constexpr int kRedPitch = 65;
+
+struct LaneReducer {
+    double *buf;   // this wave's [kRedBatch][3][kRedPitch]
+
+    __device__ __forceinline__ void put(int slot, int lane, double x, double y, double z) const
+    {
+        double *r = buf + slot * 3 * kRedPitch;
+        r[lane] = x;
+        r[kRedPitch + lane] = y;
+        r[2 * kRedPitch + lane] = z;
+    }
+
+    // out[3 base + 3 j + c] (=, or += with add) for the first nb slots
+    __device__ __forceinline__ void flush(int nb, int lane, double *__restrict__ out,
+                                          int64_t base, bool add) const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 3 * nb) {
+            const double *r = buf + lane * kRedPitch;
+            double sum = 0.0;
+#pragma unroll 8
+            for (int m = 0; m < 64; ++m) sum += r[m];
+            double *o = out + 3 * base + lane;
+            *o = add ? *o + sum : sum;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
 
 // Pass A (k lanes): per (k row, chunk) the four sums q cos e^{+kz},
 // q sin e^{+kz}, q cos e^{-kz}, q sin e^{-kz} (k = 0 row: q, q z, q, q z),
@@ -320,6 +365,8 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restri
     const double dz = satom[a1 - 1].z - satom[a0].z;
     const bool flat = kmax * dz <= 1.0;
     double *out = partB + ((int64_t)blockIdx.y * n) * 3;
+    __shared__ double red_lds[kLaneWaves][kRedBatch * 3 * kRedPitch];
+    const LaneReducer red{red_lds[wv]};
 
     double A = off.x, B = off.y;   // exclusive prefix below the atom
     double PZ = 0.0, PW = 0.0;     // inclusive in-chunk sums of the e^{-kz} pair
@@ -352,25 +399,25 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restri
         const double self = L.S1 * (Ep * Ep) - L.S2 * (Em * Em);
         const double gx = q * (gxn + L.hs * sxy);
         const double gz = q * (gzn + L.hs * szz + q * self);
-        double fx = live ? L.wx * gx : 0.0;
-        double fy = live ? L.wy * gx : 0.0;
-        double fz = live ? L.w * gz : 0.0;
+        const double fx = live ? L.wx * gx : 0.0;
+        const double fy = live ? L.wy * gx : 0.0;
+        const double fz = live ? L.w * gz : 0.0;
         A += vx;
         B += vy;
-        fx = wave_sum63(fx);
-        fy = wave_sum63(fy);
-        fz = wave_sum63(fz);
-        if (lane == 63) {
-            double *o = out + a * 3;
-            o[0] = fx;
-            o[1] = fy;
-            o[2] = fz;
-        }
+        const int slot = (int)((a - a0) % kRedBatch);
+        red.put(slot, lane, fx, fy, fz);
+        if (slot == kRedBatch - 1 || a == a1 - 1) red.flush(slot + 1, lane, out, a - slot, false);
     }
     if (flat) return;
-    // exact suffix walk: the sorted-sum terms with C, D (above the atom)
+    // exact suffix walk: the sorted-sum terms with C, D (above the atom),
+    // in batches [lo, hi] taken from the top
     double C = off.z, D = off.w;
+    int64_t lo = a1, hi = a1 - 1;
     for (int64_t a = a1 - 1; a >= a0; --a) {
+        if (a < lo) {
+            hi = a;
+            lo = a - kRedBatch + 1 < a0 ? a0 : a - kRedBatch + 1;
+        }
         const double4 at = satom[a];
         const double q = at.w;
         double c, s, Ep, Em;
@@ -379,20 +426,13 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restri
         const double qz = L.k0f * (q * at.z);
         const double gx = q * (L.hs * (sp * C - cp * D));
         const double gz = q * (L.hs * (-cp * C - sp * D));
-        double fx = live ? L.wx * gx : 0.0;
-        double fy = live ? L.wy * gx : 0.0;
-        double fz = live ? L.w * gz : 0.0;
+        const double fx = live ? L.wx * gx : 0.0;
+        const double fy = live ? L.wy * gx : 0.0;
+        const double fz = live ? L.w * gz : 0.0;
         C += q * cm;
         D += fma(q, sm, qz);
-        fx = wave_sum63(fx);
-        fy = wave_sum63(fy);
-        fz = wave_sum63(fz);
-        if (lane == 63) {
-            double *o = out + a * 3;
-            o[0] += fx;
-            o[1] += fy;
-            o[2] += fz;
-        }
+        red.put((int)(a - lo), lane, fx, fy, fz);
+        if (a == lo) red.flush((int)(hi - lo + 1), lane, out, lo, true);
     }
 }
 
