@@ -62,63 +62,84 @@ __global__ void k_gather_sorted(const double *__restrict__ xyz, const double *__
     satom[r] = make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], q[a]);
 }
 
-// Pass S: per k row, chunk offsets and totals.  One block per row; thread t
-// owns a contiguous run of chunks; run sums are combined in thread order.
+// exclusive scan of (x, y) over the CH threads of the block in thread
+// order; *tot receives the block total (the same value in every thread)
+__device__ __forceinline__ double2 block_excl_scan(double2 v, double2 *tot,
+                                                   double2 *__restrict__ sh)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double2 inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double x = __shfl_up(inc.x, d), y = __shfl_up(inc.y, d);
+        if (lane >= d) {
+            inc.x += x;
+            inc.y += y;
+        }
+    }
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    double2 base = make_double2(0.0, 0.0), all = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < CH / 64; ++i) {
+        if (i < w) {
+            base.x += sh[i].x;
+            base.y += sh[i].y;
+        }
+        all.x += sh[i].x;
+        all.y += sh[i].y;
+    }
+    __syncthreads();
+    double2 ex;
+    ex.x = __shfl_up(inc.x, 1);
+    ex.y = __shfl_up(inc.y, 1);
+    if (lane == 0) ex = make_double2(0.0, 0.0);
+    *tot = all;
+    return make_double2(base.x + ex.x, base.y + ex.y);
+}
+
+// Pass S: per k row, chunk offsets and totals.  One block per row; the
+// chunks go through block scans in tiles of CH (a fixed summation order).
 __global__ __launch_bounds__(CH) void k_long_S(const double4 *__restrict__ partA, int64_t Ptot,
                                                int nchunk, double4 *__restrict__ offA,
                                                double4 *__restrict__ totA)
 {
-    __shared__ double2 pre[CH], suf[CH];
+    __shared__ double2 sh[CH / 64];
     const int64_t p = blockIdx.x;
     const double4 *pa = partA + p * nchunk;
     double4 *oa = offA + p * nchunk;
-    const int seg = (nchunk + CH - 1) / CH;
-    const int m0 = min(nchunk, (int)threadIdx.x * seg), m1 = min(nchunk, m0 + seg);
-    double2 sp = make_double2(0.0, 0.0), ss = make_double2(0.0, 0.0);
-    for (int m = m0; m < m1; ++m) {
-        sp.x += pa[m].x;
-        sp.y += pa[m].y;
-    }
-    for (int m = m1 - 1; m >= m0; --m) {
-        ss.x += pa[m].z;
-        ss.y += pa[m].w;
-    }
-    pre[threadIdx.x] = sp;
-    suf[threadIdx.x] = ss;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // exclusive prefix over runs (ascending), exclusive suffix (descending)
-        double2 run = make_double2(0.0, 0.0);
-        for (int t = 0; t < CH; ++t) {
-            const double2 v = pre[t];
-            pre[t] = run;
-            run.x += v.x;
-            run.y += v.y;
+    // exclusive prefix of the e^{+kz} pair in ascending chunk order, tiles
+    // of CH chunks
+    double2 run = make_double2(0.0, 0.0);
+    for (int b = 0; b < nchunk; b += CH) {
+        const int m = b + (int)threadIdx.x;
+        double2 v = make_double2(0.0, 0.0);
+        if (m < nchunk) v = make_double2(pa[m].x, pa[m].y);
+        double2 tot;
+        const double2 ex = block_excl_scan(v, &tot, sh);
+        if (m < nchunk) {
+            oa[m].x = run.x + ex.x;
+            oa[m].y = run.y + ex.y;
         }
-        double2 runs = make_double2(0.0, 0.0);
-        for (int t = CH - 1; t >= 0; --t) {
-            const double2 v = suf[t];
-            suf[t] = runs;
-            runs.x += v.x;
-            runs.y += v.y;
+        run.x += tot.x;
+        run.y += tot.y;
+    }
+    // exclusive suffix of the e^{-kz} pair: the same from the top
+    double2 runs = make_double2(0.0, 0.0);
+    for (int b = 0; b < nchunk; b += CH) {
+        const int m = nchunk - 1 - (b + (int)threadIdx.x);
+        double2 v = make_double2(0.0, 0.0);
+        if (m >= 0) v = make_double2(pa[m].z, pa[m].w);
+        double2 tot;
+        const double2 ex = block_excl_scan(v, &tot, sh);
+        if (m >= 0) {
+            oa[m].z = runs.x + ex.x;
+            oa[m].w = runs.y + ex.y;
         }
-        totA[p] = make_double4(run.x, run.y, runs.x, runs.y);
+        runs.x += tot.x;
+        runs.y += tot.y;
     }
-    __syncthreads();
-    double2 a = pre[threadIdx.x];
-    for (int m = m0; m < m1; ++m) {
-        oa[m].x = a.x;
-        oa[m].y = a.y;
-        a.x += pa[m].x;
-        a.y += pa[m].y;
-    }
-    double2 b = suf[threadIdx.x];
-    for (int m = m1 - 1; m >= m0; --m) {
-        oa[m].z = b.x;
-        oa[m].w = b.y;
-        b.x += pa[m].z;
-        b.y += pa[m].w;
-    }
+    if (threadIdx.x == 0) totA[p] = make_double4(run.x, run.y, runs.x, runs.y);
 }
 
 // ---------------------------------------------------------------------------

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/qem.h"
@@ -153,6 +154,13 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision,
                               const int32_t *d_pj, int64_t npairs,
                               const double *d_xyz, const double *d_q,
                               double *d_fpair, hipStream_t s);
+// Pair records of the force path (launch_pair_forces -> launch_gather_self):
+// float4 per pair for the mixed-precision list kernel, double4 otherwise
+// (FP64, and the pole-splitting kernels).  The buffer is sized for double4.
+inline bool pair_records_f32(const ShortConsts &c, int precision)
+{
+    return precision == QEM_PREC_MIXED && !c.pole;
+}
 // Per atom: gather pair forces (i side and j side) and add the self term.
 hipError_t launch_gather_self(const ShortConsts &c, int precision,
                               const ShortWork &ws, const NeighborList &nl,

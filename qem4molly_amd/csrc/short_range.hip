@@ -125,6 +125,21 @@ struct PairForce {
     double fx, fy, fz, fzj;
 };
 
+// The per-pair record of the force path: float4 in mixed precision (the
+// pair values come from FP32 node sums; the per-atom sums of the gather
+// stay FP64), double4 in FP64.  Half the bytes for the gather to read.
+template <int PREC>
+using PairRec = typename std::conditional<PREC == QEM_PREC_MIXED, float4, double4>::type;
+
+__device__ __forceinline__ void store_rec(float4 *r, const PairForce &f)
+{
+    *r = make_float4((float)f.fx, (float)f.fy, (float)f.fz, (float)f.fzj);
+}
+__device__ __forceinline__ void store_rec(double4 *r, const PairForce &f)
+{
+    *r = make_double4(f.fx, f.fy, f.fz, f.fzj);
+}
+
 // F_short_ij for one pair (src/Force_short.jl:25-78).  Returns zeros when
 // the minimum-image distance exceeds r_cutoff (the reference's per-pair
 // 2-point neighbour list is empty then, line 36-41).
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(256) void k_pair_list(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
     const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    const double *__restrict__ q, double4 *__restrict__ fpair)
+    const double *__restrict__ q, PairRec<PREC> *__restrict__ fpair)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
@@ -292,7 +307,7 @@ __global__ __launch_bounds__(256) void k_pair_list(
         PairForce f = pair_force<PREC>(c, glx, glw, ncon, tab, q[i], q[j], xyz[3 * i],
                                        xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
                                        xyz[3 * j + 1], xyz[3 * j + 2]);
-        fpair[p] = make_double4(f.fx, f.fy, f.fz, f.fzj);
+        store_rec(fpair + p, f);
     }
 }
 
@@ -338,8 +353,9 @@ __global__ __launch_bounds__(256) void k_self_z(
 // Sixteen lanes per atom read the atom's runs of pair results in
 // contiguous strides; each lane keeps a strided partial sum and the row of
 // 16 lanes is summed in a fixed order.  No atomics.
+template <typename Rec>
 __global__ __launch_bounds__(256) void k_gather_pairs(
-    int64_t n, const double4 *__restrict__ fpair, const int32_t *__restrict__ i_start,
+    int64_t n, const Rec *__restrict__ fpair, const int32_t *__restrict__ i_start,
     const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
     const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
     const int32_t *__restrict__ j_idx, double *__restrict__ f)
@@ -357,17 +373,17 @@ __global__ __launch_bounds__(256) void k_gather_pairs(
     if (live) {
         const int32_t t1 = i_end[a];
         for (int32_t t = i_start[a] + l; t < t1; t += 16) {
-            const double4 v = fpair[i_idx ? i_idx[t] : t];
-            fx += v.x;
-            fy += v.y;
-            fz += v.z;
+            const Rec v = fpair[i_idx ? i_idx[t] : t];
+            fx += (double)v.x;
+            fy += (double)v.y;
+            fz += (double)v.z;
         }
         const int32_t u1 = j_end[a];
         for (int32_t u = j_start[a] + l; u < u1; u += 16) {
-            const double4 v = fpair[j_idx[u]];
-            fx -= v.x;
-            fy -= v.y;
-            fz += v.w;
+            const Rec v = fpair[j_idx[u]];
+            fx -= (double)v.x;
+            fy -= (double)v.y;
+            fz += (double)v.w;
         }
     }
     // every lane takes part in the row sums
@@ -406,7 +422,7 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
         k_pair_list<QEM_PREC_MIXED><<<pair_grid(npairs), 256, lds, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(),
             ws.jtab.as<float4>(), d_pi, d_pj, npairs, d_xyz, d_q,
-            reinterpret_cast<double4 *>(d_fpair));
+            reinterpret_cast<float4 *>(d_fpair));
     } else {
         if (c.jd_n > 0)
             k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs), 256, dev::lds_bytes64(c), s>>>(
@@ -478,10 +494,16 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWo
     if (e != hipSuccess) return e;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
-    k_gather_pairs<<<nb, 256, 0, s>>>(
-        n, reinterpret_cast<const double4 *>(d_fpair), nl.i_start.as<int32_t>(),
-        nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-        nl.j_idx.as<int32_t>(), d_f);
+    if (pair_records_f32(c, precision))
+        k_gather_pairs<float4><<<nb, 256, 0, s>>>(
+            n, reinterpret_cast<const float4 *>(d_fpair), nl.i_start.as<int32_t>(),
+            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+            nl.j_idx.as<int32_t>(), d_f);
+    else
+        k_gather_pairs<double4><<<nb, 256, 0, s>>>(
+            n, reinterpret_cast<const double4 *>(d_fpair), nl.i_start.as<int32_t>(),
+            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+            nl.j_idx.as<int32_t>(), d_f);
     return hipGetLastError();
 }
 
