@@ -193,7 +193,7 @@ int qem_long_terms(qem_ctx *ctx, int64_t n, const double *xyz,
  * the unbiased estimate, from the batch itself, of E|F_i - E F_i|^2 for the
  * batch force F_i (whose mean over batches is the full k sum weighted by
  * exp(-k^2/(4 alpha))).  f (may be NULL) receives qem_long_forces for the
- * same batch. */
+ * same batch.  P <= 65536 (QEM_ERR_UNSUPPORTED beyond). */
 int qem_long_variance(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
                       const int64_t *k_idx, int64_t P, double *f, double *var);
 int qem_long_variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,

@@ -1433,6 +1433,10 @@ int qem_long_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const doub
                     pick_stream(ctx, stream));
 }
 
+// the variance pass keeps one partial per 64 k rows for every atom
+// (32 B each), so its batch is bounded: 65536 k vectors
+constexpr int64_t kMaxVarianceP = 65536;
+
 // batch variance of the long-range force (and optionally the force itself)
 static int variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                         const std::vector<double> &kv, int64_t P, double *d_f, double *d_var,
@@ -1461,6 +1465,8 @@ int qem_long_variance(qem_ctx *ctx, int64_t n, const double *xyz, const double *
 {
     CTX_GUARD(ctx);
     if (P < 2 || !k_idx) return set_err(ctx, QEM_ERR_INVALID_ARG, "the batch variance needs P >= 2");
+    if (P > kMaxVarianceP)
+        return set_err(ctx, QEM_ERR_UNSUPPORTED, "the batch variance takes at most 65536 k vectors");
     if (n > 0 && (!q || !var)) return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
     int st = check_atoms(ctx, n, xyz, q);
     if (st) return st;
@@ -1490,6 +1496,8 @@ int qem_long_variance_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const do
 {
     CTX_GUARD(ctx);
     if (P < 2 || !k_idx) return set_err(ctx, QEM_ERR_INVALID_ARG, "the batch variance needs P >= 2");
+    if (P > kMaxVarianceP)
+        return set_err(ctx, QEM_ERR_UNSUPPORTED, "the batch variance takes at most 65536 k vectors");
     if (n < 0 || (n > 0 && (!d_xyz || !d_q || !d_var)))
         return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
     std::vector<double> kv;

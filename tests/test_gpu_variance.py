@@ -46,6 +46,9 @@ def test_variance_mixed_and_device_entry():
         with pytest.raises(L.QEMError) as e:
             c.long_variance(xyz, q, kidx[:1])
         assert e.value.status == L.QEM_ERR_INVALID_ARG
+        with pytest.raises(L.QEMError) as e:       # the pass's batch bound
+            c.long_variance(xyz, q, c.sample_k(7, 65537))
+        assert e.value.status == L.QEM_ERR_UNSUPPORTED
     ref = O.long_variance(oracle_params(cfg), xyz, q, K[kidx], sK / P)
     assert rel_rms(var, ref) < 1e-5
 
