@@ -658,6 +658,8 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
     for (int d = 32; d >= 1; d >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, d));
     const bool flat = kmax * (satom[a1 - 1].z - satom[a0].z) <= 1.0;
     double *out = partB + ((int64_t)blockIdx.y * n) * 3;
+    __shared__ double red_lds[kLaneWaves][kRedBatch * 3 * kRedPitch];
+    const LaneReducer red{red_lds[wv]};
 
     double A = off.x, B = off.y, dA = offt.x, dB = offt.y;
     double PZ = 0.0, PW = 0.0, dPZ = 0.0, dPW = 0.0;
@@ -706,27 +708,27 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
         const double dself = 2.0 * kvz * (L.S1 * (Ep * Ep) + L.S2 * (Em * Em));
         const double gx = q * (dgxn + L.hs * dsxy);
         const double gz = q * (dgzn + L.hs * dszz + q * dself);
-        double fx = act ? L.wx * gx : 0.0;
-        double fy = act ? L.wy * gx : 0.0;
-        double fz = act ? L.w * gz : 0.0;
+        const double fx = act ? L.wx * gx : 0.0;
+        const double fy = act ? L.wy * gx : 0.0;
+        const double fz = act ? L.w * gz : 0.0;
         A += vx;
         B += vy;
         dA += tvx;
         dB += tvy;
-        fx = wave_sum63(fx);
-        fy = wave_sum63(fy);
-        fz = wave_sum63(fz);
-        if (lane == 63) {
-            double *o = out + a * 3;
-            o[0] = fx;
-            o[1] = fy;
-            o[2] = fz;
-        }
+        const int slot = (int)((a - a0) % kRedBatch);
+        red.put(slot, lane, fx, fy, fz);
+        if (slot == kRedBatch - 1 || a == a1 - 1) red.flush(slot + 1, lane, out, a - slot, false);
     }
     if (flat) return;
-    // exact suffix walk: the sorted-sum tangent terms with C, D (above)
+    // exact suffix walk: the sorted-sum tangent terms with C, D (above),
+    // in batches [lo, hi] taken from the top
     double C = off.z, D = off.w, dC = offt.z, dD = offt.w;
+    int64_t lo = a1, hi = a1 - 1;
     for (int64_t a = a1 - 1; a >= a0; --a) {
+        if (a < lo) {
+            hi = a;
+            lo = a - kRedBatch + 1 < a0 ? a0 : a - kRedBatch + 1;
+        }
         const double4 at = satom[a];
         const double3 dv = make_double3(svx[a], svy[a], svz[a]);
         const LaneTerms X = lane_terms_t<PREC>(L, kx, ky, kabs, at, dv);
@@ -735,22 +737,15 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
         const double dszz = -X.dcp * C - X.cp * dC - X.dsp * D - X.sp * dD;
         const double gx = q * (L.hs * dsxy);
         const double gz = q * (L.hs * dszz);
-        double fx = act ? L.wx * gx : 0.0;
-        double fy = act ? L.wy * gx : 0.0;
-        double fz = act ? L.w * gz : 0.0;
+        const double fx = act ? L.wx * gx : 0.0;
+        const double fy = act ? L.wy * gx : 0.0;
+        const double fz = act ? L.w * gz : 0.0;
         C += q * X.cm;
         D += fma(q, X.sm, qz);
         dC += q * X.dcm;
         dD += q * X.dsm;
-        fx = wave_sum63(fx);
-        fy = wave_sum63(fy);
-        fz = wave_sum63(fz);
-        if (lane == 63) {
-            double *o = out + a * 3;
-            o[0] += fx;
-            o[1] += fy;
-            o[2] += fz;
-        }
+        red.put((int)(a - lo), lane, fx, fy, fz);
+        if (a == lo) red.flush((int)(hi - lo + 1), lane, out, lo, true);
     }
 }
 
