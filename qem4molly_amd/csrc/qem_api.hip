@@ -431,6 +431,12 @@ ShortConsts short_consts(const qem_ctx *ctx, int single)
     c.jd_n = ctx->sw.jd_n;
     c.jd_inv_dx = ctx->sw.jd_dx > 0 ? 1.0 / ctx->sw.jd_dx : 0.0;
     c.jdtab = ctx->sw.jdtab.as<double>();
+    c.inv_Lx = 1.0 / p.Lx;
+    c.inv_Ly = 1.0 / p.Ly;
+    c.f_2Lz = (float)(2.0 * p.Lz);
+    c.f_nlnacc = (float)(-c.ln_acc);
+    c.f_2alpha = (float)(2.0 * p.alpha);
+    c.f_m4alnacc = (float)(-4.0 * p.alpha * c.ln_acc);
     c.pole = 0;
     c.pm[0] = c.pm[1] = 0;
     c.pnode = nullptr;

@@ -44,6 +44,9 @@ struct ShortConsts {
     int32_t jd_n;        // FP64 Bessel table intervals (0: device-library j0/j1)
     double jd_inv_dx;    // its inverse interval width
     const double *jdtab; // FP64 table in device memory (staged in LDS by the kernels)
+    double inv_Lx, inv_Ly;   // 1/Lx, 1/Ly
+    float f_2Lz, f_nlnacc;   // FP32: 2 Lz, -log(accuracy)
+    float f_2alpha, f_m4alnacc;  // FP32: 2 alpha, -4 alpha log(accuracy)
     // pole splitting (gamma_1 gamma_2 > 1 with QEM_SEM_POLE_SPLIT)
     int32_t pole;             // 1: the pair and self kernels use the pole rule
     int32_t pm[2];            // nodes of the point and the gaussian set

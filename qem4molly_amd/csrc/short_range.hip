@@ -154,9 +154,11 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
     const double dx = xi - xj, dy = yi - yj;
     double mx, my, d2;
     {
+        // minimum image; dx * (1/Lx) can round differently from dx / Lx only
+        // next to a half box, where both images lie beyond r_cutoff < L/2
 #pragma clang fp contract(off)
-        mx = dx - c.Lx * rint(dx / c.Lx);
-        my = dy - c.Ly * rint(dy / c.Ly);
+        mx = dx - c.Lx * rint(dx * c.inv_Lx);
+        my = dy - c.Ly * rint(dy * c.inv_Ly);
         d2 = mx * mx + my * my;
     }
     if (d2 > c.rc2) return out;
@@ -173,14 +175,32 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
         const double b = l == 0 ? 1.0 : (l == 1 ? c.g1 : (l == 2 ? c.g2 : c.g12));
         const double s = l == 0 ? -sg : (l == 1 ? -1.0 : (l == 2 ? 1.0 : sg));
         // greens_element_ij_init limits (src/greens_function.jl:26-27)
-        const double kf1 = -c.ln_acc / (2.0 * c.Lz + a);
-        const double kf2 = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc) -
-                           2.0 * c.alpha * a;
-        const double Kp = swap ? kf1 : kf2;  // point-charge integral limit
-        const double Kg = swap ? kf2 : kf1;  // gaussian integral limit
-        // closed-form image sums (src/Force_short.jl:49,65)
+        double Kp, Kg, inv32;
         const double r2 = a * a + rho * rho;
-        const double inv32 = 1.0 / (r2 * sqrt(r2));
+        if (PREC == QEM_PREC_MIXED) {
+            // the limits only place FP32 nodes: FP32, with k_f2 in the form
+            // Y / (sqrt(X^2 + Y) + X) = sqrt(X^2 + Y) - X, X = 2 alpha a,
+            // Y = -4 alpha ln(acc), which does not cancel
+            const float af = (float)a, X = c.f_2alpha * af;
+            const float kf1 = c.f_nlnacc * __builtin_amdgcn_rcpf(c.f_2Lz + af);
+            const float kf2 =
+                c.f_m4alnacc * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(X, X, c.f_m4alnacc)) + X);
+            Kp = swap ? kf1 : kf2;
+            Kg = swap ? kf2 : kf1;
+            // r2^{-3/2} from the FP64 reciprocal square root and two Newton steps
+            double y = __builtin_amdgcn_rsq(r2);
+            y = y * fma(-0.5 * r2 * y, y, 1.5);
+            y = y * fma(-0.5 * r2 * y, y, 1.5);
+            inv32 = y * y * y;
+        } else {
+            const double kf1 = -c.ln_acc / (2.0 * c.Lz + a);
+            const double kf2 = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc) -
+                               2.0 * c.alpha * a;
+            Kp = swap ? kf1 : kf2;  // point-charge integral limit
+            Kg = swap ? kf2 : kf1;  // gaussian integral limit
+            inv32 = 1.0 / (r2 * sqrt(r2));
+        }
+        // closed-form image sums (src/Force_short.jl:49,65)
         const double cr = 0.5 * b * rho * inv32;
         const double cz = 0.5 * b * a * s * inv32;
         double pr = 0.0, pz = 0.0, gr = 0.0, gz = 0.0;
