@@ -437,6 +437,15 @@ ShortConsts short_consts(const qem_ctx *ctx, int single)
     c.f_nlnacc = (float)(-c.ln_acc);
     c.f_2alpha = (float)(2.0 * p.alpha);
     c.f_m4alnacc = (float)(-4.0 * p.alpha * c.ln_acc);
+    {
+        // the same float operations the node loop used to do per pair
+        const float log2e = 1.4426950408889634f;
+        const float c2 = -2.0f * (float)p.Lz * log2e;
+        const float cg = -log2e * (float)(0.25 / p.alpha);
+        c.f_c2 = c2;
+        c.f_rG = cg / (c2 * c2);
+        c.f_hinvLz = (float)(0.5 / p.Lz);
+    }
     c.pole = 0;
     c.pm[0] = c.pm[1] = 0;
     c.pnode = nullptr;

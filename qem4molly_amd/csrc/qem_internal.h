@@ -47,6 +47,8 @@ struct ShortConsts {
     double inv_Lx, inv_Ly;   // 1/Lx, 1/Ly
     float f_2Lz, f_nlnacc;   // FP32: 2 Lz, -log(accuracy)
     float f_2alpha, f_m4alnacc;  // FP32: 2 alpha, -4 alpha log(accuracy)
+    float f_c2, f_rG, f_hinvLz;  // FP32 node-loop constants: -2 Lz log2(e),
+                                 // cg / f_c2^2 with cg = -log2(e) / (4 alpha), 1/(2 Lz)
     // pole splitting (gamma_1 gamma_2 > 1 with QEM_SEM_POLE_SPLIT)
     int32_t pole;             // 1: the pair and self kernels use the pole rule
     int32_t pm[2];            // nodes of the point and the gaussian set

@@ -89,13 +89,12 @@ __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool ga
                                             const float4 *__restrict__ tab, float &sr, float &sz)
 {
     const float h = 0.5f * K;
-    const float c2 = -2.0f * (float)c.Lz * kLog2e;
-    const float cg = -kLog2e * (float)(0.25 / c.alpha);
+    const float c2 = c.f_c2;                       // -2 Lz log2(e)
     const float g12 = (float)c.g12;
     const float hE = h * c2;                       // e = hE u.x
-    const float rA = a * (float)(0.5 / c.Lz);      // a / 2Lz
+    const float rA = a * c.f_hinvLz;               // a / 2Lz
     const float rP = 1.0f + rA;
-    const float rG = cg / (c2 * c2);
+    const float rG = c.f_rG;                       // cg / c2^2, cg = -log2(e) / (4 alpha)
     // Bessel argument in table units; u.x < 2 keeps m <= jt_n - 1
     const float hR = fminf(h * rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
     float ar = 0.f, az = 0.f;
@@ -162,7 +161,21 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
         d2 = mx * mx + my * my;
     }
     if (d2 > c.rc2) return out;
-    const double rho = sqrt(d2);
+    // rho and 1/rho: in mixed precision from the FP64 reciprocal square root
+    // and two Newton steps (no FP64 square root or division in the set-up)
+    double rho, irho = 0.0;
+    if (PREC == QEM_PREC_MIXED) {
+        rho = 0.0;
+        if (d2 > 0.0) {
+            double y = __builtin_amdgcn_rsq(d2);
+            y = y * fma(-0.5 * d2 * y, y, 1.5);
+            y = y * fma(-0.5 * d2 * y, y, 1.5);
+            irho = y;
+            rho = d2 * y;
+        }
+    } else {
+        rho = sqrt(d2);
+    }
     const double zn = fabs(zi - zj), zp = zi + zj;
     const double sg = jl_sign(zi - zj);
     const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
@@ -176,7 +189,7 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
         const double s = l == 0 ? -sg : (l == 1 ? -1.0 : (l == 2 ? 1.0 : sg));
         // greens_element_ij_init limits (src/greens_function.jl:26-27)
         double Kp, Kg, inv32;
-        const double r2 = a * a + rho * rho;
+        const double r2 = a * a + (PREC == QEM_PREC_MIXED ? d2 : rho * rho);
         if (PREC == QEM_PREC_MIXED) {
             // the limits only place FP32 nodes: FP32, with k_f2 in the form
             // Y / (sqrt(X^2 + Y) + X) = sqrt(X^2 + Y) - X, X = 2 alpha a,
@@ -233,8 +246,13 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
     if (rho != 0.0) {
         const bool mi = (c.sem & QEM_SEM_MINIMAGE_DIR) != 0;
         const double ux = mi ? mx : dx, uy = mi ? my : dy;
-        out.fx = pref * (Fsr * ux / rho);
-        out.fy = pref * (Fsr * uy / rho);
+        if (PREC == QEM_PREC_MIXED) {
+            out.fx = pref * (Fsr * ux * irho);
+            out.fy = pref * (Fsr * uy * irho);
+        } else {
+            out.fx = pref * (Fsr * ux / rho);
+            out.fy = pref * (Fsr * uy / rho);
+        }
     }
     out.fz = pref * Fsz;
     if (c.sem & QEM_SEM_NEWTON_IMAGES)
