@@ -6,7 +6,9 @@
 //
 // Layout: atoms are bucketed into an ncx x ncy grid of cells at least
 // r_cutoff wide (stable radix sort of cell ids, so atoms inside a cell stay
-// in index order).  The sorted order numbers the atoms by slot: slot s is
+// in index order).  Cell ids follow a Morton (Z-order) curve, so that the
+// cells around a cell are mostly near it in the id order (a row-major order
+// puts the cells above and below a whole row of cells away).  The sorted order numbers the atoms by slot: slot s is
 // atom atom_idx_sorted[s], and a cell's atoms are a run of slots.  One
 // thread per slot walks the 3 x 3 block of cells around its own, first
 // counting and then writing its partners with a larger atom index (the
@@ -49,6 +51,33 @@ __device__ __forceinline__ double pair_d2(double xi, double yi, double xj,
     return mx * mx + my * my;
 }
 
+// Morton id of cell (cx, cy): the bits of cx and cy interleaved (cx, cy < 2^16)
+__host__ __device__ __forceinline__ uint32_t spread_bits(uint32_t v)
+{
+    v &= 0x0000ffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__host__ __device__ __forceinline__ uint32_t cell_id(int cx, int cy)
+{
+    return spread_bits((uint32_t)cx) | (spread_bits((uint32_t)cy) << 1);
+}
+
+// inverse of spread_bits: the even bits of v packed together
+__device__ __forceinline__ int gather_bits(uint32_t v)
+{
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return (int)v;
+}
+
 __global__ void k_cell_keys(const double *__restrict__ xyz, int64_t n, CellGrid g,
                             uint32_t *__restrict__ key, int32_t *__restrict__ idx)
 {
@@ -56,7 +85,7 @@ __global__ void k_cell_keys(const double *__restrict__ xyz, int64_t n, CellGrid 
     if (i >= n) return;
     int cx = cell_of(xyz[3 * i], g.Lx, g.ncx);
     int cy = cell_of(xyz[3 * i + 1], g.Ly, g.ncy);
-    key[i] = (uint32_t)(cy * g.ncx + cx);
+    key[i] = cell_id(cx, cy);
     idx[i] = (int32_t)i;
 }
 
@@ -94,8 +123,8 @@ __global__ void k_nl_pass(const double *__restrict__ sxyz, int64_t n, CellGrid g
     if (s >= n) return;
     const double xi = sxyz[3 * s], yi = sxyz[3 * s + 1];
     const int32_t ai = sidx[s];
-    const int key = (int)skey[s];
-    const int cx = key % g.ncx, cy = key / g.ncx;
+    const uint32_t key = skey[s];
+    const int cx = gather_bits(key), cy = gather_bits(key >> 1);
     const int nxi = g.ncx >= 3 ? 3 : g.ncx, nyi = g.ncy >= 3 ? 3 : g.ncy;
     int64_t cnt = 0;
     const int64_t base = FILL ? row_ptr[s] : 0;
@@ -103,7 +132,7 @@ __global__ void k_nl_pass(const double *__restrict__ sxyz, int64_t n, CellGrid g
         const int cyy = g.ncy >= 3 ? (cy + oy - 1 + g.ncy) % g.ncy : oy;
         for (int ox = 0; ox < nxi; ++ox) {
             const int cxx = g.ncx >= 3 ? (cx + ox - 1 + g.ncx) % g.ncx : ox;
-            const int c = cyy * g.ncx + cxx;
+            const int c = (int)cell_id(cxx, cyy);
             const int t1 = cend[c];
             for (int t = cstart[c]; t < t1; ++t) {
                 if (sidx[t] <= ai) continue;
@@ -243,7 +272,8 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
     double wx = rc * (1.0 + 1e-9), wy = rc * (1.0 + 1e-9);
     g.ncx = (int)std::min(1024.0, std::max(1.0, std::floor(Lx / wx)));
     g.ncy = (int)std::min(1024.0, std::max(1.0, std::floor(Ly / wy)));
-    const int64_t ncell = (int64_t)g.ncx * g.ncy;
+    // Morton ids run up to that of the last cell
+    const int64_t ncell = (int64_t)cell_id(g.ncx - 1, g.ncy - 1) + 1;
 
     NL_CHECK(ensure(nl.cell_key, sizeof(uint32_t) * (size_t)n));
     NL_CHECK(ensure(nl.cell_key_sorted, sizeof(uint32_t) * (size_t)n));
