@@ -263,11 +263,15 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
 }
 
 // Self-image z force F_short_i (src/Force_short.jl:82-107).  Images 1 and 4
-// have sign_a = 0 and contribute exactly zero; they are skipped.
+// have sign_a = 0 and contribute exactly zero; they are skipped.  Lane
+// `lane` of `nl` lanes takes the nodes n = lane (mod nl) of every node sum,
+// and lane 0 the closed-form image sum: the nl shares add up to the force
+// (nl = 1 gives it whole).
 template <int PREC>
 __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ glx,
                                const double *__restrict__ glw, const float *__restrict__ glxf,
-                               const float *__restrict__ glwf, double qi, double zi)
+                               const float *__restrict__ glwf, double qi, double zi,
+                               int lane = 0, int nl = 1)
 {
     const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
     double P1 = 0.0, P2 = 0.0, G = 0.0;
@@ -291,7 +295,7 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
                 if (PREC != QEM_PREC_MIXED) {
                     const double h = 0.5 * K;
                     const double inv4a = 0.25 / c.alpha;
-                    for (int n = 0; n < c.Nt; ++n) {
+                    for (int n = lane; n < c.Nt; n += nl) {
                         const double k = h + h * glx[n];
                         const double w = h * glw[n];
                         const double E = exp(-2.0 * k * c.Lz);
@@ -306,7 +310,7 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
                     const float cg = -kLog2e * (float)(0.25 / c.alpha);
                     const float g12 = (float)c.g12;
                     float af = 0.f;
-                    for (int n = 0; n < c.Nt; ++n) {
+                    for (int n = lane; n < c.Nt; n += nl) {
                         const float k = fmaf(h, glxf[n], h);
                         const float w = h * glwf[n];
                         const float E = __builtin_amdgcn_exp2f(k * c2);
@@ -322,7 +326,7 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
         }
         const double hb = 0.5 * b;
         P1 += (hb * c.g12 * s) * pz;
-        P2 += b * a * s / (a * a * a);
+        if (lane == 0) P2 += b * a * s / (a * a * a);
         G += (hb * s) * gz;
     }
     P2 *= 0.5;
@@ -390,13 +394,20 @@ __global__ __launch_bounds__(256) void k_self_z(
 // (src/QEM_short.jl:41-56), plus the self term already in f[3a + 2].
 // Sixteen lanes per atom read the atom's runs of pair results in
 // contiguous strides; each lane keeps a strided partial sum and the row of
-// 16 lanes is summed in a fixed order.  No atomics.
-template <typename Rec>
+// 16 lanes is summed in a fixed order.  No atomics.  With SELF the kernel
+// also evaluates the self term (precision SPREC), each of the 16 lanes a
+// share of its nodes, which the same row sum adds up: the node loops run
+// while the pair records stream in.  Without it the self term is already
+// in f[3a + 2].
+template <typename Rec, int SPREC, bool SELF>
 __global__ __launch_bounds__(256) void k_gather_pairs(
     int64_t n, const Rec *__restrict__ fpair, const int32_t *__restrict__ i_start,
     const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
     const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
-    const int32_t *__restrict__ j_idx, double *__restrict__ f)
+    const int32_t *__restrict__ j_idx, ShortConsts c, const double *__restrict__ glx,
+    const double *__restrict__ glw, const float *__restrict__ glxf,
+    const float *__restrict__ glwf, const double *__restrict__ xyz,
+    const double *__restrict__ q, double *__restrict__ f)
 {
     // XCD-aware block order: the hardware deals blocks to the 8 XCDs round
     // robin, so block b runs on XCD b % 8; give each XCD a contiguous range
@@ -423,6 +434,7 @@ __global__ __launch_bounds__(256) void k_gather_pairs(
             fy -= (double)v.y;
             fz += (double)v.w;
         }
+        if (SELF) fz += self_force_z<SPREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], l, 16);
     }
     // every lane takes part in the row sums
     fx = dev::row_sum16(fx);
@@ -431,7 +443,7 @@ __global__ __launch_bounds__(256) void k_gather_pairs(
     if (live && l == 0) {
         f[3 * a] = fx;
         f[3 * a + 1] = fy;
-        f[3 * a + 2] = fz + f[3 * a + 2];
+        f[3 * a + 2] = SELF ? fz : fz + f[3 * a + 2];
     }
 }
 
@@ -528,20 +540,29 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWo
                               hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipError_t e = launch_self_z(c, precision, ws, n, d_xyz, d_q, 0, d_f, s);
-    if (e != hipSuccess) return e;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
-    if (pair_records_f32(c, precision))
-        k_gather_pairs<float4><<<nb, 256, 0, s>>>(
-            n, reinterpret_cast<const float4 *>(d_fpair), nl.i_start.as<int32_t>(),
-            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-            nl.j_idx.as<int32_t>(), d_f);
-    else
-        k_gather_pairs<double4><<<nb, 256, 0, s>>>(
-            n, reinterpret_cast<const double4 *>(d_fpair), nl.i_start.as<int32_t>(),
-            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-            nl.j_idx.as<int32_t>(), d_f);
+    const int32_t *is = nl.i_start.as<int32_t>(), *ie = nl.i_end.as<int32_t>();
+    const int32_t *js = nl.j_start.as<int32_t>(), *je = nl.j_end.as<int32_t>();
+    const int32_t *jidx = nl.j_idx.as<int32_t>();
+    const double *glx = ws.glx.as<double>(), *glw = ws.glw.as<double>();
+    const float *glxf = ws.glxf.as<float>(), *glwf = ws.glwf.as<float>();
+    if (c.pole) {
+        // the pole-splitting self term has its own kernel
+        hipError_t e = launch_self_z(c, precision, ws, n, d_xyz, d_q, 0, d_f, s);
+        if (e != hipSuccess) return e;
+        k_gather_pairs<double4, QEM_PREC_FP64, false><<<nb, 256, 0, s>>>(
+            n, reinterpret_cast<const double4 *>(d_fpair), is, ie, iidx, js, je, jidx, c, glx,
+            glw, glxf, glwf, d_xyz, d_q, d_f);
+    } else if (pair_records_f32(c, precision)) {
+        k_gather_pairs<float4, QEM_PREC_MIXED, true><<<nb, 256, 0, s>>>(
+            n, reinterpret_cast<const float4 *>(d_fpair), is, ie, iidx, js, je, jidx, c, glx,
+            glw, glxf, glwf, d_xyz, d_q, d_f);
+    } else {
+        k_gather_pairs<double4, QEM_PREC_FP64, true><<<nb, 256, 0, s>>>(
+            n, reinterpret_cast<const double4 *>(d_fpair), is, ie, iidx, js, je, jidx, c, glx,
+            glw, glxf, glwf, d_xyz, d_q, d_f);
+    }
     return hipGetLastError();
 }
 
