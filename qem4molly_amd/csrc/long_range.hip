@@ -982,18 +982,10 @@ LaneGrid lane_grid(int64_t n, int64_t Ptot)
 }
 }  // namespace
 
-hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
-                       double *d_f, int accumulate, hipStream_t s, std::string &err)
+hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                              hipStream_t s, std::string &err)
 {
-    if (n <= 0) return hipSuccess;
-    if (Ptot <= 0) {
-        // nothing to add; an overwrite still has to clear the output
-        if (!accumulate) {
-            k_fill_f64<<<nblk(3 * n, 256), 256, 0, s>>>(d_f, 3 * n, 0.0);
-            LR_CHECK(hipGetLastError());
-        }
-        return hipSuccess;
-    }
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
     const int ngroup = lg.ngroup, chk = lg.chk, nchunk = lg.nchunk;
     LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
@@ -1024,10 +1016,34 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Pto
             kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>(),
             lw.offA.as<double4>(), lw.totA.as<double4>(), lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+hipError_t launch_long_sum(LongWork &lw, int64_t n, int64_t Ptot, double *d_f, int accumulate,
+                           hipStream_t s, std::string &err)
+{
+    if (n <= 0) return hipSuccess;
+    if (Ptot <= 0) {
+        // nothing to add; an overwrite still has to clear the output
+        if (!accumulate) {
+            k_fill_f64<<<nblk(3 * n, 256), 256, 0, s>>>(d_f, 3 * n, 0.0);
+            LR_CHECK(hipGetLastError());
+        }
+        return hipSuccess;
+    }
+    const int ngroup = lane_grid(n, Ptot).ngroup;
     k_long_R<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ngroup, n, lw.perm.as<int32_t>(),
                                           accumulate, d_f);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
+}
+
+hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                       double *d_f, int accumulate, hipStream_t s, std::string &err)
+{
+    hipError_t e = launch_long_passes(c, lw, n, Ptot, s, err);
+    if (e != hipSuccess) return e;
+    return launch_long_sum(lw, n, Ptot, d_f, accumulate, s, err);
 }
 
 hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,

@@ -750,12 +750,46 @@ int slot_inputs(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
 // buffers stay at (8192/64) x N x 3 doubles instead of growing with P
 constexpr int64_t kLongBatch = 8192;
 
+LongConsts long_consts(const qem_ctx *ctx, double weight, uint32_t parts);
+
+// The long range for at most kLongBatch k vectors up to its per-group
+// partial forces (lw.partB), on stream s; *Ptot receives the rows staged
+// (P, plus the k = 0 row when parts has it).
+int long_passes(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                const double *kvecs, int64_t P, double weight, uint32_t parts, hipStream_t s,
+                bool sorted, int64_t *Ptot)
+{
+    *Ptot = 0;
+    if (n == 0) return QEM_OK;
+    int st = stage_kvecs(ctx, kvecs, P, weight, (parts & QEM_LONG_K0) != 0, s, Ptot);
+    if (st) return st;
+    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
+    if (!sorted) {
+        hipError_t e0 = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
+        if (e0 != hipSuccess) return hip_err(ctx, e0, "z sort");
+    }
+    const LongConsts c = long_consts(ctx, weight, parts);
+    hipError_t e = launch_long_passes(c, ctx->lw, n, *Ptot, s, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, "long range");
+    return QEM_OK;
+}
+
+// The partial forces of long_passes (Ptot rows) summed into d_f (added with
+// accumulate), on stream s.
+int long_sum(qem_ctx *ctx, int64_t n, int64_t Ptot, double *d_f, int accumulate, hipStream_t s)
+{
+    if (n == 0) return QEM_OK;
+    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
+    hipError_t e = launch_long_sum(ctx->lw, n, Ptot, d_f, accumulate, s, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, "long range");
+    return QEM_OK;
+}
+
 int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
              const double *kvecs, int64_t P, double weight, uint32_t parts, double *d_f,
              int accumulate, hipStream_t s, bool sorted = false)
 {
     if (n == 0) return QEM_OK;
-    const bool k0 = (parts & QEM_LONG_K0) != 0;
     if (P > kLongBatch) {
         // whole batches first, the k = 0 row with the last one; one z sort
         for (int64_t off = 0; off < P; off += kLongBatch) {
@@ -769,27 +803,42 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         return QEM_OK;
     }
     int64_t Ptot = 0;
-    int st = stage_kvecs(ctx, kvecs, P, weight, k0, s, &Ptot);
+    int st = long_passes(ctx, n, d_xyz, d_q, kvecs, P, weight, parts, s, sorted, &Ptot);
     if (st) return st;
-    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
-    if (!sorted) {
-        hipError_t e0 = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
-        if (e0 != hipSuccess) return hip_err(ctx, e0, "z sort");
+    return long_sum(ctx, n, Ptot, d_f, accumulate, s);
+}
+
+// QEM_short and QEM_long of one force call on stream s (addz as in
+// short_dev).  In mixed precision the long range's passes run on the side
+// stream alongside the short range; their sum into d_f follows the short
+// range on s, so d_f holds short + long in that order, as when both run on
+// s.  The FP64 pair kernel keeps its 50 KB Bessel table in LDS, which limits
+// it to three blocks per CU; a long-range block beside them (50 KB of LDS
+// for its lane sums) would take one of the three, so in FP64 the parts run
+// one after the other.
+int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                const std::vector<double> &kv, int64_t P, double *d_f, hipStream_t s,
+                const double *addz = nullptr)
+{
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    if (n == 0 || P > kLongBatch || ctx->p.precision != QEM_PREC_MIXED) {
+        int st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz);
+        if (st) return st;
+        return long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 1, s);
     }
-    hipError_t e;
-    LongConsts c;
-    c.Lz = ctx->p.Lz;
-    c.g1 = ctx->p.gamma_1;
-    c.g2 = ctx->p.gamma_2;
-    c.g12 = c.g1 * c.g2;
-    c.inv_area_eps = 1.0 / (ctx->p.Lx * ctx->p.Ly * ctx->p.eps_0);
-    c.weight = weight;
-    c.inv4alpha = 0.25 / ctx->p.alpha;
-    c.parts = parts;
-    c.precision = ctx->p.precision;
-    e = launch_long(c, ctx->lw, n, Ptot, d_f, accumulate, s, ctx->err);
-    if (e != hipSuccess) return hip_err(ctx, e, "long range");
-    return QEM_OK;
+    API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
+    API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+    int64_t Ptot = 0;
+    int st = long_passes(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
+                         &Ptot);
+    // joined whatever happened, so that s never runs ahead of the side work
+    const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
+    if (st == QEM_OK) st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz);
+    const hipError_t ew = hipStreamWaitEvent(s, ctx->join_ev, 0);
+    if (st) return st;
+    if (ej != hipSuccess) return hip_err(ctx, ej, "join event");
+    if (ew != hipSuccess) return hip_err(ctx, ew, "join wait");
+    return long_sum(ctx, n, Ptot, d_f, 1, s);
 }
 
 // the current neighbour list, built if the context has none for n atoms
@@ -1003,7 +1052,10 @@ int qem_create(const qem_params *p, qem_ctx **out)
         return code;
     };
     if (hipSetDevice(p->device) != hipSuccess) return fail(QEM_ERR_HIP);
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess)
         return fail(QEM_ERR_HIP);
     ctx->gl = make_gl_rule(p->N_t);
     std::vector<float> xf(ctx->gl.x.begin(), ctx->gl.x.end()), wf(ctx->gl.w.begin(), ctx->gl.w.end());
@@ -1132,6 +1184,7 @@ int qem_destroy(qem_ctx *ctx)
     if (!ctx) return QEM_OK;
     (void)hipSetDevice(ctx->p.device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (DevBuf *b : all_buffers(ctx)) release(*b);
     for (int t = 0; t < LongWork::kSlots; ++t) {
         if (ctx->lw.h_kv[t]) (void)hipHostFree(ctx->lw.h_kv[t]);
@@ -1144,6 +1197,9 @@ int qem_destroy(qem_ctx *ctx)
         (void)hipEventDestroy(m.b);
     }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return QEM_OK;
@@ -1412,12 +1468,8 @@ int qem_forces(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, cons
         (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
         return st;
     API_HIP(ctx, ensure(ctx->d_f, sizeof(double) * 3 * (size_t)n));
-    st = short_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), ctx->d_f.as<double>(),
-                   ctx->stream, true);
-    if (st) return st;
-    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
-    st = long_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv.data(), P, w,
-                  QEM_LONG_ALL, ctx->d_f.as<double>(), 1, ctx->stream);
+    st = forces_core(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv, P,
+                     ctx->d_f.as<double>(), ctx->stream);
     if (st) return st;
     API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, sizeof(double) * 3 * (size_t)n,
                                 hipMemcpyDeviceToHost, ctx->stream));
@@ -1543,11 +1595,7 @@ int qem_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d
     std::vector<double> kv;
     int st = kidx_to_vecs(ctx, k_idx, P, kv);
     if (st) return st;
-    hipStream_t s = pick_stream(ctx, stream);
-    st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true);
-    if (st) return st;
-    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
-    return long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 1, s);
+    return forces_core(ctx, n, d_xyz, d_q, kv, P, d_f, pick_stream(ctx, stream));
 }
 
 int qem_short_state(qem_ctx *ctx, int64_t *npairs, int64_t *n_builds, int64_t *n_steps)
@@ -1874,9 +1922,7 @@ int qem_general_forces_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const d
     API_HIP(ctx, hipMemcpyAsync(ctx->h_flag, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     API_HIP(ctx, hipEventRecord(ctx->wall_ev, s));
     // 2. QEM_short (its final pass adds the wall z forces) and QEM_long
-    if ((st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, ctx->d_wz.as<double>()))) return st;
-    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
-    if ((st = long_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, d_f, 1, s))) return st;
+    if ((st = forces_core(ctx, n, d_xyz, d_q, kv, P, d_f, s, ctx->d_wz.as<double>()))) return st;
     // 3. the domain check waits for the wall pass only: the QEM kernels of
     //    this step are already queued behind it
     API_HIP(ctx, hipEventSynchronize(ctx->wall_ev));

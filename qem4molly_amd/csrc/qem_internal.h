@@ -231,6 +231,13 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz,
 hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n,
                        int64_t P, double *d_f, int accumulate, hipStream_t s,
                        std::string &err);
+// launch_long in two parts: the passes up to the per-group partial forces
+// (lw.partB), and their sum into d_f, which may run on another stream once
+// the passes are done.
+hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int64_t P,
+                              hipStream_t s, std::string &err);
+hipError_t launch_long_sum(LongWork &lw, int64_t n, int64_t P, double *d_f, int accumulate,
+                           hipStream_t s, std::string &err);
 
 // Batch variance of the random-batch long-range force for the k rows in
 // lw.kv (P sampled rows, plus the k = 0 row if staged, which is left out):
@@ -293,6 +300,11 @@ struct qem_ctx {
     qem_params p{};
     std::string err;
     hipStream_t stream = nullptr;
+    // the force call's long range runs on `side` alongside the short range
+    // on the call's stream: fork_ev orders it after the inputs, join_ev
+    // orders the final sum after it
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // K set (host)
     std::vector<double> kset;   // |K| x 3
     std::vector<float> kprob;   // |K|
