@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <new>
 
@@ -680,9 +681,11 @@ struct PhaseScope {
 
 // device part of Molly.forces(::QEM_short) (src/QEM_short.jl:27-59)
 // addz (optional, n doubles in atom order): added to the z forces in the
-// final pass, after the short-range sum
+// final pass, after the short-range sum; after_pair (optional) is called
+// once the pair kernel is queued, before the gather
 int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, double *d_f,
-              hipStream_t s, bool use_counter, const double *addz = nullptr)
+              hipStream_t s, bool use_counter, const double *addz = nullptr,
+              const std::function<int()> *after_pair = nullptr)
 {
     if (n == 0) return QEM_OK;
     if (use_counter) {
@@ -715,6 +718,9 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
         API_HIP(ctx, launch_pair_forces(c, ctx->p.precision, ctx->sw, nl.pi.as<int32_t>(),
                                         nl.pj.as<int32_t>(), nl.npairs, px, pq,
                                         ctx->sw.fpair.as<double>(), s));
+    }
+    if (after_pair) {
+        if (int st = (*after_pair)()) return st;
     }
     {
         PhaseScope ps_g(ctx, QEM_PHASE_GATHER, s);
@@ -752,24 +758,33 @@ constexpr int64_t kLongBatch = 8192;
 
 LongConsts long_consts(const qem_ctx *ctx, double weight, uint32_t parts);
 
-// The long range for at most kLongBatch k vectors up to its per-group
-// partial forces (lw.partB), on stream s; *Ptot receives the rows staged
-// (P, plus the k = 0 row when parts has it).
-int long_passes(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
-                const double *kvecs, int64_t P, double weight, uint32_t parts, hipStream_t s,
-                bool sorted, int64_t *Ptot)
+// The long range for at most kLongBatch k vectors, first part: the k rows
+// staged (*Ptot receives their number: P, plus the k = 0 row when parts has
+// it) and the z sort, on stream s.
+int long_prep(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+              const double *kvecs, int64_t P, double weight, uint32_t parts, hipStream_t s,
+              bool sorted, int64_t *Ptot)
 {
     *Ptot = 0;
     if (n == 0) return QEM_OK;
     int st = stage_kvecs(ctx, kvecs, P, weight, (parts & QEM_LONG_K0) != 0, s, Ptot);
     if (st) return st;
-    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
     if (!sorted) {
+        PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
         hipError_t e0 = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e0 != hipSuccess) return hip_err(ctx, e0, "z sort");
     }
+    return QEM_OK;
+}
+
+// Second part: the passes up to the per-group partial forces (lw.partB).
+int long_lanes(qem_ctx *ctx, int64_t n, int64_t Ptot, double weight, uint32_t parts,
+               hipStream_t s)
+{
+    if (n == 0) return QEM_OK;
+    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
     const LongConsts c = long_consts(ctx, weight, parts);
-    hipError_t e = launch_long_passes(c, ctx->lw, n, *Ptot, s, ctx->err);
+    hipError_t e = launch_long_passes(c, ctx->lw, n, Ptot, s, ctx->err);
     if (e != hipSuccess) return hip_err(ctx, e, "long range");
     return QEM_OK;
 }
@@ -803,19 +818,22 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         return QEM_OK;
     }
     int64_t Ptot = 0;
-    int st = long_passes(ctx, n, d_xyz, d_q, kvecs, P, weight, parts, s, sorted, &Ptot);
+    int st = long_prep(ctx, n, d_xyz, d_q, kvecs, P, weight, parts, s, sorted, &Ptot);
+    if (st == QEM_OK) st = long_lanes(ctx, n, Ptot, weight, parts, s);
     if (st) return st;
     return long_sum(ctx, n, Ptot, d_f, accumulate, s);
 }
 
 // QEM_short and QEM_long of one force call on stream s (addz as in
-// short_dev).  In mixed precision the long range's passes run on the side
-// stream alongside the short range; their sum into d_f follows the short
-// range on s, so d_f holds short + long in that order, as when both run on
-// s.  The FP64 pair kernel keeps its 50 KB Bessel table in LDS, which limits
-// it to three blocks per CU; a long-range block beside them (50 KB of LDS
-// for its lane sums) would take one of the three, so in FP64 the parts run
-// one after the other.
+// short_dev).  In mixed precision the long range runs on the side stream
+// alongside the short range: the k staging and the z sort (latency-bound,
+// few blocks) beside the pair kernel, the k-lane passes (VALU) once the pair
+// kernel is done, beside the gather (HBM).  Their sum into d_f follows the
+// short range on s, so d_f holds short + long in that order, as when both
+// run on s.  The FP64 pair kernel keeps its 50 KB Bessel table in LDS,
+// which limits it to three blocks per CU; a long-range block beside them
+// (50 KB of LDS for its lane sums) would take one of the three, so in FP64
+// the parts run one after the other.
 int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                 const std::vector<double> &kv, int64_t P, double *d_f, hipStream_t s,
                 const double *addz = nullptr)
@@ -829,11 +847,16 @@ int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
     API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
     int64_t Ptot = 0;
-    int st = long_passes(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
-                         &Ptot);
+    int st = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
+                       &Ptot);
+    const std::function<int()> lanes = [&]() -> int {
+        API_HIP(ctx, hipEventRecord(ctx->lanes_ev, s));
+        API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0));
+        return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side);
+    };
+    if (st == QEM_OK) st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz, &lanes);
     // joined whatever happened, so that s never runs ahead of the side work
     const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
-    if (st == QEM_OK) st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz);
     const hipError_t ew = hipStreamWaitEvent(s, ctx->join_ev, 0);
     if (st) return st;
     if (ej != hipSuccess) return hip_err(ctx, ej, "join event");
@@ -1055,6 +1078,7 @@ int qem_create(const qem_params *p, qem_ctx **out)
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->lanes_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess)
         return fail(QEM_ERR_HIP);
     ctx->gl = make_gl_rule(p->N_t);
@@ -1198,6 +1222,7 @@ int qem_destroy(qem_ctx *ctx)
     }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->lanes_ev) (void)hipEventDestroy(ctx->lanes_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

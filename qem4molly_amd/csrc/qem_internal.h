@@ -301,10 +301,10 @@ struct qem_ctx {
     std::string err;
     hipStream_t stream = nullptr;
     // the force call's long range runs on `side` alongside the short range
-    // on the call's stream: fork_ev orders it after the inputs, join_ev
-    // orders the final sum after it
+    // on the call's stream: fork_ev orders it after the inputs, lanes_ev its
+    // k-lane passes after the pair kernel, join_ev the final sum after it
     hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, lanes_ev = nullptr, join_ev = nullptr;
     // K set (host)
     std::vector<double> kset;   // |K| x 3
     std::vector<float> kprob;   // |K|
