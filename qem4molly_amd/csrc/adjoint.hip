@@ -458,21 +458,33 @@ __global__ __launch_bounds__(256) void k_pair_energy(
     }
 }
 
+// Per-pair tangent records: float4 in mixed precision (the values come from
+// FP32 node sums; the per-atom sums of the gather stay FP64), double4 in FP64.
+template <int PREC>
+using TanRec = typename std::conditional<PREC == QEM_PREC_MIXED, float4, double4>::type;
+
+__device__ __forceinline__ void store_tan(float4 *r, const double4 &d)
+{
+    *r = make_float4((float)d.x, (float)d.y, (float)d.z, (float)d.w);
+}
+__device__ __forceinline__ void store_tan(double4 *r, const double4 &d) { *r = d; }
+
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_tangent(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float *__restrict__ glxf, const float *__restrict__ glwf,
     const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
     const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    const double *__restrict__ q, const double *__restrict__ v, double4 *__restrict__ out)
+    const double *__restrict__ q, const double *__restrict__ v, TanRec<PREC> *__restrict__ out)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const int64_t i = pi[p], j = pj[p];
-        out[p] = pair_force_tangent<PREC>(c, glx, glw, glxf, glwf, tab, q[i], q[j], xyz + 3 * i,
-                                          xyz + 3 * j, v + 3 * i, v + 3 * j);
+        store_tan(out + p, pair_force_tangent<PREC>(c, glx, glw, glxf, glwf, tab, q[i], q[j],
+                                                    xyz + 3 * i, xyz + 3 * j, v + 3 * i,
+                                                    v + 3 * j));
     }
 }
 
@@ -555,31 +567,36 @@ __global__ __launch_bounds__(256) void k_self_tangent(
 }
 
 // per atom: scale * (tangent of the pair forces of both endpoints), added to
-// the self term already there; sixteen lanes per atom as above
+// the self term already there; sixteen lanes per atom, and the XCD-aware
+// block order of the force gather (gridDim.x a multiple of 8: the blocks
+// of one XCD take a contiguous range of atoms)
+template <typename Rec>
 __global__ __launch_bounds__(256) void k_gather_tangent(
-    int64_t n, const double4 *__restrict__ tp, const int32_t *__restrict__ i_start,
+    int64_t n, const Rec *__restrict__ tp, const int32_t *__restrict__ i_start,
     const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
     const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
     const int32_t *__restrict__ j_idx, double scale, double *__restrict__ out)
 {
-    const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int64_t per_xcd = gridDim.x / 8;
+    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
     const bool live = a < n;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (live) {
         const int32_t t1 = i_end[a];
         for (int32_t t = i_start[a] + l; t < t1; t += 16) {
-            const double4 d = tp[i_idx ? i_idx[t] : t];
-            fx += d.x;
-            fy += d.y;
-            fz += d.z;
+            const Rec d = tp[i_idx ? i_idx[t] : t];
+            fx += (double)d.x;
+            fy += (double)d.y;
+            fz += (double)d.z;
         }
         const int32_t u1 = j_end[a];
         for (int32_t t = j_start[a] + l; t < u1; t += 16) {
-            const double4 d = tp[j_idx[t]];
-            fx -= d.x;
-            fy -= d.y;
-            fz += d.w;
+            const Rec d = tp[j_idx[t]];
+            fx -= (double)d.x;
+            fy -= (double)d.y;
+            fz += (double)d.w;
         }
     }
     fx = dev::row_sum16(fx);
@@ -696,19 +713,22 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
 hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,
-                                double scale, double *d_out, hipStream_t s)
+                                double scale, double *d_out, hipStream_t s,
+                                const std::function<void()> *after_pair)
 {
     if (n <= 0) return hipSuccess;
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     double4 *tp = reinterpret_cast<double4 *>(d_tp);
+    float4 *tpf = reinterpret_cast<float4 *>(d_tp);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     if (precision == QEM_PREC_MIXED) {
         if (np > 0)
             k_pair_tangent<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
-                nl.pj.as<int32_t>(), np, d_xyz, d_q, d_v, tp);
+                nl.pj.as<int32_t>(), np, d_xyz, d_q, d_v, tpf);
+        if (after_pair) (*after_pair)();
         k_self_tangent<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);
@@ -725,13 +745,20 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                     ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
                     d_xyz, d_q, d_v, tp);
         }
+        if (after_pair) (*after_pair)();
         k_self_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);
     }
-    k_gather_tangent<<<nblk(16 * n, 256), 256, 0, s>>>(
-        n, tp, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(),
-        nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), scale, d_out);
+    const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
+    if (precision == QEM_PREC_MIXED)
+        k_gather_tangent<float4><<<nb, 256, 0, s>>>(
+            n, tpf, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), scale, d_out);
+    else
+        k_gather_tangent<double4><<<nb, 256, 0, s>>>(
+            n, tp, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), scale, d_out);
     return hipGetLastError();
 }
 

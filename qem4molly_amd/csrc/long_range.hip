@@ -1144,9 +1144,21 @@ hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int6
     return hipSuccess;
 }
 
+hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double scale,
+                                   double *d_out, int accumulate, hipStream_t s,
+                                   std::string &err)
+{
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
+    const LaneGrid lg = lane_grid(n, Ptot);
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 3,
+                                           lw.perm.as<int32_t>(), accumulate, scale, d_out);
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
 hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                const double *d_v, double scale, double *d_out, int accumulate,
-                               hipStream_t s, std::string &err)
+                               hipStream_t s, std::string &err, bool with_sum)
 {
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
@@ -1197,10 +1209,8 @@ hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int
             lw.totA.as<double4>(), lw.partT.as<double4>(), lw.offT.as<double4>(),
             lw.totT.as<double4>(), lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
-    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 3,
-                                           lw.perm.as<int32_t>(), accumulate, scale, d_out);
-    LR_CHECK(hipGetLastError());
-    return hipSuccess;
+    if (!with_sum) return hipSuccess;
+    return launch_long_tangent_sum(lw, n, Ptot, scale, d_out, accumulate, s, err);
 }
 
 }  // namespace qem

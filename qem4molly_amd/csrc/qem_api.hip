@@ -973,6 +973,35 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     if (st) return st;
     const ShortConsts c = short_consts(ctx, 0);
     API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    const LongConsts lc = long_consts(ctx, weight, QEM_LONG_NONSORT | QEM_LONG_SORT | QEM_LONG_SELF);
+    // in mixed precision the long-range tangent runs on the side stream, as
+    // in forces_core: k staging and z sort beside the pair tangent kernel,
+    // the tangent passes beside the self term and the gather, the sum into
+    // d_hv on s after the short range
+    const bool overlap = P > 0 && ctx->p.precision == QEM_PREC_MIXED;
+    int64_t Ptot = 0;
+    int lst = QEM_OK;
+    if (overlap) {
+        API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
+        API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+        lst = stage_kvecs(ctx, kvecs, P, weight, false, ctx->side, &Ptot);
+        if (lst == QEM_OK) {
+            PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
+            hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, ctx->side, ctx->err);
+            if (e != hipSuccess) lst = hip_err(ctx, e, "z sort");
+        }
+    }
+    const std::function<void()> lanes = [&]() {
+        if (lst) return;
+        hipError_t e = hipEventRecord(ctx->lanes_ev, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
+        if (e == hipSuccess) {
+            PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
+            e = launch_long_tangent(lc, ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, ctx->side,
+                                    ctx->err, false);
+        }
+        if (e != hipSuccess) lst = hip_err(ctx, e, "long-range tangent");
+    };
     {
         PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
         const double *px, *pq;
@@ -987,17 +1016,23 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
             po = nl.sout.as<double>();
         }
         API_HIP(ctx, launch_short_tangent(c, ctx->p.precision, ctx->sw, nl, n, px, pq, pv,
-                                          ctx->sw.fpair.as<double>(), -1.0, po, s));
+                                          ctx->sw.fpair.as<double>(), -1.0, po, s,
+                                          overlap ? &lanes : nullptr));
         if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, po, d_hv, s));
     }
-    if (P > 0) {
-        int64_t Ptot = 0;
+    if (overlap) {
+        API_HIP(ctx, hipEventRecord(ctx->join_ev, ctx->side));
+        API_HIP(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
+        if (lst) return lst;
+        PhaseScope ps(ctx, QEM_PHASE_LONG, s);
+        hipError_t e = launch_long_tangent_sum(ctx->lw, n, Ptot, -1.0, d_hv, 1, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "long-range tangent");
+    } else if (P > 0) {
         if ((st = stage_kvecs(ctx, kvecs, P, weight, false, s, &Ptot))) return st;
         PhaseScope ps(ctx, QEM_PHASE_LONG, s);
         hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "z sort");
-        e = launch_long_tangent(long_consts(ctx, weight, QEM_LONG_NONSORT | QEM_LONG_SORT | QEM_LONG_SELF),
-                                ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, s, ctx->err);
+        e = launch_long_tangent(lc, ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "long-range tangent");
     }
     return QEM_OK;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -250,10 +251,16 @@ hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, in
 hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                               double *d_phi, int accumulate, double *h_dgamma, hipStream_t s,
                               std::string &err);
-// Tangent of the long-range force along d_v (N x 3), times scale.
+// Tangent of the long-range force along d_v (N x 3), times scale.  Without
+// with_sum the call stops at the per-group partial tangents (lw.partB);
+// launch_long_tangent_sum then adds (or writes) them into d_out, possibly
+// on another stream once the passes are done.
 hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                const double *d_v, double scale, double *d_out, int accumulate,
-                               hipStream_t s, std::string &err);
+                               hipStream_t s, std::string &err, bool with_sum = true);
+hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double scale,
+                                   double *d_out, int accumulate, hipStream_t s,
+                                   std::string &err);
 // Short-range energy: per-pair potentials in d_pot (double4 per pair), per
 // atom phi (N) and psi (N x 2: d phi / d gamma_1,2), self terms included.
 hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
@@ -262,10 +269,13 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                                double *d_psi, hipStream_t s);
 // Short-range force tangent along d_v, times scale, into d_out (N x 3);
 // d_tp is per-pair scratch (double4 per pair).
+// after_pair (optional) is called once the pair kernel is queued, before
+// the self term and the gather.
 hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,
-                                double scale, double *d_out, hipStream_t s);
+                                double scale, double *d_out, hipStream_t s,
+                                const std::function<void()> *after_pair = nullptr);
 // out3 = (1/2 q.phi, 1/2 q.psi_1, 1/2 q.psi_2); psi may be null; d_part is
 // scratch of half_dots_scratch() doubles.
 hipError_t launch_half_dots(int64_t n, const double *d_q, const double *d_phi,
