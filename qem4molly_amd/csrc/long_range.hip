@@ -277,13 +277,17 @@ constexpr int kLaneWaves = 4;   // waves (= chunks) per block in the k-lane pass
 
 // Lane sums of three per-lane values for up to kRedBatch atoms at a time,
 // through LDS: each lane stores its values of one atom in a row slot, and
-// at a flush lane 3 j + c adds up the 64 values of slot j, component c, in
-// lane order (one FP64 add per value instead of a DPP tree per atom).  Rows
-// are kRedPitch doubles apart, so the 24 summing lanes read 24 different
-// bank pairs.  A wave only reads what it wrote itself, and the LDS executes
-// a wave's accesses in issue order; the wavefront fences keep the compiler
-// from moving them across a flush.
-constexpr int kRedBatch = 8;
+// at a flush the 3 kRedBatch rows (slot j, component c: row 3 j + c) of 64
+// values are summed by four lanes each (lane 4 r + q adds values 16 q ..
+// 16 q + 15 of row r in order), then across the quad with two DPP steps, a
+// fixed order.  Rows are kRedPitch doubles apart, so the stores of a slot
+// are conflict-free and the 48 summing lanes share each bank pair at most
+// twice.  A wave only reads what it wrote itself, and the LDS executes a
+// wave's accesses in issue order; the wavefront fences keep the compiler
+// from moving them across a flush.  Four atoms per flush keep the LDS at
+// 25 KB per block of four waves, which the VGPR budget, not the LDS, then
+// limits to five waves per SIMD.
+constexpr int kRedBatch = 4;
 constexpr int kRedPitch = 65;
 
 struct LaneReducer {
@@ -304,12 +308,18 @@ struct LaneReducer {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane < 3 * nb) {
-            const double *r = buf + lane * kRedPitch;
-            double sum = 0.0;
-#pragma unroll 8
-            for (int m = 0; m < 64; ++m) sum += r[m];
-            double *o = out + 3 * base + lane;
+        const int r = lane >> 2, q = lane & 3;
+        const bool live = r < 3 * nb;
+        double sum = 0.0;
+        if (live) {
+            const double *v = buf + r * kRedPitch + 16 * q;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) sum += v[m];
+        }
+        sum += dev::dpp_perm_f64<0xB1>(sum);   // quad_perm [1,0,3,2]
+        sum += dev::dpp_perm_f64<0x4E>(sum);   // quad_perm [2,3,0,1]
+        if (live && q == 0) {
+            double *o = out + 3 * base + r;
             *o = add ? *o + sum : sum;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
