@@ -831,9 +831,9 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
 // kernel is done, beside the gather (HBM).  Their sum into d_f follows the
 // short range on s, so d_f holds short + long in that order, as when both
 // run on s.  The FP64 pair kernel keeps its 50 KB Bessel table in LDS,
-// which limits it to three blocks per CU; a long-range block beside them
-// (50 KB of LDS for its lane sums) would take one of the three, so in FP64
-// the parts run one after the other.
+// three blocks per CU; a long-range block beside them (25 KB of LDS for its
+// lane sums) does not fit in the 160 KB and would take the place of one, so
+// in FP64 the parts run one after the other.
 int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                 const std::vector<double> &kv, int64_t P, double *d_f, hipStream_t s,
                 const double *addz = nullptr)
