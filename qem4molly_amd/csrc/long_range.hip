@@ -875,6 +875,34 @@ __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
     }
 }
 
+// k_long_R fused with the short range's output pass: for the atom a of
+// rank r, f[a] = (the slot-ordered short-range force of a, plus addz[a] on
+// z) + (the long range of rank r), in that order, as the two passes would
+// leave it.  slot_of: the slot of each atom.
+__global__ void k_long_R_slots(const double *__restrict__ partB, int ntile, int64_t n,
+                               const int32_t *__restrict__ perm,
+                               const int32_t *__restrict__ slot_of,
+                               const double *__restrict__ sout, const double *__restrict__ addz,
+                               double *__restrict__ f)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int t = 0; t < ntile; ++t) {
+        const double *o = partB + ((int64_t)t * n + r) * 3;
+        fx += o[0];
+        fy += o[1];
+        fz += o[2];
+    }
+    const int64_t a = perm[r];
+    const double *sv = sout + 3 * (int64_t)slot_of[a];
+    double z = sv[2];
+    if (addz) z = __dadd_rn(z, addz[a]);
+    f[3 * a] = sv[0] + fx;
+    f[3 * a + 1] = sv[1] + fy;
+    f[3 * a + 2] = z + fz;
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint helpers (no counterpart in the reference).
 
@@ -992,20 +1020,31 @@ LaneGrid lane_grid(int64_t n, int64_t Ptot)
 }
 }  // namespace
 
+hipError_t launch_long_krec(const LongConsts &c, LongWork &lw, int64_t Ptot, hipStream_t s,
+                            std::string &err)
+{
+    if (Ptot <= 0) return hipSuccess;
+    LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
+    k_make_krec<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot, lw.kr.as<KRec>());
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
 hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
-                              hipStream_t s, std::string &err)
+                              hipStream_t s, std::string &err, bool krec)
 {
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
     const int ngroup = lg.ngroup, chk = lg.chk, nchunk = lg.nchunk;
-    LR_CHECK(ensure(lw.kr, sizeof(double) * 16 * (size_t)Ptot));
     LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * nchunk));
     LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * nchunk));
     LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
     LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)ngroup * (size_t)n));
-    KRec *kr = lw.kr.as<KRec>();
-    k_make_krec<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot, kr);
-    LR_CHECK(hipGetLastError());
+    if (krec) {
+        const hipError_t e = launch_long_krec(c, lw, Ptot, s, err);
+        if (e != hipSuccess) return e;
+    }
+    const KRec *kr = lw.kr.as<KRec>();
     dim3 g((unsigned)((nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)ngroup);
     if (c.precision == QEM_PREC_MIXED)
         k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, lw.satom.as<double4>(), n,
@@ -1025,6 +1064,18 @@ hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int6
         k_lane_B<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
             kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>(),
             lw.offA.as<double4>(), lw.totA.as<double4>(), lw.partB.as<double>());
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+hipError_t launch_long_sum_slots(LongWork &lw, int64_t n, int64_t Ptot, const int32_t *slot_of,
+                                 const double *sout, const double *addz, double *d_f,
+                                 hipStream_t s, std::string &err)
+{
+    if (n <= 0) return hipSuccess;
+    const int ngroup = Ptot > 0 ? lane_grid(n, Ptot).ngroup : 0;
+    k_long_R_slots<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ngroup, n,
+                                                lw.perm.as<int32_t>(), slot_of, sout, addz, d_f);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }

@@ -160,6 +160,20 @@ __global__ void k_rows_to_slots(const double *__restrict__ src, const int32_t *_
     dst[e] = src[(int64_t)perm[s] * w + c];
 }
 
+// positions and charges of slot s from atom perm[s], one thread per slot
+__global__ void k_atoms_to_slots(const double *__restrict__ xyz, const double *__restrict__ q,
+                                 const int32_t *__restrict__ perm, int64_t n,
+                                 double *__restrict__ sxyz, double *__restrict__ sq)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t a = perm[s];
+    sxyz[3 * s] = xyz[3 * a];
+    sxyz[3 * s + 1] = xyz[3 * a + 1];
+    sxyz[3 * s + 2] = xyz[3 * a + 2];
+    sq[s] = q[a];
+}
+
 __global__ void k_rows_from_slots(const double *__restrict__ src,
                                   const int32_t *__restrict__ perm, int64_t n, int w,
                                   double *__restrict__ dst, const double *__restrict__ addz)
@@ -180,6 +194,15 @@ __global__ void k_row_bounds(const int64_t *__restrict__ row_ptr, int64_t n,
     if (i >= n) return;
     start[i] = (int32_t)row_ptr[i];
     end[i] = (int32_t)row_ptr[i + 1];
+}
+
+// inv[perm[t]] = t for a permutation perm of [0, m)
+__global__ void k_inverse_perm(const int32_t *__restrict__ perm, int64_t m,
+                               int32_t *__restrict__ inv)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    inv[perm[t]] = (int32_t)t;
 }
 
 __global__ void k_iota_u32key(const int32_t *__restrict__ src, int64_t m,
@@ -308,6 +331,9 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
     NL_CHECK(ensure(nl.sxyz, sizeof(double) * 3 * (size_t)n));
     k_rows_to_slots<<<nblk(3 * n, 256), 256, 0, s>>>(d_xyz, nl.atom_idx_sorted.as<int32_t>(), n, 3,
                                                       nl.sxyz.as<double>());
+    NL_CHECK(ensure(nl.slot_of, sizeof(int32_t) * (size_t)n));
+    k_inverse_perm<<<nblk(n, 256), 256, 0, s>>>(nl.atom_idx_sorted.as<int32_t>(), n,
+                                                nl.slot_of.as<int32_t>());
     // counts[n] = 0 so the exclusive scan's last entry is the total
     k_fill_i64<<<1, 64, 0, s>>>(nl.counts.as<int64_t>() + n, 1, 0);
     k_nl_pass<false><<<nblk(n, 128), 128, 0, s>>>(
@@ -384,6 +410,15 @@ hipError_t launch_rows_to_slots(const NeighborList &nl, int64_t n, int w, const 
     if (n <= 0) return hipSuccess;
     k_rows_to_slots<<<nblk(n * w, 256), 256, 0, s>>>(src, nl.atom_idx_sorted.as<int32_t>(), n, w,
                                                       dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_atoms_to_slots(const NeighborList &nl, int64_t n, const double *d_xyz,
+                                 const double *d_q, double *sxyz, double *sq, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_atoms_to_slots<<<nblk(n, 256), 256, 0, s>>>(d_xyz, d_q, nl.atom_idx_sorted.as<int32_t>(), n,
+                                                  sxyz, sq);
     return hipGetLastError();
 }
 

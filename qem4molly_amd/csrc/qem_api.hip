@@ -682,10 +682,13 @@ struct PhaseScope {
 // device part of Molly.forces(::QEM_short) (src/QEM_short.jl:27-59)
 // addz (optional, n doubles in atom order): added to the z forces in the
 // final pass, after the short-range sum; after_pair (optional) is called
-// once the pair kernel is queued, before the gather
+// once the pair kernel is queued, before the gather.  With deferred
+// non-null and a slot-ordered list, the final pass is left to the caller:
+// the forces stay in nl.sout (slot order, addz not added) and *deferred is
+// set.
 int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, double *d_f,
               hipStream_t s, bool use_counter, const double *addz = nullptr,
-              const std::function<int()> *after_pair = nullptr)
+              const std::function<int()> *after_pair = nullptr, bool *deferred = nullptr)
 {
     if (n == 0) return QEM_OK;
     if (use_counter) {
@@ -709,8 +712,8 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
             API_HIP(ctx, ensure(nl.sxyz, sizeof(double) * 3 * (size_t)n));
             API_HIP(ctx, ensure(nl.sq, sizeof(double) * (size_t)n));
             API_HIP(ctx, ensure(nl.sout, sizeof(double) * 3 * (size_t)n));
-            API_HIP(ctx, launch_rows_to_slots(nl, n, 3, d_xyz, nl.sxyz.as<double>(), s));
-            API_HIP(ctx, launch_rows_to_slots(nl, n, 1, d_q, nl.sq.as<double>(), s));
+            API_HIP(ctx, launch_atoms_to_slots(nl, n, d_xyz, d_q, nl.sxyz.as<double>(),
+                                               nl.sq.as<double>(), s));
             px = nl.sxyz.as<double>();
             pq = nl.sq.as<double>();
             pf = nl.sout.as<double>();
@@ -726,7 +729,8 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
         PhaseScope ps_g(ctx, QEM_PHASE_GATHER, s);
         API_HIP(ctx, launch_gather_self(c, ctx->p.precision, ctx->sw, nl, n, px, pq,
                                         ctx->sw.fpair.as<double>(), pf, s));
-        if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, pf, d_f, s, addz));
+        if (nl.slot_order && deferred) *deferred = true;
+        else if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, pf, d_f, s, addz));
         else if (addz) API_HIP(ctx, launch_add_z(n, addz, d_f, s));
     }
     return QEM_OK;
@@ -744,8 +748,8 @@ int slot_inputs(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     API_HIP(ctx, ensure(nl.sxyz, sizeof(double) * 3 * (size_t)n));
     API_HIP(ctx, ensure(nl.sq, sizeof(double) * (size_t)n));
     API_HIP(ctx, ensure(nl.sout, sizeof(double) * 3 * (size_t)n));
-    API_HIP(ctx, launch_rows_to_slots(nl, n, 3, d_xyz, nl.sxyz.as<double>(), s));
-    API_HIP(ctx, launch_rows_to_slots(nl, n, 1, d_q, nl.sq.as<double>(), s));
+    API_HIP(ctx, launch_atoms_to_slots(nl, n, d_xyz, d_q, nl.sxyz.as<double>(),
+                                       nl.sq.as<double>(), s));
     *px = nl.sxyz.as<double>();
     *pq = nl.sq.as<double>();
     return QEM_OK;
@@ -760,7 +764,7 @@ LongConsts long_consts(const qem_ctx *ctx, double weight, uint32_t parts);
 
 // The long range for at most kLongBatch k vectors, first part: the k rows
 // staged (*Ptot receives their number: P, plus the k = 0 row when parts has
-// it) and the z sort, on stream s.
+// it), the z sort and the per-k constants, on stream s.
 int long_prep(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
               const double *kvecs, int64_t P, double weight, uint32_t parts, hipStream_t s,
               bool sorted, int64_t *Ptot)
@@ -769,11 +773,13 @@ int long_prep(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     if (n == 0) return QEM_OK;
     int st = stage_kvecs(ctx, kvecs, P, weight, (parts & QEM_LONG_K0) != 0, s, Ptot);
     if (st) return st;
+    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
     if (!sorted) {
-        PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
         hipError_t e0 = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e0 != hipSuccess) return hip_err(ctx, e0, "z sort");
     }
+    hipError_t e = launch_long_krec(long_consts(ctx, weight, parts), ctx->lw, *Ptot, s, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, "long range");
     return QEM_OK;
 }
 
@@ -784,7 +790,7 @@ int long_lanes(qem_ctx *ctx, int64_t n, int64_t Ptot, double weight, uint32_t pa
     if (n == 0) return QEM_OK;
     PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
     const LongConsts c = long_consts(ctx, weight, parts);
-    hipError_t e = launch_long_passes(c, ctx->lw, n, Ptot, s, ctx->err);
+    hipError_t e = launch_long_passes(c, ctx->lw, n, Ptot, s, ctx->err, false);
     if (e != hipSuccess) return hip_err(ctx, e, "long range");
     return QEM_OK;
 }
@@ -854,14 +860,22 @@ int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0));
         return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side);
     };
-    if (st == QEM_OK) st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz, &lanes);
+    bool deferred = false;
+    if (st == QEM_OK) st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz, &lanes, &deferred);
     // joined whatever happened, so that s never runs ahead of the side work
     const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
     const hipError_t ew = hipStreamWaitEvent(s, ctx->join_ev, 0);
     if (st) return st;
     if (ej != hipSuccess) return hip_err(ctx, ej, "join event");
     if (ew != hipSuccess) return hip_err(ctx, ew, "join wait");
-    return long_sum(ctx, n, Ptot, d_f, 1, s);
+    if (!deferred) return long_sum(ctx, n, Ptot, d_f, 1, s);
+    // the short range's output pass and the long range's sum in one pass
+    PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
+    NeighborList &nl = ctx->nl;
+    hipError_t e = launch_long_sum_slots(ctx->lw, n, Ptot, nl.slot_of.as<int32_t>(),
+                                         nl.sout.as<double>(), addz, d_f, s, ctx->err);
+    if (e != hipSuccess) return hip_err(ctx, e, "long range");
+    return QEM_OK;
 }
 
 // the current neighbour list, built if the context has none for n atoms
@@ -1215,6 +1229,7 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
                       &ctx->sw.pnode, &ctx->sw.pnodef,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
+                      &ctx->nl.slot_of,
                       &ctx->nl.sxyz, &ctx->nl.sq, &ctx->nl.sv, &ctx->nl.sout,
                       &ctx->nl.cell_key, &ctx->nl.cell_key_sorted, &ctx->nl.atom_idx,
                       &ctx->nl.atom_idx_sorted, &ctx->nl.cell_start, &ctx->nl.cell_end,

@@ -116,6 +116,7 @@ struct NeighborList {
     // atom atom_idx_sorted[s]), so pair neighbours are near in memory; for
     // an installed list slot = atom (slot_order false).
     bool slot_order = false;
+    DevBuf slot_of;       // built lists: the slot of each atom (inverse of atom_idx_sorted)
     DevBuf sxyz, sq, sv;  // per call: positions, charges, directions in slot order
     DevBuf sout;          // per call: per-slot results before the scatter
     // scratch
@@ -130,6 +131,9 @@ hipError_t launch_rows_to_slots(const NeighborList &nl, int64_t n, int w, const 
                                 double *dst, hipStream_t s);
 hipError_t launch_rows_from_slots(const NeighborList &nl, int64_t n, int w, const double *src,
                                   double *dst, hipStream_t s, const double *addz = nullptr);
+// Positions (N x 3) and charges (N) into slot order in one pass.
+hipError_t launch_atoms_to_slots(const NeighborList &nl, int64_t n, const double *d_xyz,
+                                 const double *d_q, double *sxyz, double *sq, hipStream_t s);
 // Builds the list for device coordinates; synchronises the stream once
 // (to learn the pair count).  Returns hipSuccess or the first error.
 hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
@@ -232,11 +236,20 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz,
 hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n,
                        int64_t P, double *d_f, int accumulate, hipStream_t s,
                        std::string &err);
-// launch_long in two parts: the passes up to the per-group partial forces
-// (lw.partB), and their sum into d_f, which may run on another stream once
+// launch_long in parts: the per-k constants (lw.kr) of the rows in lw.kv;
+// the passes up to the per-group partial forces (lw.partB), with the per-k
+// constants first unless krec is false (they are then already in lw.kr for
+// these rows); and the sum into d_f, which may run on another stream once
 // the passes are done.
+hipError_t launch_long_krec(const LongConsts &c, LongWork &lw, int64_t P, hipStream_t s,
+                            std::string &err);
 hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int64_t P,
-                              hipStream_t s, std::string &err);
+                              hipStream_t s, std::string &err, bool krec = true);
+// The sum into d_f fused with the short range's output pass: d_f[a] =
+// sout[slot_of[a]] (+ addz[a] on z) + the long range of atom a.
+hipError_t launch_long_sum_slots(LongWork &lw, int64_t n, int64_t P, const int32_t *slot_of,
+                                 const double *sout, const double *addz, double *d_f,
+                                 hipStream_t s, std::string &err);
 hipError_t launch_long_sum(LongWork &lw, int64_t n, int64_t P, double *d_f, int accumulate,
                            hipStream_t s, std::string &err);
 
