@@ -62,7 +62,6 @@ def run_precision(cfg, xyz_d, q_d, f_d, precision, steps, warmup, seed0=1000):
     p.N_t, p.iter_period, p.rbe_p = cfg.N_t, 100, cfg.P
     p.precision = L.PREC_MIXED if precision == "mixed" else L.PREC_FP64
     ctx = L.Context(p)
-    ctx.timing_enable(True)
     step = 0
 
     def one():
@@ -74,39 +73,41 @@ def run_precision(cfg, xyz_d, q_d, f_d, precision, steps, warmup, seed0=1000):
     for _ in range(warmup):
         one()
     L.synchronize()
-    ctx.timing_read(reset=True)
     t0 = time.perf_counter()
     for _ in range(steps):
         one()
     L.synchronize()
     dt = time.perf_counter() - t0
-    phases_step = ctx.timing_read(reset=True)
     npairs, builds, nsteps = ctx.short_state()
     f = f_d.download()
-    # The step runs the long range on a second stream alongside the short
-    # range, so its phase events overlap.  Each part is also timed alone, on
-    # the same inputs and k batches: the short range (neighbour list, pair
-    # kernel, gather) and the long range, one launch sequence after another.
+    # Per-part times (HIP events, after the timed steps, which run without
+    # them).  The step runs the long range on a second stream beside the
+    # short range, so each part is timed alone, on the same inputs: the short
+    # range over 100 steps (one neighbour-list rebuild among them, as in the
+    # step), then the long range over 20 k batches.
     g = L.DeviceArray(np.zeros(3 * cfg.n))
-    nparts = max(1, min(steps, 20))
-    for _ in range(nparts):
+    ctx.timing_enable(True)
+    for _ in range(100):
         ctx.short_forces_dev(cfg.n, xyz_d.ptr, q_d.ptr, g.ptr)
-    for t in range(nparts):
+    L.synchronize()
+    short = ctx.timing_read(reset=True)
+    for t in range(20):
         ctx.long_forces_dev(cfg.n, xyz_d.ptr, q_d.ptr, ctx.sample_k(seed0 + t, cfg.P), g.ptr)
     L.synchronize()
-    phases = ctx.timing_read(reset=True)
+    long_ = ctx.timing_read(reset=True)
+    # part -> (ms per step, ms per launch of its events)
+    phases = {k: (short[k][0] / 100, short[k][0] / max(short[k][1], 1))
+              for k in ("neighbor", "pair", "gather")}
+    phases["long"] = (long_["long"][0] / 20, long_["long"][0] / max(long_["long"][1], 1))
     del g
     ctx.close()
-    return {"ms_per_step": dt / steps * 1e3, "phases": phases, "parts_steps": nparts,
-            "phases_step": phases_step, "npairs": npairs, "builds": builds, "f": f}
+    return {"ms_per_step": dt / steps * 1e3, "phases": phases, "npairs": npairs,
+            "builds": builds, "f": f}
 
 
-def parts_per_step(run, steps):
-    """ms per step of each part: the neighbour list from the timed steps
-    (rebuilt every 100 of them), the rest from the parts timed alone."""
-    out = {k: v[0] / run["parts_steps"] for k, v in run["phases"].items()}
-    out["neighbor"] = run["phases_step"]["neighbor"][0] / max(steps, 1)
-    return out
+def parts_per_step(run):
+    """ms per step of each part, timed alone."""
+    return {k: v[0] for k, v in run["phases"].items()}
 
 
 def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, target_s=8.0):
@@ -510,12 +511,10 @@ def main(argv=None):
     value = cfg.n / (mix["ms_per_step"] * 1e-3)
     value64 = cfg.n / (f64["ms_per_step"] * 1e-3)
 
-    pair_ms, pair_calls = mix["phases"]["pair"]
-    pair_avg_ms = pair_ms / max(pair_calls, 1)
+    pair_avg_ms = mix["phases"]["pair"][1]
     fl = pair_flops(mix["npairs"], cfg.N_t)
     achieved = fl / (pair_avg_ms * 1e-3) / 1e12
-    p64_ms, p64_calls = f64["phases"]["pair"]
-    p64_avg = p64_ms / max(p64_calls, 1)
+    p64_avg = f64["phases"]["pair"][1]
     achieved64 = fl / (p64_avg * 1e-3) / 1e12
 
     out = {
@@ -542,8 +541,8 @@ def main(argv=None):
         "rel_rms_err_vs_fp64": err,
         # each part timed alone (the step overlaps the long range with the
         # short range, so these add up to more than ms_per_step)
-        "phases_ms_per_step": parts_per_step(mix, a.steps),
-        "phases_ms_per_step_fp64": parts_per_step(f64, fp64_steps),
+        "phases_ms_per_step": parts_per_step(mix),
+        "phases_ms_per_step_fp64": parts_per_step(f64),
         "overlap": "mixed precision: long range on a second stream alongside the short range",
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
