@@ -310,23 +310,27 @@ def pole_step(steps=20, warmup=2):
     xyz, q = make_system(CONFIGS["slab_n100k"], seed=0)
     xd, qd, fd = L.DeviceArray(xyz), L.DeviceArray(q), L.DeviceArray(np.zeros_like(xyz))
     ctx = L.Context(_params(cfg, "mixed", semantics=L.SEM_CORRECTED | L.SEM_POLE_SPLIT))
-    ctx.timing_enable(True)
     for s in range(warmup):
         ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, ctx.sample_k(500 + s, cfg.P), fd.ptr)
     L.synchronize()
-    ctx.timing_read(reset=True)
     t = time.perf_counter()
     for s in range(steps):
         ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, ctx.sample_k(600 + s, cfg.P), fd.ptr)
     L.synchronize()
     dt = (time.perf_counter() - t) / steps
-    ph = ctx.timing_read(reset=True)
     npairs = ctx.short_state()[0]
     f = fd.download()
+    # the pair kernel's time from HIP events, the short range timed alone
+    g = L.DeviceArray(np.zeros_like(xyz))
+    ctx.timing_enable(True)
+    for s in range(5):
+        ctx.short_forces_dev(cfg.n, xd.ptr, qd.ptr, g.ptr)
+    L.synchronize()
+    ph = ctx.timing_read(reset=True)
     ctx.close()
     return {"workload": cfg.name, "gamma": [-2.0, -2.0], "N_t": 240, "ms_per_step": dt * 1e3,
             "value": cfg.n / dt, "unit": "atom-force evals/s", "n_pairs": npairs,
-            "pair_ms": ph["pair"][0] / steps, "finite": bool(np.all(np.isfinite(f)))}
+            "pair_ms": ph["pair"][0] / 5, "finite": bool(np.all(np.isfinite(f)))}
 
 
 LAMMPS_CHILD = r"""
