@@ -391,7 +391,7 @@ __global__ __launch_bounds__(256) void k_self_z(
 
 // Per atom: sum of pair forces with the atom as first endpoint, minus the
 // xy (and, as written, z) forces of pairs with it as second endpoint
-// (src/QEM_short.jl:41-56), plus the self term already in f[3a + 2].
+// (src/QEM_short.jl:41-56), plus the self term.
 // Sixteen lanes per atom read the atom's runs of pair results in
 // contiguous strides; each lane keeps a strided partial sum and the row of
 // 16 lanes is summed in a fixed order.  No atomics.  With SELF the kernel
