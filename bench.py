@@ -110,8 +110,9 @@ def parts_per_step(run):
     return {k: v[0] for k, v in run["phases"].items()}
 
 
-def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, target_s=8.0):
-    """Times the oracle on a bounded sample and extrapolates to one step."""
+def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, target_s=15.0):
+    """Times the oracle on a bounded sample (about target_s seconds) and
+    extrapolates to one step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -119,8 +120,9 @@ def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, targe
                  cfg.accuracy, cfg.N_t, 0)
     pairs = pairs_dev_ctx_pairs
     rng = np.random.default_rng(0)
-    # calibrate on a small sample, then size the timed sample to ~target_s
-    m0 = min(len(pairs), 2000)
+    # calibrate on a small sample (large enough to amortise the thread start),
+    # then size the timed sample to ~0.6 target_s
+    m0 = min(len(pairs), 2000 * threads)
     sel = rng.choice(len(pairs), m0, replace=False)
     t = time.perf_counter()
     O.short_pair_forces(p, xyz, q, pairs[sel], nthreads=threads)
@@ -614,7 +616,7 @@ def main(argv=None):
         out["cpu_baseline"] = cpu_baseline(cfg, xyz, q, pairs, K[kidx], sK / cfg.P, threads)
         out["cpu_baseline"]["host"] = host_cpu()
         # the same restatement on one core, and on the smaller workloads
-        more = [dict(cpu_baseline(cfg, xyz, q, pairs, K[kidx], sK / cfg.P, 1, target_s=3.0),
+        more = [dict(cpu_baseline(cfg, xyz, q, pairs, K[kidx], sK / cfg.P, 1, target_s=10.0),
                      workload=cfg.name)]
         for name in ("slab_n100", "slab_n10k"):
             c2 = CONFIGS[name]
