@@ -140,6 +140,18 @@ def test_bench_refuses_multi_gpu():
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
 
 
+def test_bench_flop_convention():
+    # the roofline numerator: 4 image terms x N_t nodes x (point 62 + gaussian
+    # 64) flop per pair, i.e. 25,200 at N_t = 50 (add/mul 1, FMA 2,
+    # transcendental 1 plus its argument, joint J0 + J1 40)
+    sys.path.insert(0, ROOT)
+    import bench
+    assert bench.pair_flops(1, 50) == 25_200.0
+    assert bench.pair_flops(15_740_142, 50) == 15_740_142 * 25_200.0
+    assert bench.parts_per_step({"phases": {"pair": (2.99, 2.99), "long": (0.32, 0.32)}}) == \
+        {"pair": 2.99, "long": 0.32}
+
+
 def test_workload_configs():
     from qem4molly_amd.workloads import CONFIGS, make_system
     xyz, q = make_system("slab_n100k")
