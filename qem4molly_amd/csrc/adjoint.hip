@@ -670,7 +670,8 @@ inline unsigned pair_grid(int64_t m)
 hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
                                const NeighborList &nl, int64_t n, const double *d_xyz,
                                const double *d_q, double *d_pot, double *d_phi,
-                               double *d_psi, hipStream_t s)
+                               double *d_psi, hipStream_t s,
+                               const std::function<void()> *after_pair)
 {
     if (n <= 0) return hipSuccess;
     const int64_t np = nl.npairs;
@@ -683,6 +684,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
                 nl.pj.as<int32_t>(), np, d_xyz, pot);
+        if (after_pair) (*after_pair)();
         k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
@@ -699,6 +701,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                     ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
                     d_xyz, pot);
         }
+        if (after_pair) (*after_pair)();
         k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);

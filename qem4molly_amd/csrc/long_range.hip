@@ -1150,11 +1150,9 @@ hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, in
     return hipSuccess;
 }
 
-hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
-                              double *d_phi, int accumulate, double *h_dgamma, hipStream_t s,
-                              std::string &err)
+hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t n,
+                                     int64_t Ptot, hipStream_t s, std::string &err)
 {
-    h_dgamma[0] = h_dgamma[1] = 0.0;
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
     const size_t pc = sizeof(double4) * (size_t)Ptot * lg.nchunk;
@@ -1188,15 +1186,37 @@ hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int6
             c, kr, Ptot, sa, n, lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
             lw.totA.as<double4>(), lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
-    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 1,
-                                           lw.perm.as<int32_t>(), accumulate, 1.0, d_phi);
-    LR_CHECK(hipGetLastError());
     k_long_dgamma<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot,
                                                   lw.totA.as<double4>(), lw.q0part.as<double2>());
     LR_CHECK(hipGetLastError());
-    std::vector<double> h(2 * (size_t)Ptot);
-    LR_CHECK(hipMemcpyAsync(h.data(), lw.q0part.p, sizeof(double2) * (size_t)Ptot,
+    return hipSuccess;
+}
+
+hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double *d_phi,
+                                  int accumulate, double *h_q0, hipStream_t s,
+                                  std::string &err)
+{
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
+    const LaneGrid lg = lane_grid(n, Ptot);
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 1,
+                                           lw.perm.as<int32_t>(), accumulate, 1.0, d_phi);
+    LR_CHECK(hipGetLastError());
+    LR_CHECK(hipMemcpyAsync(h_q0, lw.q0part.p, sizeof(double2) * (size_t)Ptot,
                             hipMemcpyDeviceToHost, s));
+    return hipSuccess;
+}
+
+hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                              double *d_phi, int accumulate, double *h_dgamma, hipStream_t s,
+                              std::string &err)
+{
+    h_dgamma[0] = h_dgamma[1] = 0.0;
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
+    std::vector<double> h(2 * (size_t)Ptot);
+    hipError_t e = launch_long_energy_passes(c, lw, n, Ptot, s, err);
+    if (e == hipSuccess)
+        e = launch_long_energy_sum(lw, n, Ptot, d_phi, accumulate, h.data(), s, err);
+    if (e != hipSuccess) return e;
     LR_CHECK(hipStreamSynchronize(s));
     for (int64_t p = 0; p < Ptot; ++p) {
         h_dgamma[0] += h[2 * (size_t)p];

@@ -939,6 +939,34 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
         phi = ctx->d_f2.as<double>();
     }
+    // in mixed precision the long-range passes run on the side stream, as in
+    // forces_core: k staging and z sort beside the pair kernel, the passes
+    // beside the self term and the gather, the phi sum and the dE/dgamma
+    // copy on s after the short range
+    const bool overlap = P >= 0 && ctx->p.precision == QEM_PREC_MIXED;
+    const LongConsts lc = long_consts(ctx, weight, QEM_LONG_ALL);
+    int64_t Ptot = 0;
+    int lst = QEM_OK;
+    if (overlap) {
+        API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
+        API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+        lst = stage_kvecs(ctx, kvecs, P, weight, true, ctx->side, &Ptot);
+        if (lst == QEM_OK) {
+            PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
+            hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, ctx->side, ctx->err);
+            if (e != hipSuccess) lst = hip_err(ctx, e, "z sort");
+        }
+    }
+    const std::function<void()> passes = [&]() {
+        if (lst) return;
+        hipError_t e = hipEventRecord(ctx->lanes_ev, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
+        if (e == hipSuccess) {
+            PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
+            e = launch_long_energy_passes(lc, ctx->lw, n, Ptot, ctx->side, ctx->err);
+        }
+        if (e != hipSuccess) lst = hip_err(ctx, e, "long-range energy");
+    };
     {
         PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
         const double *px, *pq;
@@ -948,27 +976,40 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         double *phs = nl.slot_order ? nl.sout.as<double>() : phi;
         double *pss = nl.slot_order ? nl.sout.as<double>() + n : psi;
         API_HIP(ctx, launch_short_energy(c, ctx->p.precision, ctx->sw, nl, n, px, pq,
-                                         ctx->sw.fpair.as<double>(), phs, pss, s));
+                                         ctx->sw.fpair.as<double>(), phs, pss, s,
+                                         overlap ? &passes : nullptr));
         if (nl.slot_order) {
             API_HIP(ctx, launch_rows_from_slots(nl, n, 1, phs, phi, s));
             API_HIP(ctx, launch_rows_from_slots(nl, n, 2, pss, psi, s));
         }
     }
     double dg_long[2] = {0.0, 0.0};
-    if (P >= 0) {
-        int64_t Ptot = 0;
+    std::vector<double> hq0;
+    if (overlap) {
+        API_HIP(ctx, hipEventRecord(ctx->join_ev, ctx->side));
+        API_HIP(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
+        if (lst) return lst;
+        hq0.resize(2 * (size_t)Ptot);
+        PhaseScope ps(ctx, QEM_PHASE_LONG, s);
+        hipError_t e = launch_long_energy_sum(ctx->lw, n, Ptot, phi, 1, hq0.data(), s,
+                                              ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
+    } else if (P >= 0) {
         if ((st = stage_kvecs(ctx, kvecs, P, weight, true, s, &Ptot))) return st;
         PhaseScope ps(ctx, QEM_PHASE_LONG, s);
         hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "z sort");
-        e = launch_long_energy(long_consts(ctx, weight, QEM_LONG_ALL), ctx->lw, n, Ptot, phi, 1,
-                               dg_long, s, ctx->err);
+        e = launch_long_energy(lc, ctx->lw, n, Ptot, phi, 1, dg_long, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
     }
     API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s));
     double h[3];
     API_HIP(ctx, hipMemcpyAsync(h, out3, sizeof h, hipMemcpyDeviceToHost, s));
     API_HIP(ctx, hipStreamSynchronize(s));
+    for (size_t p = 0; p < hq0.size(); p += 2) {
+        dg_long[0] += hq0[p];
+        dg_long[1] += hq0[p + 1];
+    }
     if (E) *E = h[0];
     if (dEdg) {
         dEdg[0] = h[1] + dg_long[0];

@@ -264,6 +264,15 @@ hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, in
 hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                               double *d_phi, int accumulate, double *h_dgamma, hipStream_t s,
                               std::string &err);
+// launch_long_energy in two parts that do not synchronise: the passes (with
+// the per-k dE_k/dgamma on the device), then, on the same or a joined stream,
+// the phi sum into d_phi and the copy of the per-k dE_k/dgamma into h_q0
+// (2 Ptot doubles, summed by the caller once s is done).
+hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t n,
+                                     int64_t Ptot, hipStream_t s, std::string &err);
+hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double *d_phi,
+                                  int accumulate, double *h_q0, hipStream_t s,
+                                  std::string &err);
 // Tangent of the long-range force along d_v (N x 3), times scale.  Without
 // with_sum the call stops at the per-group partial tangents (lw.partB);
 // launch_long_tangent_sum then adds (or writes) them into d_out, possibly
@@ -276,10 +285,12 @@ hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double
                                    std::string &err);
 // Short-range energy: per-pair potentials in d_pot (double4 per pair), per
 // atom phi (N) and psi (N x 2: d phi / d gamma_1,2), self terms included.
+// after_pair (optional) is called once the pair kernel is queued.
 hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
                                const NeighborList &nl, int64_t n, const double *d_xyz,
                                const double *d_q, double *d_pot, double *d_phi,
-                               double *d_psi, hipStream_t s);
+                               double *d_psi, hipStream_t s,
+                               const std::function<void()> *after_pair = nullptr);
 // Short-range force tangent along d_v, times scale, into d_out (N x 3);
 // d_tp is per-pair scratch (double4 per pair).
 // after_pair (optional) is called once the pair kernel is queued, before
