@@ -984,7 +984,8 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         }
     }
     double dg_long[2] = {0.0, 0.0};
-    std::vector<double> hq0;
+    std::vector<double> &hq0 = ctx->h_q0;
+    hq0.clear();
     if (overlap) {
         API_HIP(ctx, hipEventRecord(ctx->join_ev, ctx->side));
         API_HIP(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));

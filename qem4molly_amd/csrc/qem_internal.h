@@ -339,6 +339,10 @@ struct qem_ctx {
     // k-lane passes after the pair kernel, join_ev the final sum after it
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, lanes_ev = nullptr, join_ev = nullptr;
+    // per-k dE_k/dgamma of the energy call, copied on the call's stream; kept
+    // here, not on the stack, so an early error return leaves no copy
+    // pending into freed memory
+    std::vector<double> h_q0;
     // K set (host)
     std::vector<double> kset;   // |K| x 3
     std::vector<float> kprob;   // |K|
