@@ -18,10 +18,12 @@
 // Precision modes
 //   FP64:  everything double; J0/J1 from the device math library.
 //   MIXED: node arithmetic, exponentials (v_exp_f32), reciprocal
-//          (v_rcp_f32) and J0/J1 in FP32; J0/J1 come from a table of degree-4
-//          polynomials of J0 fitted on 0.125-wide intervals (LDS resident;
-//          J1 = -J0' from the same polynomial, p and p' by one Horner pass).  Per-pair set-up, closed-form image terms and all
-//          sums over images, pairs and atoms stay FP64.
+//          (v_rcp_f32) and J0/J1 in FP32; J0/J1 come from a table of cubic
+//          polynomials of J0 fitted on 1/32-wide intervals (1/16 when the
+//          argument range needs more than 4096 intervals; LDS resident;
+//          J1 = -J0' from the same polynomial, p and p' by one Horner
+//          pass).  Per-pair set-up, closed-form image terms and all sums
+//          over images, pairs and atoms stay FP64.
 #include "qem_device.h"
 
 namespace qem {
