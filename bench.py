@@ -485,6 +485,9 @@ def main(argv=None):
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy this many MiB three times with the library's "
                          "plain copy kernel (a known-size reference for HBM byte counters)")
+    ap.add_argument("--adjoint-only", default=None, metavar="WORKLOAD:PRECISION",
+                    help="time only the forward, the HVP and the energy of one workload "
+                         "(e.g. slab_n100k:mixed) and print that one JSON line")
     a = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -494,6 +497,13 @@ def main(argv=None):
                   f"--gpus {a.gpus} (WORLD_SIZE={world}) is not supported. How the path "
                   "would shard is described in DESIGN.md.", file=sys.stderr)
         return 2
+
+    if a.adjoint_only:
+        name, _, prec = a.adjoint_only.partition(":")
+        if name not in ("slab_n100", "slab_n10k", "slab_n100k") or prec not in ("mixed", "fp64"):
+            ap.error("--adjoint-only takes WORKLOAD:PRECISION, e.g. slab_n100k:mixed")
+        print(json.dumps(adjoint_cost(name, prec)), flush=True)
+        return 0
 
     from qem4molly_amd import _lib as L
     from qem4molly_amd.workloads import CONFIGS, make_system, rel_rms

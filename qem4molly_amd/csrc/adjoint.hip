@@ -440,26 +440,9 @@ __device__ double self_force_tangent(const ShortConsts &c, const double *__restr
 
 // ---- kernels ------------------------------------------------------------------------
 
-template <int PREC>
-__global__ __launch_bounds__(256) void k_pair_energy(
-    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
-    const float *__restrict__ glxf, const float *__restrict__ glwf,
-    const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
-    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    double4 *__restrict__ out)
-{
-    extern __shared__ float4 lds_tab[];
-    const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
-        const int64_t i = pi[p], j = pj[p];
-        out[p] = pair_energy<PREC>(c, glx, glw, glxf, glwf, tab, xyz[3 * i], xyz[3 * i + 1],
-                                   xyz[3 * i + 2], xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2]);
-    }
-}
-
-// Per-pair tangent records: float4 in mixed precision (the values come from
-// FP32 node sums; the per-atom sums of the gather stay FP64), double4 in FP64.
+// Per-pair records of the energy and tangent passes: float4 in mixed
+// precision (the values come from FP32 node sums; the per-atom sums of the
+// gathers stay FP64), double4 in FP64.
 template <int PREC>
 using TanRec = typename std::conditional<PREC == QEM_PREC_MIXED, float4, double4>::type;
 
@@ -468,6 +451,25 @@ __device__ __forceinline__ void store_tan(float4 *r, const double4 &d)
     *r = make_float4((float)d.x, (float)d.y, (float)d.z, (float)d.w);
 }
 __device__ __forceinline__ void store_tan(double4 *r, const double4 &d) { *r = d; }
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pair_energy(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float *__restrict__ glxf, const float *__restrict__ glwf,
+    const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
+    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
+    TanRec<PREC> *__restrict__ out)
+{
+    extern __shared__ float4 lds_tab[];
+    const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+        const int64_t i = pi[p], j = pj[p];
+        store_tan(out + p, pair_energy<PREC>(c, glx, glw, glxf, glwf, tab, xyz[3 * i],
+                                             xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
+                                             xyz[3 * j + 1], xyz[3 * j + 2]));
+    }
+}
 
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_tangent(
@@ -507,16 +509,21 @@ __global__ __launch_bounds__(256) void k_self_energy(
 
 // per atom: phi (dE/dq) and psi_1,2 (d phi / d gamma) from the pair
 // potentials of both endpoints, added to the self terms already there.
-// Sixteen lanes per atom, strided contiguous reads, fixed-order row sums.
+// Sixteen lanes per atom, strided contiguous reads, fixed-order row sums,
+// and the XCD-aware block order of the force gather (gridDim.x a multiple
+// of 8: the blocks of one XCD take a contiguous range of atoms).
+template <typename Rec>
 __global__ __launch_bounds__(256) void k_gather_energy(
-    int64_t n, const double *__restrict__ q, const double4 *__restrict__ pot,
+    int64_t n, const double *__restrict__ q, const Rec *__restrict__ pot,
     const int32_t *__restrict__ pi, const int32_t *__restrict__ pj,
     const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
     const int32_t *__restrict__ i_idx, const int32_t *__restrict__ j_start,
     const int32_t *__restrict__ j_end, const int32_t *__restrict__ j_idx,
     double *__restrict__ phi, double *__restrict__ psi)
 {
-    const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int64_t per_xcd = gridDim.x / 8;
+    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
     const bool live = a < n;
     double f = 0.0, s0 = 0.0, s1 = 0.0;
@@ -524,20 +531,20 @@ __global__ __launch_bounds__(256) void k_gather_energy(
         const int32_t t1 = i_end[a];
         for (int32_t t = i_start[a] + l; t < t1; t += 16) {
             const int32_t p = i_idx ? i_idx[t] : t;
-            const double4 u = pot[p];
+            const Rec u = pot[p];
             const double qb = q[pj[p]];
-            f += qb * u.x;
-            s0 += qb * u.y;
-            s1 += qb * u.z;
+            f += qb * (double)u.x;
+            s0 += qb * (double)u.y;
+            s1 += qb * (double)u.z;
         }
         const int32_t u1 = j_end[a];
         for (int32_t t = j_start[a] + l; t < u1; t += 16) {
             const int32_t p = j_idx[t];
-            const double4 u = pot[p];
+            const Rec u = pot[p];
             const double qb = q[pi[p]];
-            f += qb * u.x;
-            s0 += qb * u.y;
-            s1 += qb * u.z;
+            f += qb * (double)u.x;
+            s0 += qb * (double)u.y;
+            s1 += qb * (double)u.z;
         }
     }
     f = dev::row_sum16(f);
@@ -677,13 +684,14 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     double4 *pot = reinterpret_cast<double4 *>(d_pot);
+    float4 *potf = reinterpret_cast<float4 *>(d_pot);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     if (precision == QEM_PREC_MIXED) {
         if (np > 0)
             k_pair_energy<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
-                nl.pj.as<int32_t>(), np, d_xyz, pot);
+                nl.pj.as<int32_t>(), np, d_xyz, potf);
         if (after_pair) (*after_pair)();
         k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
@@ -706,10 +714,17 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
     }
-    k_gather_energy<<<nblk(16 * n, 256), 256, 0, s>>>(
-        n, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(),
-        nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-        nl.j_idx.as<int32_t>(), d_phi, d_psi);
+    const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
+    if (precision == QEM_PREC_MIXED)
+        k_gather_energy<float4><<<nb, 256, 0, s>>>(
+            n, d_q, potf, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(),
+            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+            nl.j_idx.as<int32_t>(), d_phi, d_psi);
+    else
+        k_gather_energy<double4><<<nb, 256, 0, s>>>(
+            n, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(),
+            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
+            nl.j_idx.as<int32_t>(), d_phi, d_psi);
     return hipGetLastError();
 }
 
