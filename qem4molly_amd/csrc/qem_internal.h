@@ -283,7 +283,8 @@ hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int
 hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double scale,
                                    double *d_out, int accumulate, hipStream_t s,
                                    std::string &err);
-// Short-range energy: per-pair potentials in d_pot (double4 per pair), per
+// Short-range energy: per-pair potentials in d_pot (room for a double4 per
+// pair; the records are float4 in mixed precision, double4 in FP64), per
 // atom phi (N) and psi (N x 2: d phi / d gamma_1,2), self terms included.
 // after_pair (optional) is called once the pair kernel is queued.
 hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
@@ -292,7 +293,8 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                                double *d_psi, hipStream_t s,
                                const std::function<void()> *after_pair = nullptr);
 // Short-range force tangent along d_v, times scale, into d_out (N x 3);
-// d_tp is per-pair scratch (double4 per pair).
+// d_tp is per-pair scratch (room for a double4 per pair; float4 records in
+// mixed precision, double4 in FP64).
 // after_pair (optional) is called once the pair kernel is queued, before
 // the self term and the gather.
 hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
