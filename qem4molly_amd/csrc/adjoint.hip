@@ -452,22 +452,34 @@ __device__ __forceinline__ void store_tan(float4 *r, const double4 &d)
 }
 __device__ __forceinline__ void store_tan(double4 *r, const double4 &d) { *r = d; }
 
+// Per-pair potentials.  FP64: one double4 record per pair, (G, dG/dgamma_1,
+// dG/dgamma_2, -).  Mixed: two float4 records, one per endpoint, already
+// weighted by the other endpoint's charge: out[p] = q_j (G, dG1, dG2, -) for
+// atom i, out[npairs + p] = q_i (G, dG1, dG2, -) for atom j, so the gather
+// reads one 16-byte record per pair and endpoint and no charges.
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_energy(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float *__restrict__ glxf, const float *__restrict__ glwf,
     const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
     const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    TanRec<PREC> *__restrict__ out)
+    const double *__restrict__ q, TanRec<PREC> *__restrict__ out)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const int64_t i = pi[p], j = pj[p];
-        store_tan(out + p, pair_energy<PREC>(c, glx, glw, glxf, glwf, tab, xyz[3 * i],
-                                             xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
-                                             xyz[3 * j + 1], xyz[3 * j + 2]));
+        const double4 u = pair_energy<PREC>(c, glx, glw, glxf, glwf, tab, xyz[3 * i],
+                                            xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
+                                            xyz[3 * j + 1], xyz[3 * j + 2]);
+        if (PREC == QEM_PREC_MIXED) {
+            const double qi = q[i], qj = q[j];
+            store_tan(out + p, make_double4(qj * u.x, qj * u.y, qj * u.z, 0.0));
+            store_tan(out + npairs + p, make_double4(qi * u.x, qi * u.y, qi * u.z, 0.0));
+        } else {
+            store_tan(out + p, u);
+        }
     }
 }
 
@@ -545,6 +557,47 @@ __global__ __launch_bounds__(256) void k_gather_energy(
             f += qb * (double)u.x;
             s0 += qb * (double)u.y;
             s1 += qb * (double)u.z;
+        }
+    }
+    f = dev::row_sum16(f);
+    s0 = dev::row_sum16(s0);
+    s1 = dev::row_sum16(s1);
+    if (live && l == 0) {
+        phi[a] += f;
+        psi[2 * a] += s0;
+        psi[2 * a + 1] += s1;
+    }
+}
+
+// k_gather_energy over the charge-weighted float4 records of the mixed
+// pair kernel: wi[p] for the pair's first atom, wj[p] for its second.
+__global__ __launch_bounds__(256) void k_gather_energy_w(
+    int64_t n, const float4 *__restrict__ wi, const float4 *__restrict__ wj,
+    const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
+    const int32_t *__restrict__ i_idx, const int32_t *__restrict__ j_start,
+    const int32_t *__restrict__ j_end, const int32_t *__restrict__ j_idx,
+    double *__restrict__ phi, double *__restrict__ psi)
+{
+    const int64_t per_xcd = gridDim.x / 8;
+    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    const bool live = a < n;
+    double f = 0.0, s0 = 0.0, s1 = 0.0;
+    if (live) {
+        const int32_t t1 = i_end[a];
+        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
+            const float4 u = wi[i_idx ? i_idx[t] : t];
+            f += (double)u.x;
+            s0 += (double)u.y;
+            s1 += (double)u.z;
+        }
+        const int32_t u1 = j_end[a];
+        for (int32_t t = j_start[a] + l; t < u1; t += 16) {
+            const float4 u = wj[j_idx[t]];
+            f += (double)u.x;
+            s0 += (double)u.y;
+            s1 += (double)u.z;
         }
     }
     f = dev::row_sum16(f);
@@ -691,7 +744,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
             k_pair_energy<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
-                nl.pj.as<int32_t>(), np, d_xyz, potf);
+                nl.pj.as<int32_t>(), np, d_xyz, d_q, potf);
         if (after_pair) (*after_pair)();
         k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
@@ -702,12 +755,12 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                 k_pair_energy<QEM_PREC_FP64><<<pair_grid(np), 256, dev::lds_bytes64(c), s>>>(
                     c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                     ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                    d_xyz, pot);
+                    d_xyz, d_q, pot);
             else
                 k_pair_energy<kPrecFP64Lib><<<pair_grid(np), 256, 0, s>>>(
                     c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                     ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                    d_xyz, pot);
+                    d_xyz, d_q, pot);
         }
         if (after_pair) (*after_pair)();
         k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
@@ -716,10 +769,10 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     }
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     if (precision == QEM_PREC_MIXED)
-        k_gather_energy<float4><<<nb, 256, 0, s>>>(
-            n, d_q, potf, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(),
-            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-            nl.j_idx.as<int32_t>(), d_phi, d_psi);
+        k_gather_energy_w<<<nb, 256, 0, s>>>(
+            n, potf, potf + np, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi,
+            d_psi);
     else
         k_gather_energy<double4><<<nb, 256, 0, s>>>(
             n, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(),

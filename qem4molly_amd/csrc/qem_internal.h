@@ -284,8 +284,9 @@ hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double
                                    double *d_out, int accumulate, hipStream_t s,
                                    std::string &err);
 // Short-range energy: per-pair potentials in d_pot (room for a double4 per
-// pair; the records are float4 in mixed precision, double4 in FP64), per
-// atom phi (N) and psi (N x 2: d phi / d gamma_1,2), self terms included.
+// pair: one double4 record in FP64, two charge-weighted float4 records in
+// mixed precision), per atom phi (N) and psi (N x 2: d phi / d gamma_1,2),
+// self terms included.
 // after_pair (optional) is called once the pair kernel is queued.
 hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
                                const NeighborList &nl, int64_t n, const double *d_xyz,
