@@ -73,13 +73,39 @@ def run_precision(cfg, xyz_d, q_d, f_d, precision, steps, warmup, seed0=1000):
     for _ in range(warmup):
         one()
     L.synchronize()
+    b0 = ctx.short_state()[1]
     t0 = time.perf_counter()
     for _ in range(steps):
         one()
     L.synchronize()
     dt = time.perf_counter() - t0
     npairs, builds, nsteps = ctx.short_state()
+    builds_timed = builds - b0
     f = f_d.download()
+    kidx_last = ctx.sample_k(seed0 + step - 1, cfg.P)
+    # The neighbour list is rebuilt at steps 1, 101, 201, ... (iter_period =
+    # 100, src/QEM_short.jl:32-36).  Two more windows after the timed steps:
+    # 20 steps holding no rebuild, then 100 consecutive steps holding exactly
+    # one (its cost amortised at its frequency).
+    while ctx.short_state()[2] % 100 != 1:
+        one()
+    L.synchronize()
+    b1 = ctx.short_state()[1]
+    t0 = time.perf_counter()
+    for _ in range(20):
+        one()
+    L.synchronize()
+    dt_none = (time.perf_counter() - t0) / 20
+    while ctx.short_state()[2] % 100 != 0:
+        one()
+    L.synchronize()
+    b2 = ctx.short_state()[1]
+    t0 = time.perf_counter()
+    for _ in range(100):
+        one()
+    L.synchronize()
+    dt_amort = (time.perf_counter() - t0) / 100
+    windows_ok = ctx.short_state()[1] - b2 == 1 and b2 == b1
     # Per-part times (HIP events, after the timed steps, which run without
     # them).  The step runs the long range on a second stream beside the
     # short range, so each part is timed alone, on the same inputs: the short
@@ -102,7 +128,9 @@ def run_precision(cfg, xyz_d, q_d, f_d, precision, steps, warmup, seed0=1000):
     del g
     ctx.close()
     return {"ms_per_step": dt / steps * 1e3, "phases": phases, "npairs": npairs,
-            "builds": builds, "f": f}
+            "builds": builds, "builds_timed": builds_timed, "f": f, "kidx_last": kidx_last,
+            "ms_per_step_rebuild_amortized": dt_amort * 1e3,
+            "ms_per_step_no_rebuild": dt_none * 1e3, "rebuild_windows_ok": windows_ok}
 
 
 def parts_per_step(run):
@@ -115,7 +143,7 @@ def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, targe
     extrapolates to one step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    O.build()
+    lib_path = O.use_avx512_build()
     p = O.params(cfg.L, cfg.r_cutoff, cfg.alpha, cfg.gamma_1, cfg.gamma_2, cfg.eps_0,
                  cfg.accuracy, cfg.N_t, 0)
     pairs = pairs_dev_ctx_pairs
@@ -152,7 +180,37 @@ def cpu_baseline(cfg, xyz, q, pairs_dev_ctx_pairs, kvecs, weight, threads, targe
                    f"linearly to one full step (C FP64 restatement, OpenMP)"),
         "seconds_per_step_est": t_step,
         "parts_s": {"pairs": t_pairs, "self": t_self, "long": t_long},
+        "build": os.path.basename(lib_path) + (" (gcc -O3 -march=x86-64-v4)" if "v4" in lib_path
+                                               else " (gcc -O3 -march=x86-64-v3)"),
     }
+
+
+def oracle_sampled_errors(cfg, xyz, q, runs, nrows=200, threads=16):
+    """rel-RMS force error of each run's last step against the FP64 oracle,
+    over a seeded sample of nrows atoms: the oracle's short range for those
+    rows (qo_short_forces_rows, each row's pairs by an O(N) scan) plus its
+    long range for the run's last k batch (all atoms; the sampled rows
+    kept).  runs: {precision: run_precision result}."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from qem4molly_amd.workloads import rel_rms
+    O.use_avx512_build()
+    t0 = time.perf_counter()
+    p = O.params(cfg.L, cfg.r_cutoff, cfg.alpha, cfg.gamma_1, cfg.gamma_2, cfg.eps_0,
+                 cfg.accuracy, cfg.N_t, 0)
+    K, _, sK, _ = O.kset(cfg.Lx, cfg.Ly, cfg.alpha, cfg.accuracy)
+    rows = np.sort(np.random.default_rng(17).choice(cfg.n, min(nrows, cfg.n), replace=False))
+    short = O.short_forces_rows(p, xyz, q, rows, nthreads=threads)
+    out, longs = {}, {}
+    for prec, run in runs.items():
+        key = run["kidx_last"].tobytes()
+        if key not in longs:
+            longs[key] = O.long_forces(p, xyz, q, K[run["kidx_last"]], weight=sK / cfg.P,
+                                       nthreads=threads)[rows]
+        out[prec] = rel_rms(run["f"][rows], short + longs[key])
+    out["rows"] = int(rows.shape[0])
+    out["oracle_seconds"] = time.perf_counter() - t0
+    return out
 
 
 def host_cpu():
@@ -422,6 +480,9 @@ def adjoint_md(name="slab_n100k", steps=100, sigma=0.01):
         z = xyz[:, 2]
         z = np.where(z < lo, 2 * lo - z, z)
         xyz[:, 2] = np.where(z > hi, 2 * hi - z, z)
+        # the previous step's HVP runs on the context's own stream, which a
+        # plain (null-stream) copy does not wait for
+        L.synchronize()
         xd.upload(xyz)
         vd.upload(rng.normal(size=xyz.shape))
         ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr)
@@ -478,6 +539,8 @@ def main(argv=None):
                     help="steps for the FP64 plain comparison (default: --steps)")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-check", action="store_true",
+                    help="skip the oracle check of the last step on 200 sampled atoms")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the full-sum, pole-splitting and LAMMPS-coupling lines")
@@ -520,6 +583,11 @@ def main(argv=None):
         # both runs end on the same k batch (same seeds, same step count)
         same_batch = (steps + warmup) == (fp64_steps + warmup)
         err = rel_rms(mix["f"], f64["f"]) if same_batch else None
+        if not a.no_oracle_check:
+            log(f"[bench] {name}: oracle check on sampled rows")
+            err_o = oracle_sampled_errors(cfg, xyz, q, {"mixed": mix, "fp64": f64})
+            mix["oracle_err"], f64["oracle_err"] = err_o["mixed"], err_o["fp64"]
+            mix["oracle_check"] = err_o
         return cfg, xyz, q, mix, f64, err
 
     fp64_steps = a.fp64_steps or a.steps
@@ -552,9 +620,18 @@ def main(argv=None):
             "N_t": cfg.N_t, "P": cfg.P, "accuracy": cfg.accuracy, "iter_period": 100,
             "n_pairs": mix["npairs"], "precision": "mixed", "parallelism": "single GPU",
         },
-        "fp64_plain": {"value": value64, "ms_per_step": f64["ms_per_step"]},
+        "fp64_plain": {"value": value64, "ms_per_step": f64["ms_per_step"],
+                       "ms_per_step_rebuild_amortized": f64["ms_per_step_rebuild_amortized"],
+                       "ms_per_step_no_rebuild": f64["ms_per_step_no_rebuild"]},
         "mixed_over_fp64": value / value64,
         "rel_rms_err_vs_fp64": err,
+        "rel_rms_err_vs_oracle_sampled": mix.get("oracle_check"),
+        # the timed steps hold mix["builds_timed"] neighbour-list rebuilds;
+        # these two windows hold exactly one in 100 steps, and none
+        "neighbor_rebuilds_in_timed_steps": mix["builds_timed"],
+        "ms_per_step_rebuild_amortized": mix["ms_per_step_rebuild_amortized"],
+        "ms_per_step_no_rebuild": mix["ms_per_step_no_rebuild"],
+        "rebuild_windows_ok": mix["rebuild_windows_ok"] and f64["rebuild_windows_ok"],
         # each part timed alone (the step overlaps the long range with the
         # short range, so these add up to more than ms_per_step)
         "phases_ms_per_step": parts_per_step(mix),
@@ -564,14 +641,16 @@ def main(argv=None):
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic_from_profiles(cfg.name),
-            "valu_issue_frac": valu_issue_from_profiles(cfg.name, "k_pair_list<mixed>"),
+            "valu_issue_frac_profiled": valu_issue_from_profiles(cfg.name, "k_pair_list<mixed>"),
+            "valu_issue_source": f"profiles/valu_issue_{cfg.name}.json (rocprofv3 --pmc, "
+                                 "not measured in this run)",
             "kernel": "k_pair_list<mixed>", "avg_ms": pair_avg_ms,
             "flop_per_launch": fl,
         },
         "roofline_fp64": {
             "bound": "valu", "achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved64 / PEAK_FP64_TFLOPS,
-            "valu_issue_frac": valu_issue_from_profiles(cfg.name, "k_pair_list<fp64>"),
+            "valu_issue_frac_profiled": valu_issue_from_profiles(cfg.name, "k_pair_list<fp64>"),
             "kernel": "k_pair_list<fp64>", "avg_ms": p64_avg,
         },
     }
@@ -600,6 +679,9 @@ def main(argv=None):
             "fp64_plain_value": c2.n / (d2["ms_per_step"] * 1e-3),
             "mixed_over_fp64": d2["ms_per_step"] / m2["ms_per_step"],
             "rel_rms_err_vs_fp64": e2,
+            "rel_rms_err_vs_oracle_sampled": m2.get("oracle_check"),
+            "ms_per_step_rebuild_amortized": m2["ms_per_step_rebuild_amortized"],
+            "ms_per_step_no_rebuild": m2["ms_per_step_no_rebuild"],
         }
 
     if not a.no_adjoint:

@@ -14,7 +14,11 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_DEFAULT_LIB = os.path.join(_HERE, "liboracle.so")
+_V4_LIB = os.path.join(_HERE, "liboracle_v4.so")
+# QEM_ORACLE_LIB names another build of the same source (the AddressSanitizer
+# build of `make -C oracle asan`, for tests/test_oracle_asan.py)
+_LIB_PATH = os.environ.get("QEM_ORACLE_LIB") or _DEFAULT_LIB
 
 SEM_SWAP_LIMITS = 1
 SEM_MINIMAGE_DIR = 2
@@ -44,10 +48,36 @@ class Elem(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with oracle/Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile liboracle.so (and liboracle_v4.so) with oracle/Makefile."""
+    if force or not (os.path.exists(_DEFAULT_LIB) and os.path.exists(_V4_LIB)):
         subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
                        check=True)
+    return _LIB_PATH
+
+
+def host_has_avx512() -> bool:
+    """Whether /proc/cpuinfo reports the x86-64-v4 extensions (AVX-512 F, BW,
+    CD, DQ, VL) that liboracle_v4.so is built for."""
+    want = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("flags"):
+                    return want <= set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return False
+
+
+def use_avx512_build() -> str:
+    """For timing: load liboracle_v4.so when the host has AVX-512 (call before
+    the first oracle call).  Same source and flags apart from -march, so the
+    results are the same.  Returns the path that will be used."""
+    global _LIB_PATH
+    if _lib is not None:
+        return _LIB_PATH
+    if os.path.exists(_V4_LIB) and host_has_avx512() and not os.environ.get("QEM_ORACLE_LIB"):
+        _LIB_PATH = _V4_LIB
     return _LIB_PATH
 
 
@@ -58,7 +88,8 @@ _P = C.POINTER
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        build()
+        if _LIB_PATH in (_DEFAULT_LIB, _V4_LIB):
+            build()
         L = C.CDLL(_LIB_PATH)
         d, i32, i64, u32 = C.c_double, C.c_int32, C.c_int64, C.c_uint32
         pd, pf, pi32, pi64 = _P(d), _P(C.c_float), _P(i32), _P(i64)
@@ -208,6 +239,24 @@ def short_forces(p: Params, xyz, q, pairs, nthreads=1):
     lib().qo_short_forces(C.byref(p), xyz.shape[0], _pd(xyz), _pd(q),
                           pairs.shape[0], pairs.ctypes.data_as(_P(C.c_int32)),
                           _pd(f), int(nthreads))
+    return f
+
+
+def short_forces_rows(p: Params, xyz, q, rows, nthreads=1):
+    """Molly.forces(::QEM_short) for the given rows only (qo_short_forces_rows):
+    each row's pairs by an O(N) scan, summed in qo_short_forces's order."""
+    L = lib()
+    if not hasattr(L.qo_short_forces_rows, "_sig"):
+        d, i64 = C.c_double, C.c_int64
+        L.qo_short_forces_rows.restype = None
+        L.qo_short_forces_rows.argtypes = [_P(Params), i64, _P(d), _P(d), i64, _P(i64), _P(d),
+                                           C.c_int]
+        L.qo_short_forces_rows._sig = True
+    xyz = _c(xyz, np.float64); q = _c(q, np.float64)
+    rows = _c(rows, np.int64).ravel()
+    f = np.zeros((rows.shape[0], 3))
+    L.qo_short_forces_rows(C.byref(p), xyz.shape[0], _pd(xyz), _pd(q), rows.shape[0],
+                           rows.ctypes.data_as(_P(C.c_int64)), _pd(f), int(nthreads))
     return f
 
 

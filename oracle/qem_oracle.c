@@ -536,6 +536,53 @@ void qo_short_forces(const qo_params *p, int64_t n, const double *xyz,
     free(fs);
 }
 
+/* Molly.forces(::QEM_short) -- src/QEM_short.jl:27-59 -- for the rows
+ * `rows` only (f is nrows x 3).  Each row's pairs are found by an O(N) scan
+ * over every other atom (minimum-image xy distance <= r_cutoff, the test of
+ * qo_neighbor_pairs, evaluated with the lower index first), and added in the
+ * order qo_short_forces adds them for that atom: the pairs (j, a), j < a, as
+ * F[j] -= F_ja (its z part the j-side force), then (a, j), j > a, as
+ * F[a] += F_aj, then the self term F_short_i.  The rows' forces are those of
+ * the full list without building it, so they can be checked at any N. */
+void qo_short_forces_rows(const qo_params *p, int64_t n, const double *xyz, const double *q,
+                          int64_t nrows, const int64_t *rows, double *f, int nthreads)
+{
+    (void)get_rule(p->N_t);
+    const double rc2 = p->r_cutoff * p->r_cutoff;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int64_t r = 0; r < nrows; ++r) {
+        const int64_t a = rows[r];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == a) continue;
+            const int64_t i0 = j < a ? j : a, j0 = j < a ? a : j;
+            double mx, my;
+            min_image(xyz[3 * i0] - xyz[3 * j0], xyz[3 * i0 + 1] - xyz[3 * j0 + 1], p->Lx,
+                      p->Ly, &mx, &my);
+            if (mx * mx + my * my > rc2) continue;
+            double out[3], fzj;
+            qo_F_short_ij(p, q[i0], q[j0], xyz + 3 * i0, xyz + 3 * j0, 0, out, &fzj);
+            if (a == i0) {
+                fx += out[0];
+                fy += out[1];
+                fz += out[2];
+            } else {
+                fx -= out[0];
+                fy -= out[1];
+                fz += fzj;
+            }
+        }
+        double fs[3];
+        qo_F_short_i(p, q[a], xyz + 3 * a, 0, fs);
+        f[3 * r] = fx;
+        f[3 * r + 1] = fy;
+        f[3 * r + 2] = fz + fs[2];
+    }
+    (void)nthreads;
+}
+
 /* ------------------------------------------------------------------------ */
 /* K_set_generator -- src/K_set_generator.jl:5-45 */
 int64_t qo_kset_size(double Lx, double Ly, double alpha, double acc,

@@ -111,6 +111,11 @@ int64_t qo_neighbor_pairs(int64_t n, const double *xyz, double Lx, double Ly,
 void qo_short_forces(const qo_params *p, int64_t n, const double *xyz,
                      const double *q, int64_t npairs, const int32_t *pairs,
                      double *f, int nthreads);
+/* The same forces for a subset of rows (f: nrows x 3), each row's pairs
+ * found by an O(N) scan, so that large systems can be checked on a sample
+ * of atoms without the O(N^2) pair list. */
+void qo_short_forces_rows(const qo_params *p, int64_t n, const double *xyz, const double *q,
+                          int64_t nrows, const int64_t *rows, double *f, int nthreads);
 /* pair forces only (4 doubles per pair: fx, fy, fz_i, fz_j) for timing */
 void qo_short_pair_forces(const qo_params *p, const double *xyz,
                           const double *q, int64_t npairs,

@@ -942,8 +942,9 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     // in mixed precision the long-range passes run on the side stream, as in
     // forces_core: k staging and z sort beside the pair kernel, the passes
     // beside the self term and the gather, the phi sum and the dE/dgamma
-    // copy on s after the short range
-    const bool overlap = P >= 0 && ctx->p.precision == QEM_PREC_MIXED;
+    // copy on s after the short range; a batch beyond kLongBatch k vectors
+    // runs on s in passes of kLongBatch rows
+    const bool overlap = P >= 0 && P <= kLongBatch && ctx->p.precision == QEM_PREC_MIXED;
     const LongConsts lc = long_consts(ctx, weight, QEM_LONG_ALL);
     int64_t Ptot = 0;
     int lst = QEM_OK;
@@ -996,12 +997,22 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                                               ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
     } else if (P >= 0) {
-        if ((st = stage_kvecs(ctx, kvecs, P, weight, true, s, &Ptot))) return st;
         PhaseScope ps(ctx, QEM_PHASE_LONG, s);
         hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "z sort");
-        e = launch_long_energy(lc, ctx->lw, n, Ptot, phi, 1, dg_long, s, ctx->err);
-        if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
+        // passes of at most kLongBatch k rows, so the per-row buffers stay
+        // bounded; the k = 0 row rides with the last one
+        for (int64_t off = 0;; off += kLongBatch) {
+            const int64_t Pb = std::min(kLongBatch, P - off);
+            const bool last = off + Pb >= P;
+            if ((st = stage_kvecs(ctx, kvecs + 3 * off, Pb, weight, last, s, &Ptot))) return st;
+            double dgb[2] = {0.0, 0.0};
+            e = launch_long_energy(lc, ctx->lw, n, Ptot, phi, 1, dgb, s, ctx->err);
+            if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
+            dg_long[0] += dgb[0];
+            dg_long[1] += dgb[1];
+            if (last) break;
+        }
     }
     API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s));
     double h[3];
@@ -1033,8 +1044,9 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     // in mixed precision the long-range tangent runs on the side stream, as
     // in forces_core: k staging and z sort beside the pair tangent kernel,
     // the tangent passes beside the self term and the gather, the sum into
-    // d_hv on s after the short range
-    const bool overlap = P > 0 && ctx->p.precision == QEM_PREC_MIXED;
+    // d_hv on s after the short range; a batch beyond kLongBatch k vectors
+    // runs on s in passes of kLongBatch rows
+    const bool overlap = P > 0 && P <= kLongBatch && ctx->p.precision == QEM_PREC_MIXED;
     int64_t Ptot = 0;
     int lst = QEM_OK;
     if (overlap) {
@@ -1084,12 +1096,16 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
         hipError_t e = launch_long_tangent_sum(ctx->lw, n, Ptot, -1.0, d_hv, 1, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "long-range tangent");
     } else if (P > 0) {
-        if ((st = stage_kvecs(ctx, kvecs, P, weight, false, s, &Ptot))) return st;
         PhaseScope ps(ctx, QEM_PHASE_LONG, s);
         hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "z sort");
-        e = launch_long_tangent(lc, ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, s, ctx->err);
-        if (e != hipSuccess) return hip_err(ctx, e, "long-range tangent");
+        // passes of at most kLongBatch k rows, each added into d_hv
+        for (int64_t off = 0; off < P; off += kLongBatch) {
+            const int64_t Pb = std::min(kLongBatch, P - off);
+            if ((st = stage_kvecs(ctx, kvecs + 3 * off, Pb, weight, false, s, &Ptot))) return st;
+            e = launch_long_tangent(lc, ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, s, ctx->err);
+            if (e != hipSuccess) return hip_err(ctx, e, "long-range tangent");
+        }
     }
     return QEM_OK;
 }

@@ -13,7 +13,9 @@ backward (no reference counterpart; the reference has forces only):
            motion of the quadrature limits, an effect at the accuracy level);
   d/dq   : gE dE/dq (the per-atom potential); a force cotangent gF with
            respect to q is not provided (raises);
-  d/dgamma: gE (dE/dgamma_1, dE/dgamma_2).  `gamma` must equal the
+  d/dgamma: gE (dE/dgamma_1, dE/dgamma_2); a force cotangent gF with
+           respect to gamma is not provided either (raises, rather than
+           returning the energy term alone).  `gamma` must equal the
            context's contrasts; it is only the handle autograd differentiates.
 The quadrature is recomputed in the backward pass: nothing per pair is kept.
 """
@@ -78,6 +80,9 @@ class QEMEnergyForces(torch.autograd.Function):
                     "gradient of the forces with respect to the charges is not provided")
             gq = gE * phi
         if fctx.needs_input_grad[2]:
+            if gF is not None and bool(torch.any(gF != 0)):
+                raise NotImplementedError(
+                    "gradient of the forces with respect to gamma is not provided")
             gg = gE * x.new_tensor(fctx.dg)
         return gx, gq, gg, None, None
 

@@ -176,3 +176,30 @@ def test_adjoint_slab_n10k_fd_and_symmetry():
         with ctx_for(cfgp, semantics=L.SEM_CORRECTED) as cp, ctx_for(cfgm, semantics=L.SEM_CORRECTED) as cm:
             d = (cp.energy(xyz, q, kidx)[0] - cm.energy(xyz, q, kidx)[0]) / 2e-5
         assert dg[w] == pytest.approx(d, rel=1e-6, abs=1e-9 * abs(E))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_energy_and_hvp_beyond_one_long_range_pass(precision):
+    # a batch of 8,292 k vectors runs its long-range passes in two parts of
+    # at most 8,192 rows; the energy equals the oracle's for the whole batch
+    # and the HVP's long range is the batch-weighted sum of its halves
+    cfg, xyz, q, c = _setup("slab_n100", 100, sem=L.SEM_CORRECTED, precision=precision)
+    v = np.random.default_rng(8).normal(size=xyz.shape)
+    P = 8192 + 100
+    with c:
+        K, _, sK = c.kset()
+        kidx = c.sample_k(21, P)
+        E, phi, dg = c.energy(xyz, q, kidx)
+        hv = c.hvp(xyz, q, kidx, v)
+        hv0 = c.hvp(xyz, q, np.zeros(0, np.int64), v)          # short range alone
+        hva = c.hvp(xyz, q, kidx[:5000], v)
+        hvb = c.hvp(xyz, q, kidx[5000:], v)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+    Eo, phio, dgo = O.energy(oracle_params(cfg, semantics=L.SEM_CORRECTED), xyz, q, pairs,
+                             K[kidx], weight=sK / P, nthreads=16)
+    tol = 1e-11 if precision == "fp64" else 1e-6
+    assert E == pytest.approx(Eo, rel=tol)
+    assert rel_rms(phi, phio) < tol
+    assert np.allclose(dg, dgo, rtol=max(tol, 1e-11) * 10, atol=1e-13 * abs(Eo))
+    want = hv0 + (5000 / P) * (hva - hv0) + ((P - 5000) / P) * (hvb - hv0)
+    assert rel_rms(hv, want) < (1e-12 if precision == "fp64" else 1e-6)

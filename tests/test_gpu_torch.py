@@ -47,6 +47,18 @@ with ctx_for(cfg, semantics=L.SEM_CORRECTED, iter_period=100000) as c:
     E2, F2 = qem_energy_forces(c, xyz2, torch.tensor(q_h, device=dev), kidx)
     (F2 * torch.tensor(v, device=dev)).sum().backward()
     out["hv"] = rel_rms(xyz2.grad.cpu().numpy(), -hv_ref)
+    # a loss on F differentiated with respect to gamma (or q) raises instead
+    # of returning the energy term alone
+    for wrt in ("gamma", "q"):
+        g3 = torch.tensor([cfg.gamma_1, cfg.gamma_2], dtype=torch.float64, device=dev,
+                          requires_grad=(wrt == "gamma"))
+        q3 = torch.tensor(q_h, device=dev, requires_grad=(wrt == "q"))
+        E3, F3 = qem_energy_forces(c, torch.tensor(xyz_h, device=dev), q3, kidx, gamma=g3)
+        try:
+            (E3 + F3.sum()).backward()
+            out["raise_" + wrt] = False
+        except NotImplementedError:
+            out["raise_" + wrt] = True
     # torch-allocated device memory through the plain device entry point
     x = torch.tensor(xyz_h, device=dev); qq = torch.tensor(q_h, device=dev)
     f = torch.empty_like(x)
@@ -69,3 +81,4 @@ def test_torch_autograd_binding():
     assert res["gx"] < 1e-13 and res["gq"] < 1e-13 and res["gg"] < 1e-12
     assert res["hv"] < 1e-13
     assert res["dev"] < 1e-13
+    assert res["raise_gamma"] and res["raise_q"]

@@ -300,3 +300,26 @@ def test_batch_weights_are_unbiased():
     k0 = O.long_forces(p, X, q, K[:0], weight=1.0)
     assert np.allclose(full, k0 + per_k, rtol=1e-6, atol=1e-9)
     assert np.isfinite(batch).all()
+
+
+# ---- the short range for a subset of rows ------------------------------------
+
+@pytest.mark.parametrize("sem", [0, O.SEM_CORRECTED])
+def test_short_forces_rows_equal_the_pair_list_sum(sem):
+    # qo_short_forces_rows finds each row's pairs by a scan and must give the
+    # rows of qo_short_forces over the O(N^2) pair list bit for bit (same
+    # pairs, same order of the adds); atoms on the periodic boundary and at
+    # exactly r_cutoff included
+    from helpers import CONFIGS, make_system, oracle_params, scaled_config
+    cfg = scaled_config(CONFIGS["slab_n10k"], 300)
+    xyz, q = make_system(cfg, seed=4)
+    xyz[0, :2] = [0.0, 0.0]
+    xyz[1, :2] = [cfg.r_cutoff, 0.0]
+    xyz[2, :2] = [cfg.Lx - 1e-12, cfg.Ly - 1e-12]
+    p = oracle_params(cfg, semantics=sem)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+    assert (0, 1) in set(map(tuple, pairs.tolist()))
+    full = O.short_forces(p, xyz, q, pairs, nthreads=4)
+    rows = np.array([0, 1, 2, 7, 150, 299])
+    got = O.short_forces_rows(p, xyz, q, rows, nthreads=4)
+    assert np.array_equal(got, full[rows])
