@@ -7,8 +7,8 @@
 // (130-161), Flz_self (164-184).
 //
 // Passes (atoms in ascending-z order):
-//   sort   stable radix sort of z with the atom index as payload; gather
-//          (x, y, z, q) into z order.
+//   sort   stable radix sort of z with the atom index as payload
+//          (radix_sort.hip), whose last pass writes (x, y, z, q) in z order.
 //   krec   per k row: derived constants (scaled k, weights, the non-sort and
 //          self factors with their exponentials and divisions).
 //   A      per (chunk, group of 64 k rows; one row per lane): the four chunk
@@ -24,8 +24,6 @@
 //   R      per atom: sum over groups, scatter back to input order.
 // The k = 0 term rides along as an extra row with (q, q z, q, q z).
 // Nothing of size N x P is stored.
-#include <hipcub/hipcub.hpp>
-
 #include "qem_device.h"
 
 namespace qem {
@@ -41,25 +39,6 @@ __global__ void k_fill_f64(double *__restrict__ p, int64_t n, double v)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-__global__ void k_extract_z(const double *__restrict__ xyz, int64_t n, double *__restrict__ z,
-                            int32_t *__restrict__ idx)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    z[i] = xyz[3 * i + 2];
-    idx[i] = (int32_t)i;
-}
-
-__global__ void k_gather_sorted(const double *__restrict__ xyz, const double *__restrict__ q,
-                                const int32_t *__restrict__ perm, int64_t n,
-                                double4 *__restrict__ satom)
-{
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int64_t a = perm[r];
-    satom[r] = make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], q[a]);
 }
 
 // exclusive scan of (x, y) over the CH threads of the block in thread
@@ -980,26 +959,11 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const doub
         err = "long range: more than 2^31-1 atoms";
         return hipErrorInvalidValue;
     }
-    const size_t nd = sizeof(double) * (size_t)n, ni = sizeof(int32_t) * (size_t)n;
-    LR_CHECK(ensure(lw.keys, nd));
-    LR_CHECK(ensure(lw.keys_sorted, nd));
-    LR_CHECK(ensure(lw.idx, ni));
-    LR_CHECK(ensure(lw.perm, ni));
-    LR_CHECK(ensure(lw.satom, 4 * nd));
-    k_extract_z<<<nblk(n, 256), 256, 0, s>>>(d_xyz, n, lw.keys.as<double>(), lw.idx.as<int32_t>());
-    size_t tbytes = 0;
-    LR_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, lw.keys.as<double>(),
-                                                lw.keys_sorted.as<double>(),
-                                                lw.idx.as<int32_t>(), lw.perm.as<int32_t>(),
-                                                (int)n, 0, 64, s));
-    LR_CHECK(ensure(lw.tmp, tbytes));
-    LR_CHECK(hipcub::DeviceRadixSort::SortPairs(lw.tmp.p, tbytes, lw.keys.as<double>(),
-                                                lw.keys_sorted.as<double>(),
-                                                lw.idx.as<int32_t>(), lw.perm.as<int32_t>(),
-                                                (int)n, 0, 64, s));
-    k_gather_sorted<<<nblk(n, 256), 256, 0, s>>>(d_xyz, d_q, lw.perm.as<int32_t>(), n,
-                                                 lw.satom.as<double4>());
-    LR_CHECK(hipGetLastError());
+    LR_CHECK(ensure(lw.perm, sizeof(int32_t) * (size_t)n));
+    LR_CHECK(ensure(lw.satom, sizeof(double4) * (size_t)n));
+    LR_CHECK(ensure_sort_scratch(lw.tmp, radix_sort_scratch_bytes(n, 8), s));
+    LR_CHECK(radix_sort_z(d_xyz, d_q, n, lw.perm.as<int32_t>(), lw.satom.as<double4>(), lw.tmp.p,
+                          s));
     return hipSuccess;
 }
 
@@ -1019,6 +983,22 @@ LaneGrid lane_grid(int64_t n, int64_t Ptot)
     return g;
 }
 }  // namespace
+
+LongSizes long_force_sizes(int64_t n, int64_t Ptot)
+{
+    LongSizes z;
+    const LaneGrid g = lane_grid(std::max<int64_t>(n, 1), std::max<int64_t>(Ptot, 1));
+    const size_t P = (size_t)std::max<int64_t>(Ptot, 1), N = (size_t)std::max<int64_t>(n, 1);
+    z.kv = sizeof(double4) * P;
+    z.kr = sizeof(double) * 16 * P;
+    z.perm = sizeof(int32_t) * N;
+    z.satom = sizeof(double4) * N;
+    z.sort = radix_sort_scratch_bytes((int64_t)N, 8);
+    z.partA = sizeof(double4) * P * (size_t)g.nchunk;
+    z.totA = sizeof(double4) * P;
+    z.partB = sizeof(double) * 3 * (size_t)g.ngroup * N;
+    return z;
+}
 
 hipError_t launch_long_krec(const LongConsts &c, LongWork &lw, int64_t Ptot, hipStream_t s,
                             std::string &err)

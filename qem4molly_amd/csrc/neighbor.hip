@@ -5,8 +5,8 @@
 // image xy distance is <= r_cutoff, reported once with i < j.
 //
 // Layout: atoms are bucketed into an ncx x ncy grid of cells at least
-// r_cutoff wide (stable radix sort of cell ids, so atoms inside a cell stay
-// in index order).  Cell ids follow a Morton (Z-order) curve, so that the
+// r_cutoff wide (stable radix sort of cell ids, radix_sort.hip, so atoms
+// inside a cell stay in index order).  Cell ids follow a Morton (Z-order) curve, so that the
 // cells around a cell are mostly near it in the id order (a row-major order
 // puts the cells above and below a whole row of cells away).  The sorted order numbers the atoms by slot: slot s is
 // atom atom_idx_sorted[s], and a cell's atoms are a run of slots.  One
@@ -19,8 +19,6 @@
 // that the per-atom reduction order is fixed (no atomics).  Since pair
 // neighbours are near each other in slot order, the per-pair position reads
 // and the gather's reads of per-pair results stay within nearby cache lines.
-#include <hipcub/hipcub.hpp>
-
 #include "qem_internal.h"
 
 namespace qem {
@@ -79,14 +77,13 @@ __device__ __forceinline__ int gather_bits(uint32_t v)
 }
 
 __global__ void k_cell_keys(const double *__restrict__ xyz, int64_t n, CellGrid g,
-                            uint32_t *__restrict__ key, int32_t *__restrict__ idx)
+                            uint32_t *__restrict__ key)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int cx = cell_of(xyz[3 * i], g.Lx, g.ncx);
     int cy = cell_of(xyz[3 * i + 1], g.Ly, g.ncy);
     key[i] = cell_id(cx, cy);
-    idx[i] = (int32_t)i;
 }
 
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
@@ -205,15 +202,6 @@ __global__ void k_inverse_perm(const int32_t *__restrict__ perm, int64_t m,
     inv[perm[t]] = (int32_t)t;
 }
 
-__global__ void k_iota_u32key(const int32_t *__restrict__ src, int64_t m,
-                              uint32_t *__restrict__ key, int32_t *__restrict__ idx)
-{
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    key[t] = (uint32_t)src[t];
-    idx[t] = (int32_t)t;
-}
-
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 inline int bits_for(uint64_t maxval)
@@ -246,23 +234,13 @@ hipError_t group_by_endpoint(NeighborList &nl, const int32_t *d_end, int64_t m, 
         k_fill_i32<<<nblk(n, 256), 256, 0, s>>>(end.as<int32_t>(), n, 0);
     }
     if (m == 0) return hipSuccess;
-    NL_CHECK(ensure(nl.cell_key, sizeof(uint32_t) * (size_t)m));
     NL_CHECK(ensure(nl.cell_key_sorted, sizeof(uint32_t) * (size_t)m));
-    NL_CHECK(ensure(nl.atom_idx, sizeof(int32_t) * (size_t)m));
     NL_CHECK(ensure(idx_out, sizeof(int32_t) * (size_t)m));
-    k_iota_u32key<<<nblk(m, 256), 256, 0, s>>>(d_end, m, nl.cell_key.as<uint32_t>(),
-                                                nl.atom_idx.as<int32_t>());
-    int ebits = bits_for((uint64_t)(n > 0 ? n - 1 : 0));
-    size_t tbytes = 0;
-    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, nl.cell_key.as<uint32_t>(),
-                                                nl.cell_key_sorted.as<uint32_t>(),
-                                                nl.atom_idx.as<int32_t>(), idx_out.as<int32_t>(),
-                                                (int)m, 0, ebits, s));
-    NL_CHECK(ensure(nl.tmp, tbytes));
-    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nl.tmp.p, tbytes, nl.cell_key.as<uint32_t>(),
-                                                nl.cell_key_sorted.as<uint32_t>(),
-                                                nl.atom_idx.as<int32_t>(), idx_out.as<int32_t>(),
-                                                (int)m, 0, ebits, s));
+    const int ebits = bits_for((uint64_t)(n > 0 ? n - 1 : 0));
+    NL_CHECK(ensure_sort_scratch(nl.tmp, radix_sort_scratch_bytes(m, 4), s));
+    NL_CHECK(radix_sort_u32(reinterpret_cast<const uint32_t *>(d_end), m, ebits,
+                            nl.cell_key_sorted.as<uint32_t>(), idx_out.as<int32_t>(), nl.tmp.p,
+                            s));
     k_run_bounds<<<nblk(m, 256), 256, 0, s>>>(nl.cell_key_sorted.as<uint32_t>(), m,
                                               start.as<int32_t>(), end.as<int32_t>());
     return hipGetLastError();
@@ -300,28 +278,19 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
 
     NL_CHECK(ensure(nl.cell_key, sizeof(uint32_t) * (size_t)n));
     NL_CHECK(ensure(nl.cell_key_sorted, sizeof(uint32_t) * (size_t)n));
-    NL_CHECK(ensure(nl.atom_idx, sizeof(int32_t) * (size_t)n));
     NL_CHECK(ensure(nl.atom_idx_sorted, sizeof(int32_t) * (size_t)n));
     NL_CHECK(ensure(nl.cell_start, sizeof(int32_t) * (size_t)ncell));
     NL_CHECK(ensure(nl.cell_end, sizeof(int32_t) * (size_t)ncell));
     NL_CHECK(ensure(nl.counts, sizeof(int64_t) * (size_t)(n + 1)));
     NL_CHECK(ensure(nl.row_ptr, sizeof(int64_t) * (size_t)(n + 1)));
 
-    k_cell_keys<<<nblk(n, 256), 256, 0, s>>>(d_xyz, n, g, nl.cell_key.as<uint32_t>(),
-                                             nl.atom_idx.as<int32_t>());
-    int kbits = bits_for((uint64_t)(ncell - 1));
-    size_t tbytes = 0;
-    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, nl.cell_key.as<uint32_t>(),
-                                                nl.cell_key_sorted.as<uint32_t>(),
-                                                nl.atom_idx.as<int32_t>(),
-                                                nl.atom_idx_sorted.as<int32_t>(), (int)n, 0,
-                                                kbits, s));
-    NL_CHECK(ensure(nl.tmp, tbytes));
-    NL_CHECK(hipcub::DeviceRadixSort::SortPairs(nl.tmp.p, tbytes, nl.cell_key.as<uint32_t>(),
-                                                nl.cell_key_sorted.as<uint32_t>(),
-                                                nl.atom_idx.as<int32_t>(),
-                                                nl.atom_idx_sorted.as<int32_t>(), (int)n, 0,
-                                                kbits, s));
+    k_cell_keys<<<nblk(n, 256), 256, 0, s>>>(d_xyz, n, g, nl.cell_key.as<uint32_t>());
+    const int kbits = bits_for((uint64_t)(ncell - 1));
+    NL_CHECK(ensure_sort_scratch(nl.tmp, std::max(radix_sort_scratch_bytes(n, 4),
+                                                  scan_scratch_bytes(n + 1)), s));
+    NL_CHECK(radix_sort_u32(nl.cell_key.as<uint32_t>(), n, kbits,
+                            nl.cell_key_sorted.as<uint32_t>(), nl.atom_idx_sorted.as<int32_t>(),
+                            nl.tmp.p, s));
     k_fill_i32<<<nblk(ncell, 256), 256, 0, s>>>(nl.cell_start.as<int32_t>(), ncell, 0);
     k_fill_i32<<<nblk(ncell, 256), 256, 0, s>>>(nl.cell_end.as<int32_t>(), ncell, 0);
     k_run_bounds<<<nblk(n, 256), 256, 0, s>>>(nl.cell_key_sorted.as<uint32_t>(), n,
@@ -341,12 +310,8 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
         nl.cell_start.as<int32_t>(), nl.cell_end.as<int32_t>(), nl.atom_idx_sorted.as<int32_t>(),
         nl.counts.as<int64_t>(), nullptr, nullptr, nullptr);
     NL_CHECK(hipGetLastError());
-    tbytes = 0;
-    NL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, nl.counts.as<int64_t>(),
-                                              nl.row_ptr.as<int64_t>(), (int)(n + 1), s));
-    NL_CHECK(ensure(nl.tmp, tbytes));
-    NL_CHECK(hipcub::DeviceScan::ExclusiveSum(nl.tmp.p, tbytes, nl.counts.as<int64_t>(),
-                                              nl.row_ptr.as<int64_t>(), (int)(n + 1), s));
+    NL_CHECK(scan_exclusive_i64(nl.counts.as<int64_t>(), nl.row_ptr.as<int64_t>(), n + 1,
+                                nl.tmp.p, s));
     int64_t total = 0;
     NL_CHECK(hipMemcpyAsync(&total, nl.row_ptr.as<int64_t>() + n, sizeof(int64_t),
                             hipMemcpyDeviceToHost, s));

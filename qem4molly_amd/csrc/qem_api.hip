@@ -499,6 +499,8 @@ void build_kset(qem_ctx *ctx)
         }
     const double sK = Sx + Sy + Sx * Sy;
     const size_t nk = (size_t)((2 * mxm + 1) * (2 * mym + 1) - 1);
+    ctx->kset_mxm = mxm;
+    ctx->kset_mym = mym;
     ctx->kset.resize(3 * nk);
     ctx->kprob.resize(nk);
     size_t t = 0;
@@ -1177,6 +1179,13 @@ int qem_create(const qem_params *p, qem_ctx **out)
     qem_ctx *ctx = new (std::nothrow) qem_ctx();
     if (!ctx) return QEM_ERR_NOMEM;
     ctx->p = *p;
+    // room for the error messages, so that setting one does not allocate
+    try {
+        ctx->err.reserve(512);
+    } catch (const std::bad_alloc &) {
+        delete ctx;
+        return QEM_ERR_NOMEM;
+    }
     auto fail = [&](int code) {
         qem_destroy(ctx);
         return code;
@@ -1289,10 +1298,10 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
                       &ctx->nl.slot_of,
                       &ctx->nl.sxyz, &ctx->nl.sq, &ctx->nl.sv, &ctx->nl.sout,
-                      &ctx->nl.cell_key, &ctx->nl.cell_key_sorted, &ctx->nl.atom_idx,
+                      &ctx->nl.cell_key, &ctx->nl.cell_key_sorted,
                       &ctx->nl.atom_idx_sorted, &ctx->nl.cell_start, &ctx->nl.cell_end,
-                      &ctx->nl.counts, &ctx->nl.row_ptr, &ctx->nl.tmp, &ctx->lw.keys,
-                      &ctx->lw.keys_sorted, &ctx->lw.idx, &ctx->lw.perm, &ctx->lw.tmp,
+                      &ctx->nl.counts, &ctx->nl.row_ptr, &ctx->nl.tmp,
+                      &ctx->lw.perm, &ctx->lw.tmp,
                       &ctx->lw.satom,
                       &ctx->lw.kr, &ctx->lw.kv,
                       &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,

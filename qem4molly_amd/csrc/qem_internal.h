@@ -99,6 +99,31 @@ std::vector<long double> fit_bessel_table(double x_max, double dx, int D, int or
 // The FP64 table: J0 interpolated at D + 1 Chebyshev points per interval.
 std::vector<long double> interp_j0_table(double x_max, double dx, int D, int *n_intervals);
 
+// ---- sorting and scanning (radix_sort.hip) --------------------------------
+// Stable LSD radix sorts (8-bit digits) and an exclusive scan, hand-written.
+// Each takes a scratch buffer of the given size whose first 256 bytes hold
+// the launches' completion counters: zero them (radix_sort_init) whenever
+// the buffer is new; every sort and scan leaves them at zero.
+size_t radix_sort_scratch_bytes(int64_t n, int key_bytes);
+hipError_t radix_sort_init(void *scratch, hipStream_t s);
+// ensure() of a sort/scan scratch buffer, with the counters zeroed when the
+// buffer is (re)allocated
+hipError_t ensure_sort_scratch(DevBuf &b, size_t bytes, hipStream_t s);
+// sortperm of the z coordinates (ascending, ties by atom index): perm[r] is
+// the atom of rank r, satom[r] its (x, y, z, q).  Scratch:
+// radix_sort_scratch_bytes(n, 8).
+hipError_t radix_sort_z(const double *d_xyz, const double *d_q, int64_t n, int32_t *perm,
+                        double4 *satom, void *scratch, hipStream_t s);
+// keys (< 2^bits) sorted stably: keys_out the sorted keys, idx_out their
+// input positions.  Scratch: radix_sort_scratch_bytes(n, 4).
+hipError_t radix_sort_u32(const uint32_t *keys, int64_t n, int bits, uint32_t *keys_out,
+                          int32_t *idx_out, void *scratch, hipStream_t s);
+// out[i] = sum of in[0 .. i-1] for m int64 values.  Scratch:
+// scan_scratch_bytes(m).
+size_t scan_scratch_bytes(int64_t m);
+hipError_t scan_exclusive_i64(const int64_t *in, int64_t *out, int64_t m, void *scratch,
+                              hipStream_t s);
+
 // ---- neighbour list (neighbor.hip) ------------------------------------
 struct NeighborList {
     int64_t n = -1;       // atom count the list was built for
@@ -120,7 +145,7 @@ struct NeighborList {
     DevBuf sxyz, sq, sv;  // per call: positions, charges, directions in slot order
     DevBuf sout;          // per call: per-slot results before the scatter
     // scratch
-    DevBuf cell_key, cell_key_sorted, atom_idx, atom_idx_sorted;
+    DevBuf cell_key, cell_key_sorted, atom_idx_sorted;
     DevBuf cell_start, cell_end, counts, row_ptr, tmp;
 };
 // Slot-ordered copies: rows of w doubles, dst[s] = src[atom_idx_sorted[s]]
@@ -211,7 +236,7 @@ struct LongConsts {
     int32_t precision;
 };
 struct LongWork {
-    DevBuf keys, keys_sorted, idx, perm, tmp;
+    DevBuf perm, tmp;        // z order (rank -> atom); the sort's scratch
     DevBuf satom;            // atoms in z order, (x, y, z, q) per atom
     DevBuf kr;               // per k row derived constants of the force passes
     DevBuf kv;               // P x 4 doubles (kx, ky, k, weight)
@@ -228,6 +253,13 @@ struct LongWork {
     hipEvent_t kv_ev[kSlots] = {nullptr, nullptr, nullptr, nullptr};
     int kv_slot = 0;
 };
+// Bytes each LongWork buffer needs for a force call over n atoms and Ptot k
+// rows (the k = 0 row included): lw.kv, lw.kr, lw.perm, lw.satom, lw.tmp
+// (the z sort), lw.partA and lw.offA (each), lw.totA, lw.partB.
+struct LongSizes {
+    size_t kv, kr, perm, satom, sort, partA, totA, partB;
+};
+LongSizes long_force_sizes(int64_t n, int64_t Ptot);
 // z-sort; fills lw.perm (int32, rank -> atom) and lw.satom (atoms in z order).
 hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz,
                         const double *d_q, hipStream_t s, std::string &err);
@@ -350,6 +382,7 @@ struct qem_ctx {
     std::vector<double> kset;   // |K| x 3
     std::vector<float> kprob;   // |K|
     double sum_K = 0.0;
+    int64_t kset_mxm = 0, kset_mym = 0;   // the enumeration's |m_x|, |m_y| bounds
     std::vector<double> alias_prob;
     std::vector<int64_t> alias_idx;
     // quadrature

@@ -1,8 +1,9 @@
 """GPU: the atoms' z order (qem_sortperm_z; src/QEM_long.jl:32-44, the
 stable sortperm of z) is bit-exact against a stable sort by (z, index), and
 against the oracle's, on uniform, layered, nearly-equal and monotone z, from
-1 atom to above 2^17 (the sizes where the device radix sort changes its
-internal algorithm)."""
+1 atom to above 2^20: one tile of the hand-written radix sort and tile
+boundaries (1024 keys per tile up to 2^20 keys, more per tile beyond), with
+eight digit passes over the 64-bit keys."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,7 @@ from zsort_cases import DISTS, LZ, stable_order, z_values
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [1, 2, 63, 4095, 4096, 4097, 10000, 65537, 100000, 131072, 131073]
+SIZES = [1, 2, 63, 1023, 1024, 1025, 4097, 10000, 65537, 100000, 131073, 1048577]
 
 
 @pytest.fixture(scope="module")
