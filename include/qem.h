@@ -288,6 +288,28 @@ int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                 const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv,
                 void *stream);
 
+/* ---- allocation-free long range ------------------------------------------
+ * No counterpart in the reference.  qem_long_forces_dev for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The call allocates nothing (no
+ * hipMalloc, hipFree, pinned staging or other allocating routine): its work
+ * buffers (z sort, per-k constants, chunk sums, per-group partial forces)
+ * live in one fixed device array of QEM_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each GPU it runs on, from the first call there until the
+ * process ends, and the k indices travel to the device as kernel arguments.
+ * Limits: n <= QEM_NOALLOC_MAX_N and P <= QEM_NOALLOC_MAX_P
+ * (QEM_ERR_CAPACITY beyond); the context's K set must exist (rbe_p > 0 at
+ * qem_create, or one qem_kset_size call; QEM_ERR_UNSUPPORTED otherwise).
+ * Every call on one GPU uses that GPU's one array, whatever the context:
+ * calls that overlap on one GPU race.  Same results as qem_long_forces_dev,
+ * bit for bit. */
+#define QEM_NOALLOC_MAX_N 131072
+#define QEM_NOALLOC_MAX_P 512
+#define QEM_NOALLOC_SCRATCH_BYTES ((size_t)72 << 20)
+int qem_long_forces_noalloc_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
+                                const double *d_q, const int64_t *k_idx, int64_t P,
+                                double *d_f, void *stream);
+
 /* ---- introspection ------------------------------------------------------ */
 /* device bytes held by the context's workspaces (they only grow: the
  * current value is the peak) */

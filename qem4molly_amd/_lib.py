@@ -55,7 +55,11 @@ EXPORTS = (
     "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
     "qem_device_bytes", "qem_substrate_lj", "qem_substrate_lj_dev", "qem_general_forces_dev",
     "qem_long_variance", "qem_long_variance_dev", "qem_long_terms_dev",
+    "qem_long_forces_noalloc_dev",
 )
+
+NOALLOC_MAX_N = 131072
+NOALLOC_MAX_P = 512
 
 PHASES = ("neighbor", "pair", "gather", "long")
 
@@ -149,6 +153,7 @@ _SIGS = {
     "qem_substrate_lj_dev": (C.c_int, [_vp, _P(Wall), _i64, _vp, _vp, _i32, _vp]),
     "qem_general_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _P(Wall), _i32,
                                          _vp, _vp]),
+    "qem_long_forces_noalloc_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp]),
 }
 
 _lib = None
@@ -412,6 +417,14 @@ class Context:
         self._check(lib().qem_long_forces_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
                                               k_idx.shape[0], d_f, stream),
                     "qem_long_forces_dev")
+
+    def long_forces_noalloc_dev(self, n, d_xyz, d_q, k_idx, d_f, stream=None):
+        """qem_long_forces_noalloc_dev: qem_long_forces_dev with no device
+        allocation (work buffers in the library's fixed per-GPU array)."""
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        self._check(lib().qem_long_forces_noalloc_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
+                                                      k_idx.shape[0], d_f, stream),
+                    "qem_long_forces_noalloc_dev")
 
     # ---- adjoint ------------------------------------------------------------
     def energy(self, xyz, q, k_idx):

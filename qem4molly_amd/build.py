@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libqem_hip.so")
 SOURCES = ["qem_api.hip", "neighbor.hip", "short_range.hip", "long_range.hip", "adjoint.hip",
-           "wall.hip", "pole.hip", "radix_sort.hip"]
+           "wall.hip", "pole.hip", "radix_sort.hip", "noalloc.hip"]
 HEADERS = ["qem_internal.h", "qem_device.h", os.path.join("..", "..", "include", "qem.h")]
 
 ARCH = os.environ.get("QEM_OFFLOAD_ARCH", "gfx950")
