@@ -278,6 +278,20 @@ int qem_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
 int qem_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz,
                    const double *d_q, const int64_t *k_idx, int64_t P,
                    double *d_dE_dq, double *E, double *dE_dgamma, void *stream);
+/* Forces, E, dE/dq and dE/dgamma of one step in one call: qem_forces_dev
+ * and qem_energy_dev together (the step counter and neighbour list advance
+ * as in qem_forces_dev).  In mixed precision the pair quadrature runs once
+ * for both, its node loops also forming the energy sums, and the long
+ * range's chunk sums serve both the force and the potential; the forces are
+ * those of qem_forces_dev bit for bit.  d_f (n x 3) and d_dE_dq (n, may be
+ * NULL) are device memory, E and dE_dgamma host outputs (the call
+ * synchronises the stream).  Needs |gamma_1 gamma_2| < 1. */
+int qem_forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                          const int64_t *k_idx, int64_t P, double *d_f, double *d_dE_dq,
+                          double *E, double *dE_dgamma, void *stream);
+int qem_forces_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                      const int64_t *k_idx, int64_t P, double *f, double *E, double *dE_dq,
+                      double *dE_dgamma);
 /* Hessian-vector product hv = -(dF/dx) v of the force as computed (short +
  * long range for the k batch; the k = 0 term is piecewise constant in z),
  * differentiating the quadrature through the z-dependent limits.  For a

@@ -55,7 +55,7 @@ EXPORTS = (
     "qem_timing_read", "qem_energy", "qem_energy_dev", "qem_hvp", "qem_hvp_dev",
     "qem_device_bytes", "qem_substrate_lj", "qem_substrate_lj_dev", "qem_general_forces_dev",
     "qem_long_variance", "qem_long_variance_dev", "qem_long_terms_dev",
-    "qem_long_forces_noalloc_dev",
+    "qem_long_forces_noalloc_dev", "qem_forces_energy_dev", "qem_forces_energy",
 )
 
 NOALLOC_MAX_N = 131072
@@ -154,6 +154,9 @@ _SIGS = {
     "qem_general_forces_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _P(Wall), _i32,
                                          _vp, _vp]),
     "qem_long_forces_noalloc_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp]),
+    "qem_forces_energy_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _pd, _pd,
+                                        _vp]),
+    "qem_forces_energy": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd, _pd, _pd]),
 }
 
 _lib = None
@@ -435,6 +438,25 @@ class Context:
         self._check(lib().qem_energy(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
                                      k_idx.shape[0], C.byref(E), ptr(phi), ptr(dg)), "qem_energy")
         return E.value, phi, dg
+
+    def forces_energy(self, xyz, q, k_idx):
+        """(F [N x 3], E, dE/dq [N], (dE/dgamma_1, dE/dgamma_2)) in one call."""
+        xyz = f64(xyz, (-1, 3)); q = f64(q)
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        f = np.zeros_like(xyz); E = _d(); phi = np.zeros(xyz.shape[0]); dg = np.zeros(2)
+        self._check(lib().qem_forces_energy(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
+                                            k_idx.shape[0], ptr(f), C.byref(E), ptr(phi),
+                                            ptr(dg)), "qem_forces_energy")
+        return f, E.value, phi, dg
+
+    def forces_energy_dev(self, n, d_xyz, d_q, k_idx, d_f, d_phi=None, stream=None):
+        """(E, dE/dgamma); forces into d_f, dE/dq into d_phi (device)."""
+        k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
+        E = _d(); dg = np.zeros(2)
+        self._check(lib().qem_forces_energy_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
+                                                k_idx.shape[0], d_f, d_phi, C.byref(E), ptr(dg),
+                                                stream), "qem_forces_energy_dev")
+        return E.value, dg
 
     def hvp(self, xyz, q, k_idx, v) -> np.ndarray:
         """-(dF/dx) v of the force as computed (N x 3)."""

@@ -781,6 +781,24 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     return hipGetLastError();
 }
 
+hipError_t launch_energy_gather(const ShortConsts &c, const ShortWork &ws,
+                                const NeighborList &nl, int64_t n, const double *d_xyz,
+                                const double *d_q, const float *d_epot, double *d_phi,
+                                double *d_psi, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
+    k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+        c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(), ws.glwf.as<float>(), n,
+        d_xyz, d_q, d_phi, d_psi);
+    const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
+    k_gather_energy<float4><<<nb, 256, 0, s>>>(
+        n, d_q, reinterpret_cast<const float4 *>(d_epot), nl.pi.as<int32_t>(),
+        nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+        nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi, d_psi);
+    return hipGetLastError();
+}
+
 hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,

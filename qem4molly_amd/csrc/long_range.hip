@@ -1140,7 +1140,7 @@ hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t 
     LR_CHECK(ensure(lw.partA, pc));
     LR_CHECK(ensure(lw.offA, pc));
     LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
-    LR_CHECK(ensure(lw.partB, sizeof(double) * (size_t)lg.ngroup * (size_t)n));
+    LR_CHECK(ensure(lw.partE, sizeof(double) * (size_t)lg.ngroup * (size_t)n));
     LR_CHECK(ensure(lw.q0part, sizeof(double2) * (size_t)Ptot));
     KRec *kr = lw.kr.as<KRec>();
     k_make_krec<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot, kr);
@@ -1157,14 +1157,27 @@ hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t 
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, lg.nchunk,
                                            lw.offA.as<double4>(), lw.totA.as<double4>());
     LR_CHECK(hipGetLastError());
+    return launch_long_energy_B(c, lw, n, Ptot, s, err);
+}
+
+hipError_t launch_long_energy_B(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                                hipStream_t s, std::string &err)
+{
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
+    const LaneGrid lg = lane_grid(n, Ptot);
+    LR_CHECK(ensure(lw.partE, sizeof(double) * (size_t)lg.ngroup * (size_t)n));
+    LR_CHECK(ensure(lw.q0part, sizeof(double2) * (size_t)Ptot));
+    const KRec *kr = lw.kr.as<KRec>();
+    const dim3 g((unsigned)((lg.nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)lg.ngroup);
+    const double4 *sa = lw.satom.as<double4>();
     if (c.precision == QEM_PREC_MIXED)
         k_lane_B_energy<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
             c, kr, Ptot, sa, n, lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
-            lw.totA.as<double4>(), lw.partB.as<double>());
+            lw.totA.as<double4>(), lw.partE.as<double>());
     else
         k_lane_B_energy<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
             c, kr, Ptot, sa, n, lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
-            lw.totA.as<double4>(), lw.partB.as<double>());
+            lw.totA.as<double4>(), lw.partE.as<double>());
     LR_CHECK(hipGetLastError());
     k_long_dgamma<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot,
                                                   lw.totA.as<double4>(), lw.q0part.as<double2>());
@@ -1178,7 +1191,7 @@ hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double 
 {
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
-    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 1,
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partE.as<double>(), lg.ngroup, n, 1,
                                            lw.perm.as<int32_t>(), accumulate, 1.0, d_phi);
     LR_CHECK(hipGetLastError());
     LR_CHECK(hipMemcpyAsync(h_q0, lw.q0part.p, sizeof(double2) * (size_t)Ptot,

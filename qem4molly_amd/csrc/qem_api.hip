@@ -429,6 +429,7 @@ ShortConsts short_consts(const qem_ctx *ctx, int single)
     c.jt_dx = ctx->sw.jt_dx;
     c.jt_inv_dx = ctx->sw.jt_dx > 0 ? 1.0f / ctx->sw.jt_dx : 0.0f;
     c.ncon = ctx->sw.ncon.as<float4>();
+    c.nconi = ctx->sw.nconi.as<float>();
     c.jd_n = ctx->sw.jd_n;
     c.jd_inv_dx = ctx->sw.jd_dx > 0 ? 1.0 / ctx->sw.jd_dx : 0.0;
     c.jdtab = ctx->sw.jdtab.as<double>();
@@ -681,6 +682,22 @@ struct PhaseScope {
     }
 };
 
+// One force call's step: the counter advances and the neighbour list is
+// rebuilt when (n_steps - 1) % iter_period == 0 (src/QEM_short.jl:32-36),
+// or when n changed or the list was installed by the caller.
+int advance_step(qem_ctx *ctx, int64_t n, const double *d_xyz, hipStream_t s)
+{
+    ctx->n_steps += 1;
+    const bool due = ((ctx->n_steps - 1) % ctx->p.iter_period) == 0;
+    if (due || ctx->nl.n != n || ctx->nl.has_i_idx) {
+        PhaseScope ps(ctx, QEM_PHASE_NEIGHBOR, s);
+        hipError_t e = build_neighbor_list(ctx->nl, n, d_xyz, ctx->p.Lx, ctx->p.Ly,
+                                           ctx->p.r_cutoff, s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, ctx->err.c_str());
+    }
+    return QEM_OK;
+}
+
 // device part of Molly.forces(::QEM_short) (src/QEM_short.jl:27-59)
 // addz (optional, n doubles in atom order): added to the z forces in the
 // final pass, after the short-range sum; after_pair (optional) is called
@@ -694,14 +711,7 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
 {
     if (n == 0) return QEM_OK;
     if (use_counter) {
-        ctx->n_steps += 1;
-        const bool due = ((ctx->n_steps - 1) % ctx->p.iter_period) == 0;
-        if (due || ctx->nl.n != n || ctx->nl.has_i_idx) {
-            PhaseScope ps(ctx, QEM_PHASE_NEIGHBOR, s);
-            hipError_t e = build_neighbor_list(ctx->nl, n, d_xyz, ctx->p.Lx, ctx->p.Ly,
-                                               ctx->p.r_cutoff, s, ctx->err);
-            if (e != hipSuccess) return hip_err(ctx, e, ctx->err.c_str());
-        }
+        if (int st = advance_step(ctx, n, d_xyz, s)) return st;
     }
     const ShortConsts c = short_consts(ctx, 0);
     API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
@@ -1112,6 +1122,124 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     return QEM_OK;
 }
 
+// Forces, energy, dE/dq and dE/dgamma of one step.  Mixed precision (batch
+// of at most kLongBatch k vectors): one pair pass for both, the force's node
+// loops also forming the energy sums; on the side stream the long range's
+// passes A and S serve both the force pass B and the potential pass.  The
+// forces are those of forces_core bit for bit.  Otherwise: forces_core,
+// then energy_dev.  The step counter advances as for a force call.
+int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                      const std::vector<double> &kv, int64_t P, double *d_f, double *d_phi,
+                      double *E, double *dEdg, hipStream_t s)
+{
+    if (int st0 = check_adjoint(ctx)) return st0;
+    const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
+    if (n == 0) {
+        if (E) *E = 0.0;
+        if (dEdg) dEdg[0] = dEdg[1] = 0.0;
+        return QEM_OK;
+    }
+    if (ctx->p.precision != QEM_PREC_MIXED || P > kLongBatch || pole_active(ctx->p)) {
+        int st = forces_core(ctx, n, d_xyz, d_q, kv, P, d_f, s);
+        if (st) return st;
+        return energy_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, d_phi, E, dEdg, s);
+    }
+    if (int st = advance_step(ctx, n, d_xyz, s)) return st;
+    NeighborList &nl = ctx->nl;
+    const ShortConsts c = short_consts(ctx, 0);
+    const LongConsts lc = long_consts(ctx, w, QEM_LONG_ALL);
+    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, nl.npairs)));
+    API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (2 * (size_t)n + 4 + half_dots_scratch())));
+    double *psi = ctx->d_adj.as<double>();          // N x 2
+    double *out3 = psi + 2 * n;                      // 3 (+1 pad)
+    double *part = out3 + 4;                         // reduction scratch
+    double *phi = d_phi;
+    if (!phi) {
+        API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
+        phi = ctx->d_f2.as<double>();
+    }
+    // side stream: k rows, z sort, per-k constants beside the pair kernel
+    API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
+    API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+    int64_t Ptot = 0;
+    int lst = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
+                        &Ptot);
+    int st = QEM_OK;
+    const double *px = d_xyz, *pq = d_q;
+    {
+        PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
+        st = slot_inputs(ctx, n, d_xyz, d_q, &px, &pq, s);
+        if (st == QEM_OK) {
+            hipError_t e = ensure(nl.sout2, sizeof(double) * 3 * (size_t)n);
+            if (e == hipSuccess)
+                e = launch_pair_forces_energy(c, ctx->sw, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
+                                              nl.npairs, px, pq, ctx->sw.fpair.as<double>(), s);
+            if (e != hipSuccess) st = hip_err(ctx, e, "fused pair kernel");
+        }
+    }
+    // side stream, once the pair kernel is done: the force passes, then the
+    // potential pass on their chunk sums
+    if (lst == QEM_OK && st == QEM_OK) {
+        hipError_t e = hipEventRecord(ctx->lanes_ev, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
+        if (e != hipSuccess) lst = hip_err(ctx, e, "lanes event");
+        if (lst == QEM_OK) lst = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side);
+        if (lst == QEM_OK) {
+            PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
+            e = launch_long_energy_B(lc, ctx->lw, n, Ptot, ctx->side, ctx->err);
+            if (e != hipSuccess) lst = hip_err(ctx, e, "long-range energy");
+        }
+    }
+    // call's stream: the force gather (self term included), then the energy
+    // self terms and gather, all in slot order
+    if (st == QEM_OK) {
+        PhaseScope ps(ctx, QEM_PHASE_GATHER, s);
+        const float *epot = reinterpret_cast<const float *>(
+            reinterpret_cast<const float4 *>(ctx->sw.fpair.p) + nl.npairs);
+        hipError_t e = launch_gather_self(c, QEM_PREC_MIXED, ctx->sw, nl, n, px, pq,
+                                          ctx->sw.fpair.as<double>(), nl.sout.as<double>(), s);
+        if (e == hipSuccess)
+            e = launch_energy_gather(c, ctx->sw, nl, n, px, pq, epot, nl.sout2.as<double>(),
+                                     nl.sout2.as<double>() + n, s);
+        if (e != hipSuccess) st = hip_err(ctx, e, "gathers");
+    }
+    // joined whatever happened, so that s never runs ahead of the side work
+    const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
+    const hipError_t ew = hipStreamWaitEvent(s, ctx->join_ev, 0);
+    if (st) return st;
+    if (lst) return lst;
+    if (ej != hipSuccess) return hip_err(ctx, ej, "join event");
+    if (ew != hipSuccess) return hip_err(ctx, ew, "join wait");
+    std::vector<double> &hq0 = ctx->h_q0;
+    hq0.assign(2 * (size_t)Ptot, 0.0);
+    {
+        PhaseScope ps(ctx, QEM_PHASE_LONG, s);
+        hipError_t e = launch_long_sum_slots(ctx->lw, n, Ptot, nl.slot_of.as<int32_t>(),
+                                             nl.sout.as<double>(), nullptr, d_f, s, ctx->err);
+        if (e == hipSuccess) e = launch_rows_from_slots(nl, n, 1, nl.sout2.as<double>(), phi, s);
+        if (e == hipSuccess)
+            e = launch_rows_from_slots(nl, n, 2, nl.sout2.as<double>() + n, psi, s);
+        if (e == hipSuccess)
+            e = launch_long_energy_sum(ctx->lw, n, Ptot, phi, 1, hq0.data(), s, ctx->err);
+        if (e != hipSuccess) return hip_err(ctx, e, "output passes");
+    }
+    API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s));
+    double h[3];
+    API_HIP(ctx, hipMemcpyAsync(h, out3, sizeof h, hipMemcpyDeviceToHost, s));
+    API_HIP(ctx, hipStreamSynchronize(s));
+    double dg_long[2] = {0.0, 0.0};
+    for (size_t q = 0; q < hq0.size(); q += 2) {
+        dg_long[0] += hq0[q];
+        dg_long[1] += hq0[q + 1];
+    }
+    if (E) *E = h[0];
+    if (dEdg) {
+        dEdg[0] = h[1] + dg_long[0];
+        dEdg[1] = h[2] + dg_long[1];
+    }
+    return QEM_OK;
+}
+
 #define CTX_GUARD(ctx)                                                        \
     do {                                                                      \
         if (!(ctx)) return QEM_ERR_INVALID_ARG;                               \
@@ -1208,6 +1336,9 @@ int qem_create(const qem_params *p, qem_ctx **out)
         nc[4 * (size_t)n + 3] = (float)std::log2(ctx->gl.w[(size_t)n] * u);
     }
     if (upload(ctx, ctx->sw.ncon, nc.data(), sizeof(float) * nc.size())) return fail(QEM_ERR_HIP);
+    std::vector<float> nci((size_t)p->N_t);
+    for (int n = 0; n < p->N_t; ++n) nci[(size_t)n] = (float)(1.0 / (1.0 + ctx->gl.x[(size_t)n]));
+    if (upload(ctx, ctx->sw.nconi, nci.data(), sizeof(float) * nci.size())) return fail(QEM_ERR_HIP);
     if (upload(ctx, ctx->sw.glx, ctx->gl.x.data(), sizeof(double) * p->N_t) ||
         upload(ctx, ctx->sw.glw, ctx->gl.w.data(), sizeof(double) * p->N_t) ||
         upload(ctx, ctx->sw.glxf, xf.data(), sizeof(float) * p->N_t) ||
@@ -1292,7 +1423,7 @@ int qem_create(const qem_params *p, qem_ctx **out)
 
 static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
 {
-    return {&ctx->sw.ncon, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab, &ctx->sw.jdtab,
+    return {&ctx->sw.ncon, &ctx->sw.nconi, &ctx->nl.sout2, &ctx->lw.partE, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab, &ctx->sw.jdtab,
                       &ctx->sw.pnode, &ctx->sw.pnodef,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
@@ -1790,6 +1921,52 @@ int qem_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d
     const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
     return energy_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, d_dE_dq, E, dE_dgamma,
                       pick_stream(ctx, stream));
+}
+
+int qem_forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                          const int64_t *k_idx, int64_t P, double *d_f, double *d_dE_dq,
+                          double *E, double *dE_dgamma, void *stream)
+{
+    CTX_GUARD(ctx);
+    if (n < 0 || P < 0 || (P > 0 && !k_idx) || (n > 0 && (!d_xyz || !d_q || !d_f)))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    std::vector<double> kv;
+    int st = kidx_to_vecs(ctx, k_idx, P, kv);
+    if (st) return st;
+    return forces_energy_dev(ctx, n, d_xyz, d_q, kv, P, d_f, d_dE_dq, E, dE_dgamma,
+                             pick_stream(ctx, stream));
+}
+
+int qem_forces_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                      const int64_t *k_idx, int64_t P, double *f, double *E, double *dE_dq,
+                      double *dE_dgamma)
+{
+    CTX_GUARD(ctx);
+    if ((n > 0 && (!q || !f)) || P < 0 || (P > 0 && !k_idx))
+        return set_err(ctx, QEM_ERR_INVALID_ARG, "bad arguments");
+    int st = check_atoms(ctx, n, xyz, q);
+    if (st) return st;
+    std::vector<double> kv;
+    if ((st = kidx_to_vecs(ctx, k_idx, P, kv))) return st;
+    if (n == 0)
+        return forces_energy_dev(ctx, 0, nullptr, nullptr, kv, P, nullptr, nullptr, E,
+                                 dE_dgamma, ctx->stream);
+    const size_t b3 = sizeof(double) * 3 * (size_t)n;
+    if ((st = upload(ctx, ctx->d_xyz, xyz, b3)) ||
+        (st = upload(ctx, ctx->d_q, q, sizeof(double) * (size_t)n)))
+        return st;
+    API_HIP(ctx, ensure(ctx->d_f, b3));
+    API_HIP(ctx, ensure(ctx->d_aux, sizeof(double) * (size_t)n));
+    st = forces_energy_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv, P,
+                           ctx->d_f.as<double>(), ctx->d_aux.as<double>(), E, dE_dgamma,
+                           ctx->stream);
+    if (st) return st;
+    API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, b3, hipMemcpyDeviceToHost, ctx->stream));
+    if (dE_dq)
+        API_HIP(ctx, hipMemcpyAsync(dE_dq, ctx->d_aux.p, sizeof(double) * (size_t)n,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    API_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QEM_OK;
 }
 
 int qem_hvp(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, const int64_t *k_idx,

@@ -42,6 +42,7 @@ struct ShortConsts {
     int32_t jt_n;        // Bessel table intervals (mixed precision)
     float jt_dx, jt_inv_dx;  // interval width and its inverse
     const float4 *ncon;  // device node constants (mixed precision), or null
+    const float *nconi;  // per node 1 / (1 + x_n) (the fused energy sums), or null
     int32_t jd_n;        // FP64 Bessel table intervals (0: device-library j0/j1)
     double jd_inv_dx;    // its inverse interval width
     const double *jdtab; // FP64 table in device memory (staged in LDS by the kernels)
@@ -144,6 +145,7 @@ struct NeighborList {
     DevBuf slot_of;       // built lists: the slot of each atom (inverse of atom_idx_sorted)
     DevBuf sxyz, sq, sv;  // per call: positions, charges, directions in slot order
     DevBuf sout;          // per call: per-slot results before the scatter
+    DevBuf sout2;         // per call: per-slot potentials (phi, psi) of the fused call
     // scratch
     DevBuf cell_key, cell_key_sorted, atom_idx_sorted;
     DevBuf cell_start, cell_end, counts, row_ptr, tmp;
@@ -174,6 +176,7 @@ struct ShortWork {
     DevBuf glxf, glwf;    // float nodes / weights
     DevBuf ncon;          // float4 per node: (1 + x_n, (1 + x_n)^2, log2 w_n,
                           //                   log2(w_n (1 + x_n)))
+    DevBuf nconi;         // float per node: 1 / (1 + x_n)
     DevBuf jtab;          // one float4 (cubic coefficients) per interval
     int jt_n = 0;
     float jt_dx = 0.f;
@@ -189,6 +192,14 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision,
                               const int32_t *d_pj, int64_t npairs,
                               const double *d_xyz, const double *d_q,
                               double *d_fpair, hipStream_t s);
+// Mixed precision: the force records of launch_pair_forces (float4, d_fpair
+// as float4[npairs]) and, after them, the pair potentials without the
+// charges (float4 (G, dG/dgamma_1, dG/dgamma_2, 0), float4[npairs]) from the
+// same node loops; d_fpair has room for a double4 per pair.
+hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
+                                     const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
+                                     const double *d_xyz, const double *d_q, double *d_fpair,
+                                     hipStream_t s);
 // Pair records of the force path (launch_pair_forces -> launch_gather_self):
 // float4 per pair for the mixed-precision list kernel, double4 otherwise
 // (FP64, and the pole-splitting kernels).  The buffer is sized for double4.
@@ -245,6 +256,7 @@ struct LongWork {
     DevBuf totA;             // [P][4] totals
     DevBuf q0part;           // per k row: dE_k/dgamma_1,2 (energy)
     DevBuf partB;            // [ntile][n][3]
+    DevBuf partE;            // [ntile][n] (energy: the potential of each atom)
     DevBuf svx, svy, svz;    // tangent directions in z order
     DevBuf partT, offT, totT;  // tangent chunk sums / offsets / totals
     static constexpr int kSlots = 4;   // pinned staging ring for k vectors
@@ -305,6 +317,18 @@ hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t 
 hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double *d_phi,
                                   int accumulate, double *h_q0, hipStream_t s,
                                   std::string &err);
+// The potential pass (and the per-k dE_k/dgamma) alone, for k rows whose
+// chunk sums, offsets and totals the force passes (launch_long_passes)
+// have just formed: the two share passes A and S.
+hipError_t launch_long_energy_B(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
+                                hipStream_t s, std::string &err);
+// Short-range energy from the pair potentials of launch_pair_forces_energy
+// (mixed precision, float4 per pair without the charges): the self terms
+// into d_phi, d_psi (overwrite), then every atom's pair terms added.
+hipError_t launch_energy_gather(const ShortConsts &c, const ShortWork &ws,
+                                const NeighborList &nl, int64_t n, const double *d_xyz,
+                                const double *d_q, const float *d_epot, double *d_phi,
+                                double *d_psi, hipStream_t s);
 // Tangent of the long-range force along d_v (N x 3), times scale.  Without
 // with_sum the call stops at the per-group partial tangents (lw.partB);
 // launch_long_tangent_sum then adds (or writes) them into d_out, possibly

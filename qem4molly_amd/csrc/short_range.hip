@@ -86,9 +86,19 @@ __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool 
 // Table bound: k < K and rho <= r_cutoff, and the table spans
 // r_cutoff * max(K) plus two intervals, so x / dx stays below the last
 // interval; hR is clamped once per set so that holds for any input.
+//
+// With ENERGY the same loop also forms the energy node sums of the set
+// (the J0 transforms of the image factors, weight w instead of w k): the
+// summand of sz divided by u.x (a wave-uniform node constant, 1/(1 + x_n))
+// is w e^{..} / D J0 / h, so
+//   ev = h sum (t / u.x) J0,   ed = h sum (t / u.x) J0 {1/D point, E/D gauss},
+// ed being what d/dgamma of 1/D brings down; the force operations are the
+// same, so the forces are the same bits with or without it.
+template <bool ENERGY>
 __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool gauss,
                                             const ShortConsts &c, const float4 *__restrict__ ncon,
-                                            const float4 *__restrict__ tab, float &sr, float &sz)
+                                            const float4 *__restrict__ tab, float &sr, float &sz,
+                                            float &ev, float &ed)
 {
     const float h = 0.5f * K;
     const float c2 = c.f_c2;                       // -2 Lz log2(e)
@@ -99,14 +109,15 @@ __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool ga
     const float rG = c.f_rG;                       // cg / c2^2, cg = -log2(e) / (4 alpha)
     // Bessel argument in table units; u.x < 2 keeps m <= jt_n - 1
     const float hR = fminf(h * rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
-    float ar = 0.f, az = 0.f;
+    float ar = 0.f, az = 0.f, av = 0.f, ad = 0.f;
     for (int n = 0; n < c.Nt; ++n) {
         const float4 u = ncon[n];
         const float e = hE * u.x;
         const float E = __builtin_amdgcn_exp2f(e);
         const float num = gauss ? __builtin_amdgcn_exp2f(fmaf(e, fmaf(e, rG, rA), u.w))
                                 : __builtin_amdgcn_exp2f(fmaf(e, rP, u.w));
-        const float t = num * __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+        const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+        const float t = num * rD;
         const float xb = hR * u.x;
         const float tt = __builtin_amdgcn_fractf(xb);
         const float4 cc = tab[(int)xb];
@@ -116,10 +127,19 @@ __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool ga
         dp = fmaf(dp, tt, p); p = fmaf(p, tt, cc.x);
         ar = fmaf(t, dp, ar);   // J1 = -dp / dx
         az = fmaf(t, p, az);
+        if (ENERGY) {
+            const float tp = (t * c.nconi[n]) * p;
+            av += tp;
+            ad = fmaf(tp, gauss ? E * rD : rD, ad);
+        }
     }
     const float h2 = h * h;
     sr = -h2 * c.jt_inv_dx * ar;
     sz = h2 * az;
+    if (ENERGY) {
+        ev = h * av;
+        ed = h * ad;
+    }
 }
 
 struct PairForce {
@@ -141,17 +161,34 @@ __device__ __forceinline__ void store_rec(double4 *r, const PairForce &f)
     *r = make_double4(f.fx, f.fy, f.fz, f.fzj);
 }
 
+// d b_l / d gamma_w (w = 0: gamma_1, 1: gamma_2) of image l
+__device__ __forceinline__ double db_dgamma(const ShortConsts &c, int l, int w)
+{
+    if (l == 1) return w == 0 ? 1.0 : 0.0;
+    if (l == 2) return w == 1 ? 1.0 : 0.0;
+    if (l == 3) return w == 0 ? c.g2 : c.g1;
+    return 0.0;
+}
+
 // F_short_ij for one pair (src/Force_short.jl:25-78).  Returns zeros when
 // the minimum-image distance exceeds r_cutoff (the reference's per-pair
 // 2-point neighbour list is empty then, line 36-41).
-template <int PREC>
+// With ENERGY (mixed precision) *pot also receives the pair potential
+// without q_i q_j and its gamma derivatives, (G, dG/dgamma_1, dG/dgamma_2),
+// from the same node loops: per image the closed form b/(2 sqrt(a^2 +
+// rho^2)) minus the point-charge J0 transform of Gamma_2 plus the gaussian
+// one of Gamma_1 e^{-k^2/4 alpha}; the derivatives take d b / d gamma and
+// d(1/D)/d gamma = -g_other E / D^2 (src/greens_function.jl:68-88).
+template <int PREC, bool ENERGY = false>
 __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__ glx,
                                 const double *__restrict__ glw, const float4 *__restrict__ ncon,
                                 const float4 *__restrict__ tab,
                                 double qi, double qj, double xi, double yi, double zi, double xj,
-                                double yj, double zj)
+                                double yj, double zj, double4 *pot = nullptr)
 {
+    static_assert(!ENERGY || PREC == QEM_PREC_MIXED, "the fused energy is mixed precision");
     PairForce out{0.0, 0.0, 0.0, 0.0};
+    if (ENERGY) *pot = make_double4(0.0, 0.0, 0.0, 0.0);
     const double dx = xi - xj, dy = yi - yj;
     double mx, my, d2;
     {
@@ -184,13 +221,14 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
 
     double Fsr = 0.0, Fsz = 0.0;
     double part[4];
+    double Eu = 0.0, Ed0 = 0.0, Ed1 = 0.0;
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
         const double a = l == 0 ? zn : (l == 1 ? zp : (l == 2 ? 2.0 * c.Lz - zp : 2.0 * c.Lz - zn));
         const double b = l == 0 ? 1.0 : (l == 1 ? c.g1 : (l == 2 ? c.g2 : c.g12));
         const double s = l == 0 ? -sg : (l == 1 ? -1.0 : (l == 2 ? 1.0 : sg));
         // greens_element_ij_init limits (src/greens_function.jl:26-27)
-        double Kp, Kg, inv32;
+        double Kp, Kg, inv32, inv_r = 0.0;
         const double r2 = a * a + (PREC == QEM_PREC_MIXED ? d2 : rho * rho);
         if (PREC == QEM_PREC_MIXED) {
             // the limits only place FP32 nodes: FP32, with k_f2 in the form
@@ -207,6 +245,7 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
             y = y * fma(-0.5 * r2 * y, y, 1.5);
             y = y * fma(-0.5 * r2 * y, y, 1.5);
             inv32 = y * y * y;
+            inv_r = y;
         } else {
             const double kf1 = -c.ln_acc / (2.0 * c.Lz + a);
             const double kf2 = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc) -
@@ -219,21 +258,30 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
         const double cr = 0.5 * b * rho * inv32;
         const double cz = 0.5 * b * a * s * inv32;
         double pr = 0.0, pz = 0.0, gr = 0.0, gz = 0.0;
-        if (b != 0.0) {
+        double Pv = 0.0, Pd = 0.0, Gv = 0.0, Gd = 0.0;
+        const double db0 = ENERGY ? db_dgamma(c, l, 0) : 0.0;
+        const double db1 = ENERGY ? db_dgamma(c, l, 1) : 0.0;
+        if (b != 0.0 || db0 != 0.0 || db1 != 0.0) {
             if (PREC != QEM_PREC_MIXED) {
                 constexpr bool TAB = PREC == QEM_PREC_FP64;
                 const double *tab64 = reinterpret_cast<const double *>(tab);
                 nodes_fp64<TAB>(Kp, a, rho, false, c, glx, glw, tab64, pr, pz);
                 if (!c.single) nodes_fp64<TAB>(Kg, a, rho, true, c, glx, glw, tab64, gr, gz);
             } else {
-                float fr, fz;
-                nodes_mixed((float)Kp, (float)a, (float)rho, false, c, ncon, tab, fr, fz);
+                float fr, fz, fv = 0.f, fd = 0.f;
+                nodes_mixed<ENERGY>((float)Kp, (float)a, (float)rho, false, c, ncon, tab, fr, fz,
+                                    fv, fd);
                 pr = fr;
                 pz = fz;
+                Pv = fv;
+                Pd = fd;
                 if (!c.single) {
-                    nodes_mixed((float)Kg, (float)a, (float)rho, true, c, ncon, tab, fr, fz);
+                    nodes_mixed<ENERGY>((float)Kg, (float)a, (float)rho, true, c, ncon, tab, fr,
+                                        fz, fv, fd);
                     gr = fr;
                     gz = fz;
+                    Gv = fv;
+                    Gd = fd;
                 }
             }
         }
@@ -243,7 +291,14 @@ __device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__
         // F_sz: +sum GL[dz Gamma_2 J0] - closed - sum GL[dz Gamma_1 e^{..} J0]
         part[l] = (hb * c.g12 * s) * pz - cz - (hb * s) * gz;
         Fsz += part[l];
+        if (ENERGY) {
+            const double img = inv_r - c.g12 * Pv + Gv;
+            Eu += hb * img;
+            Ed0 += 0.5 * db0 * img + hb * c.g2 * (Pd - Gd);
+            Ed1 += 0.5 * db1 * img + hb * c.g1 * (Pd - Gd);
+        }
     }
+    if (ENERGY) *pot = make_double4(c.inv2pieps * Eu, c.inv2pieps * Ed0, c.inv2pieps * Ed1, 0.0);
     const double pref = qi * qj * c.inv2pieps;
     if (rho != 0.0) {
         const bool mi = (c.sem & QEM_SEM_MINIMAGE_DIR) != 0;
@@ -352,6 +407,30 @@ __global__ __launch_bounds__(256) void k_pair_list(
                                        xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
                                        xyz[3 * j + 1], xyz[3 * j + 2]);
         store_rec(fpair + p, f);
+    }
+}
+
+// The force pair kernel with the pair potentials (mixed precision): fpair[p]
+// the force record of k_pair_list, epot[p] the pair's (G, dG/dgamma_1,
+// dG/dgamma_2, 0), without the charges.
+__global__ __launch_bounds__(256) void k_pair_list_fe(
+    ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
+    const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
+    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
+    const double *__restrict__ q, float4 *__restrict__ fpair, float4 *__restrict__ epot)
+{
+    extern __shared__ float4 lds_tab[];
+    const float4 *tab = dev::stage_tables<QEM_PREC_MIXED>(c, gtab, lds_tab);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+        const int64_t i = pi[p], j = pj[p];
+        double4 u;
+        PairForce f = pair_force<QEM_PREC_MIXED, true>(c, glx, glw, ncon, tab, q[i], q[j],
+                                                       xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
+                                                       xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2],
+                                                       &u);
+        store_rec(fpair + p, f);
+        epot[p] = make_float4((float)u.x, (float)u.y, (float)u.z, 0.0f);
     }
 }
 
@@ -487,6 +566,21 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
                 d_pj, npairs, d_xyz, d_q,
                 reinterpret_cast<double4 *>(d_fpair));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
+                                     const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
+                                     const double *d_xyz, const double *d_q, double *d_fpair,
+                                     hipStream_t s)
+{
+    if (npairs <= 0) return hipSuccess;
+    if (c.pole || !c.nconi) return hipErrorInvalidValue;
+    float4 *rec = reinterpret_cast<float4 *>(d_fpair);
+    const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
+    k_pair_list_fe<<<pair_grid(npairs), 256, lds, s>>>(
+        c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), ws.jtab.as<float4>(),
+        d_pi, d_pj, npairs, d_xyz, d_q, rec, rec + npairs);
     return hipGetLastError();
 }
 

@@ -3,9 +3,10 @@
     from qem4molly_amd.torch_qem import qem_energy_forces
     E, F = qem_energy_forces(ctx, xyz, q, k_idx, gamma=gamma)   # cuda float64
 
-forward  runs qem_forces_dev and qem_energy_dev on torch's current stream
-         (device pointers of the tensors; nothing is copied to the host except
-         the k indices and the scalar results).
+forward  runs qem_forces_energy_dev on torch's current stream: forces, E,
+         dE/dq and dE/dgamma in one call (in mixed precision one pair
+         quadrature for both), on the tensors' device pointers; nothing is
+         copied to the host except the k indices and the scalar results.
 backward (no reference counterpart; the reference has forces only):
   d/dxyz : -gE F  +  J_F^T gF, with J_F^T gF taken as -H gF from the
            Hessian-vector product kernel (qem_hvp_dev): the force Jacobian is
@@ -51,8 +52,8 @@ class QEMEnergyForces(torch.autograd.Function):
         F = torch.empty_like(x)
         phi = torch.empty_like(qq)
         s = _stream()
-        qctx.forces_dev(n, x.data_ptr(), qq.data_ptr(), k_idx, F.data_ptr(), s)
-        E, dg = qctx.energy_dev(n, x.data_ptr(), qq.data_ptr(), k_idx, phi.data_ptr(), s)
+        E, dg = qctx.forces_energy_dev(n, x.data_ptr(), qq.data_ptr(), k_idx, F.data_ptr(),
+                                       phi.data_ptr(), s)
         fctx.save_for_backward(x, qq, F, phi)
         fctx.qctx = qctx
         fctx.k_idx = np.array(k_idx, dtype=np.int64)

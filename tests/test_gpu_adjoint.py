@@ -203,3 +203,33 @@ def test_energy_and_hvp_beyond_one_long_range_pass(precision):
     assert np.allclose(dg, dgo, rtol=max(tol, 1e-11) * 10, atol=1e-13 * abs(Eo))
     want = hv0 + (5000 / P) * (hva - hv0) + ((P - 5000) / P) * (hvb - hv0)
     assert rel_rms(hv, want) < (1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_forces_energy_in_one_call(precision):
+    # qem_forces_energy: the forces of qem_forces bit for bit and the energy,
+    # dE/dq, dE/dgamma of qem_energy, in one call that advances the step
+    # counter once (FP64: the two calls' own bits; mixed: one pair
+    # quadrature for both, checked against the oracle's energy)
+    cfg, xyz, q, c = _setup("slab_n10k", 2000, sem=L.SEM_CORRECTED, precision=precision)
+    with c, ctx_for(cfg, semantics=L.SEM_CORRECTED, precision=precision,
+                    iter_period=100000) as c2:
+        K, _, sK = c.kset()
+        kidx = c.sample_k(9, cfg.P)
+        F, E, phi, dg = c.forces_energy(xyz, q, kidx)
+        steps = c.short_state()[2]
+        F2 = c2.forces(xyz, q, kidx)
+        E2, phi2, dg2 = c2.energy(xyz, q, kidx)
+    assert steps == 1
+    assert np.array_equal(F, F2)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+    Eo, phio, dgo = O.energy(oracle_params(cfg, semantics=L.SEM_CORRECTED), xyz, q, pairs,
+                             K[kidx], weight=sK / cfg.P, nthreads=16)
+    if precision == "fp64":
+        assert E == E2 and np.array_equal(phi, phi2) and np.array_equal(dg, dg2)
+        assert E == pytest.approx(Eo, rel=1e-12)
+    else:
+        assert E == pytest.approx(Eo, rel=1e-6)
+        assert rel_rms(phi, phio) < 1e-6
+        assert np.allclose(dg, dgo, rtol=1e-5)
+        assert E == pytest.approx(E2, rel=1e-6)

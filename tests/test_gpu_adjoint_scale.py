@@ -11,7 +11,9 @@ context on the same inputs (forces, HVP, E and dE/dq within 1e-6; dE/dgamma,
 a sum of FP32 node sums over every pair, within 1e-5, the bound of the
 small-system mixed-precision energy test);
 every step's outputs are finite, the last step's are bitwise repeatable,
-and the context's device memory stays under 1.5 GB.
+the one-call forces + energy (qem_forces_energy_dev) gives the same forces
+bit for bit and the energy within the same bars, and the context's device
+memory stays under 1.5 GB.
 """
 import numpy as np
 import pytest
@@ -77,6 +79,16 @@ def test_adjoint_slab_n100k_perturbed_steps():
         F2, hv2, E2, phi2, dg2 = _step(c, cfg.n, xd, qd, vd, fd, hd, pd, kidx)
         assert np.array_equal(F2, F) and np.array_equal(hv2, hv) and E2 == E
         assert np.array_equal(phi2, phi) and np.array_equal(dg2, dg)
+        # forces and energy in one call (one pair quadrature for both): the
+        # same forces bit for bit, the energy within the mixed bars of FP64
+        E3, dg3 = c.forces_energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr, pd.ptr)
+        L.synchronize()
+        F3, phi3, dg3 = fd.download(), pd.download(), np.array(dg3)
+        print(f"fused vs fp64: E {abs(E3 - E64) / abs(E64):.2e} phi {rel_rms(phi3, phi64):.2e}"
+              f" dg {np.abs(dg3 - dg64) / np.abs(dg64)}", flush=True)
+        assert np.array_equal(F3, F)
+        assert abs(E3 - E64) <= 1e-6 * abs(E64) and rel_rms(phi3, phi64) < 1e-6
+        assert np.allclose(dg3, dg64, rtol=1e-5, atol=1e-9 * abs(E64))
         mem = c.device_bytes()
         print(f"peak device bytes (mixed context): {mem}", flush=True)
         assert mem < MAX_DEVICE_BYTES
