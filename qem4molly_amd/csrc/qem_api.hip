@@ -430,6 +430,13 @@ ShortConsts short_consts(const qem_ctx *ctx, int single)
     c.jt_inv_dx = ctx->sw.jt_dx > 0 ? 1.0f / ctx->sw.jt_dx : 0.0f;
     c.ncon = ctx->sw.ncon.as<float4>();
     c.nconi = ctx->sw.nconi.as<float>();
+    c.ncon64 = ctx->sw.ncon64.as<double2>();
+    {
+        const double log2e = 1.4426950408889634;
+        c.d_c2 = -2.0 * p.Lz * log2e;
+        c.d_rG = (-log2e * (0.25 / p.alpha)) / (c.d_c2 * c.d_c2);
+        c.d_hinvLz = 0.5 / p.Lz;
+    }
     c.jd_n = ctx->sw.jd_n;
     c.jd_inv_dx = ctx->sw.jd_dx > 0 ? 1.0 / ctx->sw.jd_dx : 0.0;
     c.jdtab = ctx->sw.jdtab.as<double>();
@@ -1336,6 +1343,17 @@ int qem_create(const qem_params *p, qem_ctx **out)
         nc[4 * (size_t)n + 3] = (float)std::log2(ctx->gl.w[(size_t)n] * u);
     }
     if (upload(ctx, ctx->sw.ncon, nc.data(), sizeof(float) * nc.size())) return fail(QEM_ERR_HIP);
+    {
+        // FP64 node constants from long double: 1 + x_n and log2(w_n (1 + x_n))
+        std::vector<double> n64(2 * (size_t)p->N_t);
+        for (int n = 0; n < p->N_t; ++n) {
+            const long double u = 1.0L + (long double)ctx->gl.x[(size_t)n];
+            n64[2 * (size_t)n] = (double)u;
+            n64[2 * (size_t)n + 1] = (double)log2l((long double)ctx->gl.w[(size_t)n] * u);
+        }
+        if (upload(ctx, ctx->sw.ncon64, n64.data(), sizeof(double) * n64.size()))
+            return fail(QEM_ERR_HIP);
+    }
     std::vector<float> nci((size_t)p->N_t);
     for (int n = 0; n < p->N_t; ++n) nci[(size_t)n] = (float)(1.0 / (1.0 + ctx->gl.x[(size_t)n]));
     if (upload(ctx, ctx->sw.nconi, nci.data(), sizeof(float) * nci.size())) return fail(QEM_ERR_HIP);
@@ -1423,7 +1441,7 @@ int qem_create(const qem_params *p, qem_ctx **out)
 
 static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
 {
-    return {&ctx->sw.ncon, &ctx->sw.nconi, &ctx->nl.sout2, &ctx->lw.partE, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab, &ctx->sw.jdtab,
+    return {&ctx->sw.ncon, &ctx->sw.nconi, &ctx->sw.ncon64, &ctx->nl.sout2, &ctx->lw.partE, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab, &ctx->sw.jdtab,
                       &ctx->sw.pnode, &ctx->sw.pnodef,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
                       &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,

@@ -30,6 +30,46 @@ __device__ __forceinline__ void bessel01_tab(float x, const float4 *__restrict__
     J1 = -dp * inv_dx;
 }
 
+// 2^y in FP64: y = m + f with m = rint(y) and |f| <= 1/2 (exact), 2^f from
+// the degree-10 polynomial interpolating 2^f at the 11 Chebyshev points of
+// [-1/2, 1/2] (near-minimax; coefficients from csrc/gen_exp2_coeffs.py,
+// relative error 4.7e-16 in double Horner), scaled by 2^m through the
+// exponent (v_ldexp_f64; results below the FP64 range go to zero).
+constexpr double kExp2C[11] = {
+    1.0,
+    0.69314718055995,
+    0.24022650695910097,
+    0.05550410866444772,
+    0.009618129107606888,
+    0.0013333558230164974,
+    0.0001540353044173605,
+    1.5252657260200837e-05,
+    1.321544258792169e-06,
+    1.0208690299958306e-07,
+    7.072585949269223e-09};
+
+__device__ __forceinline__ double exp2_fp64(double y)
+{
+    const double m = rint(y);
+    const double f = y - m;
+    double p = kExp2C[10];
+#pragma unroll
+    for (int i = 9; i >= 0; --i) p = fma(p, f, kExp2C[i]);
+    return ldexp(p, (int)m);
+}
+
+// 1/d in FP64: the hardware reciprocal refined by two Newton steps
+// r <- r + r (1 - d r) (for |d| >= 1 - |gamma_1 gamma_2| > 0, the kernels'
+// denominators, no special cases arise).
+__device__ __forceinline__ double rcp_fp64(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
 // FP64 J0/J1 from the FP64 table: per interval of width dx a degree-7
 // polynomial p(t) ~ J0(x0 + t dx) (8 doubles in a row of 9), J1 = -p'(t) / dx
 // from the same polynomial.  The index is clamped to the last interval.

@@ -43,6 +43,9 @@ struct ShortConsts {
     float jt_dx, jt_inv_dx;  // interval width and its inverse
     const float4 *ncon;  // device node constants (mixed precision), or null
     const float *nconi;  // per node 1 / (1 + x_n) (the fused energy sums), or null
+    const double2 *ncon64;   // FP64 node constants: (1 + x_n, log2(w_n (1 + x_n)))
+    double d_c2, d_rG, d_hinvLz;  // FP64: -2 Lz log2(e), cg / d_c2^2 with
+                                  // cg = -log2(e) / (4 alpha), 1 / (2 Lz)
     int32_t jd_n;        // FP64 Bessel table intervals (0: device-library j0/j1)
     double jd_inv_dx;    // its inverse interval width
     const double *jdtab; // FP64 table in device memory (staged in LDS by the kernels)
@@ -177,6 +180,7 @@ struct ShortWork {
     DevBuf ncon;          // float4 per node: (1 + x_n, (1 + x_n)^2, log2 w_n,
                           //                   log2(w_n (1 + x_n)))
     DevBuf nconi;         // float per node: 1 / (1 + x_n)
+    DevBuf ncon64;        // double2 per node: (1 + x_n, log2(w_n (1 + x_n)))
     DevBuf jtab;          // one float4 (cubic coefficients) per interval
     int jt_n = 0;
     float jt_dx = 0.f;

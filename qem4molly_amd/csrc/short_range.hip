@@ -40,6 +40,12 @@ using dev::kLog2e;
 // with D(k) = g12 e^{-2kLz} - 1 and the nodes mapped to [0, K].
 // The constant factors 0.5 b_l (g12 for point) and s_l are applied by the
 // caller.
+// FP64 with the node constants folded in, as in the mixed loop below: per
+// node u = (1 + x_n, log2(w_n (1 + x_n))) (from long double), k = h u.x,
+// w k = h^2 w_n u.x, and every exponent goes through e = -2 Lz k log2(e):
+//   point: e^{-k (a + 2 Lz)} w k / h^2 = 2^{e (1 + a/2Lz) + u.y}
+//   gauss: e^{-k a - k^2/4 alpha} w k / h^2 = 2^{e (a/2Lz + e cg/c2^2) + u.y}
+// with the FP64 2^y and 1/D of qem_device.h.
 // TAB: J0/J1 from the FP64 Bessel table staged in LDS (tab64), else from
 // the device library (argument ranges beyond the table's capacity).
 template <bool TAB>
@@ -49,29 +55,48 @@ __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool 
                                            const double *__restrict__ tab64, double &sr,
                                            double &sz)
 {
+    (void)glx;
+    (void)glw;
+    const double2 *__restrict__ nc = c.ncon64;
     const double h = 0.5 * K;
-    const double inv4a = 0.25 / c.alpha;
+    const double hE = h * c.d_c2;                  // e = hE u.x
+    const double rA = a * c.d_hinvLz;              // a / 2Lz
+    const double rP = 1.0 + rA;
+    const double rG = c.d_rG;
+    // Bessel argument in table units (TAB; clamped once per set, u.x < 2
+    // keeps the interval below jd_n - 1, as in the mixed loop) or the
+    // argument itself
+    const double hR = TAB ? fmin(h * rho * c.jd_inv_dx, 0.5 * (double)(c.jd_n - 1)) : h * rho;
     double ar = 0.0, az = 0.0;
     for (int n = 0; n < c.Nt; ++n) {
-        const double k = h + h * glx[n];
-        const double w = h * glw[n];
-        const double E = exp(-2.0 * k * c.Lz);
-        const double D = c.g12 * E - 1.0;
-        const double num = gauss ? exp(-k * a - k * k * inv4a) : exp(-k * a) * E;
-        const double wkt = w * k * (num / D);
-        const double x = k * rho;
+        const double2 u = nc[n];
+        const double e = hE * u.x;
+        const double E = dev::exp2_fp64(e);
+        const double y = gauss ? fma(e, fma(e, rG, rA), u.y) : fma(e, rP, u.y);
+        const double t = dev::exp2_fp64(y) * dev::rcp_fp64(fma(c.g12, E, -1.0));
+        const double xb = hR * u.x;
         double J0, J1;
         if (TAB) {
-            dev::bessel01_tab64(x, tab64, c.jd_n, c.jd_inv_dx, J0, J1);
+            const double tt = __builtin_amdgcn_fract(xb);
+            const double *__restrict__ cc = tab64 + 9 * (int)xb;
+            double p = cc[7], dp = 0.0;
+#pragma unroll
+            for (int k = 6; k >= 0; --k) {
+                dp = fma(dp, tt, p);
+                p = fma(p, tt, cc[k]);
+            }
+            J0 = p;
+            J1 = dp;   // -J1 dx, scaled below
         } else {
-            J0 = j0(x);
-            J1 = j1(x);
+            J0 = j0(xb);
+            J1 = j1(xb);
         }
-        ar += wkt * J1;
-        az += wkt * J0;
+        ar = fma(t, J1, ar);
+        az = fma(t, J0, az);
     }
-    sr = ar;
-    sz = az;
+    const double h2 = h * h;
+    sr = TAB ? -h2 * c.jd_inv_dx * ar : h2 * ar;
+    sz = h2 * az;
 }
 
 // Mixed-precision node sums with the node constants folded in: for the
@@ -350,16 +375,18 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
                 const double K = gauss ? Kg : Kp;
                 double acc = 0.0;
                 if (PREC != QEM_PREC_MIXED) {
+                    // the folded FP64 node terms of nodes_fp64 with J0(0) = 1
                     const double h = 0.5 * K;
-                    const double inv4a = 0.25 / c.alpha;
+                    const double hE = h * c.d_c2, rA = a * c.d_hinvLz, rP = 1.0 + rA;
+                    double ad = 0.0;
                     for (int n = lane; n < c.Nt; n += nl) {
-                        const double k = h + h * glx[n];
-                        const double w = h * glw[n];
-                        const double E = exp(-2.0 * k * c.Lz);
-                        const double D = c.g12 * E - 1.0;
-                        const double num = gauss ? exp(-k * a - k * k * inv4a) : exp(-k * a) * E;
-                        acc += w * k * (num / D);
+                        const double2 u = c.ncon64[n];
+                        const double e = hE * u.x;
+                        const double E = dev::exp2_fp64(e);
+                        const double y = gauss ? fma(e, fma(e, c.d_rG, rA), u.y) : fma(e, rP, u.y);
+                        ad = fma(dev::exp2_fp64(y), dev::rcp_fp64(fma(c.g12, E, -1.0)), ad);
                     }
+                    acc = h * h * ad;
                 } else {
                     const float h = 0.5f * (float)K;
                     const float c2 = -2.0f * (float)c.Lz * kLog2e;
@@ -391,8 +418,15 @@ __device__ double self_force_z(const ShortConsts &c, const double *__restrict__ 
     return -(qi * qi) * c.inv2pieps * Fsz;
 }
 
+// FP64 blocks are 512 threads: the FP64 Bessel table (~47 KB at the 100k
+// slab) is staged once per block, so two blocks of eight waves fill a CU to
+// the VGPR limit (4 waves per SIMD) where three blocks of four waves would
+// leave it at 3.
 template <int PREC>
-__global__ __launch_bounds__(256) void k_pair_list(
+constexpr int pair_block() { return PREC == QEM_PREC_FP64 ? 512 : 256; }
+
+template <int PREC>
+__global__ __launch_bounds__(pair_block<PREC>()) void k_pair_list(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
     const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
@@ -532,10 +566,10 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // grid for grid-stride pair kernels: enough blocks to fill 256 CUs several
 // times over, without relaunching the LDS table stage per 256 pairs
-inline unsigned pair_grid(int64_t m)
+inline unsigned pair_grid(int64_t m, int block = 256)
 {
-    const int64_t want = (m + 255) / 256;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
+    const int64_t want = (m + block - 1) / block;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16 * 256 / block));
 }
 
 }  // namespace
@@ -556,7 +590,7 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
             reinterpret_cast<float4 *>(d_fpair));
     } else {
         if (c.jd_n > 0)
-            k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs), 256, dev::lds_bytes64(c), s>>>(
+            k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs, 512), 512, dev::lds_bytes64(c), s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, d_pi,
                 d_pj, npairs, d_xyz, d_q,
                 reinterpret_cast<double4 *>(d_fpair));

@@ -162,3 +162,27 @@ def test_workload_configs():
     assert q[0] == -1.0 and q[1] == 1.0 and q.sum() == 0.0   # (-1)^i, i = 1..N
     for c in CONFIGS.values():
         assert 2 * c.r_cutoff < min(c.Lx, c.Ly)
+
+
+def test_fp64_exp2_coefficients_match_their_generator():
+    # dev::exp2_fp64 (qem_device.h) holds the degree-10 polynomial
+    # interpolating 2^f at the Chebyshev points of [-1/2, 1/2], as
+    # csrc/gen_exp2_coeffs.py computes it in 50-digit arithmetic; its double
+    # Horner evaluation stays within 5e-16 relative of 2^f
+    import re
+    import mpmath as mp
+    csrc = os.path.join(ROOT, "qem4molly_amd", "csrc")
+    sys.path.insert(0, csrc)
+    from gen_exp2_coeffs import coefficients
+    src = open(os.path.join(csrc, "qem_device.h")).read()
+    body = re.search(r"kExp2C\[11\] = \{(.*?)\};", src, re.S).group(1)
+    held = [float(v) for v in body.replace("\n", " ").split(",")]
+    want = coefficients()
+    assert held == want
+    f = np.linspace(-0.5, 0.5, 2001)
+    p = np.zeros_like(f)
+    for v in reversed(held):
+        p = p * f + v
+    mp.mp.dps = 30
+    ref = np.array([float(mp.power(2, mp.mpf(x))) for x in f])
+    assert np.max(np.abs(p - ref) / ref) < 5e-16
