@@ -22,8 +22,9 @@
 //          the atom's own term give Fl_non_sort; Flz_self; each atom's force
 //          is summed over the group's rows (fixed DPP order).
 //   R      per atom: sum over groups, scatter back to input order.
-// The k = 0 term rides along as an extra row with (q, q z, q, q z).
-// Nothing of size N x P is stored.
+// The k = 0 term (Flz_sort_k0) is not a k lane: it is a prefix sum of the
+// charges in z order (tile sums, a scan, a per-atom pass; k_k0_*), run
+// right after the sort and added in R.  Nothing of size N x P is stored.
 #include "qem_device.h"
 
 namespace qem {
@@ -132,12 +133,12 @@ __global__ __launch_bounds__(CH) void k_long_S(const double4 *__restrict__ partA
 
 // derived constants of one k row (k_make_krec)
 struct KRec {
-    double kx, ky, k, k0f;    // k vector; k0f = 1 on the k = 0 row
+    double kx, ky, k, pad;    // k vector (|k| > 0); pad keeps 16 doubles
     double kxt, kyt, kl, w;   // k_x / 2pi, k_y / 2pi, |k| log2 e; z weight
     double wx, wy;            // w k_x / |k|, w k_y / |k| (xy weights)
     double N1, N2, N3;        // non-sort: g g2 e2, g g12 e2, g g1 (g = 1 / 2(g12 e2 - 1))
     double S1, S2;            // self: g g2 e2, g g1
-    double hs;                // sorted-sum factor: 1/2, or 1 on the k = 0 row
+    double hs;                // sorted-sum factor: 1/2 (0 without the sorted part)
 };
 static_assert(sizeof(KRec) == 16 * sizeof(double), "KRec layout");
 
@@ -155,34 +156,28 @@ __global__ void k_make_krec(LongConsts lc, const double4 *__restrict__ kv, int64
     r.kyt = kk.y * kInv2Pi;
     r.kl = kk.z * kLog2eD;
     r.w = kk.w;
-    r.wx = r.wy = r.N1 = r.N2 = r.N3 = r.S1 = r.S2 = 0.0;
-    if (kk.z == 0.0) {
-        // Flz_sort_k0: w q (Q_below - Q_above), carried by the sorted sums
-        r.k0f = 1.0;
-        r.hs = (lc.parts & QEM_LONG_K0) ? 1.0 : 0.0;
-    } else {
-        r.k0f = 0.0;
-        const double e2 = exp(-2.0 * kk.z * lc.Lz);
-        const double g = 0.5 / (lc.g12 * e2 - 1.0);
-        r.wx = kk.w * (kk.x / kk.z);
-        r.wy = kk.w * (kk.y / kk.z);
-        if (lc.parts & QEM_LONG_NONSORT) {
-            r.N1 = g * (lc.g2 * e2);
-            r.N2 = g * (lc.g12 * e2);
-            r.N3 = g * lc.g1;
-        }
-        if (lc.parts & QEM_LONG_SELF) {
-            r.S1 = g * (lc.g2 * e2);
-            r.S2 = g * lc.g1;
-        }
-        r.hs = (lc.parts & QEM_LONG_SORT) ? 0.5 : 0.0;
+    r.pad = 0.0;
+    r.N1 = r.N2 = r.N3 = r.S1 = r.S2 = 0.0;
+    const double e2 = exp(-2.0 * kk.z * lc.Lz);
+    const double g = 0.5 / (lc.g12 * e2 - 1.0);
+    r.wx = kk.w * (kk.x / kk.z);
+    r.wy = kk.w * (kk.y / kk.z);
+    if (lc.parts & QEM_LONG_NONSORT) {
+        r.N1 = g * (lc.g2 * e2);
+        r.N2 = g * (lc.g12 * e2);
+        r.N3 = g * lc.g1;
     }
+    if (lc.parts & QEM_LONG_SELF) {
+        r.S1 = g * (lc.g2 * e2);
+        r.S2 = g * lc.g1;
+    }
+    r.hs = (lc.parts & QEM_LONG_SORT) ? 0.5 : 0.0;
     kr[p] = r;
 }
 
 // one lane's row constants
 struct LaneK {
-    double kx, ky, k, k0f, w, wx, wy, N1, N2, N3, S1, S2, hs;
+    double kx, ky, k, w, wx, wy, N1, N2, N3, S1, S2, hs;
 };
 
 __device__ __forceinline__ LaneK load_lane(const KRec &r, bool mixed)
@@ -191,7 +186,6 @@ __device__ __forceinline__ LaneK load_lane(const KRec &r, bool mixed)
     L.kx = mixed ? r.kxt : r.kx;
     L.ky = mixed ? r.kyt : r.ky;
     L.k = mixed ? r.kl : r.k;
-    L.k0f = r.k0f;
     L.w = r.w;
     L.wx = r.wx;
     L.wy = r.wy;
@@ -307,37 +301,137 @@ struct LaneReducer {
     }
 };
 
+// The last group of k rows holds Ptot mod 64 rows; when that is at most 32
+// (or 16), its waves take 32 rows x 2 consecutive chunks (16 x 4) instead
+// of 64 rows x 1 chunk, so that no lane idles: SPLIT sub-chunks per wave,
+// lane = sub * (64 / SPLIT) + row.  The other groups use SPLIT = 1.
+__host__ __device__ constexpr int lane_split_of(int64_t rem)
+{
+    return rem == 0 ? 1 : rem <= 16 ? 4 : rem <= 32 ? 2 : 1;
+}
+
+template <int SPLIT>
+__device__ constexpr int RW_of() { return 64 / SPLIT; }
+
+// One wave's place in the k-lane grid: its rows, its sub-chunk, the atoms.
+template <int SPLIT>
+struct LaneSite {
+    static constexpr int RW = 64 / SPLIT;   // rows per sub-chunk
+    int64_t wfc;      // the wave's first chunk
+    int64_t chunk;    // this lane's chunk (>= nchunk: none)
+    int64_t p;        // this lane's k row (>= Ptot: none)
+    int64_t a0, a1;   // this lane's atoms [a0, a1) (empty without a chunk)
+    int64_t lmax;     // the longest sub-chunk of the wave (its first)
+    __device__ __forceinline__ LaneSite(int wv, int lane, int64_t n, int chk, int nchunk)
+    {
+        const int sub = SPLIT == 1 ? 0 : lane / RW, rl = SPLIT == 1 ? lane : lane % RW;
+        wfc = ((int64_t)blockIdx.x * kLaneWaves + wv) * SPLIT;
+        chunk = wfc + sub;
+        p = (int64_t)blockIdx.y * 64 + rl;
+        a0 = chunk * chk;
+        a1 = chunk < nchunk ? min(n, a0 + chk) : a0;
+        lmax = min((int64_t)chk, n - wfc * chk);
+    }
+    // the atom of step t (clamped into the array for a lane past its end)
+    __device__ __forceinline__ int64_t at(int64_t t, int64_t n) const
+    {
+        return min(a0 + t, n - 1);
+    }
+};
+
+// Lane sums for a SPLIT-wave: the flush of LaneReducer with the 16-value
+// row shares of lane 4 r + q summed over the quad (SPLIT 1), over pairs of
+// the quad (SPLIT 2) or not at all (SPLIT 4), since the values 16 q ..
+// 16 q + 15 of a row are then sub-chunk q's rows (or half of them).  Slot j
+// of the batch that starts at step t0 is the atom a0 + t0 + j of each
+// sub-chunk (up), or a1 - 1 - (t0 + j) (down); atoms past a sub-chunk's
+// end are not written.
+template <int SPLIT>
+__device__ __forceinline__ void lane_flush(const LaneReducer &red, int nb, int lane,
+                                           double *__restrict__ out, int64_t wfc, int chk,
+                                           int64_t n, int nchunk, int64_t t0, bool down,
+                                           bool add)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int r = lane >> 2, q = lane & 3;
+    const bool live = r < 3 * nb;
+    double sum = 0.0;
+    if (live) {
+        const double *v = red.buf + r * kRedPitch + 16 * q;
+        sum = v[0];
+#pragma unroll
+        for (int m = 1; m < 16; ++m) sum += v[m];
+    }
+    if (SPLIT <= 2) sum += dev::dpp_perm_f64<0xB1>(sum);   // quad_perm [1,0,3,2]
+    if (SPLIT == 1) sum += dev::dpp_perm_f64<0x4E>(sum);   // quad_perm [2,3,0,1]
+    constexpr int QPS = 4 / SPLIT;   // quad lanes per sub-chunk
+    if (live && q % QPS == 0) {
+        const int64_t ch = wfc + q / QPS;
+        if (ch < nchunk) {
+            const int64_t a0 = ch * chk, a1 = min(n, a0 + chk);
+            const int j = r / 3, c = r - 3 * j;
+            const int64_t a = down ? a1 - 1 - (t0 + j) : a0 + t0 + j;
+            if (a >= a0 && a < a1) {
+                double *o = out + 3 * a + c;
+                *o = add ? *o + sum : sum;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Pass A (k lanes): per (k row, chunk) the four sums q cos e^{+kz},
-// q sin e^{+kz}, q cos e^{-kz}, q sin e^{-kz} (k = 0 row: q, q z, q, q z),
-// accumulated in ascending z.
+// q sin e^{+kz}, q cos e^{-kz}, q sin e^{-kz}, accumulated in ascending z.
+template <int PREC, int SPLIT>
+__device__ __forceinline__ void lane_A_body(const KRec *__restrict__ kr, int64_t Ptot,
+                                            const double4 *__restrict__ satom, int64_t n,
+                                            int chk, int nchunk, double4 *__restrict__ partA,
+                                            int wv, int lane)
+{
+    const LaneSite<SPLIT> w(wv, lane, n, chk, nchunk);
+    if (w.wfc >= nchunk) return;
+    const bool live = w.p < Ptot && w.chunk < nchunk;
+    const LaneK L = load_lane(kr[w.p < Ptot ? w.p : Ptot - 1], PREC == QEM_PREC_MIXED);
+    const int64_t len = w.a1 - w.a0;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
+    // the next atom's record is loaded one iteration ahead (a uniform,
+    // scalar load for SPLIT = 1), so it is in flight while this atom's
+    // terms are computed
+    double4 nx = satom[w.at(0, n)];
+    for (int64_t t = 0; t < w.lmax; ++t) {
+        const double4 at = nx;
+        if (t + 1 < w.lmax) nx = satom[w.at(t + 1, n)];
+        if (SPLIT == 1 || t < len) {
+            double c, s, Ep, Em;
+            lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+            // the same rounded values and adds as pass B's running sums
+            sx = add_nc(sx, at.w * (c * Ep));
+            sy = add_nc(sy, at.w * (s * Ep));
+            sz = add_nc(sz, at.w * (c * Em));
+            sw = add_nc(sw, at.w * (s * Em));
+        }
+    }
+    if (live) partA[w.p * nchunk + w.chunk] = make_double4(sx, sy, sz, sw);
+}
+
 template <int PREC>
 __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_A(const KRec *__restrict__ kr,
                                                              int64_t Ptot,
                                                              const double4 *__restrict__ satom,
                                                              int64_t n, int chk, int nchunk,
+                                                             int split_last,
                                                              double4 *__restrict__ partA)
 {
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
-    if (chunk >= nchunk) return;
-    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
-    const bool live = p < Ptot;
-    const LaneK L = load_lane(kr[live ? p : Ptot - 1], PREC == QEM_PREC_MIXED);
-    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
-    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
-    for (int64_t a = a0; a < a1; ++a) {
-        const double4 at = satom[a];
-        double c, s, Ep, Em;
-        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
-        const double qz = L.k0f * (at.w * at.z);
-        // the same rounded values and adds as pass B's running sums
-        sx = add_nc(sx, at.w * (c * Ep));
-        sy = add_nc(sy, fma(at.w, s * Ep, qz));
-        sz = add_nc(sz, at.w * (c * Em));
-        sw = add_nc(sw, fma(at.w, s * Em, qz));
-    }
-    if (live) partA[p * nchunk + chunk] = make_double4(sx, sy, sz, sw);
+    const int split = blockIdx.y + 1 == gridDim.y ? split_last : 1;
+    if (split == 4) lane_A_body<PREC, 4>(kr, Ptot, satom, n, chk, nchunk, partA, wv, lane);
+    else if (split == 2) lane_A_body<PREC, 2>(kr, Ptot, satom, n, chk, nchunk, partA, wv, lane);
+    else lane_A_body<PREC, 1>(kr, Ptot, satom, n, chk, nchunk, partA, wv, lane);
 }
 
 // Pass B (k lanes): forces of one chunk from one group of 64 k rows into
@@ -346,104 +440,159 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_A(const KRec *__restri
 // inclusive sum when the chunk is flat in e^{-kz} (k_max dz <= 1 over the
 // chunk, so the subtraction loses at most a factor e in relative accuracy);
 // otherwise a second walk in descending z accumulates them from the top.
+template <int PREC, int SPLIT>
+__device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t Ptot,
+                                            const double4 *__restrict__ satom, int64_t n,
+                                            int chk, int nchunk,
+                                            const double4 *__restrict__ partA,
+                                            const double4 *__restrict__ offA,
+                                            const double4 *__restrict__ totA, int own_term,
+                                            double *__restrict__ partB, double *red_buf, int wv,
+                                            int lane)
+{
+    const LaneSite<SPLIT> w(wv, lane, n, chk, nchunk);
+    if (w.wfc >= nchunk) return;
+    const bool live = w.p < Ptot;
+    const int64_t pl = live ? w.p : Ptot - 1;
+    const int64_t cl = w.chunk < nchunk ? w.chunk : nchunk - 1;
+    LaneK L = load_lane(kr[pl], PREC == QEM_PREC_MIXED);
+    const double kabs = live ? kr[pl].k : 0.0;
+    if (!live) L.w = L.wx = L.wy = 0.0;   // a lane past the last row adds zeros
+    const double4 T = totA[pl];
+    // Fl_non_sort over all atoms, the atom's own term included: that term
+    // is zero in xy (sin 0 = 0) and q e^{+-kz} (c^2 + s^2) times the z
+    // factors, so the non-sort force is e^{+kz} (c u1 - s u2) + e^{-kz}
+    // (c v1 - s v2) in xy and e^{+kz} (c u2 + s u1) - e^{-kz} (c v2 + s v1)
+    // in z, with per-row constants from the totals; the own term,
+    // -q (N1 e^{2kz} - N3 e^{-2kz}), and Flz_self, q (S1 e^{2kz} - S2
+    // e^{-2kz}), combine into q (E1 e^{2kz} - E2 e^{-2kz}), which is zero
+    // when both parts are on (S1 = N1, S2 = N3) and skipped then (own_term 0)
+    const double u1 = fma(L.N1, T.y, L.N2 * T.w), u2 = fma(L.N1, T.x, L.N2 * T.z);
+    const double v1 = fma(L.N2, T.y, L.N3 * T.w), v2 = fma(L.N2, T.x, L.N3 * T.z);
+    const double E1 = L.S1 - L.N1, E2 = L.S2 - L.N3;
+    const double4 off = offA[pl * nchunk + cl];
+    const double4 own = partA[pl * nchunk + cl];
+    const int64_t len = w.a1 - w.a0;
+    // flat: k_max dz <= 1 over every sub-chunk of the wave (rows repeat
+    // across sub-chunks, so the lane maximum of k is the rows' maximum)
+    double kmax = kabs;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, d));
+    double dz = len > 0 ? satom[w.a1 - 1].z - satom[w.a0].z : 0.0;
+#pragma unroll
+    for (int d = 32; d >= RW_of<SPLIT>(); d >>= 1) dz = fmax(dz, __shfl_xor(dz, d));
+    const bool flat = __builtin_amdgcn_readfirstlane((int)(kmax * dz <= 1.0)) != 0;
+    double *out = partB + ((int64_t)blockIdx.y * n) * 3;
+    const LaneReducer red{red_buf};
+
+    double A = off.x, B = off.y;   // exclusive prefix below the atom
+    double PZ = 0.0, PW = 0.0;     // inclusive in-chunk sums of the e^{-kz} pair
+    // the walk up the chunk, with the chunk's flatness and own_term (both
+    // wave-uniform) as compile-time flags, so neither costs a select per atom
+    const auto walk = [&](auto flat_c, auto own_c) {
+        constexpr bool FLAT = decltype(flat_c)::value, OWN = decltype(own_c)::value;
+        double4 nx = satom[w.at(0, n)];   // the next atom, loaded one iteration ahead
+        for (int64_t t = 0; t < w.lmax; ++t) {
+            const double4 at = nx;
+            if (t + 1 < w.lmax) nx = satom[w.at(t + 1, n)];
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            if (SPLIT == 1 || t < len) {
+                const double q = at.w;
+                double c, s, Ep, Em;
+                lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+                const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+                const double vx = q * cp, vy = q * sp;
+                // Fl_non_sort (src/Force_long.jl:5-70), own term included
+                const double gxn = Ep * (c * u1 - s * u2) + Em * (c * v1 - s * v2);
+                const double gzn = Ep * (c * u2 + s * u1) - Em * (c * v2 + s * v1);
+                // Fl_sort (src/Force_long.jl:72-127): A, B below the atom; C, D
+                // above it here for a flat chunk, else from the second walk
+                double sxy = sm * A - cm * B, szz = cm * A + sm * B;
+                if constexpr (FLAT) {
+                    PZ = add_nc(PZ, q * cm);
+                    PW = add_nc(PW, q * sm);
+                    const double C = off.z + (own.z - PZ), D = off.w + (own.w - PW);
+                    sxy += sp * C - cp * D;
+                    szz -= cp * C + sp * D;
+                }
+                double gzs = gzn + L.hs * szz;
+                // Flz_self (src/Force_long.jl:164-184) minus the own term above
+                if constexpr (OWN) gzs += q * (E1 * (Ep * Ep) - E2 * (Em * Em));
+                const double gx = q * (gxn + L.hs * sxy);
+                const double gz = q * gzs;
+                fx = L.wx * gx;
+                fy = L.wy * gx;
+                fz = L.w * gz;
+                A += vx;
+                B += vy;
+            }
+            const int slot = (int)(t % kRedBatch);
+            red.put(slot, lane, fx, fy, fz);
+            if (slot == kRedBatch - 1 || t + 1 == w.lmax)
+                lane_flush<SPLIT>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk, t - slot,
+                                  false, false);
+        }
+    };
+    if (flat) {
+        if (own_term) walk(std::true_type{}, std::true_type{});
+        else walk(std::true_type{}, std::false_type{});
+    } else {
+        if (own_term) walk(std::false_type{}, std::true_type{});
+        else walk(std::false_type{}, std::false_type{});
+    }
+    if (flat) return;
+    // exact suffix walk: the sorted-sum terms with C, D (above the atom),
+    // from the top, added to the first walk's values
+    double C = off.z, D = off.w;
+    for (int64_t t = 0; t < w.lmax; ++t) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        if (SPLIT == 1 || t < len) {
+            const double4 at = satom[w.a1 - 1 - t];
+            const double q = at.w;
+            double c, s, Ep, Em;
+            lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+            const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+            const double gx = q * (L.hs * (sp * C - cp * D));
+            const double gz = q * (L.hs * (-cp * C - sp * D));
+            fx = L.wx * gx;
+            fy = L.wy * gx;
+            fz = L.w * gz;
+            C += q * cm;
+            D += q * sm;
+        }
+        const int slot = (int)(t % kRedBatch);
+        red.put(slot, lane, fx, fy, fz);
+        if (slot == kRedBatch - 1 || t + 1 == w.lmax)
+            lane_flush<SPLIT>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk, t - slot, true,
+                              true);
+    }
+}
+
 template <int PREC>
 __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restrict__ kr,
                                                              int64_t Ptot,
                                                              const double4 *__restrict__ satom,
                                                              int64_t n, int chk, int nchunk,
+                                                             int split_last,
                                                              const double4 *__restrict__ partA,
                                                              const double4 *__restrict__ offA,
                                                              const double4 *__restrict__ totA,
+                                                             int own_term,
                                                              double *__restrict__ partB)
 {
+    __shared__ double red_lds[kLaneWaves][kRedBatch * 3 * kRedPitch];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int64_t chunk = (int64_t)blockIdx.x * kLaneWaves + wv;
-    if (chunk >= nchunk) return;
-    const int64_t p = (int64_t)blockIdx.y * 64 + lane;
-    const bool live = p < Ptot;
-    const int64_t pl = live ? p : Ptot - 1;
-    const LaneK L = load_lane(kr[pl], PREC == QEM_PREC_MIXED);
-    const double kabs = live ? kr[pl].k : 0.0;
-    const double4 T = totA[pl];
-    const double4 off = offA[pl * nchunk + chunk];
-    const double4 own = partA[pl * nchunk + chunk];
-    const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
-    double kmax = kabs;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, d));
-    const double dz = satom[a1 - 1].z - satom[a0].z;
-    const bool flat = kmax * dz <= 1.0;
-    double *out = partB + ((int64_t)blockIdx.y * n) * 3;
-    __shared__ double red_lds[kLaneWaves][kRedBatch * 3 * kRedPitch];
-    const LaneReducer red{red_lds[wv]};
-
-    double A = off.x, B = off.y;   // exclusive prefix below the atom
-    double PZ = 0.0, PW = 0.0;     // inclusive in-chunk sums of the e^{-kz} pair
-    for (int64_t a = a0; a < a1; ++a) {
-        const double4 at = satom[a];
-        const double q = at.w;
-        double c, s, Ep, Em;
-        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
-        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-        const double qz = L.k0f * (q * at.z);
-        const double vx = q * cp, vy = fma(q, sp, qz), vz = q * cm, vw = fma(q, sm, qz);
-        PZ = add_nc(PZ, vz);
-        PW = add_nc(PW, vw);
-        double C = 0.0, D = 0.0;
-        if (flat) {
-            C = off.z + (own.z - PZ);
-            D = off.w + (own.w - PW);
-        }
-        // Fl_non_sort (src/Force_long.jl:5-70), own term removed
-        const double f1 = T.x - vx, f2 = T.y - vy, f3 = T.z - vz, f4 = T.w - vw;
-        const double X1 = c * f2 - s * f1, X2 = c * f4 - s * f3;
-        const double Z1 = c * f1 + s * f2, Z2 = c * f3 + s * f4;
-        const double gu = L.N1 * Ep + L.N2 * Em, gw = L.N3 * Em + L.N2 * Ep;
-        const double guz = L.N1 * Ep - L.N2 * Em, gwz = L.N2 * Ep - L.N3 * Em;
-        const double gxn = gu * X1 + gw * X2, gzn = guz * Z1 + gwz * Z2;
-        // Fl_sort (src/Force_long.jl:72-127) and Flz_sort_k0 (130-161)
-        const double sxy = sm * A - cm * B + sp * C - cp * D;
-        const double szz = cm * A + sm * B - cp * C - sp * D;
-        // Flz_self (src/Force_long.jl:164-184)
-        const double self = L.S1 * (Ep * Ep) - L.S2 * (Em * Em);
-        const double gx = q * (gxn + L.hs * sxy);
-        const double gz = q * (gzn + L.hs * szz + q * self);
-        const double fx = live ? L.wx * gx : 0.0;
-        const double fy = live ? L.wy * gx : 0.0;
-        const double fz = live ? L.w * gz : 0.0;
-        A += vx;
-        B += vy;
-        const int slot = (int)((a - a0) % kRedBatch);
-        red.put(slot, lane, fx, fy, fz);
-        if (slot == kRedBatch - 1 || a == a1 - 1) red.flush(slot + 1, lane, out, a - slot, false);
-    }
-    if (flat) return;
-    // exact suffix walk: the sorted-sum terms with C, D (above the atom),
-    // in batches [lo, hi] taken from the top
-    double C = off.z, D = off.w;
-    int64_t lo = a1, hi = a1 - 1;
-    for (int64_t a = a1 - 1; a >= a0; --a) {
-        if (a < lo) {
-            hi = a;
-            lo = a - kRedBatch + 1 < a0 ? a0 : a - kRedBatch + 1;
-        }
-        const double4 at = satom[a];
-        const double q = at.w;
-        double c, s, Ep, Em;
-        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
-        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-        const double qz = L.k0f * (q * at.z);
-        const double gx = q * (L.hs * (sp * C - cp * D));
-        const double gz = q * (L.hs * (-cp * C - sp * D));
-        const double fx = live ? L.wx * gx : 0.0;
-        const double fy = live ? L.wy * gx : 0.0;
-        const double fz = live ? L.w * gz : 0.0;
-        C += q * cm;
-        D += fma(q, sm, qz);
-        red.put((int)(a - lo), lane, fx, fy, fz);
-        if (a == lo) red.flush((int)(hi - lo + 1), lane, out, lo, true);
-    }
+    const int split = blockIdx.y + 1 == gridDim.y ? split_last : 1;
+    if (split == 4)
+        lane_B_body<PREC, 4>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA, own_term, partB,
+                             red_lds[wv], wv, lane);
+    else if (split == 2)
+        lane_B_body<PREC, 2>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA, own_term, partB,
+                             red_lds[wv], wv, lane);
+    else
+        lane_B_body<PREC, 1>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA, own_term, partB,
+                             red_lds[wv], wv, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -473,7 +622,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_V(const KRec *__restri
     const bool live = p < Ptot;
     const int64_t pl = live ? p : Ptot - 1;
     const LaneK L = load_lane(kr[pl], PREC == QEM_PREC_MIXED);
-    const bool real = live && L.k0f == 0.0;   // the k = 0 row is not sampled
+    const bool real = live;
     const double4 T = totA[pl];
     const double4 off = offA[pl * nchunk + chunk];
     const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
@@ -485,7 +634,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_V(const KRec *__restri
         lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
         sCD[wv][a - a0][lane] = make_double2(C, D);
         C += at.w * (c * Em);
-        D += fma(at.w, s * Em, L.k0f * (at.w * at.z));
+        D += at.w * (s * Em);
     }
     double4 *out = partV + (int64_t)blockIdx.y * n;
     double A = off.x, B = off.y;
@@ -495,8 +644,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_V(const KRec *__restri
         double c, s, Ep, Em;
         lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
         const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-        const double qz = L.k0f * (q * at.z);
-        const double vx = q * cp, vy = fma(q, sp, qz), vz = q * cm, vw = fma(q, sm, qz);
+        const double vx = q * cp, vy = q * sp, vz = q * cm, vw = q * sm;
         const double2 cd = sCD[wv][a - a0][lane];
         // the force terms of k_lane_B with exact suffix sums
         const double f1 = T.x - vx, f2 = T.y - vy, f3 = T.z - vz, f4 = T.w - vw;
@@ -548,8 +696,8 @@ __global__ void k_var_R(const double4 *__restrict__ partV, int ngroup, int64_t n
 //   d(s e^{+kz}) =  c e^{+kz} dth + s e^{+kz} k v_z,
 //   d(c e^{-kz}) = -s e^{-kz} dth - c e^{-kz} k v_z,
 //   d(s e^{-kz}) =  c e^{-kz} dth - s e^{-kz} k v_z,
-// and every sum carries its tangent.  The k = 0 row has no tangent (its z
-// force is piecewise constant in z); its lane contributes nothing.
+// and every sum carries its tangent.  The k = 0 term has no tangent (its z
+// force is piecewise constant in z).
 
 // values and tangents of the four per-atom channels (without the charge)
 struct LaneTerms {
@@ -594,15 +742,22 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_A_t(
     const double kx = R.kx, ky = R.ky, kabs = R.k;
     const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
     double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0, tx = 0.0, ty = 0.0, tz = 0.0, tw = 0.0;
+    // the next atom and its direction, loaded one iteration ahead
+    double4 nx = satom[a0];
+    double3 nv = make_double3(svx[a0], svy[a0], svz[a0]);
     for (int64_t a = a0; a < a1; ++a) {
-        const double4 at = satom[a];
-        const double3 dv = make_double3(svx[a], svy[a], svz[a]);
+        const double4 at = nx;
+        const double3 dv = nv;
+        if (a + 1 < a1) {
+            nx = satom[a + 1];
+            nv = make_double3(svx[a + 1], svy[a + 1], svz[a + 1]);
+        }
         const LaneTerms T = lane_terms_t<PREC>(L, kx, ky, kabs, at, dv);
-        const double q = at.w, qz = L.k0f * (q * at.z);
+        const double q = at.w;
         sx = add_nc(sx, q * T.cp);
-        sy = add_nc(sy, fma(q, T.sp, qz));
+        sy = add_nc(sy, q * T.sp);
         sz = add_nc(sz, q * T.cm);
-        sw = add_nc(sw, fma(q, T.sm, qz));
+        sw = add_nc(sw, q * T.sm);
         tx = add_nc(tx, q * T.dcp);
         ty = add_nc(ty, q * T.dsp);
         tz = add_nc(tz, q * T.dcm);
@@ -636,8 +791,8 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
     const KRec &R = kr[pl];
     const LaneK L = load_lane(R, PREC == QEM_PREC_MIXED);
     const double kx = R.kx, ky = R.ky, kabs = R.k;
-    // the k = 0 row and idle lanes add nothing
-    const bool act = live && R.k0f == 0.0;
+    // idle lanes add nothing
+    const bool act = live;
     const double4 T = totA[pl], dT = totT[pl];
     const double4 off = offA[pl * nchunk + chunk], offt = offT[pl * nchunk + chunk];
     const double4 own = partA[pl * nchunk + chunk], ownt = partT[pl * nchunk + chunk];
@@ -652,12 +807,19 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
 
     double A = off.x, B = off.y, dA = offt.x, dB = offt.y;
     double PZ = 0.0, PW = 0.0, dPZ = 0.0, dPW = 0.0;
+    // the next atom and its direction, loaded one iteration ahead
+    double4 nx = satom[a0];
+    double3 nv = make_double3(svx[a0], svy[a0], svz[a0]);
     for (int64_t a = a0; a < a1; ++a) {
-        const double4 at = satom[a];
-        const double3 dv = make_double3(svx[a], svy[a], svz[a]);
+        const double4 at = nx;
+        const double3 dv = nv;
+        if (a + 1 < a1) {
+            nx = satom[a + 1];
+            nv = make_double3(svx[a + 1], svy[a + 1], svz[a + 1]);
+        }
         const LaneTerms X = lane_terms_t<PREC>(L, kx, ky, kabs, at, dv);
-        const double q = at.w, qz = L.k0f * (q * at.z);
-        const double vx = q * X.cp, vy = fma(q, X.sp, qz), vz = q * X.cm, vw = fma(q, X.sm, qz);
+        const double q = at.w;
+        const double vx = q * X.cp, vy = q * X.sp, vz = q * X.cm, vw = q * X.sm;
         const double tvx = q * X.dcp, tvy = q * X.dsp, tvz = q * X.dcm, tvw = q * X.dsm;
         PZ = add_nc(PZ, vz);
         PW = add_nc(PW, vw);
@@ -721,7 +883,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
         const double4 at = satom[a];
         const double3 dv = make_double3(svx[a], svy[a], svz[a]);
         const LaneTerms X = lane_terms_t<PREC>(L, kx, ky, kabs, at, dv);
-        const double q = at.w, qz = L.k0f * (q * at.z);
+        const double q = at.w;
         const double dsxy = X.dsp * C + X.sp * dC - X.dcp * D - X.cp * dD;
         const double dszz = -X.dcp * C - X.cp * dC - X.dsp * D - X.sp * dD;
         const double gx = q * (L.hs * dsxy);
@@ -730,7 +892,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
         const double fy = act ? L.wy * gx : 0.0;
         const double fz = act ? L.w * gz : 0.0;
         C += q * X.cm;
-        D += fma(q, X.sm, qz);
+        D += q * X.sm;
         dC += q * X.dcm;
         dD += q * X.dsm;
         red.put((int)(a - lo), lane, fx, fy, fz);
@@ -745,8 +907,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
 // with the prefix/suffix sums A..D (the direct pair term) and P1..P4 the
 // per-k derivative of the image term I_k = [g1 |S-|^2 + g2 e2 |S+|^2
 // + 2 g1 g2 e2 Re(S+ conj S-)] / (2 (1 - g1 g2 e2)) with respect to q_i,
-// divided by the atom's own terms; for k = 0,
-//   phi_0(i) = -w0 [z_i (Q_below - Q_above) - (QZ_below - QZ_above)].
+// divided by the atom's own terms.  (The k = 0 potential is k_k0_apply's.)
 template <int PREC>
 __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_energy(
     LongConsts lc, const KRec *__restrict__ kr, int64_t Ptot, const double4 *__restrict__ satom,
@@ -763,21 +924,18 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_energy(
     const int64_t pl = live ? p : Ptot - 1;
     const KRec &R = kr[pl];
     const LaneK L = load_lane(R, PREC == QEM_PREC_MIXED);
-    const bool k0 = R.k0f != 0.0;
     const double4 T = totA[pl];
     const double4 off = offA[pl * nchunk + chunk];
     const double4 own = partA[pl * nchunk + chunk];
-    // per-row factors: w/(2k) and the image-term derivative coefficients
-    double we = 0.0, P1 = 0.0, P2 = 0.0, P3 = 0.0, P4 = 0.0;
-    if (!k0) {
-        const double e2 = exp(-2.0 * R.k * lc.Lz);
-        const double F = 1.0 / (1.0 - lc.g12 * e2);   // 2 / (2 (1 - g12 e2))
-        we = R.w / (2.0 * R.k);
-        P1 = F * (lc.g1 * T.z + lc.g12 * e2 * T.x);
-        P2 = F * (lc.g1 * T.w + lc.g12 * e2 * T.y);
-        P3 = F * e2 * (lc.g2 * T.x + lc.g12 * T.z);
-        P4 = F * e2 * (lc.g2 * T.y + lc.g12 * T.w);
-    }
+    // per-row factors: w/(2k) (0 on an idle lane) and the image-term
+    // derivative coefficients
+    const double e2 = exp(-2.0 * R.k * lc.Lz);
+    const double F = 1.0 / (1.0 - lc.g12 * e2);   // 2 / (2 (1 - g12 e2))
+    const double we = live ? R.w / (2.0 * R.k) : 0.0;
+    const double P1 = F * (lc.g1 * T.z + lc.g12 * e2 * T.x);
+    const double P2 = F * (lc.g1 * T.w + lc.g12 * e2 * T.y);
+    const double P3 = F * e2 * (lc.g2 * T.x + lc.g12 * T.z);
+    const double P4 = F * e2 * (lc.g2 * T.y + lc.g12 * T.w);
     const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
     double kmax = live ? R.k : 0.0;
 #pragma unroll
@@ -786,14 +944,15 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_energy(
     double *out = partB + (int64_t)blockIdx.y * n;
 
     double A = off.x, B = off.y, PZ = 0.0, PW = 0.0;
+    double4 nx = satom[a0];        // the next atom, loaded one iteration ahead
     for (int64_t a = a0; a < a1; ++a) {
-        const double4 at = satom[a];
+        const double4 at = nx;
+        if (a + 1 < a1) nx = satom[a + 1];
         const double q = at.w;
         double c, s, Ep, Em;
         lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
         const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-        const double qz = L.k0f * (q * at.z);
-        const double vx = q * cp, vy = fma(q, sp, qz), vz = q * cm, vw = fma(q, sm, qz);
+        const double vx = q * cp, vy = q * sp, vz = q * cm, vw = q * sm;
         PZ = add_nc(PZ, vz);
         PW = add_nc(PW, vw);
         double C = 0.0, D = 0.0;
@@ -801,9 +960,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_energy(
             C = off.z + (own.z - PZ);
             D = off.w + (own.w - PW);
         }
-        double ph = k0 ? -L.w * (at.z * (A - C) - (B - D))
-                       : we * (cm * (A + P1) + sm * (B + P2) + cp * (C + P3) + sp * (D + P4));
-        if (!live) ph = 0.0;
+        double ph = we * (cm * (A + P1) + sm * (B + P2) + cp * (C + P3) + sp * (D + P4));
         A += vx;
         B += vy;
         ph = wave_sum63(ph);
@@ -818,20 +975,103 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_energy(
         double c, s, Ep, Em;
         lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
         const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-        const double qz = L.k0f * (q * at.z);
-        double ph = k0 ? L.w * (at.z * C - D) : we * (cp * C + sp * D);
-        if (!live) ph = 0.0;
+        double ph = we * (cp * C + sp * D);
         C += q * cm;
-        D += fma(q, sm, qz);
+        D += q * sm;
         ph = wave_sum63(ph);
         if (lane == 63) out[a] += ph;
     }
 }
 
-// Pass R: sum the k tiles and scatter to input order.
+// ---------------------------------------------------------------------------
+// The k = 0 term (Flz_sort_k0, src/Force_long.jl:130-161) and its potential,
+// off the k lanes.  For the atom of rank r in z order, with Q_b, QZ_b the
+// sums of q and q z over the atoms before it and Q_a, QZ_a over those after,
+//   F0_z(r) = w0 q (Q_b - Q_a),
+//   phi0(r) = -w0 [z (Q_b - Q_a) - (QZ_b - QZ_a)],   w0 = 1 / (2 Lx Ly eps_0),
+// into k0[r] = (F0_z, phi0).  Tiles of kK0Tile ranks, kK0Items consecutive
+// ranks per thread: the tile sums, their exclusive scan (one block), then
+// the per-atom pass, where the sums after an atom are the total minus those
+// up to it (charges of either sign and no exponentials: no cancellation
+// beyond the rounding of the totals).
+constexpr int kK0Items = 8, kK0Tile = CH * kK0Items;
+
+__global__ __launch_bounds__(CH) void k_k0_tiles(const double4 *__restrict__ satom, int64_t n,
+                                                 double2 *__restrict__ tiles)
+{
+    __shared__ double2 sh[CH / 64];
+    const int64_t base = (int64_t)blockIdx.x * kK0Tile + (int64_t)threadIdx.x * kK0Items;
+    double2 v = make_double2(0.0, 0.0);
+    for (int i = 0; i < kK0Items; ++i) {
+        if (base + i < n) {
+            const double4 at = satom[base + i];
+            v.x += at.w;
+            v.y += at.w * at.z;
+        }
+    }
+    double2 tot;
+    (void)block_excl_scan(v, &tot, sh);
+    if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
+}
+
+// in place: tiles[t] becomes the sum of the tiles before t, tiles[ntile]
+// the total
+__global__ __launch_bounds__(CH) void k_k0_scan(double2 *__restrict__ tiles, int ntile)
+{
+    __shared__ double2 sh[CH / 64];
+    double2 run = make_double2(0.0, 0.0);
+    for (int b = 0; b < ntile; b += CH) {
+        const int m = b + (int)threadIdx.x;
+        const double2 v = m < ntile ? tiles[m] : make_double2(0.0, 0.0);
+        double2 tot;
+        const double2 ex = block_excl_scan(v, &tot, sh);
+        if (m < ntile) tiles[m] = make_double2(run.x + ex.x, run.y + ex.y);
+        run.x += tot.x;
+        run.y += tot.y;
+    }
+    if (threadIdx.x == 0) tiles[ntile] = run;
+}
+
+__global__ __launch_bounds__(CH) void k_k0_apply(const double4 *__restrict__ satom, int64_t n,
+                                                 const double2 *__restrict__ tiles, int ntile,
+                                                 double w0, double2 *__restrict__ k0)
+{
+    __shared__ double2 sh[CH / 64];
+    const int64_t base = (int64_t)blockIdx.x * kK0Tile + (int64_t)threadIdx.x * kK0Items;
+    double q[kK0Items], z[kK0Items];
+    double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < kK0Items; ++i) {
+        q[i] = z[i] = 0.0;
+        if (base + i < n) {
+            const double4 at = satom[base + i];
+            q[i] = at.w;
+            z[i] = at.z;
+        }
+        v.x += q[i];
+        v.y += q[i] * z[i];
+    }
+    double2 tot;
+    const double2 ex = block_excl_scan(v, &tot, sh);
+    const double2 T = tiles[ntile], t0 = tiles[blockIdx.x];
+    double Qb = t0.x + ex.x, QZb = t0.y + ex.y;
+#pragma unroll
+    for (int i = 0; i < kK0Items; ++i) {
+        const double qz = q[i] * z[i];
+        const double Qa = (T.x - Qb) - q[i], QZa = (T.y - QZb) - qz;
+        const double dQ = Qb - Qa;
+        if (base + i < n)
+            k0[base + i] = make_double2(w0 * (q[i] * dQ), -w0 * (z[i] * dQ - (QZb - QZa)));
+        Qb += q[i];
+        QZb += qz;
+    }
+}
+
+// Pass R: sum the k tiles (plus the k = 0 term when k0 is given) and
+// scatter to input order.
 __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
-                         const int32_t *__restrict__ perm, int accumulate,
-                         double *__restrict__ f)
+                         const int32_t *__restrict__ perm, const double2 *__restrict__ k0,
+                         int accumulate, double *__restrict__ f)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -842,6 +1082,7 @@ __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
         fy += o[1];
         fz += o[2];
     }
+    if (k0) fz += k0[r].x;
     const int64_t a = perm[r];
     if (accumulate) {
         f[3 * a] += fx;
@@ -860,6 +1101,7 @@ __global__ void k_long_R(const double *__restrict__ partB, int ntile, int64_t n,
 // leave it.  slot_of: the slot of each atom.
 __global__ void k_long_R_slots(const double *__restrict__ partB, int ntile, int64_t n,
                                const int32_t *__restrict__ perm,
+                               const double2 *__restrict__ k0,
                                const int32_t *__restrict__ slot_of,
                                const double *__restrict__ sout, const double *__restrict__ addz,
                                double *__restrict__ f)
@@ -873,6 +1115,7 @@ __global__ void k_long_R_slots(const double *__restrict__ partB, int ntile, int6
         fy += o[1];
         fz += o[2];
     }
+    if (k0) fz += k0[r].x;
     const int64_t a = perm[r];
     const double *sv = sout + 3 * (int64_t)slot_of[a];
     double z = sv[2];
@@ -892,10 +1135,6 @@ __global__ void k_long_dgamma(LongConsts lc, const double4 *__restrict__ kv, int
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Ptot) return;
     const double4 kk = kv[p];
-    if (kk.z == 0.0) {
-        out[p] = make_double2(0.0, 0.0);
-        return;
-    }
     const double k = kk.z;
     const double4 S = totA[p];
     const double e2 = exp(-2.0 * k * lc.Lz);
@@ -922,10 +1161,12 @@ __global__ void k_gather_sorted3(const double *__restrict__ v, const int32_t *__
     svz[r] = v[3 * a + 2];
 }
 
-// sum the k tiles of a width-w per-atom quantity, scale, scatter to input order
+// sum the k tiles of a width-w per-atom quantity (for the potential,
+// width 1, plus the k = 0 term's when k0 is given), scale, scatter to
+// input order
 __global__ void k_long_Rw(const double *__restrict__ partB, int ntile, int64_t n, int width,
-                          const int32_t *__restrict__ perm, int accumulate, double scale,
-                          double *__restrict__ f)
+                          const int32_t *__restrict__ perm, const double2 *__restrict__ k0,
+                          int accumulate, double scale, double *__restrict__ f)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -933,6 +1174,7 @@ __global__ void k_long_Rw(const double *__restrict__ partB, int ntile, int64_t n
     for (int c = 0; c < width; ++c) {
         double acc = 0.0;
         for (int t = 0; t < ntile; ++t) acc += partB[((int64_t)t * n + r) * width + c];
+        if (k0 && width == 1) acc += k0[r].y;
         acc *= scale;
         if (accumulate) f[width * a + c] += acc;
         else f[width * a + c] = acc;
@@ -967,17 +1209,40 @@ hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz, const doub
     return hipSuccess;
 }
 
+hipError_t launch_long_k0(const LongConsts &c, LongWork &lw, int64_t n, hipStream_t s,
+                          std::string &err)
+{
+    if (n <= 0) return hipSuccess;
+    const int ntile = (int)((n + kK0Tile - 1) / kK0Tile);
+    LR_CHECK(ensure(lw.k0tile, sizeof(double2) * ((size_t)ntile + 1)));
+    LR_CHECK(ensure(lw.k0out, sizeof(double2) * (size_t)n));
+    double2 *tiles = lw.k0tile.as<double2>();
+    const double4 *sa = lw.satom.as<double4>();
+    k_k0_tiles<<<(unsigned)ntile, CH, 0, s>>>(sa, n, tiles);
+    LR_CHECK(hipGetLastError());
+    k_k0_scan<<<1, CH, 0, s>>>(tiles, ntile);
+    LR_CHECK(hipGetLastError());
+    k_k0_apply<<<(unsigned)ntile, CH, 0, s>>>(sa, n, tiles, ntile, 0.5 * c.inv_area_eps,
+                                               lw.k0out.as<double2>());
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
 namespace {
 // k-lane passes: k rows in groups of 64 (one per lane); chunks of chk atoms
 // sized so that the grid holds about 8192 waves
+// (the last group's waves take split_last chunks each, lane_split_of)
 struct LaneGrid {
-    int ngroup, chk, nchunk;
+    int ngroup, chk, nchunk, split_last;
 };
 LaneGrid lane_grid(int64_t n, int64_t Ptot)
 {
     LaneGrid g;
     g.ngroup = (int)((Ptot + 63) / 64);
-    const int64_t want = (n * g.ngroup + 8191) / 8192;
+    g.split_last = lane_split_of(Ptot % 64);
+    // chunks per group-wave row: (ngroup - 1) full groups plus 1 / split_last
+    const int64_t units = (int64_t)(g.ngroup - 1) * g.split_last + 1;
+    const int64_t want = (n * units + 8192LL * g.split_last - 1) / (8192LL * g.split_last);
     g.chk = (int)std::min<int64_t>(256, std::max<int64_t>(16, want));
     g.nchunk = (int)((n + g.chk - 1) / g.chk);
     return g;
@@ -997,6 +1262,8 @@ LongSizes long_force_sizes(int64_t n, int64_t Ptot)
     z.partA = sizeof(double4) * P * (size_t)g.nchunk;
     z.totA = sizeof(double4) * P;
     z.partB = sizeof(double) * 3 * (size_t)g.ngroup * N;
+    z.k0tile = sizeof(double2) * ((N + kK0Tile - 1) / kK0Tile + 1);
+    z.k0out = sizeof(double2) * N;
     return z;
 }
 
@@ -1011,11 +1278,11 @@ hipError_t launch_long_krec(const LongConsts &c, LongWork &lw, int64_t Ptot, hip
 }
 
 hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
-                              hipStream_t s, std::string &err, bool krec)
+                              hipStream_t s, std::string &err, bool krec, int which)
 {
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
-    const int ngroup = lg.ngroup, chk = lg.chk, nchunk = lg.nchunk;
+    const int ngroup = lg.ngroup, chk = lg.chk, nchunk = lg.nchunk, sl = lg.split_last;
     LR_CHECK(ensure(lw.partA, sizeof(double4) * (size_t)Ptot * nchunk));
     LR_CHECK(ensure(lw.offA, sizeof(double4) * (size_t)Ptot * nchunk));
     LR_CHECK(ensure(lw.totA, sizeof(double4) * (size_t)Ptot));
@@ -1026,24 +1293,30 @@ hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int6
     }
     const KRec *kr = lw.kr.as<KRec>();
     dim3 g((unsigned)((nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)ngroup);
-    if (c.precision == QEM_PREC_MIXED)
-        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, lw.satom.as<double4>(), n,
-                                                               chk, nchunk, lw.partA.as<double4>());
-    else
-        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, lw.satom.as<double4>(), n,
-                                                              chk, nchunk, lw.partA.as<double4>());
-    LR_CHECK(hipGetLastError());
-    k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
-                                           lw.offA.as<double4>(), lw.totA.as<double4>());
-    LR_CHECK(hipGetLastError());
+    if (which & kLongPassAS) {
+        if (c.precision == QEM_PREC_MIXED)
+            k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
+                kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>());
+        else
+            k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
+                kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>());
+        LR_CHECK(hipGetLastError());
+        k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
+                                               lw.offA.as<double4>(), lw.totA.as<double4>());
+        LR_CHECK(hipGetLastError());
+    }
+    if (!(which & kLongPassB)) return hipSuccess;
+    // the self term differs from the non-sort sum's own term unless both
+    // parts are on or both off
+    const int own_term = ((c.parts & QEM_LONG_NONSORT) != 0) != ((c.parts & QEM_LONG_SELF) != 0);
     if (c.precision == QEM_PREC_MIXED)
         k_lane_B<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
-            kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>(),
-            lw.offA.as<double4>(), lw.totA.as<double4>(), lw.partB.as<double>());
+            kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>(),
+            lw.offA.as<double4>(), lw.totA.as<double4>(), own_term, lw.partB.as<double>());
     else
         k_lane_B<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
-            kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>(),
-            lw.offA.as<double4>(), lw.totA.as<double4>(), lw.partB.as<double>());
+            kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>(),
+            lw.offA.as<double4>(), lw.totA.as<double4>(), own_term, lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }
@@ -1054,8 +1327,9 @@ hipError_t launch_long_sum_slots(LongWork &lw, int64_t n, int64_t Ptot, const in
 {
     if (n <= 0) return hipSuccess;
     const int ngroup = Ptot > 0 ? lane_grid(n, Ptot).ngroup : 0;
-    k_long_R_slots<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ngroup, n,
-                                                lw.perm.as<int32_t>(), slot_of, sout, addz, d_f);
+    k_long_R_slots<<<nblk(n, 256), 256, 0, s>>>(
+        lw.partB.as<double>(), ngroup, n, lw.perm.as<int32_t>(),
+        lw.k0_on ? lw.k0out.as<double2>() : nullptr, slot_of, sout, addz, d_f);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }
@@ -1064,7 +1338,7 @@ hipError_t launch_long_sum(LongWork &lw, int64_t n, int64_t Ptot, double *d_f, i
                            hipStream_t s, std::string &err)
 {
     if (n <= 0) return hipSuccess;
-    if (Ptot <= 0) {
+    if (Ptot <= 0 && !lw.k0_on) {
         // nothing to add; an overwrite still has to clear the output
         if (!accumulate) {
             k_fill_f64<<<nblk(3 * n, 256), 256, 0, s>>>(d_f, 3 * n, 0.0);
@@ -1072,8 +1346,9 @@ hipError_t launch_long_sum(LongWork &lw, int64_t n, int64_t Ptot, double *d_f, i
         }
         return hipSuccess;
     }
-    const int ngroup = lane_grid(n, Ptot).ngroup;
+    const int ngroup = Ptot > 0 ? lane_grid(n, Ptot).ngroup : 0;
     k_long_R<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), ngroup, n, lw.perm.as<int32_t>(),
+                                          lw.k0_on ? lw.k0out.as<double2>() : nullptr,
                                           accumulate, d_f);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
@@ -1108,10 +1383,10 @@ hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, in
     const dim3 g((unsigned)((nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)ngroup);
     const double4 *sa = lw.satom.as<double4>();
     if (c.precision == QEM_PREC_MIXED)
-        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk,
+        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk, 1,
                                                                lw.partA.as<double4>());
     else
-        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk,
+        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk, 1,
                                                               lw.partA.as<double4>());
     LR_CHECK(hipGetLastError());
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
@@ -1148,11 +1423,11 @@ hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t 
     const dim3 g((unsigned)((lg.nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)lg.ngroup);
     const double4 *sa = lw.satom.as<double4>();
     if (c.precision == QEM_PREC_MIXED)
-        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, lg.chk,
-                                                               lg.nchunk, lw.partA.as<double4>());
+        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, n, lg.chk, lg.nchunk, lg.split_last, lw.partA.as<double4>());
     else
-        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, lg.chk,
-                                                              lg.nchunk, lw.partA.as<double4>());
+        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, sa, n, lg.chk, lg.nchunk, lg.split_last, lw.partA.as<double4>());
     LR_CHECK(hipGetLastError());
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, lg.nchunk,
                                            lw.offA.as<double4>(), lw.totA.as<double4>());
@@ -1189,13 +1464,16 @@ hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double 
                                   int accumulate, double *h_q0, hipStream_t s,
                                   std::string &err)
 {
-    if (n <= 0 || Ptot <= 0) return hipSuccess;
-    const LaneGrid lg = lane_grid(n, Ptot);
-    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partE.as<double>(), lg.ngroup, n, 1,
-                                           lw.perm.as<int32_t>(), accumulate, 1.0, d_phi);
+    if (n <= 0 || (Ptot <= 0 && !lw.k0_on)) return hipSuccess;
+    const int ngroup = Ptot > 0 ? lane_grid(n, Ptot).ngroup : 0;
+    k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partE.as<double>(), ngroup, n, 1,
+                                           lw.perm.as<int32_t>(),
+                                           lw.k0_on ? lw.k0out.as<double2>() : nullptr,
+                                           accumulate, 1.0, d_phi);
     LR_CHECK(hipGetLastError());
-    LR_CHECK(hipMemcpyAsync(h_q0, lw.q0part.p, sizeof(double2) * (size_t)Ptot,
-                            hipMemcpyDeviceToHost, s));
+    if (Ptot > 0)
+        LR_CHECK(hipMemcpyAsync(h_q0, lw.q0part.p, sizeof(double2) * (size_t)Ptot,
+                                hipMemcpyDeviceToHost, s));
     return hipSuccess;
 }
 
@@ -1204,9 +1482,10 @@ hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int6
                               std::string &err)
 {
     h_dgamma[0] = h_dgamma[1] = 0.0;
-    if (n <= 0 || Ptot <= 0) return hipSuccess;
-    std::vector<double> h(2 * (size_t)Ptot);
-    hipError_t e = launch_long_energy_passes(c, lw, n, Ptot, s, err);
+    if (n <= 0 || (Ptot <= 0 && !lw.k0_on)) return hipSuccess;
+    std::vector<double> h(2 * (size_t)std::max<int64_t>(Ptot, 0));
+    hipError_t e = lw.k0_on ? launch_long_k0(c, lw, n, s, err) : hipSuccess;
+    if (e == hipSuccess) e = launch_long_energy_passes(c, lw, n, Ptot, s, err);
     if (e == hipSuccess)
         e = launch_long_energy_sum(lw, n, Ptot, d_phi, accumulate, h.data(), s, err);
     if (e != hipSuccess) return e;
@@ -1225,7 +1504,8 @@ hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
     k_long_Rw<<<nblk(n, 256), 256, 0, s>>>(lw.partB.as<double>(), lg.ngroup, n, 3,
-                                           lw.perm.as<int32_t>(), accumulate, scale, d_out);
+                                           lw.perm.as<int32_t>(), nullptr, accumulate, scale,
+                                           d_out);
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }

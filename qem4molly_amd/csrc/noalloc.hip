@@ -5,7 +5,8 @@
 //
 // Its work buffers -- the staged k rows, the z sort (keys, values,
 // histograms, counters), the z-ordered atoms, the per-k constants, the chunk
-// sums and offsets, the per-group partial forces -- are carved from one
+// sums and offsets, the per-group partial forces, the k = 0 term's tile sums
+// and per-atom values -- are carved from one
 // file-scope device array, `scratch`, of QEM_NOALLOC_SCRATCH_BYTES.  The
 // array exists once per GPU, from the time the library's code object is
 // loaded on it until the process ends, and every call on that GPU shares it:
@@ -60,9 +61,7 @@ struct KIdxChunk {
 struct KSetGeom {
     double Lx, Ly;
     double wrow;   // batch weight / (Lx Ly eps_0), as the host's staged rows
-    double w0;     // the k = 0 row's 1 / (2 Lx Ly eps_0)
     int32_t mxm, mym;
-    int32_t k0_row;  // write the k = 0 row at this position (-1: not in this launch)
 };
 
 __global__ void k_rows_from_idx(KIdxChunk ch, KSetGeom g, double4 *__restrict__ kv)
@@ -80,7 +79,6 @@ __global__ void k_rows_from_idx(KIdxChunk ch, KSetGeom g, double4 *__restrict__ 
         const double k = sqrt(kx * kx + ky * ky);
         kv[ch.off + t] = make_double4(kx, ky, k, g.wrow);
     }
-    if (t == 0 && g.k0_row >= 0) kv[g.k0_row] = make_double4(0.0, 0.0, 0.0, g.w0);
 }
 
 inline size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -126,7 +124,7 @@ extern "C" int qem_long_forces_noalloc_dev(qem_ctx *ctx, int64_t n, const double
         return fail(ctx, QEM_ERR_HIP, "hipGetSymbolAddress(scratch)");
 
     // carve the scratch: the sort's counters first, at a fixed offset
-    const int64_t Ptot = P + 1;   // the sampled rows, then the k = 0 row
+    const int64_t Ptot = P;   // the sampled rows (the k = 0 term has no row)
     const LongSizes z = long_force_sizes(n, Ptot);
     char *b = static_cast<char *>(base);
     size_t off = 0;
@@ -145,6 +143,9 @@ extern "C" int qem_long_forces_noalloc_dev(qem_ctx *ctx, int64_t n, const double
     take(lw.offA, z.partA);
     take(lw.totA, z.totA);
     take(lw.partB, z.partB);
+    take(lw.k0tile, z.k0tile);
+    take(lw.k0out, z.k0out);
+    lw.k0_on = true;
     if (off > kScratchBytes)
         return fail(ctx, QEM_ERR_CAPACITY, "qem_long_forces_noalloc_dev: scratch too small");
 
@@ -153,25 +154,22 @@ extern "C" int qem_long_forces_noalloc_dev(qem_ctx *ctx, int64_t n, const double
     if (radix_sort_init(lw.tmp.p, s) != hipSuccess)
         return fail(ctx, QEM_ERR_HIP, "radix_sort_init");
 
-    // k rows from the indices, then the k = 0 row
+    // k rows from the indices
     const qem_params &p = ctx->p;
     KSetGeom g;
     g.Lx = p.Lx;
     g.Ly = p.Ly;
     const double inv_ae = 1.0 / (p.Lx * p.Ly * p.eps_0);
     g.wrow = (P > 0 ? ctx->sum_K / (double)P : 0.0) * inv_ae;
-    g.w0 = 1.0 / (2.0 * p.Lx * p.Ly * p.eps_0);
     g.mxm = (int32_t)ctx->kset_mxm;
     g.mym = (int32_t)ctx->kset_mym;
     double4 *kv = lw.kv.as<double4>();
-    for (int64_t o = 0; o < P || o == 0; o += kIdxPerLaunch) {
+    for (int64_t o = 0; o < P; o += kIdxPerLaunch) {
         KIdxChunk ch;
         ch.off = (int32_t)o;
         ch.count = (int32_t)std::min<int64_t>(kIdxPerLaunch, P - o);
         for (int t = 0; t < ch.count; ++t) ch.idx[t] = (int32_t)k_idx[o + t];
-        g.k0_row = o + kIdxPerLaunch >= P ? (int32_t)P : -1;
         k_rows_from_idx<<<1, kIdxPerLaunch, 0, s>>>(ch, g, kv);
-        if (g.k0_row >= 0) break;
     }
     if (hipGetLastError() != hipSuccess) return fail(ctx, QEM_ERR_HIP, "k rows");
 
@@ -188,6 +186,7 @@ extern "C" int qem_long_forces_noalloc_dev(qem_ctx *ctx, int64_t n, const double
     // the same launches as qem_long_forces_dev; every buffer above already
     // has the capacity these ask for, so nothing is allocated
     if (launch_zsort(lw, n, d_xyz, d_q, s, ctx->err) != hipSuccess ||
+        launch_long_k0(c, lw, n, s, ctx->err) != hipSuccess ||
         launch_long_krec(c, lw, Ptot, s, ctx->err) != hipSuccess ||
         launch_long_passes(c, lw, n, Ptot, s, ctx->err, false) != hipSuccess ||
         launch_long_sum(lw, n, Ptot, d_f, 0, s, ctx->err) != hipSuccess)

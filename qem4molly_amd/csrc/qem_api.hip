@@ -580,13 +580,16 @@ int upload(qem_ctx *ctx, DevBuf &b, const void *h, size_t bytes)
     return QEM_OK;
 }
 
-// Stages P k rows (plus the k = 0 row when requested) in the pinned buffer
-// and copies them to lw.kv: (k_x, k_y, |k|, weight / (Lx Ly eps_0)).
+// Stages P k rows in the pinned buffer and copies them to lw.kv: (k_x, k_y,
+// |k|, weight / (Lx Ly eps_0)); with_k0 marks the batch as carrying the
+// k = 0 term (lw.k0_on), which has no row: launch_long_k0 computes it and
+// the sums add it.
 int stage_kvecs(qem_ctx *ctx, const double *kvecs, int64_t P, double weight, bool with_k0,
                 hipStream_t s, int64_t *Ptot)
 {
     const qem_params &p = ctx->p;
-    const int64_t rows = P + (with_k0 ? 1 : 0);
+    const int64_t rows = P;
+    ctx->lw.k0_on = with_k0;
     const size_t bytes = sizeof(double) * 4 * (size_t)(rows > 0 ? rows : 1);
     LongWork &lw = ctx->lw;
     // ring of pinned staging slots; a slot is rewritten only after the copy
@@ -614,14 +617,8 @@ int stage_kvecs(qem_ctx *ctx, const double *kvecs, int64_t P, double weight, boo
         h[4 * t + 2] = k;
         h[4 * t + 3] = wk * inv_ae;
     }
-    if (with_k0) {
-        h[4 * P] = 0.0;
-        h[4 * P + 1] = 0.0;
-        h[4 * P + 2] = 0.0;
-        h[4 * P + 3] = 1.0 / (2.0 * p.Lx * p.Ly * p.eps_0);
-    }
     API_HIP(ctx, ensure(lw.kv, bytes));
-    API_HIP(ctx, hipMemcpyAsync(lw.kv.p, h, bytes, hipMemcpyHostToDevice, s));
+    if (rows > 0) API_HIP(ctx, hipMemcpyAsync(lw.kv.p, h, bytes, hipMemcpyHostToDevice, s));
     API_HIP(ctx, hipEventRecord(lw.kv_ev[slot], s));
     *Ptot = rows;
     return QEM_OK;
@@ -782,8 +779,8 @@ constexpr int64_t kLongBatch = 8192;
 LongConsts long_consts(const qem_ctx *ctx, double weight, uint32_t parts);
 
 // The long range for at most kLongBatch k vectors, first part: the k rows
-// staged (*Ptot receives their number: P, plus the k = 0 row when parts has
-// it), the z sort and the per-k constants, on stream s.
+// staged (*Ptot receives their number, P), the z sort, the k = 0 term when
+// parts has it, and the per-k constants, on stream s.
 int long_prep(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
               const double *kvecs, int64_t P, double weight, uint32_t parts, hipStream_t s,
               bool sorted, int64_t *Ptot)
@@ -797,19 +794,22 @@ int long_prep(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         hipError_t e0 = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e0 != hipSuccess) return hip_err(ctx, e0, "z sort");
     }
-    hipError_t e = launch_long_krec(long_consts(ctx, weight, parts), ctx->lw, *Ptot, s, ctx->err);
+    const LongConsts c = long_consts(ctx, weight, parts);
+    hipError_t e = ctx->lw.k0_on ? launch_long_k0(c, ctx->lw, n, s, ctx->err) : hipSuccess;
+    if (e == hipSuccess) e = launch_long_krec(c, ctx->lw, *Ptot, s, ctx->err);
     if (e != hipSuccess) return hip_err(ctx, e, "long range");
     return QEM_OK;
 }
 
-// Second part: the passes up to the per-group partial forces (lw.partB).
+// Second part: the passes up to the per-group partial forces (lw.partB), or
+// (which) only their chunk sums and offsets, or only the forces from those.
 int long_lanes(qem_ctx *ctx, int64_t n, int64_t Ptot, double weight, uint32_t parts,
-               hipStream_t s)
+               hipStream_t s, int which = kLongPassAll)
 {
     if (n == 0) return QEM_OK;
     PhaseScope ps_long(ctx, QEM_PHASE_LONG, s);
     const LongConsts c = long_consts(ctx, weight, parts);
-    hipError_t e = launch_long_passes(c, ctx->lw, n, Ptot, s, ctx->err, false);
+    hipError_t e = launch_long_passes(c, ctx->lw, n, Ptot, s, ctx->err, false, which);
     if (e != hipSuccess) return hip_err(ctx, e, "long range");
     return QEM_OK;
 }
@@ -831,7 +831,7 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
 {
     if (n == 0) return QEM_OK;
     if (P > kLongBatch) {
-        // whole batches first, the k = 0 row with the last one; one z sort
+        // whole batches first, the k = 0 term with the last one; one z sort
         for (int64_t off = 0; off < P; off += kLongBatch) {
             const int64_t Pb = std::min(kLongBatch, P - off);
             const bool last = off + Pb >= P;
@@ -851,9 +851,11 @@ int long_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
 
 // QEM_short and QEM_long of one force call on stream s (addz as in
 // short_dev).  In mixed precision the long range runs on the side stream
-// alongside the short range: the k staging and the z sort (latency-bound,
-// few blocks) beside the pair kernel, the k-lane passes (VALU) once the pair
-// kernel is done, beside the gather (HBM).  Their sum into d_f follows the
+// alongside the short range: the k staging, the z sort (latency-bound, few
+// blocks), the k = 0 term and the per-k constants beside the pair kernel,
+// the k-lane passes (VALU) once the pair kernel is done, beside the gather
+// (HBM); beside the pair kernel, which is VALU-bound, the passes slow it
+// by more than their own time.  Their sum into d_f follows the
 // short range on s, so d_f holds short + long in that order, as when both
 // run on s.  The FP64 pair kernel keeps its 50 KB Bessel table in LDS,
 // three blocks per CU; a long-range block beside them (25 KB of LDS for its
@@ -974,6 +976,7 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         if (lst == QEM_OK) {
             PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
             hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, ctx->side, ctx->err);
+            if (e == hipSuccess) e = launch_long_k0(lc, ctx->lw, n, ctx->side, ctx->err);
             if (e != hipSuccess) lst = hip_err(ctx, e, "z sort");
         }
     }
@@ -1020,7 +1023,7 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "z sort");
         // passes of at most kLongBatch k rows, so the per-row buffers stay
-        // bounded; the k = 0 row rides with the last one
+        // bounded; the k = 0 term comes with the last one
         for (int64_t off = 0;; off += kLongBatch) {
             const int64_t Pb = std::min(kLongBatch, P - off);
             const bool last = off + Pb >= P;
@@ -1165,7 +1168,8 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
         API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
         phi = ctx->d_f2.as<double>();
     }
-    // side stream: k rows, z sort, per-k constants beside the pair kernel
+    // side stream: k rows, z sort, the k = 0 term, per-k constants beside
+    // the pair kernel
     API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
     API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
     int64_t Ptot = 0;
@@ -1451,7 +1455,7 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
                       &ctx->nl.atom_idx_sorted, &ctx->nl.cell_start, &ctx->nl.cell_end,
                       &ctx->nl.counts, &ctx->nl.row_ptr, &ctx->nl.tmp,
                       &ctx->lw.perm, &ctx->lw.tmp,
-                      &ctx->lw.satom,
+                      &ctx->lw.satom, &ctx->lw.k0tile, &ctx->lw.k0out,
                       &ctx->lw.kr, &ctx->lw.kv,
                       &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,
                       &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,

@@ -263,6 +263,9 @@ struct LongWork {
     DevBuf partE;            // [ntile][n] (energy: the potential of each atom)
     DevBuf svx, svy, svz;    // tangent directions in z order
     DevBuf partT, offT, totT;  // tangent chunk sums / offsets / totals
+    DevBuf k0tile;           // k = 0 term: tile sums of (q, q z), then their prefix
+    DevBuf k0out;            // [n] (F0_z, phi0) of each atom in z order
+    bool k0_on = false;      // the staged batch carries the k = 0 term
     static constexpr int kSlots = 4;   // pinned staging ring for k vectors
     void *h_kv[kSlots] = {nullptr, nullptr, nullptr, nullptr};
     size_t h_kv_cap[kSlots] = {0, 0, 0, 0};
@@ -270,15 +273,21 @@ struct LongWork {
     int kv_slot = 0;
 };
 // Bytes each LongWork buffer needs for a force call over n atoms and Ptot k
-// rows (the k = 0 row included): lw.kv, lw.kr, lw.perm, lw.satom, lw.tmp
-// (the z sort), lw.partA and lw.offA (each), lw.totA, lw.partB.
+// rows (k != 0): lw.kv, lw.kr, lw.perm, lw.satom, lw.tmp (the z sort),
+// lw.partA and lw.offA (each), lw.totA, lw.partB, and the k = 0 term's
+// lw.k0tile and lw.k0out.
 struct LongSizes {
-    size_t kv, kr, perm, satom, sort, partA, totA, partB;
+    size_t kv, kr, perm, satom, sort, partA, totA, partB, k0tile, k0out;
 };
 LongSizes long_force_sizes(int64_t n, int64_t Ptot);
 // z-sort; fills lw.perm (int32, rank -> atom) and lw.satom (atoms in z order).
 hipError_t launch_zsort(LongWork &lw, int64_t n, const double *d_xyz,
                         const double *d_q, hipStream_t s, std::string &err);
+// The k = 0 term of the atoms in lw.satom: lw.k0out[r] = (F0_z, phi0) of
+// the atom of rank r (force and potential, weight 1 / (2 Lx Ly eps_0)).  The
+// sums (launch_long_sum*, launch_long_energy_sum) add it when lw.k0_on.
+hipError_t launch_long_k0(const LongConsts &c, LongWork &lw, int64_t n, hipStream_t s,
+                          std::string &err);
 // Full long-range force for the k vectors already in lw.kv (P rows).
 // accumulate != 0 adds into d_f instead of overwriting.
 hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n,
@@ -291,8 +300,12 @@ hipError_t launch_long(const LongConsts &c, LongWork &lw, int64_t n,
 // the passes are done.
 hipError_t launch_long_krec(const LongConsts &c, LongWork &lw, int64_t P, hipStream_t s,
                             std::string &err);
+// which: the chunk sums and offsets (passes A and S, kLongPassAS), the
+// per-group partial forces from them (pass B, kLongPassB), or both.
+constexpr int kLongPassAS = 1, kLongPassB = 2, kLongPassAll = 3;
 hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int64_t P,
-                              hipStream_t s, std::string &err, bool krec = true);
+                              hipStream_t s, std::string &err, bool krec = true,
+                              int which = kLongPassAll);
 // The sum into d_f fused with the short range's output pass: d_f[a] =
 // sout[slot_of[a]] (+ addz[a] on z) + the long range of atom a.
 hipError_t launch_long_sum_slots(LongWork &lw, int64_t n, int64_t P, const int32_t *slot_of,
@@ -302,7 +315,7 @@ hipError_t launch_long_sum(LongWork &lw, int64_t n, int64_t P, double *d_f, int 
                            hipStream_t s, std::string &err);
 
 // Batch variance of the random-batch long-range force for the k rows in
-// lw.kv (P sampled rows, plus the k = 0 row if staged, which is left out):
+// lw.kv (P sampled rows; the k = 0 term is not sampled and is left out):
 // d_var[atom] = (P sum_p |X_p|^2 - |sum_p X_p|^2) / (P - 1), X_p the row's
 // weighted force on the atom.  P >= 2.
 hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,

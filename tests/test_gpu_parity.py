@@ -201,6 +201,35 @@ def test_long_forces_scaled_n10k_fp64():
     assert rel_rms(got, ref) < FP64_TOL
 
 
+@pytest.mark.parametrize("P", [10, 30, 64, 90, 150])
+@pytest.mark.parametrize("n", [300, 2500])
+def test_long_forces_row_groups_fp64(n, P):
+    """k batches whose last group of 64 rows is full (64), holds 17-32 rows (two
+    chunks per wave) or at most 16 (four chunks per wave), one group among
+    several or alone; at n = 300 the chunks are walked a second time from the
+    top, at n = 2500 most are flat."""
+    cfg = scaled_config(CONFIGS["slab_n10k"], n)
+    xyz, q = make_system(cfg, seed=3)
+    with ctx_for(cfg) as c:
+        K, Pk, sK = c.kset()
+        kidx = c.sample_k(11, P)
+        got = c.long_forces(xyz, q, kidx)
+    ref = O.long_forces(oracle_params(cfg), xyz, q, K[kidx], weight=sK / P, nthreads=16)
+    assert rel_rms(got, ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("P", [30, 90, 150])
+def test_long_forces_row_groups_mixed(P):
+    cfg = scaled_config(CONFIGS["slab_n100k"], 3000)
+    xyz, q = make_system(cfg, seed=4)
+    with ctx_for(cfg, precision="mixed") as c:
+        K, Pk, sK = c.kset()
+        kidx = c.sample_k(12, P)
+        got = c.long_forces(xyz, q, kidx)
+    ref = O.long_forces(oracle_params(cfg), xyz, q, K[kidx], weight=sK / P, nthreads=16)
+    assert rel_rms(got, ref) < MIXED_TOL
+
+
 # ---- both halves, mixed precision ---------------------------------------------
 
 @pytest.mark.parametrize("name,n", [("slab_n100", 100), ("slab_n10k", 1500),
