@@ -542,8 +542,27 @@ __global__ __launch_bounds__(256) void k_gather_pairs(
             fy += (double)v.y;
             fz += (double)v.z;
         }
+        // j side: four records in flight per lane (their indices first), added
+        // in order
         const int32_t u1 = j_end[a];
-        for (int32_t u = j_start[a] + l; u < u1; u += 16) {
+        int32_t u = j_start[a] + l;
+        for (; u + 48 < u1; u += 64) {
+            const int32_t p0 = j_idx[u], p1 = j_idx[u + 16], p2 = j_idx[u + 32], p3 = j_idx[u + 48];
+            const Rec v0 = fpair[p0], v1 = fpair[p1], v2 = fpair[p2], v3 = fpair[p3];
+            fx -= (double)v0.x;
+            fy -= (double)v0.y;
+            fz += (double)v0.w;
+            fx -= (double)v1.x;
+            fy -= (double)v1.y;
+            fz += (double)v1.w;
+            fx -= (double)v2.x;
+            fy -= (double)v2.y;
+            fz += (double)v2.w;
+            fx -= (double)v3.x;
+            fy -= (double)v3.y;
+            fz += (double)v3.w;
+        }
+        for (; u < u1; u += 16) {
             const Rec v = fpair[j_idx[u]];
             fx -= (double)v.x;
             fy -= (double)v.y;
