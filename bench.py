@@ -254,14 +254,20 @@ def adjoint_cost(name, precision, reps=10):
     ctx = L.Context(_params(cfg, precision, semantics=7, iter_period=1000000))
     kidx = ctx.sample_k(1, cfg.P)
 
-    def timed(fn):
+    def timed(fn, blocks=3):
+        # the fastest of three blocks of reps calls (each block's mean), so a
+        # stray host delay in one block does not move the ratios
         fn()
         L.synchronize()
-        t = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        L.synchronize()
-        return (time.perf_counter() - t) / reps * 1e3
+        best = None
+        for _ in range(blocks):
+            t = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            L.synchronize()
+            ms = (time.perf_counter() - t) / reps * 1e3
+            best = ms if best is None else min(best, ms)
+        return best
 
     t_f = timed(lambda: ctx.forces_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr))
     t_h = timed(lambda: ctx.hvp_dev(cfg.n, xd.ptr, qd.ptr, kidx, vd.ptr, hd.ptr))

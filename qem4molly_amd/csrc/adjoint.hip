@@ -235,12 +235,12 @@ __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a,
                                 : __builtin_amdgcn_exp2f(fmaf(e, rP, u.w));
         const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
         const float t = num * rD;
-        const float wq = (E * rD) * u.x;
+        const float tu = t * u.x, tq = tu * (E * rD);   // u.x t, u.x (E/D) t
         if (self) {  // J0 = 1, J1 = 0
             S0 += t;
-            U0 = fmaf(u.x, t, U0);
-            Q0 = fmaf(wq, t, Q0);
-            if (gauss) R0 = fmaf(u.y, t, R0);
+            U0 += tu;
+            Q0 += tq;
+            if (gauss) R0 = fmaf(u.x, tu, R0);
             continue;
         }
         const float xb = hR * u.x;
@@ -250,16 +250,17 @@ __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a,
         float dp = cc.w;
         dp = fmaf(dp, tt, p); p = fmaf(p, tt, cc.y);
         dp = fmaf(dp, tt, p); p = fmaf(p, tt, cc.x);
-        const float A0 = t * p, A1 = t * dp;   // t J0, -t J1 dx
-        S0 += A0;
-        S1 += A1;
-        U0 = fmaf(u.x, A0, U0);
-        U1 = fmaf(u.x, A1, U1);
-        Q0 = fmaf(wq, A0, Q0);
-        Q1 = fmaf(wq, A1, Q1);
+        // p = J0, dp = -J1 dx
+        S0 = fmaf(t, p, S0);
+        S1 = fmaf(t, dp, S1);
+        U0 = fmaf(tu, p, U0);
+        U1 = fmaf(tu, dp, U1);
+        Q0 = fmaf(tq, p, Q0);
+        Q1 = fmaf(tq, dp, Q1);
         if (gauss) {
-            R0 = fmaf(u.y, A0, R0);
-            R1 = fmaf(u.y, A1, R1);
+            const float tr = tu * u.x;   // u.x^2 t
+            R0 = fmaf(tr, p, R0);
+            R1 = fmaf(tr, dp, R1);
         }
     }
     const float A = gauss ? -r * a - da : -r * (a + Lz2) - da;
@@ -549,8 +550,29 @@ __global__ __launch_bounds__(256) void k_gather_energy(
             s0 += qb * (double)u.y;
             s1 += qb * (double)u.z;
         }
+        // j side: four pairs in flight per lane (indices, then records and
+        // charges), added in order
         const int32_t u1 = j_end[a];
-        for (int32_t t = j_start[a] + l; t < u1; t += 16) {
+        int32_t t = j_start[a] + l;
+        for (; t + 48 < u1; t += 64) {
+            int32_t p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) p[k] = j_idx[t + 16 * k];
+            Rec u[4];
+            double qb[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                u[k] = pot[p[k]];
+                qb[k] = q[pi[p[k]]];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                f += qb[k] * (double)u[k].x;
+                s0 += qb[k] * (double)u[k].y;
+                s1 += qb[k] * (double)u[k].z;
+            }
+        }
+        for (; t < u1; t += 16) {
             const int32_t p = j_idx[t];
             const Rec u = pot[p];
             const double qb = q[pi[p]];
@@ -592,8 +614,24 @@ __global__ __launch_bounds__(256) void k_gather_energy_w(
             s0 += (double)u.y;
             s1 += (double)u.z;
         }
+        // j side: four records in flight per lane, added in order
         const int32_t u1 = j_end[a];
-        for (int32_t t = j_start[a] + l; t < u1; t += 16) {
+        int32_t t = j_start[a] + l;
+        for (; t + 48 < u1; t += 64) {
+            int32_t p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) p[k] = j_idx[t + 16 * k];
+            float4 u[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) u[k] = wj[p[k]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                f += (double)u[k].x;
+                s0 += (double)u[k].y;
+                s1 += (double)u[k].z;
+            }
+        }
+        for (; t < u1; t += 16) {
             const float4 u = wj[j_idx[t]];
             f += (double)u.x;
             s0 += (double)u.y;
@@ -651,8 +689,24 @@ __global__ __launch_bounds__(256) void k_gather_tangent(
             fy += (double)d.y;
             fz += (double)d.z;
         }
+        // j side: four records in flight per lane, added in order
         const int32_t u1 = j_end[a];
-        for (int32_t t = j_start[a] + l; t < u1; t += 16) {
+        int32_t t = j_start[a] + l;
+        for (; t + 48 < u1; t += 64) {
+            int32_t p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) p[k] = j_idx[t + 16 * k];
+            Rec d[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d[k] = tp[p[k]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                fx -= (double)d[k].x;
+                fy -= (double)d[k].y;
+                fz += (double)d[k].w;
+            }
+        }
+        for (; t < u1; t += 16) {
             const Rec d = tp[j_idx[t]];
             fx -= (double)d.x;
             fy -= (double)d.y;
