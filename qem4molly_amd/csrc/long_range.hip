@@ -260,7 +260,7 @@ constexpr int kLaneWaves = 4;   // waves (= chunks) per block in the k-lane pass
 // from moving them across a flush.  Four atoms per flush keep the LDS at
 // 25 KB per block of four waves, which the VGPR budget, not the LDS, then
 // limits to five waves per SIMD.
-constexpr int kRedBatch = 4;
+constexpr int kRedBatch = 4;   // a power of two (slots are step & (kRedBatch - 1))
 constexpr int kRedPitch = 65;
 
 struct LaneReducer {
@@ -302,9 +302,12 @@ struct LaneReducer {
 };
 
 // The last group of k rows holds Ptot mod 64 rows; when that is at most 32
-// (or 16), its waves take 32 rows x 2 consecutive chunks (16 x 4) instead
-// of 64 rows x 1 chunk, so that no lane idles: SPLIT sub-chunks per wave,
-// lane = sub * (64 / SPLIT) + row.  The other groups use SPLIT = 1.
+// (or 16), pass B's waves there take 32 rows x 2 consecutive chunks
+// (16 x 4) instead of 64 rows x 1 chunk, so that no lane idles: SPLIT
+// sub-chunks per wave, lane = sub * (64 / SPLIT) + row.  The other groups,
+// and pass A everywhere, use SPLIT = 1: pass A's short loop does not hide
+// the per-lane atom loads of a split wave (measured: 58 us without the
+// split, 61 with it, at N = 100k), pass B's does (125 -> 114 us).
 __host__ __device__ constexpr int lane_split_of(int64_t rem)
 {
     return rem == 0 ? 1 : rem <= 16 ? 4 : rem <= 32 ? 2 : 1;
@@ -384,54 +387,79 @@ __device__ __forceinline__ void lane_flush(const LaneReducer &red, int nb, int l
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The lane sums of LaneReducer for one value per atom: up to kRedBatch1
+// atoms per flush, one row of 64 values per atom, summed the same way (four
+// lanes per row, 16 values each in order, then the quad).
+constexpr int kRedBatch1 = 12;
+
+struct LaneReducer1 {
+    double *buf;   // this wave's [kRedBatch1][kRedPitch]
+
+    __device__ __forceinline__ void put(int slot, int lane, double v) const
+    {
+        buf[slot * kRedPitch + lane] = v;
+    }
+
+    // out[base + j] (=, or += with add) for the first nb slots
+    __device__ __forceinline__ void flush(int nb, int lane, double *__restrict__ out,
+                                          int64_t base, bool add) const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int r = lane >> 2, q = lane & 3;
+        const bool live = r < nb;
+        double sum = 0.0;
+        if (live) {
+            const double *v = buf + r * kRedPitch + 16 * q;
+            sum = v[0];
+#pragma unroll
+            for (int m = 1; m < 16; ++m) sum += v[m];
+        }
+        sum += dev::dpp_perm_f64<0xB1>(sum);   // quad_perm [1,0,3,2]
+        sum += dev::dpp_perm_f64<0x4E>(sum);   // quad_perm [2,3,0,1]
+        if (live && q == 0) {
+            double *o = out + base + r;
+            *o = add ? *o + sum : sum;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+static_assert(kRedBatch1 * kRedPitch <= kRedBatch * 3 * kRedPitch, "LDS of the lane sums");
+
 // Pass A (k lanes): per (k row, chunk) the four sums q cos e^{+kz},
 // q sin e^{+kz}, q cos e^{-kz}, q sin e^{-kz}, accumulated in ascending z.
-template <int PREC, int SPLIT>
-__device__ __forceinline__ void lane_A_body(const KRec *__restrict__ kr, int64_t Ptot,
-                                            const double4 *__restrict__ satom, int64_t n,
-                                            int chk, int nchunk, double4 *__restrict__ partA,
-                                            int wv, int lane)
-{
-    const LaneSite<SPLIT> w(wv, lane, n, chk, nchunk);
-    if (w.wfc >= nchunk) return;
-    const bool live = w.p < Ptot && w.chunk < nchunk;
-    const LaneK L = load_lane(kr[w.p < Ptot ? w.p : Ptot - 1], PREC == QEM_PREC_MIXED);
-    const int64_t len = w.a1 - w.a0;
-    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
-    // the next atom's record is loaded one iteration ahead (a uniform,
-    // scalar load for SPLIT = 1), so it is in flight while this atom's
-    // terms are computed
-    double4 nx = satom[w.at(0, n)];
-    for (int64_t t = 0; t < w.lmax; ++t) {
-        const double4 at = nx;
-        if (t + 1 < w.lmax) nx = satom[w.at(t + 1, n)];
-        if (SPLIT == 1 || t < len) {
-            double c, s, Ep, Em;
-            lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
-            // the same rounded values and adds as pass B's running sums
-            sx = add_nc(sx, at.w * (c * Ep));
-            sy = add_nc(sy, at.w * (s * Ep));
-            sz = add_nc(sz, at.w * (c * Em));
-            sw = add_nc(sw, at.w * (s * Em));
-        }
-    }
-    if (live) partA[w.p * nchunk + w.chunk] = make_double4(sx, sy, sz, sw);
-}
-
 template <int PREC>
 __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_A(const KRec *__restrict__ kr,
                                                              int64_t Ptot,
                                                              const double4 *__restrict__ satom,
                                                              int64_t n, int chk, int nchunk,
-                                                             int split_last,
                                                              double4 *__restrict__ partA)
 {
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int split = blockIdx.y + 1 == gridDim.y ? split_last : 1;
-    if (split == 4) lane_A_body<PREC, 4>(kr, Ptot, satom, n, chk, nchunk, partA, wv, lane);
-    else if (split == 2) lane_A_body<PREC, 2>(kr, Ptot, satom, n, chk, nchunk, partA, wv, lane);
-    else lane_A_body<PREC, 1>(kr, Ptot, satom, n, chk, nchunk, partA, wv, lane);
+    const LaneSite<1> w(wv, lane, n, chk, nchunk);
+    if (w.wfc >= nchunk) return;
+    const bool live = w.p < Ptot;
+    const LaneK L = load_lane(kr[live ? w.p : Ptot - 1], PREC == QEM_PREC_MIXED);
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
+    // the next atom's (uniform, scalar) record is loaded one iteration
+    // ahead, so it is in flight while this atom's terms are computed
+    double4 nx = satom[w.a0];
+    for (int64_t a = w.a0; a < w.a1; ++a) {
+        const double4 at = nx;
+        if (a + 1 < w.a1) nx = satom[a + 1];
+        double c, s, Ep, Em;
+        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+        // the same rounded values and adds as pass B's running sums
+        sx = add_nc(sx, at.w * (c * Ep));
+        sy = add_nc(sy, at.w * (s * Ep));
+        sz = add_nc(sz, at.w * (c * Em));
+        sw = add_nc(sw, at.w * (s * Em));
+    }
+    if (live) partA[w.p * nchunk + w.chunk] = make_double4(sx, sy, sz, sw);
 }
 
 // Pass B (k lanes): forces of one chunk from one group of 64 k rows into
@@ -526,7 +554,7 @@ __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t
                 A += vx;
                 B += vy;
             }
-            const int slot = (int)(t % kRedBatch);
+            const int slot = (int)t & (kRedBatch - 1);
             red.put(slot, lane, fx, fy, fz);
             if (slot == kRedBatch - 1 || t + 1 == w.lmax)
                 lane_flush<SPLIT>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk, t - slot,
@@ -560,7 +588,7 @@ __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t
             C += q * cm;
             D += q * sm;
         }
-        const int slot = (int)(t % kRedBatch);
+        const int slot = (int)t & (kRedBatch - 1);
         red.put(slot, lane, fx, fy, fz);
         if (slot == kRedBatch - 1 || t + 1 == w.lmax)
             lane_flush<SPLIT>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk, t - slot, true,
@@ -778,7 +806,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
     const double *__restrict__ svz, int64_t n, int chk, int nchunk,
     const double4 *__restrict__ partA, const double4 *__restrict__ offA,
     const double4 *__restrict__ totA, const double4 *__restrict__ partT,
-    const double4 *__restrict__ offT, const double4 *__restrict__ totT,
+    const double4 *__restrict__ offT, const double4 *__restrict__ totT, int own_term,
     double *__restrict__ partB)
 {
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -789,11 +817,17 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
     const bool live = p < Ptot;
     const int64_t pl = live ? p : Ptot - 1;
     const KRec &R = kr[pl];
-    const LaneK L = load_lane(R, PREC == QEM_PREC_MIXED);
+    LaneK L = load_lane(R, PREC == QEM_PREC_MIXED);
     const double kx = R.kx, ky = R.ky, kabs = R.k;
-    // idle lanes add nothing
-    const bool act = live;
+    if (!live) L.w = L.wx = L.wy = 0.0;   // a lane past the last row adds zeros
     const double4 T = totA[pl], dT = totT[pl];
+    // the non-sort term with the atom's own term kept, as in k_lane_B: per
+    // row the constants from the totals and from their tangents
+    const double u1 = fma(L.N1, T.y, L.N2 * T.w), u2 = fma(L.N1, T.x, L.N2 * T.z);
+    const double v1 = fma(L.N2, T.y, L.N3 * T.w), v2 = fma(L.N2, T.x, L.N3 * T.z);
+    const double du1 = fma(L.N1, dT.y, L.N2 * dT.w), du2 = fma(L.N1, dT.x, L.N2 * dT.z);
+    const double dv1 = fma(L.N2, dT.y, L.N3 * dT.w), dv2 = fma(L.N2, dT.x, L.N3 * dT.z);
+    const double E1 = L.S1 - L.N1, E2 = L.S2 - L.N3;
     const double4 off = offA[pl * nchunk + chunk], offt = offT[pl * nchunk + chunk];
     const double4 own = partA[pl * nchunk + chunk], ownt = partT[pl * nchunk + chunk];
     const int64_t a0 = chunk * chk, a1 = min(n, a0 + chk);
@@ -834,39 +868,36 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
         }
         const double c = X.c, s = X.s, Ep = X.Ep, Em = X.Em;
         const double dth = kx * dv.x + ky * dv.y, kvz = kabs * dv.z;
-        const double dc = -s * dth, ds = c * dth, dEp = kvz * Ep, dEm = -kvz * Em;
-        // Fl_non_sort (src/Force_long.jl:5-70), own term removed
-        const double f1 = T.x - vx, f2 = T.y - vy, f3 = T.z - vz, f4 = T.w - vw;
-        const double df1 = dT.x - tvx, df2 = dT.y - tvy, df3 = dT.z - tvz, df4 = dT.w - tvw;
-        const double X1 = c * f2 - s * f1, X2 = c * f4 - s * f3;
-        const double Z1 = c * f1 + s * f2, Z2 = c * f3 + s * f4;
-        const double dX1 = dc * f2 + c * df2 - ds * f1 - s * df1;
-        const double dX2 = dc * f4 + c * df4 - ds * f3 - s * df3;
-        const double dZ1 = dc * f1 + c * df1 + ds * f2 + s * df2;
-        const double dZ2 = dc * f3 + c * df3 + ds * f4 + s * df4;
-        const double gu = L.N1 * Ep + L.N2 * Em, gw = L.N3 * Em + L.N2 * Ep;
-        const double guz = L.N1 * Ep - L.N2 * Em, gwz = L.N2 * Ep - L.N3 * Em;
-        const double dgu = L.N1 * dEp + L.N2 * dEm, dgw = L.N3 * dEm + L.N2 * dEp;
-        const double dguz = L.N1 * dEp - L.N2 * dEm, dgwz = L.N2 * dEp - L.N3 * dEm;
-        const double dgxn = dgu * X1 + gu * dX1 + dgw * X2 + gw * dX2;
-        const double dgzn = dguz * Z1 + guz * dZ1 + dgwz * Z2 + gwz * dZ2;
+        // Fl_non_sort (src/Force_long.jl:5-70), own term included: the
+        // force is Ep (c u1 - s u2) + Em (c v1 - s v2) in xy and
+        // Ep (c u2 + s u1) - Em (c v2 + s v1) in z; along v, dc = -s dth,
+        // ds = c dth, dEp = kvz Ep, dEm = -kvz Em and the per-row constants
+        // move with the totals' tangents
+        const double xu = c * u1 - s * u2, xv = c * v1 - s * v2;
+        const double zu = c * u2 + s * u1, zv = c * v2 + s * v1;
+        const double pu = du1 - dth * u2, qu = du2 + dth * u1;
+        const double pv = dv1 - dth * v2, qv = dv2 + dth * v1;
+        const double dxu = c * pu - s * qu, dxv = c * pv - s * qv;
+        const double dzu = c * qu + s * pu, dzv = c * qv + s * pv;
+        const double dgxn = Ep * fma(kvz, xu, dxu) + Em * (dxv - kvz * xv);
+        const double dgzn = Ep * fma(kvz, zu, dzu) + Em * (kvz * zv - dzv);
         // Fl_sort (src/Force_long.jl:72-127)
         const double dsxy = X.dsm * A + X.sm * dA - X.dcm * B - X.cm * dB + X.dsp * C +
                             X.sp * dC - X.dcp * D - X.cp * dD;
         const double dszz = X.dcm * A + X.cm * dA + X.dsm * B + X.sm * dB - X.dcp * C -
                             X.cp * dC - X.dsp * D - X.sp * dD;
-        // Flz_self (src/Force_long.jl:164-184)
-        const double dself = 2.0 * kvz * (L.S1 * (Ep * Ep) + L.S2 * (Em * Em));
+        double gzs = dgzn + L.hs * dszz;
+        // Flz_self (src/Force_long.jl:164-184) minus the own term above:
+        // the tangent of q (E1 Ep^2 - E2 Em^2)
+        if (own_term) gzs += q * (2.0 * kvz * (E1 * (Ep * Ep) + E2 * (Em * Em)));
         const double gx = q * (dgxn + L.hs * dsxy);
-        const double gz = q * (dgzn + L.hs * dszz + q * dself);
-        const double fx = act ? L.wx * gx : 0.0;
-        const double fy = act ? L.wy * gx : 0.0;
-        const double fz = act ? L.w * gz : 0.0;
+        const double gz = q * gzs;
+        const double fx = L.wx * gx, fy = L.wy * gx, fz = L.w * gz;
         A += vx;
         B += vy;
         dA += tvx;
         dB += tvy;
-        const int slot = (int)((a - a0) % kRedBatch);
+        const int slot = (int)(a - a0) & (kRedBatch - 1);
         red.put(slot, lane, fx, fy, fz);
         if (slot == kRedBatch - 1 || a == a1 - 1) red.flush(slot + 1, lane, out, a - slot, false);
     }
@@ -888,9 +919,7 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_t(
         const double dszz = -X.dcp * C - X.cp * dC - X.dsp * D - X.sp * dD;
         const double gx = q * (L.hs * dsxy);
         const double gz = q * (L.hs * dszz);
-        const double fx = act ? L.wx * gx : 0.0;
-        const double fy = act ? L.wy * gx : 0.0;
-        const double fz = act ? L.w * gz : 0.0;
+        const double fx = L.wx * gx, fy = L.wy * gx, fz = L.w * gz;
         C += q * X.cm;
         D += q * X.sm;
         dC += q * X.dcm;
@@ -940,46 +969,63 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B_energy(
     double kmax = live ? R.k : 0.0;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, d));
-    const bool flat = kmax * (satom[a1 - 1].z - satom[a0].z) <= 1.0;
+    const bool flat =
+        __builtin_amdgcn_readfirstlane((int)(kmax * (satom[a1 - 1].z - satom[a0].z) <= 1.0)) != 0;
     double *out = partB + (int64_t)blockIdx.y * n;
+    __shared__ double red_lds[kLaneWaves][kRedBatch1 * kRedPitch];
+    const LaneReducer1 red{red_lds[wv]};
 
     double A = off.x, B = off.y, PZ = 0.0, PW = 0.0;
-    double4 nx = satom[a0];        // the next atom, loaded one iteration ahead
-    for (int64_t a = a0; a < a1; ++a) {
-        const double4 at = nx;
-        if (a + 1 < a1) nx = satom[a + 1];
-        const double q = at.w;
-        double c, s, Ep, Em;
-        lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
-        const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-        const double vx = q * cp, vy = q * sp, vz = q * cm, vw = q * sm;
-        PZ = add_nc(PZ, vz);
-        PW = add_nc(PW, vw);
-        double C = 0.0, D = 0.0;
-        if (flat) {
-            C = off.z + (own.z - PZ);
-            D = off.w + (own.w - PW);
+    // the walk up the chunk, the chunk's flatness a compile-time flag
+    const auto walk = [&](auto flat_c) {
+        constexpr bool FLAT = decltype(flat_c)::value;
+        double4 nx = satom[a0];    // the next atom, loaded one iteration ahead
+        int slot = 0;              // (a - a0) mod kRedBatch1
+        for (int64_t a = a0; a < a1; ++a) {
+            const double4 at = nx;
+            if (a + 1 < a1) nx = satom[a + 1];
+            const double q = at.w;
+            double c, s, Ep, Em;
+            lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
+            const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
+            double t = cm * (A + P1) + sm * (B + P2) + cp * P3 + sp * P4;
+            if constexpr (FLAT) {
+                PZ = add_nc(PZ, q * cm);
+                PW = add_nc(PW, q * sm);
+                t += cp * (off.z + (own.z - PZ)) + sp * (off.w + (own.w - PW));
+            }
+            A += q * cp;
+            B += q * sp;
+            red.put(slot, lane, we * t);
+            if (slot == kRedBatch1 - 1 || a == a1 - 1) {
+                red.flush(slot + 1, lane, out, a - slot, false);
+                slot = 0;
+            } else {
+                ++slot;
+            }
         }
-        double ph = we * (cm * (A + P1) + sm * (B + P2) + cp * (C + P3) + sp * (D + P4));
-        A += vx;
-        B += vy;
-        ph = wave_sum63(ph);
-        if (lane == 63) out[a] = ph;
-    }
+    };
+    if (flat) walk(std::true_type{});
+    else walk(std::false_type{});
     if (flat) return;
-    // exact suffix walk: the terms with C, D (above the atom)
+    // exact suffix walk: the terms with C, D (above the atom), in batches
+    // [lo, hi] taken from the top, added to the first walk's values
     double C = off.z, D = off.w;
+    int64_t lo = a1, hi = a1 - 1;
     for (int64_t a = a1 - 1; a >= a0; --a) {
+        if (a < lo) {
+            hi = a;
+            lo = a - kRedBatch1 + 1 < a0 ? a0 : a - kRedBatch1 + 1;
+        }
         const double4 at = satom[a];
         const double q = at.w;
         double c, s, Ep, Em;
         lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
         const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-        double ph = we * (cp * C + sp * D);
+        red.put((int)(a - lo), lane, we * (cp * C + sp * D));
         C += q * cm;
         D += q * sm;
-        ph = wave_sum63(ph);
-        if (lane == 63) out[a] += ph;
+        if (a == lo) red.flush((int)(hi - lo + 1), lane, out, lo, true);
     }
 }
 
@@ -1296,10 +1342,10 @@ hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int6
     if (which & kLongPassAS) {
         if (c.precision == QEM_PREC_MIXED)
             k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
-                kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>());
+                kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>());
         else
             k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
-                kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>());
+                kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, lw.partA.as<double4>());
         LR_CHECK(hipGetLastError());
         k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
                                                lw.offA.as<double4>(), lw.totA.as<double4>());
@@ -1383,10 +1429,10 @@ hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, in
     const dim3 g((unsigned)((nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)ngroup);
     const double4 *sa = lw.satom.as<double4>();
     if (c.precision == QEM_PREC_MIXED)
-        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk, 1,
+        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk,
                                                                lw.partA.as<double4>());
     else
-        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk, 1,
+        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, chk, nchunk,
                                                               lw.partA.as<double4>());
     LR_CHECK(hipGetLastError());
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, nchunk,
@@ -1423,11 +1469,11 @@ hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t 
     const dim3 g((unsigned)((lg.nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)lg.ngroup);
     const double4 *sa = lw.satom.as<double4>();
     if (c.precision == QEM_PREC_MIXED)
-        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
-            kr, Ptot, sa, n, lg.chk, lg.nchunk, lg.split_last, lw.partA.as<double4>());
+        k_lane_A<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, lg.chk,
+                                                               lg.nchunk, lw.partA.as<double4>());
     else
-        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
-            kr, Ptot, sa, n, lg.chk, lg.nchunk, lg.split_last, lw.partA.as<double4>());
+        k_lane_A<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(kr, Ptot, sa, n, lg.chk,
+                                                              lg.nchunk, lw.partA.as<double4>());
     LR_CHECK(hipGetLastError());
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, lg.nchunk,
                                            lw.offA.as<double4>(), lw.totA.as<double4>());
@@ -1550,18 +1596,21 @@ hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partT.as<double4>(), Ptot, lg.nchunk,
                                            lw.offT.as<double4>(), lw.totT.as<double4>());
     LR_CHECK(hipGetLastError());
+    // the self term differs from the non-sort sum's own term unless both
+    // parts are on or both off
+    const int own_term = ((c.parts & QEM_LONG_NONSORT) != 0) != ((c.parts & QEM_LONG_SELF) != 0);
     if (c.precision == QEM_PREC_MIXED)
         k_lane_B_t<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
             kr, Ptot, sa, lw.svx.as<double>(), lw.svy.as<double>(), lw.svz.as<double>(), n,
             lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
             lw.totA.as<double4>(), lw.partT.as<double4>(), lw.offT.as<double4>(),
-            lw.totT.as<double4>(), lw.partB.as<double>());
+            lw.totT.as<double4>(), own_term, lw.partB.as<double>());
     else
         k_lane_B_t<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
             kr, Ptot, sa, lw.svx.as<double>(), lw.svy.as<double>(), lw.svz.as<double>(), n,
             lg.chk, lg.nchunk, lw.partA.as<double4>(), lw.offA.as<double4>(),
             lw.totA.as<double4>(), lw.partT.as<double4>(), lw.offT.as<double4>(),
-            lw.totT.as<double4>(), lw.partB.as<double>());
+            lw.totT.as<double4>(), own_term, lw.partB.as<double>());
     LR_CHECK(hipGetLastError());
     if (!with_sum) return hipSuccess;
     return launch_long_tangent_sum(lw, n, Ptot, scale, d_out, accumulate, s, err);
