@@ -116,7 +116,7 @@ __device__ __forceinline__ double db_dg(const ShortConsts &c, int l, int w)
 
 // Pair potential u_ij (without q_i q_j) and du/dgamma_1, du/dgamma_2.
 template <int PREC>
-__device__ double4 pair_energy(const ShortConsts &c, const double *__restrict__ glx,
+__device__ __forceinline__ double4 pair_energy(const ShortConsts &c, const double *__restrict__ glx,
                                const double *__restrict__ glw, const float *__restrict__ glxf,
                                const float *__restrict__ glwf, const float4 *__restrict__ tab,
                                double xi, double yi, double zi, double xj, double yj, double zj)
@@ -340,7 +340,7 @@ __device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, do
 
 // Tangent of F_short_ij (force on i; z force on j) along (vi, vj).
 template <int PREC>
-__device__ double4 pair_force_tangent(const ShortConsts &c, const double *__restrict__ glx,
+__device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, const double *__restrict__ glx,
                                       const double *__restrict__ glw,
                                       const float *__restrict__ glxf,
                                       const float *__restrict__ glwf,
@@ -447,60 +447,49 @@ __device__ double self_force_tangent(const ShortConsts &c, const double *__restr
 template <int PREC>
 using TanRec = typename std::conditional<PREC == QEM_PREC_MIXED, float4, double4>::type;
 
-__device__ __forceinline__ void store_tan(float4 *r, const double4 &d)
-{
-    *r = make_float4((float)d.x, (float)d.y, (float)d.z, (float)d.w);
-}
-__device__ __forceinline__ void store_tan(double4 *r, const double4 &d) { *r = d; }
 
-// Per-pair potentials.  FP64: one double4 record per pair, (G, dG/dgamma_1,
-// dG/dgamma_2, -).  Mixed: two float4 records, one per endpoint, already
-// weighted by the other endpoint's charge: out[p] = q_j (G, dG1, dG2, -) for
-// atom i, out[npairs + p] = q_i (G, dG1, dG2, -) for atom j, so the gather
-// reads one 16-byte record per pair and endpoint and no charges.
+// Potential records (endpoint records, placed as the force records): the
+// first atom's is q_j (G, dG/dgamma_1, dG/dgamma_2, 0), the second's
+// q_i (...), so the gather reads one record per pair and endpoint and no
+// charges.
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_energy(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float *__restrict__ glxf, const float *__restrict__ glwf,
     const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
-    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    const double *__restrict__ q, TanRec<PREC> *__restrict__ out)
+    const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
+    const double *__restrict__ xyz, const double *__restrict__ q, TanRec<PREC> *__restrict__ out)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
-        const int64_t i = pi[p], j = pj[p];
-        const double4 u = pair_energy<PREC>(c, glx, glw, glxf, glwf, tab, xyz[3 * i],
-                                            xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
-                                            xyz[3 * j + 1], xyz[3 * j + 2]);
-        if (PREC == QEM_PREC_MIXED) {
-            const double qi = q[i], qj = q[j];
-            store_tan(out + p, make_double4(qj * u.x, qj * u.y, qj * u.z, 0.0));
-            store_tan(out + npairs + p, make_double4(qi * u.x, qi * u.y, qi * u.z, 0.0));
-        } else {
-            store_tan(out + p, u);
-        }
-    }
+    dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
+        const double4 u =
+            pair_energy<PREC>(c, glx, glw, glxf, glwf, tab, o.xi, o.yi, o.zi, o.xj, o.yj, o.zj);
+        dev::store4(out + p, o.qj * u.x, o.qj * u.y, o.qj * u.z, 0.0);
+        dev::store4(out + npairs + o.jp, o.qi * u.x, o.qi * u.y, o.qi * u.z, 0.0);
+    });
 }
 
+// Tangent records: the first atom's (dfx, dfy, dfz_i, 0), the second's
+// (-dfx, -dfy, dfz_j, 0), placed as the force records.
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_tangent(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float *__restrict__ glxf, const float *__restrict__ glwf,
     const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
-    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    const double *__restrict__ q, const double *__restrict__ v, TanRec<PREC> *__restrict__ out)
+    const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
+    const double *__restrict__ xyz, const double *__restrict__ q, const double *__restrict__ v,
+    TanRec<PREC> *__restrict__ out)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
-        const int64_t i = pi[p], j = pj[p];
-        store_tan(out + p, pair_force_tangent<PREC>(c, glx, glw, glxf, glwf, tab, q[i], q[j],
-                                                    xyz + 3 * i, xyz + 3 * j, v + 3 * i,
-                                                    v + 3 * j));
-    }
+    dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
+        const int64_t i = o.i, j = o.j;
+        const double4 d = pair_force_tangent<PREC>(c, glx, glw, glxf, glwf, tab, o.qi, o.qj,
+                                                   xyz + 3 * i, xyz + 3 * j, v + 3 * i, v + 3 * j);
+        dev::store4(out + p, d.x, d.y, d.z, 0.0);
+        dev::store4(out + npairs + o.jp, -d.x, -d.y, d.w, 0.0);
+    });
 }
 
 // self terms per atom: phi = 2 q s_i, psi_w = 2 q ds_i/dgamma_w (overwrite)
@@ -520,127 +509,28 @@ __global__ __launch_bounds__(256) void k_self_energy(
     psi[2 * a + 1] = 2.0 * qa * se.z;
 }
 
-// per atom: phi (dE/dq) and psi_1,2 (d phi / d gamma) from the pair
-// potentials of both endpoints, added to the self terms already there.
-// Sixteen lanes per atom, strided contiguous reads, fixed-order row sums,
-// and the XCD-aware block order of the force gather (gridDim.x a multiple
-// of 8: the blocks of one XCD take a contiguous range of atoms).
+// per atom: phi (dE/dq) and psi_1,2 (d phi / d gamma) from the potential
+// records of its pairs (both endpoints), added to the self terms already
+// there.  Sixteen lanes per atom, contiguous strided reads of the atom's two
+// record runs, fixed-order row sums, and the XCD-aware atom order of the
+// force gather.
 template <typename Rec>
 __global__ __launch_bounds__(256) void k_gather_energy(
-    int64_t n, const double *__restrict__ q, const Rec *__restrict__ pot,
-    const int32_t *__restrict__ pi, const int32_t *__restrict__ pj,
-    const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
-    const int32_t *__restrict__ i_idx, const int32_t *__restrict__ j_start,
-    const int32_t *__restrict__ j_end, const int32_t *__restrict__ j_idx,
+    int64_t n, int64_t npairs, const Rec *__restrict__ pot, const int32_t *__restrict__ i_start,
+    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
+    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
     double *__restrict__ phi, double *__restrict__ psi)
 {
-    const int64_t per_xcd = gridDim.x / 8;
-    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
+    const int64_t a = dev::gather_atom();
     const int l = threadIdx.x & 15;
     const bool live = a < n;
-    double f = 0.0, s0 = 0.0, s1 = 0.0;
-    if (live) {
-        const int32_t t1 = i_end[a];
-        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
-            const int32_t p = i_idx ? i_idx[t] : t;
-            const Rec u = pot[p];
-            const double qb = q[pj[p]];
-            f += qb * (double)u.x;
-            s0 += qb * (double)u.y;
-            s1 += qb * (double)u.z;
-        }
-        // j side: four pairs in flight per lane (indices, then records and
-        // charges), added in order
-        const int32_t u1 = j_end[a];
-        int32_t t = j_start[a] + l;
-        for (; t + 48 < u1; t += 64) {
-            int32_t p[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) p[k] = j_idx[t + 16 * k];
-            Rec u[4];
-            double qb[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                u[k] = pot[p[k]];
-                qb[k] = q[pi[p[k]]];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                f += qb[k] * (double)u[k].x;
-                s0 += qb[k] * (double)u[k].y;
-                s1 += qb[k] * (double)u[k].z;
-            }
-        }
-        for (; t < u1; t += 16) {
-            const int32_t p = j_idx[t];
-            const Rec u = pot[p];
-            const double qb = q[pi[p]];
-            f += qb * (double)u.x;
-            s0 += qb * (double)u.y;
-            s1 += qb * (double)u.z;
-        }
-    }
-    f = dev::row_sum16(f);
-    s0 = dev::row_sum16(s0);
-    s1 = dev::row_sum16(s1);
-    if (live && l == 0) {
-        phi[a] += f;
-        psi[2 * a] += s0;
-        psi[2 * a + 1] += s1;
-    }
-}
-
-// k_gather_energy over the charge-weighted float4 records of the mixed
-// pair kernel: wi[p] for the pair's first atom, wj[p] for its second.
-__global__ __launch_bounds__(256) void k_gather_energy_w(
-    int64_t n, const float4 *__restrict__ wi, const float4 *__restrict__ wj,
-    const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
-    const int32_t *__restrict__ i_idx, const int32_t *__restrict__ j_start,
-    const int32_t *__restrict__ j_end, const int32_t *__restrict__ j_idx,
-    double *__restrict__ phi, double *__restrict__ psi)
-{
-    const int64_t per_xcd = gridDim.x / 8;
-    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
-    const int l = threadIdx.x & 15;
-    const bool live = a < n;
-    double f = 0.0, s0 = 0.0, s1 = 0.0;
-    if (live) {
-        const int32_t t1 = i_end[a];
-        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
-            const float4 u = wi[i_idx ? i_idx[t] : t];
-            f += (double)u.x;
-            s0 += (double)u.y;
-            s1 += (double)u.z;
-        }
-        // j side: four records in flight per lane, added in order
-        const int32_t u1 = j_end[a];
-        int32_t t = j_start[a] + l;
-        for (; t + 48 < u1; t += 64) {
-            int32_t p[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) p[k] = j_idx[t + 16 * k];
-            float4 u[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) u[k] = wj[p[k]];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                f += (double)u[k].x;
-                s0 += (double)u[k].y;
-                s1 += (double)u[k].z;
-            }
-        }
-        for (; t < u1; t += 16) {
-            const float4 u = wj[j_idx[t]];
-            f += (double)u.x;
-            s0 += (double)u.y;
-            s1 += (double)u.z;
-        }
-    }
-    f = dev::row_sum16(f);
-    s0 = dev::row_sum16(s0);
-    s1 = dev::row_sum16(s1);
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (live)
+        dev::add_endpoint_runs<Rec, 3>(acc, pot, npairs, a, i_start, i_end, i_idx, j_start, j_end,
+                                       l);
+    const double f = dev::row_sum16(acc[0]);
+    const double s0 = dev::row_sum16(acc[1]);
+    const double s1 = dev::row_sum16(acc[2]);
     if (live && l == 0) {
         phi[a] += f;
         psi[2 * a] += s0;
@@ -664,58 +554,26 @@ __global__ __launch_bounds__(256) void k_self_tangent(
         scale * self_force_tangent<PREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], v[3 * a + 2]);
 }
 
-// per atom: scale * (tangent of the pair forces of both endpoints), added to
-// the self term already there; sixteen lanes per atom, and the XCD-aware
-// block order of the force gather (gridDim.x a multiple of 8: the blocks
-// of one XCD take a contiguous range of atoms)
+// per atom: scale * (the tangent records of its pairs, both endpoints),
+// added to the self term already there; sixteen lanes per atom and the
+// XCD-aware atom order of the force gather
 template <typename Rec>
 __global__ __launch_bounds__(256) void k_gather_tangent(
-    int64_t n, const Rec *__restrict__ tp, const int32_t *__restrict__ i_start,
+    int64_t n, int64_t npairs, const Rec *__restrict__ tp, const int32_t *__restrict__ i_start,
     const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
-    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
-    const int32_t *__restrict__ j_idx, double scale, double *__restrict__ out)
+    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end, double scale,
+    double *__restrict__ out)
 {
-    const int64_t per_xcd = gridDim.x / 8;
-    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
+    const int64_t a = dev::gather_atom();
     const int l = threadIdx.x & 15;
     const bool live = a < n;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (live) {
-        const int32_t t1 = i_end[a];
-        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
-            const Rec d = tp[i_idx ? i_idx[t] : t];
-            fx += (double)d.x;
-            fy += (double)d.y;
-            fz += (double)d.z;
-        }
-        // j side: four records in flight per lane, added in order
-        const int32_t u1 = j_end[a];
-        int32_t t = j_start[a] + l;
-        for (; t + 48 < u1; t += 64) {
-            int32_t p[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) p[k] = j_idx[t + 16 * k];
-            Rec d[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) d[k] = tp[p[k]];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                fx -= (double)d[k].x;
-                fy -= (double)d[k].y;
-                fz += (double)d[k].w;
-            }
-        }
-        for (; t < u1; t += 16) {
-            const Rec d = tp[j_idx[t]];
-            fx -= (double)d.x;
-            fy -= (double)d.y;
-            fz += (double)d.w;
-        }
-    }
-    fx = dev::row_sum16(fx);
-    fy = dev::row_sum16(fy);
-    fz = dev::row_sum16(fz);
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (live)
+        dev::add_endpoint_runs<Rec, 3>(acc, tp, npairs, a, i_start, i_end, i_idx, j_start, j_end,
+                                       l);
+    const double fx = dev::row_sum16(acc[0]);
+    const double fy = dev::row_sum16(acc[1]);
+    const double fz = dev::row_sum16(acc[2]);
     if (live && l == 0) {
         out[3 * a] = scale * fx;
         out[3 * a + 1] = scale * fy;
@@ -773,11 +631,7 @@ __global__ void k_half_dots_final(int nb, const double *__restrict__ part,
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-inline unsigned pair_grid(int64_t m)
-{
-    const int64_t want = (m + 255) / 256;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
-}
+using dev::pair_grid;
 
 }  // namespace
 
@@ -790,6 +644,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     if (n <= 0) return hipSuccess;
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
+    const int32_t *jpos = nl.jpos.as<int32_t>();
     double4 *pot = reinterpret_cast<double4 *>(d_pot);
     float4 *potf = reinterpret_cast<float4 *>(d_pot);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
@@ -798,7 +653,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
             k_pair_energy<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
-                nl.pj.as<int32_t>(), np, d_xyz, d_q, potf);
+                nl.pj.as<int32_t>(), jpos, np, d_xyz, d_q, potf);
         if (after_pair) (*after_pair)();
         k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
@@ -808,13 +663,13 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
             if (c.jd_n > 0)
                 k_pair_energy<QEM_PREC_FP64><<<pair_grid(np), 256, dev::lds_bytes64(c), s>>>(
                     c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                    d_xyz, d_q, pot);
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), jpos,
+                    np, d_xyz, d_q, pot);
             else
                 k_pair_energy<kPrecFP64Lib><<<pair_grid(np), 256, 0, s>>>(
                     c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                    d_xyz, d_q, pot);
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), jpos,
+                    np, d_xyz, d_q, pot);
         }
         if (after_pair) (*after_pair)();
         k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
@@ -823,15 +678,13 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     }
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     if (precision == QEM_PREC_MIXED)
-        k_gather_energy_w<<<nb, 256, 0, s>>>(
-            n, potf, potf + np, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
-            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi,
-            d_psi);
+        k_gather_energy<float4><<<nb, 256, 0, s>>>(
+            n, np, potf, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), d_phi, d_psi);
     else
         k_gather_energy<double4><<<nb, 256, 0, s>>>(
-            n, d_q, pot, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(),
-            nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(),
-            nl.j_idx.as<int32_t>(), d_phi, d_psi);
+            n, np, pot, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), d_phi, d_psi);
     return hipGetLastError();
 }
 
@@ -847,9 +700,9 @@ hipError_t launch_energy_gather(const ShortConsts &c, const ShortWork &ws,
         d_xyz, d_q, d_phi, d_psi);
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     k_gather_energy<float4><<<nb, 256, 0, s>>>(
-        n, d_q, reinterpret_cast<const float4 *>(d_epot), nl.pi.as<int32_t>(),
-        nl.pj.as<int32_t>(), nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
-        nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), d_phi, d_psi);
+        n, nl.npairs, reinterpret_cast<const float4 *>(d_epot), nl.i_start.as<int32_t>(),
+        nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), d_phi,
+        d_psi);
     return hipGetLastError();
 }
 
@@ -862,6 +715,7 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
     if (n <= 0) return hipSuccess;
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
+    const int32_t *jpos = nl.jpos.as<int32_t>();
     double4 *tp = reinterpret_cast<double4 *>(d_tp);
     float4 *tpf = reinterpret_cast<float4 *>(d_tp);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
@@ -870,7 +724,7 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
             k_pair_tangent<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
-                nl.pj.as<int32_t>(), np, d_xyz, d_q, d_v, tpf);
+                nl.pj.as<int32_t>(), jpos, np, d_xyz, d_q, d_v, tpf);
         if (after_pair) (*after_pair)();
         k_self_tangent<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
@@ -880,13 +734,13 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
             if (c.jd_n > 0)
                 k_pair_tangent<QEM_PREC_FP64><<<pair_grid(np), 256, dev::lds_bytes64(c), s>>>(
                     c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                    d_xyz, d_q, d_v, tp);
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), jpos,
+                    np, d_xyz, d_q, d_v, tp);
             else
                 k_pair_tangent<kPrecFP64Lib><<<pair_grid(np), 256, 0, s>>>(
                     c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
-                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), np,
-                    d_xyz, d_q, d_v, tp);
+                    ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), jpos,
+                    np, d_xyz, d_q, d_v, tp);
         }
         if (after_pair) (*after_pair)();
         k_self_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
@@ -896,12 +750,12 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     if (precision == QEM_PREC_MIXED)
         k_gather_tangent<float4><<<nb, 256, 0, s>>>(
-            n, tpf, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
-            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), scale, d_out);
+            n, np, tpf, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), scale, d_out);
     else
         k_gather_tangent<double4><<<nb, 256, 0, s>>>(
-            n, tp, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
-            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), nl.j_idx.as<int32_t>(), scale, d_out);
+            n, np, tp, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
+            nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), scale, d_out);
     return hipGetLastError();
 }
 

@@ -246,6 +246,16 @@ hipError_t group_by_endpoint(NeighborList &nl, const int32_t *d_end, int64_t m, 
     return hipGetLastError();
 }
 
+// jpos[p]: pair p's place in the second-endpoint grouping, where the pair
+// kernels write the record of its second atom (the inverse of j_idx).
+hipError_t second_endpoint_places(NeighborList &nl, int64_t m, hipStream_t s, std::string &err)
+{
+    NL_CHECK(ensure(nl.jpos, sizeof(int32_t) * (size_t)(m > 0 ? m : 1)));
+    if (m == 0) return hipSuccess;
+    k_inverse_perm<<<nblk(m, 256), 256, 0, s>>>(nl.j_idx.as<int32_t>(), m, nl.jpos.as<int32_t>());
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz, double Lx,
@@ -336,7 +346,7 @@ hipError_t build_neighbor_list(NeighborList &nl, int64_t n, const double *d_xyz,
     NL_CHECK(hipGetLastError());
     NL_CHECK(group_by_endpoint(nl, nl.pj.as<int32_t>(), total, n, nl.j_idx, nl.j_start,
                                nl.j_end, s, err));
-    return hipSuccess;
+    return second_endpoint_places(nl, total, s, err);
 }
 
 hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs, int64_t npairs,
@@ -364,6 +374,7 @@ hipError_t install_pairs(NeighborList &nl, int64_t n, const int32_t *h_pairs, in
                                nl.i_end, s, err));
     NL_CHECK(group_by_endpoint(nl, nl.pj.as<int32_t>(), npairs, n, nl.j_idx, nl.j_start,
                                nl.j_end, s, err));
+    NL_CHECK(second_endpoint_places(nl, npairs, s, err));
     // the host vectors must outlive the asynchronous copies
     NL_CHECK(hipStreamSynchronize(s));
     return hipSuccess;

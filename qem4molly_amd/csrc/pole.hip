@@ -199,7 +199,7 @@ struct PolePairForce {
 
 // F_short_ij with the pole rule (src/Force_short.jl:25-78 assembly)
 template <int PREC>
-__device__ PolePairForce pole_pair_force(const ShortConsts &c, const float4 *__restrict__ tab,
+__device__ __forceinline__ PolePairForce pole_pair_force(const ShortConsts &c, const float4 *__restrict__ tab,
                                          double qi, double qj, double xi, double yi, double zi,
                                          double xj, double yj, double zj)
 {
@@ -281,19 +281,19 @@ __device__ double pole_self_z(const ShortConsts &c, double qi, double zi)
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_list_pole(
     ShortConsts c, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
-    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    const double *__restrict__ q, double4 *__restrict__ fpair)
+    const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
+    const double *__restrict__ xyz, const double *__restrict__ q, double4 *__restrict__ fpair)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
-        const int64_t i = pi[p], j = pj[p];
-        const PolePairForce f = pole_pair_force<PREC>(
-            c, tab, q[i], q[j], xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
-            xyz[3 * j + 1], xyz[3 * j + 2]);
-        fpair[p] = make_double4(f.fx, f.fy, f.fz, f.fzj);
-    }
+    dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
+        const PolePairForce f = pole_pair_force<PREC>(c, tab, o.qi, o.qj, o.xi, o.yi, o.zi, o.xj,
+                                                      o.yj, o.zj);
+        // endpoint records: (fx, fy, fz) for the first atom, (-fx, -fy, fz_j)
+        // for the second, at its place in the second-endpoint grouping
+        fpair[p] = make_double4(f.fx, f.fy, f.fz, 0.0);
+        fpair[npairs + o.jp] = make_double4(-f.fx, -f.fy, f.fzj, 0.0);
+    });
 }
 
 template <int PREC>
@@ -339,21 +339,21 @@ inline unsigned grid_for(int64_t m)
 }  // namespace
 
 hipError_t launch_pair_pole(const ShortConsts &c, int precision, const ShortWork &ws,
-                            const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
-                            const double *d_xyz, const double *d_q, double *d_fpair,
-                            hipStream_t s)
+                            const int32_t *d_pi, const int32_t *d_pj, const int32_t *d_jpos,
+                            int64_t npairs, const double *d_xyz, const double *d_q,
+                            double *d_fpair, hipStream_t s)
 {
     if (npairs <= 0) return hipSuccess;
     double4 *out = reinterpret_cast<double4 *>(d_fpair);
     if (precision == QEM_PREC_MIXED)
-        k_pair_list_pole<QEM_PREC_MIXED><<<grid_for(npairs), 256, sizeof(float4) * ws.jt_n, s>>>(
-            c, ws.jtab.as<float4>(), d_pi, d_pj, npairs, d_xyz, d_q, out);
+        k_pair_list_pole<QEM_PREC_MIXED><<<dev::pair_grid(npairs), 256, sizeof(float4) * ws.jt_n, s>>>(
+            c, ws.jtab.as<float4>(), d_pi, d_pj, d_jpos, npairs, d_xyz, d_q, out);
     else if (c.jd_n > 0)
-        k_pair_list_pole<QEM_PREC_FP64><<<grid_for(npairs), 256, dev::lds_bytes64(c), s>>>(
-            c, nullptr, d_pi, d_pj, npairs, d_xyz, d_q, out);
+        k_pair_list_pole<QEM_PREC_FP64><<<dev::pair_grid(npairs), 256, dev::lds_bytes64(c), s>>>(
+            c, nullptr, d_pi, d_pj, d_jpos, npairs, d_xyz, d_q, out);
     else
-        k_pair_list_pole<kPrecFP64Lib><<<grid_for(npairs), 256, 0, s>>>(
-            c, nullptr, d_pi, d_pj, npairs, d_xyz, d_q, out);
+        k_pair_list_pole<kPrecFP64Lib><<<dev::pair_grid(npairs), 256, 0, s>>>(
+            c, nullptr, d_pi, d_pj, d_jpos, npairs, d_xyz, d_q, out);
     return hipGetLastError();
 }
 

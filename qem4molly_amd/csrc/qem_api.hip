@@ -718,7 +718,7 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
         if (int st = advance_step(ctx, n, d_xyz, s)) return st;
     }
     const ShortConsts c = short_consts(ctx, 0);
-    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, pair_records_f32(c, ctx->p.precision) ? 16 : 32)));
     NeighborList &nl = ctx->nl;
     const double *px = d_xyz, *pq = d_q;
     double *pf = d_f;
@@ -735,7 +735,8 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
             pf = nl.sout.as<double>();
         }
         API_HIP(ctx, launch_pair_forces(c, ctx->p.precision, ctx->sw, nl.pi.as<int32_t>(),
-                                        nl.pj.as<int32_t>(), nl.npairs, px, pq,
+                                        nl.pj.as<int32_t>(), nl.jpos.as<int32_t>(), nl.npairs,
+                                        px, pq,
                                         ctx->sw.fpair.as<double>(), s));
     }
     if (after_pair) {
@@ -950,7 +951,7 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     int st = ensure_list(ctx, n, d_xyz, s);
     if (st) return st;
     const ShortConsts c = short_consts(ctx, 0);
-    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, ctx->p.precision == QEM_PREC_MIXED ? 16 : 32)));
     API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (2 * (size_t)n + 4 + half_dots_scratch())));
     double *psi = ctx->d_adj.as<double>();          // N x 2
     double *out3 = psi + 2 * n;                      // 3 (+1 pad)
@@ -1061,7 +1062,7 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     int st = ensure_list(ctx, n, d_xyz, s);
     if (st) return st;
     const ShortConsts c = short_consts(ctx, 0);
-    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, ctx->nl.npairs)));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, ctx->p.precision == QEM_PREC_MIXED ? 16 : 32)));
     const LongConsts lc = long_consts(ctx, weight, QEM_LONG_NONSORT | QEM_LONG_SORT | QEM_LONG_SELF);
     // in mixed precision the long-range tangent runs on the side stream, as
     // in forces_core: k staging and z sort beside the pair tangent kernel,
@@ -1158,7 +1159,7 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
     NeighborList &nl = ctx->nl;
     const ShortConsts c = short_consts(ctx, 0);
     const LongConsts lc = long_consts(ctx, w, QEM_LONG_ALL);
-    API_HIP(ctx, ensure(ctx->sw.fpair, sizeof(double) * 4 * (size_t)std::max<int64_t>(1, nl.npairs)));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(nl.npairs, 16, 2)));
     API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (2 * (size_t)n + 4 + half_dots_scratch())));
     double *psi = ctx->d_adj.as<double>();          // N x 2
     double *out3 = psi + 2 * n;                      // 3 (+1 pad)
@@ -1184,7 +1185,8 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             hipError_t e = ensure(nl.sout2, sizeof(double) * 3 * (size_t)n);
             if (e == hipSuccess)
                 e = launch_pair_forces_energy(c, ctx->sw, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
-                                              nl.npairs, px, pq, ctx->sw.fpair.as<double>(), s);
+                                              nl.jpos.as<int32_t>(), nl.npairs, px, pq,
+                                              ctx->sw.fpair.as<double>(), s);
             if (e != hipSuccess) st = hip_err(ctx, e, "fused pair kernel");
         }
     }
@@ -1206,7 +1208,7 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
     if (st == QEM_OK) {
         PhaseScope ps(ctx, QEM_PHASE_GATHER, s);
         const float *epot = reinterpret_cast<const float *>(
-            reinterpret_cast<const float4 *>(ctx->sw.fpair.p) + nl.npairs);
+            reinterpret_cast<const float4 *>(ctx->sw.fpair.p) + 2 * nl.npairs);
         hipError_t e = launch_gather_self(c, QEM_PREC_MIXED, ctx->sw, nl, n, px, pq,
                                           ctx->sw.fpair.as<double>(), nl.sout.as<double>(), s);
         if (e == hipSuccess)
@@ -1448,7 +1450,7 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
     return {&ctx->sw.ncon, &ctx->sw.nconi, &ctx->sw.ncon64, &ctx->nl.sout2, &ctx->lw.partE, &ctx->sw.glx, &ctx->sw.glw, &ctx->sw.glxf, &ctx->sw.glwf, &ctx->sw.jtab, &ctx->sw.jdtab,
                       &ctx->sw.pnode, &ctx->sw.pnodef,
                       &ctx->sw.fpair, &ctx->nl.pi, &ctx->nl.pj, &ctx->nl.i_start, &ctx->nl.i_end,
-                      &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx,
+                      &ctx->nl.i_idx, &ctx->nl.j_start, &ctx->nl.j_end, &ctx->nl.j_idx, &ctx->nl.jpos,
                       &ctx->nl.slot_of,
                       &ctx->nl.sxyz, &ctx->nl.sq, &ctx->nl.sv, &ctx->nl.sout,
                       &ctx->nl.cell_key, &ctx->nl.cell_key_sorted,

@@ -197,5 +197,161 @@ __device__ __forceinline__ double row_sum16(double v)
     return v;
 }
 
+// ---- endpoint records --------------------------------------------------------
+// A pair kernel over a list writes, for pair p, the first atom's record at
+// rec[p] and the second atom's at rec[npairs + jpos[p]] (jpos: the pair's
+// place in the second-endpoint grouping).  Each record holds what its atom
+// adds up, so a gather reads every atom's records as two contiguous runs
+// and never follows a pair index.
+__device__ __forceinline__ void store4(float4 *r, double x, double y, double z, double w)
+{
+    *r = make_float4((float)x, (float)y, (float)z, (float)w);
+}
+__device__ __forceinline__ void store4(double4 *r, double x, double y, double z, double w)
+{
+    *r = make_double4(x, y, z, w);
+}
+
+template <typename Rec, int NC>
+__device__ __forceinline__ void add_rec(double *acc, const Rec &r)
+{
+    acc[0] += (double)r.x;
+    acc[1] += (double)r.y;
+    acc[2] += (double)r.z;
+    if (NC > 3) acc[3] += (double)r.w;
+}
+
+// Lane l of an atom's sixteen adds records t = s + l, s + l + 16, ... of
+// run [s, e) in that order, four loads in flight.
+template <typename Rec, int NC>
+__device__ __forceinline__ void add_run(double *acc, const Rec *__restrict__ rec, int32_t s,
+                                        int32_t e, int l)
+{
+    int32_t t = s + l;
+    for (; t + 48 < e; t += 64) {
+        const Rec r0 = rec[t], r1 = rec[t + 16], r2 = rec[t + 32], r3 = rec[t + 48];
+        add_rec<Rec, NC>(acc, r0);
+        add_rec<Rec, NC>(acc, r1);
+        add_rec<Rec, NC>(acc, r2);
+        add_rec<Rec, NC>(acc, r3);
+    }
+    for (; t < e; t += 16) add_rec<Rec, NC>(acc, rec[t]);
+}
+
+// This lane's share of atom a's records: its first-endpoint run (through
+// i_idx for a list that is not grouped by first atom), then its
+// second-endpoint run in the second half.
+template <typename Rec, int NC>
+__device__ __forceinline__ void add_endpoint_runs(double *acc, const Rec *__restrict__ rec,
+                                                  int64_t npairs, int64_t a,
+                                                  const int32_t *__restrict__ i_start,
+                                                  const int32_t *__restrict__ i_end,
+                                                  const int32_t *__restrict__ i_idx,
+                                                  const int32_t *__restrict__ j_start,
+                                                  const int32_t *__restrict__ j_end, int l)
+{
+    if (i_idx) {
+        const int32_t e = i_end[a];
+        for (int32_t t = i_start[a] + l; t < e; t += 16) add_rec<Rec, NC>(acc, rec[i_idx[t]]);
+    } else {
+        add_run<Rec, NC>(acc, rec, i_start[a], i_end[a], l);
+    }
+    add_run<Rec, NC>(acc, rec + npairs, j_start[a], j_end[a], l);
+}
+
+// XCD-aware atom order of the gathers: the hardware deals blocks to the 8
+// XCDs round robin, so block b runs on XCD b % 8; each XCD takes a
+// contiguous range of atoms, whose records then share that XCD's L2
+// (gridDim.x a multiple of 8).  Sixteen lanes per atom.
+__device__ __forceinline__ int64_t gather_atom()
+{
+    const int64_t per_xcd = gridDim.x / 8;
+    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    return (blk * blockDim.x + threadIdx.x) >> 4;
+}
+
+// One pair's endpoints, positions and charges.
+struct PairOps {
+    int32_t i, j;
+    int32_t jp;   // the pair's place in the second-endpoint grouping
+    double qi, qj, xi, yi, zi, xj, yj, zj;
+};
+
+__device__ __forceinline__ void load_ends(PairOps &o, const int32_t *__restrict__ pi,
+                                          const int32_t *__restrict__ pj,
+                                          const int32_t *__restrict__ jpos, int64_t p)
+{
+    o.i = pi[p];
+    o.j = pj[p];
+    o.jp = jpos[p];
+}
+
+__device__ __forceinline__ void load_atoms(PairOps &o, const double *__restrict__ xyz,
+                                           const double *__restrict__ q)
+{
+    const int64_t i = o.i, j = o.j;
+    o.qi = q[i];
+    o.qj = q[j];
+    o.xi = xyz[3 * i];
+    o.yi = xyz[3 * i + 1];
+    o.zi = xyz[3 * i + 2];
+    o.xj = xyz[3 * j];
+    o.yj = xyz[3 * j + 1];
+    o.zj = xyz[3 * j + 2];
+}
+
+// Walk over the pair list, one pair per lane at a time, in an XCD-aware
+// order: the hardware deals blocks to the 8 XCDs round robin (block b runs
+// on XCD b % 8), and XCD x takes the x-th eighth of the list, which its
+// blocks stride through in order.  The pairs of nearby first atoms then run
+// on one XCD, and so do most writes into a second atom's record run: each
+// L2 gathers whole lines of second-endpoint records instead of eight L2s
+// holding pieces of the same lines.
+// body(p, ops) evaluates pair p.
+template <class Body>
+__device__ __forceinline__ void walk_pairs(const int32_t *__restrict__ pi,
+                                           const int32_t *__restrict__ pj,
+                                           const int32_t *__restrict__ jpos, int64_t npairs,
+                                           const double *__restrict__ xyz,
+                                           const double *__restrict__ q, Body body)
+{
+    const int64_t per_xcd = (int64_t)(gridDim.x / 8) * blockDim.x;
+    int64_t p, e, stride;
+    if (per_xcd == 0) {
+        // fewer than 8 blocks: a plain grid stride
+        p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        e = npairs;
+        stride = (int64_t)gridDim.x * blockDim.x;
+    } else {
+        const int x = blockIdx.x % 8;
+        const int64_t share = (npairs + 7) / 8;
+        const int64_t b = x * share;
+        e = b + share < npairs ? b + share : npairs;
+        // blocks past the last multiple of 8 join the last XCD's range
+        const int64_t local = blockIdx.x < (gridDim.x / 8) * 8 ? blockIdx.x / 8 : -1;
+        if (local < 0) return;
+        p = b + local * blockDim.x + threadIdx.x;
+        stride = per_xcd;
+    }
+    for (; p < e; p += stride) {
+        PairOps o;
+        load_ends(o, pi, pj, jpos, p);
+        load_atoms(o, xyz, q);
+        body(p, o);
+    }
+}
+
+
+// Grid of a pair-list kernel (blocks of `block` lanes, about two pairs per
+// lane at the largest lists): small blocks let the last ones end together
+// (a block of ~15 pairs per lane ran ~0.75 ms, and the last round of them
+// left CUs idle); a multiple of 8 from 8 blocks on, since walk_pairs gives
+// each XCD an eighth of them.
+inline unsigned pair_grid(int64_t m, int block = 256)
+{
+    const int64_t want = std::min<int64_t>((m + block - 1) / block, (int64_t)256 * 128 * 256 / block);
+    return (unsigned)(want >= 8 ? want / 8 * 8 : std::max<int64_t>(1, want));
+}
+
 }  // namespace dev
 }  // namespace qem

@@ -140,6 +140,9 @@ struct NeighborList {
     bool has_i_idx = false;
     DevBuf i_start, i_end, i_idx;
     DevBuf j_start, j_end, j_idx;
+    // per pair p: its place in the second-endpoint grouping (jpos[j_idx[u]]
+    // = u), where the pair kernels write the record of its second atom
+    DevBuf jpos;
     // Endpoints, i_*/j_* and per-atom kernel rows are in slot numbering:
     // for a built list the slots are the atoms in cell order (slot s is
     // atom atom_idx_sorted[s]), so pair neighbours are near in memory; for
@@ -187,26 +190,40 @@ struct ShortWork {
     DevBuf jdtab;         // FP64: 9 doubles per interval (degree-7 J0 coefficients, pad)
     int jd_n = 0;
     double jd_dx = 0.0;
-    DevBuf fpair;         // double4 per pair
+    DevBuf fpair;         // endpoint records, pair_rec_bytes(npairs)
     DevBuf pnode, pnodef; // pole-splitting nodes (double2, float2)
 };
-// Evaluates every pair of the list into ws.fpair (fx, fy, fz_i, fz_j).
+// Endpoint records.  A pair kernel over a list writes two records per pair
+// p, one per endpoint, each holding what that atom adds up: rec[p] for the
+// first atom and rec[npairs + jpos[p]] for the second.  An atom's records
+// are then two contiguous runs, [i_start, i_end) of the first half (through
+// i_idx for an installed list) and [j_start, j_end) of the second, which
+// the gathers read in order.  Bytes for the records of npairs pairs, rec
+// bytes each (16 for float4, 32 for double4): two per pair, or four for the
+// forces + energies kernel (two record sets).
+inline size_t pair_rec_bytes(int64_t npairs, size_t rec, int sets = 1)
+{
+    return 2 * (size_t)sets * rec * (size_t)(npairs > 0 ? npairs : 1);
+}
+// Evaluates every pair of the list into endpoint records in d_fpair: first
+// atom (fx, fy, fz_i, 0), second atom (-fx, -fy, fz_j, 0).
 hipError_t launch_pair_forces(const ShortConsts &c, int precision,
                               const ShortWork &ws, const int32_t *d_pi,
-                              const int32_t *d_pj, int64_t npairs,
+                              const int32_t *d_pj, const int32_t *d_jpos, int64_t npairs,
                               const double *d_xyz, const double *d_q,
                               double *d_fpair, hipStream_t s);
-// Mixed precision: the force records of launch_pair_forces (float4, d_fpair
-// as float4[npairs]) and, after them, the pair potentials without the
-// charges (float4 (G, dG/dgamma_1, dG/dgamma_2, 0), float4[npairs]) from the
-// same node loops; d_fpair has room for a double4 per pair.
+// Mixed precision: the force records of launch_pair_forces (float4,
+// 2 npairs of them) and, after them, the potential records (float4,
+// 2 npairs: first atom q_j (G, dG/dgamma_1, dG/dgamma_2, 0), second atom
+// q_i (...)) from the same node loops.
 hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
-                                     const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
+                                     const int32_t *d_pi, const int32_t *d_pj,
+                                     const int32_t *d_jpos, int64_t npairs,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
                                      hipStream_t s);
 // Pair records of the force path (launch_pair_forces -> launch_gather_self):
-// float4 per pair for the mixed-precision list kernel, double4 otherwise
-// (FP64, and the pole-splitting kernels).  The buffer is sized for double4.
+// float4 for the mixed-precision list kernel, double4 otherwise (FP64, and
+// the pole-splitting kernels).
 inline bool pair_records_f32(const ShortConsts &c, int precision)
 {
     return precision == QEM_PREC_MIXED && !c.pole;
@@ -220,7 +237,8 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision,
 // The pole-splitting variants (pole.hip): same outputs as the kernels they
 // replace when ShortConsts::pole is set.
 hipError_t launch_pair_pole(const ShortConsts &c, int precision, const ShortWork &ws,
-                            const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
+                            const int32_t *d_pi, const int32_t *d_pj, const int32_t *d_jpos,
+                            int64_t npairs,
                             const double *d_xyz, const double *d_q, double *d_fpair,
                             hipStream_t s);
 hipError_t launch_pair_direct_pole(const ShortConsts &c, int precision, const ShortWork &ws,
@@ -339,9 +357,9 @@ hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double 
 // have just formed: the two share passes A and S.
 hipError_t launch_long_energy_B(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                 hipStream_t s, std::string &err);
-// Short-range energy from the pair potentials of launch_pair_forces_energy
-// (mixed precision, float4 per pair without the charges): the self terms
-// into d_phi, d_psi (overwrite), then every atom's pair terms added.
+// Short-range energy from the potential records of
+// launch_pair_forces_energy (mixed precision): the self terms into d_phi,
+// d_psi (overwrite), then every atom's pair terms added.
 hipError_t launch_energy_gather(const ShortConsts &c, const ShortWork &ws,
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const float *d_epot, double *d_phi,
@@ -356,10 +374,10 @@ hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int
 hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double scale,
                                    double *d_out, int accumulate, hipStream_t s,
                                    std::string &err);
-// Short-range energy: per-pair potentials in d_pot (room for a double4 per
-// pair: one double4 record in FP64, two charge-weighted float4 records in
-// mixed precision), per atom phi (N) and psi (N x 2: d phi / d gamma_1,2),
-// self terms included.
+// Short-range energy: potential records in d_pot (pair_rec_bytes: per
+// endpoint the other atom's charge times (G, dG/dgamma_1, dG/dgamma_2, 0),
+// float4 in mixed precision, double4 in FP64), per atom phi (N) and psi
+// (N x 2: d phi / d gamma_1,2), self terms included.
 // after_pair (optional) is called once the pair kernel is queued.
 hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
                                const NeighborList &nl, int64_t n, const double *d_xyz,
@@ -367,8 +385,9 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                                double *d_psi, hipStream_t s,
                                const std::function<void()> *after_pair = nullptr);
 // Short-range force tangent along d_v, times scale, into d_out (N x 3);
-// d_tp is per-pair scratch (room for a double4 per pair; float4 records in
-// mixed precision, double4 in FP64).
+// d_tp holds the tangent records (pair_rec_bytes: first atom (dfx, dfy,
+// dfz_i, 0), second atom (-dfx, -dfy, dfz_j, 0); float4 in mixed precision,
+// double4 in FP64).
 // after_pair (optional) is called once the pair kernel is queued, before
 // the self term and the gather.
 hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,

@@ -177,13 +177,14 @@ struct PairForce {
 template <int PREC>
 using PairRec = typename std::conditional<PREC == QEM_PREC_MIXED, float4, double4>::type;
 
-__device__ __forceinline__ void store_rec(float4 *r, const PairForce &f)
+// The pair's two endpoint records: the first atom adds (fx, fy, fz), the
+// second (-fx, -fy, fz_j).
+template <typename Rec>
+__device__ __forceinline__ void store_recs(Rec *rec, int64_t npairs, int64_t p, int32_t jp,
+                                           const PairForce &f)
 {
-    *r = make_float4((float)f.fx, (float)f.fy, (float)f.fz, (float)f.fzj);
-}
-__device__ __forceinline__ void store_rec(double4 *r, const PairForce &f)
-{
-    *r = make_double4(f.fx, f.fy, f.fz, f.fzj);
+    dev::store4(rec + p, f.fx, f.fy, f.fz, 0.0);
+    dev::store4(rec + npairs + jp, -f.fx, -f.fy, f.fzj, 0.0);
 }
 
 // d b_l / d gamma_w (w = 0: gamma_1, 1: gamma_2) of image l
@@ -205,7 +206,7 @@ __device__ __forceinline__ double db_dgamma(const ShortConsts &c, int l, int w)
 // one of Gamma_1 e^{-k^2/4 alpha}; the derivatives take d b / d gamma and
 // d(1/D)/d gamma = -g_other E / D^2 (src/greens_function.jl:68-88).
 template <int PREC, bool ENERGY = false>
-__device__ PairForce pair_force(const ShortConsts &c, const double *__restrict__ glx,
+__device__ __forceinline__ PairForce pair_force(const ShortConsts &c, const double *__restrict__ glx,
                                 const double *__restrict__ glw, const float4 *__restrict__ ncon,
                                 const float4 *__restrict__ tab,
                                 double qi, double qj, double xi, double yi, double zi, double xj,
@@ -429,43 +430,40 @@ template <int PREC>
 __global__ __launch_bounds__(pair_block<PREC>()) void k_pair_list(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
-    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    const double *__restrict__ q, PairRec<PREC> *__restrict__ fpair)
+    const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
+    const double *__restrict__ xyz, const double *__restrict__ q,
+    PairRec<PREC> *__restrict__ fpair)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<PREC>(c, gtab, lds_tab);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
-        const int64_t i = pi[p], j = pj[p];
-        PairForce f = pair_force<PREC>(c, glx, glw, ncon, tab, q[i], q[j], xyz[3 * i],
-                                       xyz[3 * i + 1], xyz[3 * i + 2], xyz[3 * j],
-                                       xyz[3 * j + 1], xyz[3 * j + 2]);
-        store_rec(fpair + p, f);
-    }
+    dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
+        const PairForce f = pair_force<PREC>(c, glx, glw, ncon, tab, o.qi, o.qj, o.xi, o.yi,
+                                             o.zi, o.xj, o.yj, o.zj);
+        store_recs(fpair, npairs, p, o.jp, f);
+    });
 }
 
-// The force pair kernel with the pair potentials (mixed precision): fpair[p]
-// the force record of k_pair_list, epot[p] the pair's (G, dG/dgamma_1,
-// dG/dgamma_2, 0), without the charges.
+// The force pair kernel with the pair potentials (mixed precision): the
+// force records of k_pair_list in fpair, and in epot the potential records
+// q_j (G, dG/dgamma_1, dG/dgamma_2, 0) for the first atom and q_i (...) for
+// the second, placed like the force records.
 __global__ __launch_bounds__(256) void k_pair_list_fe(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
-    const int32_t *__restrict__ pj, int64_t npairs, const double *__restrict__ xyz,
-    const double *__restrict__ q, float4 *__restrict__ fpair, float4 *__restrict__ epot)
+    const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
+    const double *__restrict__ xyz, const double *__restrict__ q, float4 *__restrict__ fpair,
+    float4 *__restrict__ epot)
 {
     extern __shared__ float4 lds_tab[];
     const float4 *tab = dev::stage_tables<QEM_PREC_MIXED>(c, gtab, lds_tab);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
-        const int64_t i = pi[p], j = pj[p];
+    dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
         double4 u;
-        PairForce f = pair_force<QEM_PREC_MIXED, true>(c, glx, glw, ncon, tab, q[i], q[j],
-                                                       xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
-                                                       xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2],
-                                                       &u);
-        store_rec(fpair + p, f);
-        epot[p] = make_float4((float)u.x, (float)u.y, (float)u.z, 0.0f);
-    }
+        const PairForce f = pair_force<QEM_PREC_MIXED, true>(
+            c, glx, glw, ncon, tab, o.qi, o.qj, o.xi, o.yi, o.zi, o.xj, o.yj, o.zj, &u);
+        store_recs(fpair, npairs, p, o.jp, f);
+        dev::store4(epot + p, o.qj * u.x, o.qj * u.y, o.qj * u.z, 0.0);
+        dev::store4(epot + npairs + o.jp, o.qi * u.x, o.qi * u.y, o.qi * u.z, 0.0);
+    });
 }
 
 template <int PREC>
@@ -504,76 +502,39 @@ __global__ __launch_bounds__(256) void k_self_z(
     }
 }
 
-// Per atom: sum of pair forces with the atom as first endpoint, minus the
-// xy (and, as written, z) forces of pairs with it as second endpoint
-// (src/QEM_short.jl:41-56), plus the self term.
-// Sixteen lanes per atom read the atom's runs of pair results in
-// contiguous strides; each lane keeps a strided partial sum and the row of
-// 16 lanes is summed in a fixed order.  No atomics.  With SELF the kernel
-// also evaluates the self term (precision SPREC), each of the 16 lanes a
-// share of its nodes, which the same row sum adds up: the node loops run
-// while the pair records stream in.  Without it the self term is already
-// in f[3a + 2].
+// Per atom: the force records of its pairs, first endpoint (+F_ij) and
+// second (-F_ij in xy; in z what the semantics give the second atom)
+// (src/QEM_short.jl:41-56), plus the self term.  Sixteen lanes per atom
+// read the atom's two record runs in contiguous strides; each lane keeps a
+// strided partial sum and the row of 16 lanes is summed in a fixed order.
+// No atomics.  With SELF the kernel also evaluates the self term
+// (precision SPREC), each of the 16 lanes a share of its nodes, which the
+// same row sum adds up: the node loops run while the records stream in.
+// Without it the self term is already in f[3a + 2].
 template <typename Rec, int SPREC, bool SELF>
 __global__ __launch_bounds__(256) void k_gather_pairs(
-    int64_t n, const Rec *__restrict__ fpair, const int32_t *__restrict__ i_start,
-    const int32_t *__restrict__ i_end, const int32_t *__restrict__ i_idx,
-    const int32_t *__restrict__ j_start, const int32_t *__restrict__ j_end,
-    const int32_t *__restrict__ j_idx, ShortConsts c, const double *__restrict__ glx,
+    int64_t n, int64_t npairs, const Rec *__restrict__ fpair,
+    const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
+    const int32_t *__restrict__ i_idx, const int32_t *__restrict__ j_start,
+    const int32_t *__restrict__ j_end, ShortConsts c, const double *__restrict__ glx,
     const double *__restrict__ glw, const float *__restrict__ glxf,
     const float *__restrict__ glwf, const double *__restrict__ xyz,
     const double *__restrict__ q, double *__restrict__ f)
 {
-    // XCD-aware block order: the hardware deals blocks to the 8 XCDs round
-    // robin, so block b runs on XCD b % 8; give each XCD a contiguous range
-    // of atoms, whose pairs' results then share that XCD's L2 (gridDim.x is
-    // a multiple of 8)
-    const int64_t per_xcd = gridDim.x / 8;
-    const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    const int64_t a = (blk * blockDim.x + threadIdx.x) >> 4;
+    const int64_t a = dev::gather_atom();
     const int l = threadIdx.x & 15;
     const bool live = a < n;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
     if (live) {
-        const int32_t t1 = i_end[a];
-        for (int32_t t = i_start[a] + l; t < t1; t += 16) {
-            const Rec v = fpair[i_idx ? i_idx[t] : t];
-            fx += (double)v.x;
-            fy += (double)v.y;
-            fz += (double)v.z;
-        }
-        // j side: four records in flight per lane (their indices first), added
-        // in order
-        const int32_t u1 = j_end[a];
-        int32_t u = j_start[a] + l;
-        for (; u + 48 < u1; u += 64) {
-            const int32_t p0 = j_idx[u], p1 = j_idx[u + 16], p2 = j_idx[u + 32], p3 = j_idx[u + 48];
-            const Rec v0 = fpair[p0], v1 = fpair[p1], v2 = fpair[p2], v3 = fpair[p3];
-            fx -= (double)v0.x;
-            fy -= (double)v0.y;
-            fz += (double)v0.w;
-            fx -= (double)v1.x;
-            fy -= (double)v1.y;
-            fz += (double)v1.w;
-            fx -= (double)v2.x;
-            fy -= (double)v2.y;
-            fz += (double)v2.w;
-            fx -= (double)v3.x;
-            fy -= (double)v3.y;
-            fz += (double)v3.w;
-        }
-        for (; u < u1; u += 16) {
-            const Rec v = fpair[j_idx[u]];
-            fx -= (double)v.x;
-            fy -= (double)v.y;
-            fz += (double)v.w;
-        }
-        if (SELF) fz += self_force_z<SPREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], l, 16);
+        dev::add_endpoint_runs<Rec, 3>(acc, fpair, npairs, a, i_start, i_end, i_idx, j_start,
+                                       j_end, l);
+        if (SELF)
+            acc[2] += self_force_z<SPREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], l, 16);
     }
     // every lane takes part in the row sums
-    fx = dev::row_sum16(fx);
-    fy = dev::row_sum16(fy);
-    fz = dev::row_sum16(fz);
+    const double fx = dev::row_sum16(acc[0]);
+    const double fy = dev::row_sum16(acc[1]);
+    const double fz = dev::row_sum16(acc[2]);
     if (live && l == 0) {
         f[3 * a] = fx;
         f[3 * a + 1] = fy;
@@ -585,45 +546,43 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // grid for grid-stride pair kernels: enough blocks to fill 256 CUs several
 // times over, without relaunching the LDS table stage per 256 pairs
-inline unsigned pair_grid(int64_t m, int block = 256)
-{
-    const int64_t want = (m + block - 1) / block;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16 * 256 / block));
-}
+using dev::pair_grid;
 
 }  // namespace
 
 hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWork &ws,
-                              const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
-                              const double *d_xyz, const double *d_q, double *d_fpair,
-                              hipStream_t s)
+                              const int32_t *d_pi, const int32_t *d_pj, const int32_t *d_jpos,
+                              int64_t npairs, const double *d_xyz, const double *d_q,
+                              double *d_fpair, hipStream_t s)
 {
     if (npairs <= 0) return hipSuccess;
     if (c.pole)
-        return launch_pair_pole(c, precision, ws, d_pi, d_pj, npairs, d_xyz, d_q, d_fpair, s);
+        return launch_pair_pole(c, precision, ws, d_pi, d_pj, d_jpos, npairs, d_xyz, d_q,
+                                d_fpair, s);
     if (precision == QEM_PREC_MIXED) {
         const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
         k_pair_list<QEM_PREC_MIXED><<<pair_grid(npairs), 256, lds, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(),
-            ws.jtab.as<float4>(), d_pi, d_pj, npairs, d_xyz, d_q,
+            ws.jtab.as<float4>(), d_pi, d_pj, d_jpos, npairs, d_xyz, d_q,
             reinterpret_cast<float4 *>(d_fpair));
     } else {
         if (c.jd_n > 0)
             k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs, 512), 512, dev::lds_bytes64(c), s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, d_pi,
-                d_pj, npairs, d_xyz, d_q,
+                d_pj, d_jpos, npairs, d_xyz, d_q,
                 reinterpret_cast<double4 *>(d_fpair));
         else
             k_pair_list<kPrecFP64Lib><<<pair_grid(npairs), 256, 0, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), nullptr, d_pi,
-                d_pj, npairs, d_xyz, d_q,
+                d_pj, d_jpos, npairs, d_xyz, d_q,
                 reinterpret_cast<double4 *>(d_fpair));
     }
     return hipGetLastError();
 }
 
 hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
-                                     const int32_t *d_pi, const int32_t *d_pj, int64_t npairs,
+                                     const int32_t *d_pi, const int32_t *d_pj,
+                                     const int32_t *d_jpos, int64_t npairs,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
                                      hipStream_t s)
 {
@@ -633,7 +592,7 @@ hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     k_pair_list_fe<<<pair_grid(npairs), 256, lds, s>>>(
         c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), ws.jtab.as<float4>(),
-        d_pi, d_pj, npairs, d_xyz, d_q, rec, rec + npairs);
+        d_pi, d_pj, d_jpos, npairs, d_xyz, d_q, rec, rec + 2 * npairs);
     return hipGetLastError();
 }
 
@@ -693,7 +652,7 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWo
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     const int32_t *is = nl.i_start.as<int32_t>(), *ie = nl.i_end.as<int32_t>();
     const int32_t *js = nl.j_start.as<int32_t>(), *je = nl.j_end.as<int32_t>();
-    const int32_t *jidx = nl.j_idx.as<int32_t>();
+    const int64_t np = nl.npairs;
     const double *glx = ws.glx.as<double>(), *glw = ws.glw.as<double>();
     const float *glxf = ws.glxf.as<float>(), *glwf = ws.glwf.as<float>();
     if (c.pole) {
@@ -701,15 +660,15 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWo
         hipError_t e = launch_self_z(c, precision, ws, n, d_xyz, d_q, 0, d_f, s);
         if (e != hipSuccess) return e;
         k_gather_pairs<double4, QEM_PREC_FP64, false><<<nb, 256, 0, s>>>(
-            n, reinterpret_cast<const double4 *>(d_fpair), is, ie, iidx, js, je, jidx, c, glx,
+            n, np, reinterpret_cast<const double4 *>(d_fpair), is, ie, iidx, js, je, c, glx,
             glw, glxf, glwf, d_xyz, d_q, d_f);
     } else if (pair_records_f32(c, precision)) {
         k_gather_pairs<float4, QEM_PREC_MIXED, true><<<nb, 256, 0, s>>>(
-            n, reinterpret_cast<const float4 *>(d_fpair), is, ie, iidx, js, je, jidx, c, glx,
+            n, np, reinterpret_cast<const float4 *>(d_fpair), is, ie, iidx, js, je, c, glx,
             glw, glxf, glwf, d_xyz, d_q, d_f);
     } else {
         k_gather_pairs<double4, QEM_PREC_FP64, true><<<nb, 256, 0, s>>>(
-            n, reinterpret_cast<const double4 *>(d_fpair), is, ie, iidx, js, je, jidx, c, glx,
+            n, np, reinterpret_cast<const double4 *>(d_fpair), is, ie, iidx, js, je, c, glx,
             glw, glxf, glwf, d_xyz, d_q, d_f);
     }
     return hipGetLastError();
