@@ -29,6 +29,8 @@ using dev::kLog2e;
 // kind 2 (gauss, multiple reflections of the charge itself, a = 0):
 //                   t = e^{-k^2/4a} E / D,          v += w t,     d += w t / D
 // with E = e^{-2kLz}, D = g12 E - 1.  d is what d/dgamma of D brings down.
+// Lane `lane` of `nl` lanes takes the nodes n = lane (mod nl); the nl
+// shares add up to the sums (nl = 1 gives them whole).
 template <int PREC, bool SELF>
 __device__ __forceinline__ void nodes_energy(double K, double a, double rho, int kind,
                                              const ShortConsts &c,
@@ -36,12 +38,13 @@ __device__ __forceinline__ void nodes_energy(double K, double a, double rho, int
                                              const double *__restrict__ glw,
                                              const float *__restrict__ glxf,
                                              const float *__restrict__ glwf,
-                                             const float4 *__restrict__ tab, double &v, double &d)
+                                             const float4 *__restrict__ tab, double &v, double &d,
+                                             int lane = 0, int nl = 1)
 {
     if (PREC != QEM_PREC_MIXED) {
         const double h = 0.5 * K, inv4a = 0.25 / c.alpha;
         double av = 0.0, ad = 0.0;
-        for (int n = 0; n < c.Nt; ++n) {
+        for (int n = lane; n < c.Nt; n += nl) {
             const double k = h + h * glx[n];
             const double w = h * glw[n];
             const double E = exp(-2.0 * k * c.Lz);
@@ -82,7 +85,7 @@ __device__ __forceinline__ void nodes_energy(double K, double a, double rho, int
         const float hR = SELF ? 0.0f
                               : fminf(h * (float)rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
         float av = 0.f, ad = 0.f;
-        for (int n = 0; n < c.Nt; ++n) {
+        for (int n = lane; n < c.Nt; n += nl) {
             const float4 u = ncon[n];
             const float e = hE * u.x;
             const float E = __builtin_amdgcn_exp2f(e);
@@ -150,11 +153,16 @@ __device__ __forceinline__ double4 pair_energy(const ShortConsts &c, const doubl
 
 // Self energy factor s_i (E_self = q_i^2 s_i) and its gamma derivatives:
 // the images of the charge itself (image 1 keeps only its multiple
-// reflections), split by exp(-k^2/4a) like the pair terms.
+// reflections), split by exp(-k^2/4a) like the pair terms.  Lane `lane` of
+// `nl` lanes takes the nodes n = lane (mod nl) of every node sum and lane 0
+// the closed-form terms: the nl shares add up to the factor.
 template <int PREC>
-__device__ double3 self_energy(const ShortConsts &c, const double *__restrict__ glx,
-                               const double *__restrict__ glw, const float *__restrict__ glxf,
-                               const float *__restrict__ glwf, double zi)
+__device__ __forceinline__ double3 self_energy(const ShortConsts &c,
+                                               const double *__restrict__ glx,
+                                               const double *__restrict__ glw,
+                                               const float *__restrict__ glxf,
+                                               const float *__restrict__ glwf, double zi,
+                                               int lane = 0, int nl = 1)
 {
     const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
     double s = 0.0, ds0 = 0.0, ds1 = 0.0;
@@ -167,20 +175,24 @@ __device__ double3 self_energy(const ShortConsts &c, const double *__restrict__ 
         if (b == 0.0 && db0 == 0.0 && db1 == 0.0) continue;
         const double hb = 0.5 * b;
         double Pv, Pd, Gv, Gd;
-        nodes_energy<PREC, true>(Kp, a, 0.0, 0, c, glx, glw, glxf, glwf, nullptr, Pv, Pd);
+        nodes_energy<PREC, true>(Kp, a, 0.0, 0, c, glx, glw, glxf, glwf, nullptr, Pv, Pd, lane,
+                                 nl);
         s += -hb * c.g12 * Pv;
         ds0 += -0.5 * db0 * c.g12 * Pv + hb * c.g2 * Pd;
         ds1 += -0.5 * db1 * c.g12 * Pv + hb * c.g1 * Pd;
         if (l == 0) {
-            nodes_energy<PREC, true>(Kg, 0.0, 0.0, 2, c, glx, glw, glxf, glwf, nullptr, Gv, Gd);
+            nodes_energy<PREC, true>(Kg, 0.0, 0.0, 2, c, glx, glw, glxf, glwf, nullptr, Gv, Gd,
+                                     lane, nl);
             s += hb * c.g12 * Gv;
             ds0 += -hb * c.g2 * Gd;
             ds1 += -hb * c.g1 * Gd;
         } else {
-            nodes_energy<PREC, true>(Kg, a, 0.0, 1, c, glx, glw, glxf, glwf, nullptr, Gv, Gd);
-            s += hb * Gv + hb / a;
-            ds0 += 0.5 * db0 * (Gv + 1.0 / a) - hb * c.g2 * Gd;
-            ds1 += 0.5 * db1 * (Gv + 1.0 / a) - hb * c.g1 * Gd;
+            nodes_energy<PREC, true>(Kg, a, 0.0, 1, c, glx, glw, glxf, glwf, nullptr, Gv, Gd,
+                                     lane, nl);
+            const double ia = lane == 0 ? 1.0 / a : 0.0;   // the closed form on lane 0
+            s += hb * Gv + hb * ia;
+            ds0 += 0.5 * db0 * (Gv + ia) - hb * c.g2 * Gd;
+            ds1 += 0.5 * db1 * (Gv + ia) - hb * c.g1 * Gd;
         }
     }
     const double f = 0.5 * c.inv2pieps;
@@ -538,6 +550,53 @@ __global__ __launch_bounds__(256) void k_gather_energy(
     }
 }
 
+// The fused forces + energy call's gather (mixed precision): per atom the
+// force records of its pairs (x, y, z: the force; w: the pair potentials
+// times the other charge) and the self terms, the self force and self
+// energy each split over the atom's 16 lanes by nodes.  Writes the force,
+// phi = dE/dq and psi = the self terms' d phi / d gamma (the pairs' share
+// of dE/dgamma is summed in the pair kernel).
+__global__ __launch_bounds__(256) void k_gather_fe(
+    int64_t n, int64_t npairs, const float4 *__restrict__ rec,
+    const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
+    const int32_t *__restrict__ i_idx, const int32_t *__restrict__ j_start,
+    const int32_t *__restrict__ j_end, ShortConsts c, const double *__restrict__ glx,
+    const double *__restrict__ glw, const float *__restrict__ glxf,
+    const float *__restrict__ glwf, const double *__restrict__ xyz,
+    const double *__restrict__ q, double *__restrict__ f, double *__restrict__ phi,
+    double *__restrict__ psi)
+{
+    const int64_t a = dev::gather_atom();
+    const int l = threadIdx.x & 15;
+    const bool live = a < n;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double s0 = 0.0, s1 = 0.0;
+    if (live) {
+        dev::add_endpoint_runs<float4, 4>(acc, rec, npairs, a, i_start, i_end, i_idx, j_start,
+                                          j_end, l);
+        const double qa = q[a], za = xyz[3 * a + 2];
+        acc[2] += dev::self_force_z<QEM_PREC_MIXED>(c, glx, glw, glxf, glwf, qa, za, l, 16);
+        const double3 se = self_energy<QEM_PREC_MIXED>(c, glx, glw, glxf, glwf, za, l, 16);
+        acc[3] += 2.0 * qa * se.x;
+        s0 = 2.0 * qa * se.y;
+        s1 = 2.0 * qa * se.z;
+    }
+    const double fx = dev::row_sum16(acc[0]);
+    const double fy = dev::row_sum16(acc[1]);
+    const double fz = dev::row_sum16(acc[2]);
+    const double ph = dev::row_sum16(acc[3]);
+    s0 = dev::row_sum16(s0);
+    s1 = dev::row_sum16(s1);
+    if (live && l == 0) {
+        f[3 * a] = fx;
+        f[3 * a + 1] = fy;
+        f[3 * a + 2] = fz;
+        phi[a] = ph;
+        psi[2 * a] = s0;
+        psi[2 * a + 1] = s1;
+    }
+}
+
 // self term of the force tangent per atom: out = scale (0, 0, dF_self,z)
 template <int PREC>
 __global__ __launch_bounds__(256) void k_self_tangent(
@@ -585,15 +644,19 @@ __global__ __launch_bounds__(256) void k_gather_tangent(
 // block b sums atoms [b*chunk, (b+1)*chunk) into part[b], one block sums the
 // parts in block order
 constexpr int kDotBlocks = 128;
+// Block b: the sums over atoms [b*chunk, (b+1)*chunk) of q phi and q psi_w
+// and, with gpart, the sums of its entries [b*gchunk, (b+1)*gchunk), each
+// thread a strided subset in order, the block in a fixed order.
 __global__ __launch_bounds__(256) void k_half_dots_part(int64_t n, const double *__restrict__ q,
                                                         const double *__restrict__ phi,
                                                         const double *__restrict__ psi,
+                                                        const double *__restrict__ gpart, int ng,
                                                         double *__restrict__ part)
 {
-    __shared__ double s[3][256];
+    __shared__ double s[5][256];
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-    double a = 0.0, b = 0.0, d = 0.0;
+    double a = 0.0, b = 0.0, d = 0.0, g1 = 0.0, g2 = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         a += q[i] * phi[i];
         if (psi) {
@@ -601,32 +664,46 @@ __global__ __launch_bounds__(256) void k_half_dots_part(int64_t n, const double 
             d += q[i] * psi[2 * i + 1];
         }
     }
+    if (gpart) {
+        const int gchunk = (ng + gridDim.x - 1) / gridDim.x;
+        const int k0 = blockIdx.x * gchunk, k1 = min(ng, k0 + gchunk);
+        for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+            g1 += gpart[2 * k];
+            g2 += gpart[2 * k + 1];
+        }
+    }
     s[0][threadIdx.x] = a;
     s[1][threadIdx.x] = b;
     s[2][threadIdx.x] = d;
+    s[3][threadIdx.x] = g1;
+    s[4][threadIdx.x] = g2;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int r = 0; r < 3; ++r) s[r][threadIdx.x] += s[r][threadIdx.x + w];
+            for (int r = 0; r < 5; ++r) s[r][threadIdx.x] += s[r][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int r = 0; r < 3; ++r) part[3 * blockIdx.x + r] = s[r][0];
+        for (int r = 0; r < 5; ++r) part[5 * blockIdx.x + r] = s[r][0];
 }
 
+// out3 = (1/2 sum q phi, 1/2 sum q psi_1 + sum gpart_1, 1/2 sum q psi_2 +
+// sum gpart_2) from the block parts, in block order
 __global__ void k_half_dots_final(int nb, const double *__restrict__ part,
                                   double *__restrict__ out3)
 {
     if (threadIdx.x != 0) return;
-    double a = 0.0, b = 0.0, d = 0.0;
+    double a = 0.0, b = 0.0, d = 0.0, g1 = 0.0, g2 = 0.0;
     for (int i = 0; i < nb; ++i) {
-        a += part[3 * i];
-        b += part[3 * i + 1];
-        d += part[3 * i + 2];
+        a += part[5 * i];
+        b += part[5 * i + 1];
+        d += part[5 * i + 2];
+        g1 += part[5 * i + 3];
+        g2 += part[5 * i + 4];
     }
     out3[0] = 0.5 * a;
-    out3[1] = 0.5 * b;
-    out3[2] = 0.5 * d;
+    out3[1] = 0.5 * b + g1;
+    out3[2] = 0.5 * d + g2;
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -688,21 +765,19 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     return hipGetLastError();
 }
 
-hipError_t launch_energy_gather(const ShortConsts &c, const ShortWork &ws,
-                                const NeighborList &nl, int64_t n, const double *d_xyz,
-                                const double *d_q, const float *d_epot, double *d_phi,
-                                double *d_psi, hipStream_t s)
+hipError_t launch_gather_forces_energy(const ShortConsts &c, const ShortWork &ws,
+                                       const NeighborList &nl, int64_t n, const double *d_xyz,
+                                       const double *d_q, const double *d_rec, double *d_f,
+                                       double *d_phi, double *d_psi, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
-    k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
-        c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(), ws.glwf.as<float>(), n,
-        d_xyz, d_q, d_phi, d_psi);
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
-    k_gather_energy<float4><<<nb, 256, 0, s>>>(
-        n, nl.npairs, reinterpret_cast<const float4 *>(d_epot), nl.i_start.as<int32_t>(),
-        nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), d_phi,
-        d_psi);
+    k_gather_fe<<<nb, 256, 0, s>>>(
+        n, nl.npairs, reinterpret_cast<const float4 *>(d_rec), nl.i_start.as<int32_t>(),
+        nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), c,
+        ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(), ws.glwf.as<float>(),
+        d_xyz, d_q, d_f, d_phi, d_psi);
     return hipGetLastError();
 }
 
@@ -760,13 +835,14 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
 }
 
 hipError_t launch_half_dots(int64_t n, const double *d_q, const double *d_phi,
-                            const double *d_psi, double *d_out3, double *d_part, hipStream_t s)
+                            const double *d_psi, double *d_out3, double *d_part, hipStream_t s,
+                            const double *d_gpart, int ngpart)
 {
-    k_half_dots_part<<<kDotBlocks, 256, 0, s>>>(n, d_q, d_phi, d_psi, d_part);
+    k_half_dots_part<<<kDotBlocks, 256, 0, s>>>(n, d_q, d_phi, d_psi, d_gpart, ngpart, d_part);
     k_half_dots_final<<<1, 64, 0, s>>>(kDotBlocks, d_part, d_out3);
     return hipGetLastError();
 }
 
-int half_dots_scratch() { return 3 * kDotBlocks; }
+int half_dots_scratch() { return 5 * kDotBlocks; }
 
 }  // namespace qem

@@ -875,15 +875,20 @@ int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
     API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
     int64_t Ptot = 0;
-    int st = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
-                       &Ptot);
+    // queued once the pair kernel is: the side stream's preparation (k rows,
+    // z sort, k = 0 term, per-k constants) runs beside it, and a caller that
+    // waits for each call does not see the GPU idle while the host queues
+    // that chain ahead of the pair kernel
     const std::function<int()> lanes = [&]() -> int {
+        const int st = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side,
+                                 false, &Ptot);
+        if (st) return st;
         API_HIP(ctx, hipEventRecord(ctx->lanes_ev, s));
         API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0));
         return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side);
     };
     bool deferred = false;
-    if (st == QEM_OK) st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz, &lanes, &deferred);
+    int st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz, &lanes, &deferred);
     // joined whatever happened, so that s never runs ahead of the side work
     const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
     const hipError_t ew = hipStreamWaitEvent(s, ctx->join_ev, 0);
@@ -973,16 +978,21 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     if (overlap) {
         API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
         API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+    }
+    // queued once the pair kernel is, and run beside it: k rows, z sort and
+    // the k = 0 term; then, once it is done, the potential passes
+    const std::function<void()> passes = [&]() {
         lst = stage_kvecs(ctx, kvecs, P, weight, true, ctx->side, &Ptot);
-        if (lst == QEM_OK) {
+        if (lst) return;
+        {
             PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
             hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, ctx->side, ctx->err);
             if (e == hipSuccess) e = launch_long_k0(lc, ctx->lw, n, ctx->side, ctx->err);
-            if (e != hipSuccess) lst = hip_err(ctx, e, "z sort");
+            if (e != hipSuccess) {
+                lst = hip_err(ctx, e, "z sort");
+                return;
+            }
         }
-    }
-    const std::function<void()> passes = [&]() {
-        if (lst) return;
         hipError_t e = hipEventRecord(ctx->lanes_ev, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
         if (e == hipSuccess) {
@@ -1075,15 +1085,20 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     if (overlap) {
         API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
         API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+    }
+    // queued once the pair tangent kernel is, and run beside it: k rows and
+    // z sort; then, once it is done, the tangent passes
+    const std::function<void()> lanes = [&]() {
         lst = stage_kvecs(ctx, kvecs, P, weight, false, ctx->side, &Ptot);
-        if (lst == QEM_OK) {
+        if (lst) return;
+        {
             PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
             hipError_t e = launch_zsort(ctx->lw, n, d_xyz, d_q, ctx->side, ctx->err);
-            if (e != hipSuccess) lst = hip_err(ctx, e, "z sort");
+            if (e != hipSuccess) {
+                lst = hip_err(ctx, e, "z sort");
+                return;
+            }
         }
-    }
-    const std::function<void()> lanes = [&]() {
-        if (lst) return;
         hipError_t e = hipEventRecord(ctx->lanes_ev, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
         if (e == hipSuccess) {
@@ -1159,23 +1174,23 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
     NeighborList &nl = ctx->nl;
     const ShortConsts c = short_consts(ctx, 0);
     const LongConsts lc = long_consts(ctx, w, QEM_LONG_ALL);
-    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(nl.npairs, 16, 2)));
-    API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (2 * (size_t)n + 4 + half_dots_scratch())));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(nl.npairs, 16)));
+    const int ngp = pair_fe_blocks(nl.npairs);
+    API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (2 * (size_t)n + 4 + half_dots_scratch() +
+                                                      2 * (size_t)ngp)));
     double *psi = ctx->d_adj.as<double>();          // N x 2
     double *out3 = psi + 2 * n;                      // 3 (+1 pad)
     double *part = out3 + 4;                         // reduction scratch
+    double *gpart = part + half_dots_scratch();      // the pair kernel's dE/dgamma per block
     double *phi = d_phi;
     if (!phi) {
         API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
         phi = ctx->d_f2.as<double>();
     }
-    // side stream: k rows, z sort, the k = 0 term, per-k constants beside
-    // the pair kernel
     API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
     API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
     int64_t Ptot = 0;
-    int lst = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
-                        &Ptot);
+    int lst = QEM_OK;
     int st = QEM_OK;
     const double *px = d_xyz, *pq = d_q;
     {
@@ -1186,10 +1201,15 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             if (e == hipSuccess)
                 e = launch_pair_forces_energy(c, ctx->sw, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
                                               nl.jpos.as<int32_t>(), nl.npairs, px, pq,
-                                              ctx->sw.fpair.as<double>(), s);
+                                              ctx->sw.fpair.as<double>(), gpart, s);
             if (e != hipSuccess) st = hip_err(ctx, e, "fused pair kernel");
         }
     }
+    // side stream, queued after the pair kernel and run beside it: k rows,
+    // z sort, the k = 0 term, per-k constants
+    if (st == QEM_OK)
+        lst = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
+                        &Ptot);
     // side stream, once the pair kernel is done: the force passes, then the
     // potential pass on their chunk sums
     if (lst == QEM_OK && st == QEM_OK) {
@@ -1203,18 +1223,15 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             if (e != hipSuccess) lst = hip_err(ctx, e, "long-range energy");
         }
     }
-    // call's stream: the force gather (self term included), then the energy
-    // self terms and gather, all in slot order
+    // call's stream: one gather for the forces, dE/dq and the self terms, in
+    // slot order
     if (st == QEM_OK) {
         PhaseScope ps(ctx, QEM_PHASE_GATHER, s);
-        const float *epot = reinterpret_cast<const float *>(
-            reinterpret_cast<const float4 *>(ctx->sw.fpair.p) + 2 * nl.npairs);
-        hipError_t e = launch_gather_self(c, QEM_PREC_MIXED, ctx->sw, nl, n, px, pq,
-                                          ctx->sw.fpair.as<double>(), nl.sout.as<double>(), s);
-        if (e == hipSuccess)
-            e = launch_energy_gather(c, ctx->sw, nl, n, px, pq, epot, nl.sout2.as<double>(),
-                                     nl.sout2.as<double>() + n, s);
-        if (e != hipSuccess) st = hip_err(ctx, e, "gathers");
+        hipError_t e = launch_gather_forces_energy(c, ctx->sw, nl, n, px, pq,
+                                                   ctx->sw.fpair.as<double>(),
+                                                   nl.sout.as<double>(), nl.sout2.as<double>(),
+                                                   nl.sout2.as<double>() + n, s);
+        if (e != hipSuccess) st = hip_err(ctx, e, "gather");
     }
     // joined whatever happened, so that s never runs ahead of the side work
     const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
@@ -1236,7 +1253,7 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             e = launch_long_energy_sum(ctx->lw, n, Ptot, phi, 1, hq0.data(), s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "output passes");
     }
-    API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s));
+    API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s, gpart, ngp));
     double h[3];
     API_HIP(ctx, hipMemcpyAsync(h, out3, sizeof h, hipMemcpyDeviceToHost, s));
     API_HIP(ctx, hipStreamSynchronize(s));

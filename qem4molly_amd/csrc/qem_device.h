@@ -353,5 +353,110 @@ inline unsigned pair_grid(int64_t m, int block = 256)
     return (unsigned)(want >= 8 ? want / 8 * 8 : std::max<int64_t>(1, want));
 }
 
+// Self-image z force F_short_i (src/Force_short.jl:82-107).  Images 1 and 4
+// have sign_a = 0 and contribute exactly zero; they are skipped.  Lane
+// `lane` of `nl` lanes takes the nodes n = lane (mod nl) of every node sum,
+// and lane 0 the closed-form image sum: the nl shares add up to the force
+// (nl = 1 gives it whole).
+template <int PREC>
+__device__ __forceinline__ double self_force_z(const ShortConsts &c, const double *__restrict__ glx,
+                               const double *__restrict__ glw, const float *__restrict__ glxf,
+                               const float *__restrict__ glwf, double qi, double zi,
+                               int lane = 0, int nl = 1)
+{
+    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
+    double P1 = 0.0, P2 = 0.0, G = 0.0;
+#pragma unroll
+    for (int l = 1; l <= 2; ++l) {
+        const double a = l == 1 ? 2.0 * zi : 2.0 * c.Lz - 2.0 * zi;
+        const double b = l == 1 ? c.g1 : c.g2;
+        const double s = l == 1 ? -1.0 : 1.0;
+        const double kf1 = -c.ln_acc / (2.0 * c.Lz + a);
+        const double kf2 = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc) -
+                           2.0 * c.alpha * a;
+        const double Kp = swap ? kf1 : kf2;
+        const double Kg = swap ? kf2 : kf1;
+        double pz = 0.0, gz = 0.0;
+        if (b != 0.0) {
+            // J0(0) = 1: the node sums reduce to sum w k num / D
+            for (int pass = 0; pass < (c.single ? 1 : 2); ++pass) {
+                const bool gauss = pass == 1;
+                const double K = gauss ? Kg : Kp;
+                double acc = 0.0;
+                if (PREC != QEM_PREC_MIXED) {
+                    // the folded FP64 node terms of nodes_fp64 with J0(0) = 1
+                    const double h = 0.5 * K;
+                    const double hE = h * c.d_c2, rA = a * c.d_hinvLz, rP = 1.0 + rA;
+                    double ad = 0.0;
+                    for (int n = lane; n < c.Nt; n += nl) {
+                        const double2 u = c.ncon64[n];
+                        const double e = hE * u.x;
+                        const double E = exp2_fp64(e);
+                        const double y = gauss ? fma(e, fma(e, c.d_rG, rA), u.y) : fma(e, rP, u.y);
+                        ad = fma(exp2_fp64(y), rcp_fp64(fma(c.g12, E, -1.0)), ad);
+                    }
+                    acc = h * h * ad;
+                } else {
+                    const float h = 0.5f * (float)K;
+                    const float c2 = -2.0f * (float)c.Lz * kLog2e;
+                    const float ca = -(float)a * kLog2e;
+                    const float cg = -kLog2e * (float)(0.25 / c.alpha);
+                    const float g12 = (float)c.g12;
+                    float af = 0.f;
+                    for (int n = lane; n < c.Nt; n += nl) {
+                        const float k = fmaf(h, glxf[n], h);
+                        const float w = h * glwf[n];
+                        const float E = __builtin_amdgcn_exp2f(k * c2);
+                        const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
+                        const float num = gauss ? __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca))
+                                                : __builtin_amdgcn_exp2f(k * ca) * E;
+                        af = fmaf(w * k, num * rD, af);
+                    }
+                    acc = af;
+                }
+                if (gauss) gz = acc; else pz = acc;
+            }
+        }
+        const double hb = 0.5 * b;
+        P1 += (hb * c.g12 * s) * pz;
+        if (lane == 0) P2 += b * a * s / (a * a * a);
+        G += (hb * s) * gz;
+    }
+    P2 *= 0.5;
+    const double Fsz = -P1 + P2 + G;
+    return -(qi * qi) * c.inv2pieps * Fsz;
+}
+
+
+// Sums of a and b over the block in a fixed order (rows of 16 lanes, the
+// four rows of a wave, the waves in order); the results land in thread 0.
+// red holds two doubles per wave; every thread of the block must call.
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *red)
+{
+    a = row_sum16(a);
+    b = row_sum16(b);
+    auto lane_d = [](double v, int lane) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+        return __hiloint2double(hi, lo);
+    };
+    const double wa = ((lane_d(a, 0) + lane_d(a, 16)) + lane_d(a, 32)) + lane_d(a, 48);
+    const double wb = ((lane_d(b, 0) + lane_d(b, 16)) + lane_d(b, 32)) + lane_d(b, 48);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = wa;
+        red[2 * w + 1] = wb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = red[0];
+        b = red[1];
+        for (int k = 1; k < nw; ++k) {
+            a += red[2 * k];
+            b += red[2 * k + 1];
+        }
+    }
+}
+
 }  // namespace dev
 }  // namespace qem

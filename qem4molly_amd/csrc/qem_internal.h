@@ -212,15 +212,17 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision,
                               const int32_t *d_pj, const int32_t *d_jpos, int64_t npairs,
                               const double *d_xyz, const double *d_q,
                               double *d_fpair, hipStream_t s);
-// Mixed precision: the force records of launch_pair_forces (float4,
-// 2 npairs of them) and, after them, the potential records (float4,
-// 2 npairs: first atom q_j (G, dG/dgamma_1, dG/dgamma_2, 0), second atom
-// q_i (...)) from the same node loops.
+// Mixed precision: the force records of launch_pair_forces (float4), whose
+// w carries the pair potential times the other endpoint's charge (first
+// atom q_j G, second atom q_i G), from the same node loops; and per block of
+// the launch (pair_fe_blocks of them) the pairs' sum of
+// (q_i q_j dG/dgamma_1, q_i q_j dG/dgamma_2) in d_gpart (2 doubles each).
+int pair_fe_blocks(int64_t npairs);
 hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                      const int32_t *d_pi, const int32_t *d_pj,
                                      const int32_t *d_jpos, int64_t npairs,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
-                                     hipStream_t s);
+                                     double *d_gpart, hipStream_t s);
 // Pair records of the force path (launch_pair_forces -> launch_gather_self):
 // float4 for the mixed-precision list kernel, double4 otherwise (FP64, and
 // the pole-splitting kernels).
@@ -357,13 +359,14 @@ hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double 
 // have just formed: the two share passes A and S.
 hipError_t launch_long_energy_B(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                 hipStream_t s, std::string &err);
-// Short-range energy from the potential records of
-// launch_pair_forces_energy (mixed precision): the self terms into d_phi,
-// d_psi (overwrite), then every atom's pair terms added.
-hipError_t launch_energy_gather(const ShortConsts &c, const ShortWork &ws,
-                                const NeighborList &nl, int64_t n, const double *d_xyz,
-                                const double *d_q, const float *d_epot, double *d_phi,
-                                double *d_psi, hipStream_t s);
+// Per atom from the records of launch_pair_forces_energy (mixed precision)
+// and the self terms: the short-range force into d_f (N x 3), dE/dq into
+// d_phi (N) and the self terms' d phi / d gamma_1,2 into d_psi (N x 2), all
+// overwritten.
+hipError_t launch_gather_forces_energy(const ShortConsts &c, const ShortWork &ws,
+                                       const NeighborList &nl, int64_t n, const double *d_xyz,
+                                       const double *d_q, const double *d_rec, double *d_f,
+                                       double *d_phi, double *d_psi, hipStream_t s);
 // Tangent of the long-range force along d_v (N x 3), times scale.  Without
 // with_sum the call stops at the per-group partial tangents (lw.partB);
 // launch_long_tangent_sum then adds (or writes) them into d_out, possibly
@@ -396,9 +399,11 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                                 double scale, double *d_out, hipStream_t s,
                                 const std::function<void()> *after_pair = nullptr);
 // out3 = (1/2 q.phi, 1/2 q.psi_1, 1/2 q.psi_2); psi may be null; d_part is
-// scratch of half_dots_scratch() doubles.
+// scratch of half_dots_scratch() doubles.  With d_gpart, the sums of its
+// ngpart pairs of doubles are added to out3[1] and out3[2].
 hipError_t launch_half_dots(int64_t n, const double *d_q, const double *d_phi,
-                            const double *d_psi, double *d_out3, double *d_part, hipStream_t s);
+                            const double *d_psi, double *d_out3, double *d_part, hipStream_t s,
+                            const double *d_gpart = nullptr, int ngpart = 0);
 int half_dots_scratch();
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);

@@ -345,79 +345,7 @@ __device__ __forceinline__ PairForce pair_force(const ShortConsts &c, const doub
     return out;
 }
 
-// Self-image z force F_short_i (src/Force_short.jl:82-107).  Images 1 and 4
-// have sign_a = 0 and contribute exactly zero; they are skipped.  Lane
-// `lane` of `nl` lanes takes the nodes n = lane (mod nl) of every node sum,
-// and lane 0 the closed-form image sum: the nl shares add up to the force
-// (nl = 1 gives it whole).
-template <int PREC>
-__device__ double self_force_z(const ShortConsts &c, const double *__restrict__ glx,
-                               const double *__restrict__ glw, const float *__restrict__ glxf,
-                               const float *__restrict__ glwf, double qi, double zi,
-                               int lane = 0, int nl = 1)
-{
-    const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
-    double P1 = 0.0, P2 = 0.0, G = 0.0;
-#pragma unroll
-    for (int l = 1; l <= 2; ++l) {
-        const double a = l == 1 ? 2.0 * zi : 2.0 * c.Lz - 2.0 * zi;
-        const double b = l == 1 ? c.g1 : c.g2;
-        const double s = l == 1 ? -1.0 : 1.0;
-        const double kf1 = -c.ln_acc / (2.0 * c.Lz + a);
-        const double kf2 = sqrt(4.0 * c.alpha * c.alpha * a * a - 4.0 * c.alpha * c.ln_acc) -
-                           2.0 * c.alpha * a;
-        const double Kp = swap ? kf1 : kf2;
-        const double Kg = swap ? kf2 : kf1;
-        double pz = 0.0, gz = 0.0;
-        if (b != 0.0) {
-            // J0(0) = 1: the node sums reduce to sum w k num / D
-            for (int pass = 0; pass < (c.single ? 1 : 2); ++pass) {
-                const bool gauss = pass == 1;
-                const double K = gauss ? Kg : Kp;
-                double acc = 0.0;
-                if (PREC != QEM_PREC_MIXED) {
-                    // the folded FP64 node terms of nodes_fp64 with J0(0) = 1
-                    const double h = 0.5 * K;
-                    const double hE = h * c.d_c2, rA = a * c.d_hinvLz, rP = 1.0 + rA;
-                    double ad = 0.0;
-                    for (int n = lane; n < c.Nt; n += nl) {
-                        const double2 u = c.ncon64[n];
-                        const double e = hE * u.x;
-                        const double E = dev::exp2_fp64(e);
-                        const double y = gauss ? fma(e, fma(e, c.d_rG, rA), u.y) : fma(e, rP, u.y);
-                        ad = fma(dev::exp2_fp64(y), dev::rcp_fp64(fma(c.g12, E, -1.0)), ad);
-                    }
-                    acc = h * h * ad;
-                } else {
-                    const float h = 0.5f * (float)K;
-                    const float c2 = -2.0f * (float)c.Lz * kLog2e;
-                    const float ca = -(float)a * kLog2e;
-                    const float cg = -kLog2e * (float)(0.25 / c.alpha);
-                    const float g12 = (float)c.g12;
-                    float af = 0.f;
-                    for (int n = lane; n < c.Nt; n += nl) {
-                        const float k = fmaf(h, glxf[n], h);
-                        const float w = h * glwf[n];
-                        const float E = __builtin_amdgcn_exp2f(k * c2);
-                        const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
-                        const float num = gauss ? __builtin_amdgcn_exp2f(k * fmaf(k, cg, ca))
-                                                : __builtin_amdgcn_exp2f(k * ca) * E;
-                        af = fmaf(w * k, num * rD, af);
-                    }
-                    acc = af;
-                }
-                if (gauss) gz = acc; else pz = acc;
-            }
-        }
-        const double hb = 0.5 * b;
-        P1 += (hb * c.g12 * s) * pz;
-        if (lane == 0) P2 += b * a * s / (a * a * a);
-        G += (hb * s) * gz;
-    }
-    P2 *= 0.5;
-    const double Fsz = -P1 + P2 + G;
-    return -(qi * qi) * c.inv2pieps * Fsz;
-}
+using dev::self_force_z;
 
 // FP64 blocks are 512 threads: the FP64 Bessel table (~47 KB at the 100k
 // slab) is staged once per block, so two blocks of eight waves fill a CU to
@@ -443,27 +371,35 @@ __global__ __launch_bounds__(pair_block<PREC>()) void k_pair_list(
     });
 }
 
-// The force pair kernel with the pair potentials (mixed precision): the
-// force records of k_pair_list in fpair, and in epot the potential records
-// q_j (G, dG/dgamma_1, dG/dgamma_2, 0) for the first atom and q_i (...) for
-// the second, placed like the force records.
+// The force pair kernel with the pair potentials (mixed precision).  The
+// force records of k_pair_list carry in w what the pair adds to the
+// endpoint's dE/dq: q_j G for the first atom, q_i G for the second (G the
+// pair potential without the charges).  The pair's share of dE/dgamma_1,2,
+// q_i q_j dG/dgamma, is summed per lane, then per block in a fixed order,
+// into gpart[blockIdx.x].
 __global__ __launch_bounds__(256) void k_pair_list_fe(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
     const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
     const double *__restrict__ xyz, const double *__restrict__ q, float4 *__restrict__ fpair,
-    float4 *__restrict__ epot)
+    double2 *__restrict__ gpart)
 {
     extern __shared__ float4 lds_tab[];
+    __shared__ double red[8];
     const float4 *tab = dev::stage_tables<QEM_PREC_MIXED>(c, gtab, lds_tab);
+    double g1 = 0.0, g2 = 0.0;
     dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
         double4 u;
         const PairForce f = pair_force<QEM_PREC_MIXED, true>(
             c, glx, glw, ncon, tab, o.qi, o.qj, o.xi, o.yi, o.zi, o.xj, o.yj, o.zj, &u);
-        store_recs(fpair, npairs, p, o.jp, f);
-        dev::store4(epot + p, o.qj * u.x, o.qj * u.y, o.qj * u.z, 0.0);
-        dev::store4(epot + npairs + o.jp, o.qi * u.x, o.qi * u.y, o.qi * u.z, 0.0);
+        dev::store4(fpair + p, f.fx, f.fy, f.fz, o.qj * u.x);
+        dev::store4(fpair + npairs + o.jp, -f.fx, -f.fy, f.fzj, o.qi * u.x);
+        const double qq = o.qi * o.qj;
+        g1 = fma(qq, u.y, g1);
+        g2 = fma(qq, u.z, g2);
     });
+    dev::block_sum2(g1, g2, red);
+    if (threadIdx.x == 0) gpart[blockIdx.x] = make_double2(g1, g2);
 }
 
 template <int PREC>
@@ -580,19 +516,21 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
     return hipGetLastError();
 }
 
+int pair_fe_blocks(int64_t npairs) { return npairs > 0 ? (int)pair_grid(npairs) : 0; }
+
 hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                      const int32_t *d_pi, const int32_t *d_pj,
                                      const int32_t *d_jpos, int64_t npairs,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
-                                     hipStream_t s)
+                                     double *d_gpart, hipStream_t s)
 {
     if (npairs <= 0) return hipSuccess;
     if (c.pole || !c.nconi) return hipErrorInvalidValue;
-    float4 *rec = reinterpret_cast<float4 *>(d_fpair);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     k_pair_list_fe<<<pair_grid(npairs), 256, lds, s>>>(
         c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), ws.jtab.as<float4>(),
-        d_pi, d_pj, d_jpos, npairs, d_xyz, d_q, rec, rec + 2 * npairs);
+        d_pi, d_pj, d_jpos, npairs, d_xyz, d_q, reinterpret_cast<float4 *>(d_fpair),
+        reinterpret_cast<double2 *>(d_gpart));
     return hipGetLastError();
 }
 

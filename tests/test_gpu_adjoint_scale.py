@@ -13,8 +13,8 @@ small-system mixed-precision energy test);
 every step's outputs are finite, the last step's are bitwise repeatable,
 the one-call forces + energy (qem_forces_energy_dev) gives the same forces
 bit for bit and the energy within the same bars, and the context's device
-memory stays under 2.2 GB (the endpoint records of the fused forces +
-energy call take 64 bytes per pair, 1.0 GB at 15.7 million pairs).
+memory stays under 1.6 GB (measured 1.49 GB: the endpoint records take
+32 bytes per pair, 0.5 GB at 15.7 million pairs).
 """
 import numpy as np
 import pytest
@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 STEPS = 10
 SIGMA = 0.01
-MAX_DEVICE_BYTES = 2.2e9
+MAX_DEVICE_BYTES = 1.6e9
 
 
 def _perturb(xyz, rng, cfg):
