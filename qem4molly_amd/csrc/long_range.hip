@@ -1452,7 +1452,8 @@ hipError_t launch_long_variance(const LongConsts &c, LongWork &lw, int64_t n, in
 }
 
 hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t n,
-                                     int64_t Ptot, hipStream_t s, std::string &err)
+                                     int64_t Ptot, hipStream_t s, std::string &err,
+                                     const std::function<hipError_t()> *between)
 {
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
@@ -1478,6 +1479,7 @@ hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t 
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partA.as<double4>(), Ptot, lg.nchunk,
                                            lw.offA.as<double4>(), lw.totA.as<double4>());
     LR_CHECK(hipGetLastError());
+    if (between) LR_CHECK((*between)());
     return launch_long_energy_B(c, lw, n, Ptot, s, err);
 }
 
@@ -1558,7 +1560,8 @@ hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double
 
 hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                const double *d_v, double scale, double *d_out, int accumulate,
-                               hipStream_t s, std::string &err, bool with_sum)
+                               hipStream_t s, std::string &err, bool with_sum,
+                               const std::function<hipError_t()> *between)
 {
     if (n <= 0 || Ptot <= 0) return hipSuccess;
     const LaneGrid lg = lane_grid(n, Ptot);
@@ -1596,6 +1599,7 @@ hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int
     k_long_S<<<(unsigned)Ptot, CH, 0, s>>>(lw.partT.as<double4>(), Ptot, lg.nchunk,
                                            lw.offT.as<double4>(), lw.totT.as<double4>());
     LR_CHECK(hipGetLastError());
+    if (between) LR_CHECK((*between)());
     // the self term differs from the non-sort sum's own term unless both
     // parts are on or both off
     const int own_term = ((c.parts & QEM_LONG_NONSORT) != 0) != ((c.parts & QEM_LONG_SELF) != 0);

@@ -880,12 +880,18 @@ int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     // waits for each call does not see the GPU idle while the host queues
     // that chain ahead of the pair kernel
     const std::function<int()> lanes = [&]() -> int {
-        const int st = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side,
-                                 false, &Ptot);
+        int st = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
+                           &Ptot);
         if (st) return st;
         API_HIP(ctx, hipEventRecord(ctx->lanes_ev, s));
         API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0));
-        return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side);
+        // passes A and S with the GPU to themselves, then the gather (on s)
+        // beside pass B: beside the gather's waves, the latency-bound chunk
+        // scan waits for wave slots
+        if ((st = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassAS))) return st;
+        API_HIP(ctx, hipEventRecord(ctx->sums_ev, ctx->side));
+        API_HIP(ctx, hipStreamWaitEvent(s, ctx->sums_ev, 0));
+        return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassB);
     };
     bool deferred = false;
     int st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz, &lanes, &deferred);
@@ -995,9 +1001,14 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         }
         hipError_t e = hipEventRecord(ctx->lanes_ev, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
+        // the self term and the gather on s wait for passes A and S
+        const std::function<hipError_t()> sums = [&]() -> hipError_t {
+            hipError_t r = hipEventRecord(ctx->sums_ev, ctx->side);
+            return r == hipSuccess ? hipStreamWaitEvent(s, ctx->sums_ev, 0) : r;
+        };
         if (e == hipSuccess) {
             PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
-            e = launch_long_energy_passes(lc, ctx->lw, n, Ptot, ctx->side, ctx->err);
+            e = launch_long_energy_passes(lc, ctx->lw, n, Ptot, ctx->side, ctx->err, &sums);
         }
         if (e != hipSuccess) lst = hip_err(ctx, e, "long-range energy");
     };
@@ -1101,10 +1112,15 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
         }
         hipError_t e = hipEventRecord(ctx->lanes_ev, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
+        // the self term and the gather on s wait for passes A and S
+        const std::function<hipError_t()> sums = [&]() -> hipError_t {
+            hipError_t r = hipEventRecord(ctx->sums_ev, ctx->side);
+            return r == hipSuccess ? hipStreamWaitEvent(s, ctx->sums_ev, 0) : r;
+        };
         if (e == hipSuccess) {
             PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
             e = launch_long_tangent(lc, ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, ctx->side,
-                                    ctx->err, false);
+                                    ctx->err, false, &sums);
         }
         if (e != hipSuccess) lst = hip_err(ctx, e, "long-range tangent");
     };
@@ -1210,13 +1226,22 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
     if (st == QEM_OK)
         lst = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
                         &Ptot);
-    // side stream, once the pair kernel is done: the force passes, then the
-    // potential pass on their chunk sums
+    // side stream, once the pair kernel is done: passes A and S with the GPU
+    // to themselves, then (beside the gather, which waits for them) the
+    // force pass B and the potential pass on their chunk sums
     if (lst == QEM_OK && st == QEM_OK) {
         hipError_t e = hipEventRecord(ctx->lanes_ev, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
         if (e != hipSuccess) lst = hip_err(ctx, e, "lanes event");
-        if (lst == QEM_OK) lst = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side);
+        if (lst == QEM_OK)
+            lst = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassAS);
+        if (lst == QEM_OK) {
+            e = hipEventRecord(ctx->sums_ev, ctx->side);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->sums_ev, 0);
+            if (e != hipSuccess) lst = hip_err(ctx, e, "sums event");
+        }
+        if (lst == QEM_OK)
+            lst = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassB);
         if (lst == QEM_OK) {
             PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
             e = launch_long_energy_B(lc, ctx->lw, n, Ptot, ctx->side, ctx->err);
@@ -1353,7 +1378,8 @@ int qem_create(const qem_params *p, qem_ctx **out)
         hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->lanes_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->sums_ev, hipEventDisableTiming) != hipSuccess)
         return fail(QEM_ERR_HIP);
     ctx->gl = make_gl_rule(p->N_t);
     std::vector<float> xf(ctx->gl.x.begin(), ctx->gl.x.end()), wf(ctx->gl.w.begin(), ctx->gl.w.end());
@@ -1513,6 +1539,7 @@ int qem_destroy(qem_ctx *ctx)
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->lanes_ev) (void)hipEventDestroy(ctx->lanes_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->sums_ev) (void)hipEventDestroy(ctx->sums_ev);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

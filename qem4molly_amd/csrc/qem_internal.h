@@ -350,7 +350,8 @@ hipError_t launch_long_energy(const LongConsts &c, LongWork &lw, int64_t n, int6
 // the phi sum into d_phi and the copy of the per-k dE_k/dgamma into h_q0
 // (2 Ptot doubles, summed by the caller once s is done).
 hipError_t launch_long_energy_passes(const LongConsts &c, LongWork &lw, int64_t n,
-                                     int64_t Ptot, hipStream_t s, std::string &err);
+                                     int64_t Ptot, hipStream_t s, std::string &err,
+                                     const std::function<hipError_t()> *between = nullptr);
 hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double *d_phi,
                                   int accumulate, double *h_q0, hipStream_t s,
                                   std::string &err);
@@ -367,13 +368,16 @@ hipError_t launch_gather_forces_energy(const ShortConsts &c, const ShortWork &ws
                                        const NeighborList &nl, int64_t n, const double *d_xyz,
                                        const double *d_q, const double *d_rec, double *d_f,
                                        double *d_phi, double *d_psi, hipStream_t s);
-// Tangent of the long-range force along d_v (N x 3), times scale.  Without
-// with_sum the call stops at the per-group partial tangents (lw.partB);
+// Tangent of the long-range force along d_v (N x 3), times scale; between,
+// when given, is called once the chunk sums and offsets are queued, before
+// pass B.  Without with_sum the call stops at the per-group partial
+// tangents (lw.partB);
 // launch_long_tangent_sum then adds (or writes) them into d_out, possibly
 // on another stream once the passes are done.
 hipError_t launch_long_tangent(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                const double *d_v, double scale, double *d_out, int accumulate,
-                               hipStream_t s, std::string &err, bool with_sum = true);
+                               hipStream_t s, std::string &err, bool with_sum = true,
+                               const std::function<hipError_t()> *between = nullptr);
 hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double scale,
                                    double *d_out, int accumulate, hipStream_t s,
                                    std::string &err);
@@ -438,7 +442,7 @@ struct qem_ctx {
     // on the call's stream: fork_ev orders it after the inputs, lanes_ev its
     // k-lane passes after the pair kernel, join_ev the final sum after it
     hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, lanes_ev = nullptr, join_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, lanes_ev = nullptr, join_ev = nullptr, sums_ev = nullptr;
     // per-k dE_k/dgamma of the energy call, copied on the call's stream; kept
     // here, not on the stack, so an early error return leaves no copy
     // pending into freed memory
