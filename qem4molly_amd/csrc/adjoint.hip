@@ -370,8 +370,8 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
     const double zn = fabs(ri[2] - rj[2]), zp = ri[2] + rj[2], sg = jl_sign(ri[2] - rj[2]);
     const double dzn = sg * (vi[2] - vj[2]), dzp = vi[2] + vj[2];
     const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
-    double Fsr = 0.0, dFsr = 0.0, Fsz = 0.0, dFsz = 0.0;
-    double part[4], dpart[4];
+    double Fsr = 0.0, dFsr = 0.0, dFsz = 0.0;
+    double dFszj = 0.0;   // the z tangent on j of the as-written Newton semantics
     for (int l = 0; l < 4; ++l) {
         double a, b, s;
         dev::image(c, l, zn, zp, sg, a, b, s);
@@ -384,7 +384,7 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
         const double dinv32 = -3.0 * inv32 * (a * da + rho * drho) / r2;
         const double hb = 0.5 * b;
         const double cr = hb * rho * inv32, dcr = hb * (drho * inv32 + rho * dinv32);
-        const double cz = hb * a * s * inv32, dcz = hb * s * (da * inv32 + a * dinv32);
+        const double dcz = hb * s * (da * inv32 + a * dinv32);
         Dual4 P{0.0, 0.0, 0.0, 0.0}, G{0.0, 0.0, 0.0, 0.0};
         if (b != 0.0) {
             P = nodes_tangent<PREC>(Kp, dKp, a, da, rho, drho, false, c, glx, glw, glxf, glwf,
@@ -395,10 +395,10 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
         }
         Fsr += -(hb * c.g12) * P.sr + cr + hb * G.sr;
         dFsr += -(hb * c.g12) * P.dsr + dcr + hb * G.dsr;
-        part[l] = (hb * c.g12 * s) * P.sz - cz - (hb * s) * G.sz;
-        dpart[l] = (hb * c.g12 * s) * P.dsz - dcz - (hb * s) * G.dsz;
-        Fsz += part[l];
-        dFsz += dpart[l];
+        const double dpart = (hb * c.g12 * s) * P.dsz - dcz - (hb * s) * G.dsz;
+        dFsz += dpart;
+        // images 0 and 3 flip sign on the second atom, 1 and 2 do not
+        dFszj += (l == 0 || l == 3) ? -dpart : dpart;
     }
     const double pref = qi * qj * c.inv2pieps;
     double4 o = make_double4(0.0, 0.0, pref * dFsz, 0.0);
@@ -410,10 +410,9 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
         o.y = pref * (dFsr * uy * ir + Fsr * (ddy * ir - uy * drho * ir * ir));
     }
     if (c.sem & QEM_SEM_NEWTON_IMAGES)
-        o.w = pref * (-dpart[0] + dpart[1] + dpart[2] - dpart[3]);
+        o.w = pref * dFszj;
     else
         o.w = -o.z;
-    (void)Fsz;
     return o;
 }
 

@@ -221,27 +221,27 @@ __device__ __forceinline__ void add_rec(double *acc, const Rec &r)
     if (NC > 3) acc[3] += (double)r.w;
 }
 
-// Lane l of an atom's sixteen adds records t = s + l, s + l + 16, ... of
+// Lane l of an atom's L lanes adds records t = s + l, s + l + L, ... of
 // run [s, e) in that order, four loads in flight.
-template <typename Rec, int NC>
+template <typename Rec, int NC, int L = 16>
 __device__ __forceinline__ void add_run(double *acc, const Rec *__restrict__ rec, int32_t s,
                                         int32_t e, int l)
 {
     int32_t t = s + l;
-    for (; t + 48 < e; t += 64) {
-        const Rec r0 = rec[t], r1 = rec[t + 16], r2 = rec[t + 32], r3 = rec[t + 48];
+    for (; t + 3 * L < e; t += 4 * L) {
+        const Rec r0 = rec[t], r1 = rec[t + L], r2 = rec[t + 2 * L], r3 = rec[t + 3 * L];
         add_rec<Rec, NC>(acc, r0);
         add_rec<Rec, NC>(acc, r1);
         add_rec<Rec, NC>(acc, r2);
         add_rec<Rec, NC>(acc, r3);
     }
-    for (; t < e; t += 16) add_rec<Rec, NC>(acc, rec[t]);
+    for (; t < e; t += L) add_rec<Rec, NC>(acc, rec[t]);
 }
 
 // This lane's share of atom a's records: its first-endpoint run (through
 // i_idx for a list that is not grouped by first atom), then its
 // second-endpoint run in the second half.
-template <typename Rec, int NC>
+template <typename Rec, int NC, int L = 16>
 __device__ __forceinline__ void add_endpoint_runs(double *acc, const Rec *__restrict__ rec,
                                                   int64_t npairs, int64_t a,
                                                   const int32_t *__restrict__ i_start,
@@ -252,22 +252,36 @@ __device__ __forceinline__ void add_endpoint_runs(double *acc, const Rec *__rest
 {
     if (i_idx) {
         const int32_t e = i_end[a];
-        for (int32_t t = i_start[a] + l; t < e; t += 16) add_rec<Rec, NC>(acc, rec[i_idx[t]]);
+        for (int32_t t = i_start[a] + l; t < e; t += L) add_rec<Rec, NC>(acc, rec[i_idx[t]]);
     } else {
-        add_run<Rec, NC>(acc, rec, i_start[a], i_end[a], l);
+        add_run<Rec, NC, L>(acc, rec, i_start[a], i_end[a], l);
     }
-    add_run<Rec, NC>(acc, rec + npairs, j_start[a], j_end[a], l);
+    add_run<Rec, NC, L>(acc, rec + npairs, j_start[a], j_end[a], l);
+}
+
+// Sum over each group of L (8 or 16) lanes in a fixed order (pairs, quads,
+// half rows[, rows]); every lane of the group receives its group's sum.
+template <int L>
+__device__ __forceinline__ double lane_group_sum(double v)
+{
+    static_assert(L == 8 || L == 16, "lanes per atom");
+    v += dpp_perm_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_perm_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_perm_f64<0x141>(v);  // row_half_mirror
+    if (L == 16) v += dpp_perm_f64<0x140>(v);  // row_mirror
+    return v;
 }
 
 // XCD-aware atom order of the gathers: the hardware deals blocks to the 8
 // XCDs round robin, so block b runs on XCD b % 8; each XCD takes a
 // contiguous range of atoms, whose records then share that XCD's L2
-// (gridDim.x a multiple of 8).  Sixteen lanes per atom.
+// (gridDim.x a multiple of 8).  L lanes per atom.
+template <int L = 16>
 __device__ __forceinline__ int64_t gather_atom()
 {
     const int64_t per_xcd = gridDim.x / 8;
     const int64_t blk = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    return (blk * blockDim.x + threadIdx.x) >> 4;
+    return (blk * blockDim.x + threadIdx.x) / L;
 }
 
 // One pair's endpoints, positions and charges.

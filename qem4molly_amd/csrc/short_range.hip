@@ -455,6 +455,8 @@ __global__ __launch_bounds__(256) void k_self_z(
 // (precision SPREC), each of the 16 lanes a share of its nodes, which the
 // same row sum adds up: the node loops run while the records stream in.
 // Without it the self term is already in f[3a + 2].
+constexpr int kGatherLanes = 16;   // lanes per atom in the force gather
+
 template <typename Rec, int SPREC, bool SELF>
 __global__ __launch_bounds__(256) void k_gather_pairs(
     int64_t n, int64_t npairs, const Rec *__restrict__ fpair,
@@ -465,20 +467,21 @@ __global__ __launch_bounds__(256) void k_gather_pairs(
     const float *__restrict__ glwf, const double *__restrict__ xyz,
     const double *__restrict__ q, double *__restrict__ f)
 {
-    const int64_t a = dev::gather_atom();
-    const int l = threadIdx.x & 15;
+    const int64_t a = dev::gather_atom<kGatherLanes>();
+    const int l = threadIdx.x & (kGatherLanes - 1);
     const bool live = a < n;
     double acc[3] = {0.0, 0.0, 0.0};
     if (live) {
-        dev::add_endpoint_runs<Rec, 3>(acc, fpair, npairs, a, i_start, i_end, i_idx, j_start,
-                                       j_end, l);
+        dev::add_endpoint_runs<Rec, 3, kGatherLanes>(acc, fpair, npairs, a, i_start, i_end, i_idx,
+                                                     j_start, j_end, l);
         if (SELF)
-            acc[2] += self_force_z<SPREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], l, 16);
+            acc[2] += self_force_z<SPREC>(c, glx, glw, glxf, glwf, q[a], xyz[3 * a + 2], l,
+                                          kGatherLanes);
     }
-    // every lane takes part in the row sums
-    const double fx = dev::row_sum16(acc[0]);
-    const double fy = dev::row_sum16(acc[1]);
-    const double fz = dev::row_sum16(acc[2]);
+    // every lane takes part in the group sums
+    const double fx = dev::lane_group_sum<kGatherLanes>(acc[0]);
+    const double fy = dev::lane_group_sum<kGatherLanes>(acc[1]);
+    const double fz = dev::lane_group_sum<kGatherLanes>(acc[2]);
     if (live && l == 0) {
         f[3 * a] = fx;
         f[3 * a + 1] = fy;
@@ -595,7 +598,7 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWo
 {
     if (n <= 0) return hipSuccess;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
-    const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
+    const unsigned nb = (nblk(kGatherLanes * n, 256) + 7) / 8 * 8;
     const int32_t *is = nl.i_start.as<int32_t>(), *ie = nl.i_end.as<int32_t>();
     const int32_t *js = nl.j_start.as<int32_t>(), *je = nl.j_end.as<int32_t>();
     const int64_t np = nl.npairs;
