@@ -531,8 +531,9 @@ __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t
                 const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
                 const double vx = q * cp, vy = q * sp;
                 // Fl_non_sort (src/Force_long.jl:5-70), own term included
-                const double gxn = Ep * (c * u1 - s * u2) + Em * (c * v1 - s * v2);
-                const double gzn = Ep * (c * u2 + s * u1) - Em * (c * v2 + s * v1);
+                // in the products cp, sp, cm, sm: four FMAs each
+                const double gxn = fma(cp, u1, fma(-sp, u2, fma(cm, v1, -sm * v2)));
+                const double gzn = fma(cp, u2, fma(sp, u1, -fma(cm, v2, sm * v1)));
                 // Fl_sort (src/Force_long.jl:72-127): A, B below the atom; C, D
                 // above it here for a flat chunk, else from the second walk
                 double sxy = sm * A - cm * B, szz = cm * A + sm * B;

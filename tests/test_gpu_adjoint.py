@@ -218,9 +218,14 @@ def test_forces_energy_in_one_call(precision):
         kidx = c.sample_k(9, cfg.P)
         F, E, phi, dg = c.forces_energy(xyz, q, kidx)
         steps = c.short_state()[2]
+        # a second call repeats every output bit for bit (fixed-order sums,
+        # dE/dgamma's per-block partials included)
+        F3, E3, phi3, dg3 = c.forces_energy(xyz, q, kidx)
         F2 = c2.forces(xyz, q, kidx)
         E2, phi2, dg2 = c2.energy(xyz, q, kidx)
     assert steps == 1
+    assert np.array_equal(F3, F) and E3 == E
+    assert np.array_equal(phi3, phi) and np.array_equal(dg3, dg)
     assert np.array_equal(F, F2)
     pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
     Eo, phio, dgo = O.energy(oracle_params(cfg, semantics=L.SEM_CORRECTED), xyz, q, pairs,
