@@ -551,10 +551,10 @@ __global__ __launch_bounds__(256) void k_gather_energy(
 
 // The fused forces + energy call's gather (mixed precision): per atom the
 // force records of its pairs (x, y, z: the force; w: the pair potentials
-// times the other charge) and the self terms, the self force and self
-// energy each split over the atom's 16 lanes by nodes.  Writes the force,
-// phi = dE/dq and psi = the self terms' d phi / d gamma (the pairs' share
-// of dE/dgamma is summed in the pair kernel).
+// times the other charge) and the self force, split over the atom's 16
+// lanes by nodes.  Writes the force and adds the pairs' share of phi =
+// dE/dq to the self energy's, which k_self_energy has written (the pairs'
+// share of dE/dgamma is summed in the pair kernel).
 __global__ __launch_bounds__(256) void k_gather_fe(
     int64_t n, int64_t npairs, const float4 *__restrict__ rec,
     const int32_t *__restrict__ i_start, const int32_t *__restrict__ i_end,
@@ -562,37 +562,27 @@ __global__ __launch_bounds__(256) void k_gather_fe(
     const int32_t *__restrict__ j_end, ShortConsts c, const double *__restrict__ glx,
     const double *__restrict__ glw, const float *__restrict__ glxf,
     const float *__restrict__ glwf, const double *__restrict__ xyz,
-    const double *__restrict__ q, double *__restrict__ f, double *__restrict__ phi,
-    double *__restrict__ psi)
+    const double *__restrict__ q, double *__restrict__ f, double *__restrict__ phi)
 {
     const int64_t a = dev::gather_atom();
     const int l = threadIdx.x & 15;
     const bool live = a < n;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    double s0 = 0.0, s1 = 0.0;
     if (live) {
         dev::add_endpoint_runs<float4, 4>(acc, rec, npairs, a, i_start, i_end, i_idx, j_start,
                                           j_end, l);
-        const double qa = q[a], za = xyz[3 * a + 2];
-        acc[2] += dev::self_force_z<QEM_PREC_MIXED>(c, glx, glw, glxf, glwf, qa, za, l, 16);
-        const double3 se = self_energy<QEM_PREC_MIXED>(c, glx, glw, glxf, glwf, za, l, 16);
-        acc[3] += 2.0 * qa * se.x;
-        s0 = 2.0 * qa * se.y;
-        s1 = 2.0 * qa * se.z;
+        acc[2] += dev::self_force_z<QEM_PREC_MIXED>(c, glx, glw, glxf, glwf, q[a],
+                                                    xyz[3 * a + 2], l, 16);
     }
     const double fx = dev::row_sum16(acc[0]);
     const double fy = dev::row_sum16(acc[1]);
     const double fz = dev::row_sum16(acc[2]);
     const double ph = dev::row_sum16(acc[3]);
-    s0 = dev::row_sum16(s0);
-    s1 = dev::row_sum16(s1);
     if (live && l == 0) {
         f[3 * a] = fx;
         f[3 * a + 1] = fy;
         f[3 * a + 2] = fz;
-        phi[a] = ph;
-        psi[2 * a] = s0;
-        psi[2 * a + 1] = s1;
+        phi[a] += ph;
     }
 }
 
@@ -715,9 +705,10 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                                const NeighborList &nl, int64_t n, const double *d_xyz,
                                const double *d_q, double *d_pot, double *d_phi,
                                double *d_psi, hipStream_t s,
-                               const std::function<void()> *after_pair)
+                               const std::function<void()> *after_pair, hipStream_t self_s)
 {
     if (n <= 0) return hipSuccess;
+    if (!self_s) self_s = s;
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     const int32_t *jpos = nl.jpos.as<int32_t>();
@@ -730,10 +721,10 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
                 nl.pj.as<int32_t>(), jpos, np, d_xyz, d_q, potf);
-        if (after_pair) (*after_pair)();
-        k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, self_s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
+        if (after_pair) (*after_pair)();
     } else {
         if (np > 0) {
             if (c.jd_n > 0)
@@ -747,10 +738,10 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                     ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), jpos,
                     np, d_xyz, d_q, pot);
         }
-        if (after_pair) (*after_pair)();
-        k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, self_s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
+        if (after_pair) (*after_pair)();
     }
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     if (precision == QEM_PREC_MIXED)
@@ -764,10 +755,26 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     return hipGetLastError();
 }
 
+hipError_t launch_self_energy(const ShortConsts &c, int precision, const ShortWork &ws,
+                              int64_t n, const double *d_xyz, const double *d_q, double *d_phi,
+                              double *d_psi, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (precision == QEM_PREC_MIXED)
+        k_self_energy<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
+    else
+        k_self_energy<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
+            ws.glwf.as<float>(), n, d_xyz, d_q, d_phi, d_psi);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                        const NeighborList &nl, int64_t n, const double *d_xyz,
                                        const double *d_q, const double *d_rec, double *d_f,
-                                       double *d_phi, double *d_psi, hipStream_t s)
+                                       double *d_phi, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
@@ -776,7 +783,7 @@ hipError_t launch_gather_forces_energy(const ShortConsts &c, const ShortWork &ws
         n, nl.npairs, reinterpret_cast<const float4 *>(d_rec), nl.i_start.as<int32_t>(),
         nl.i_end.as<int32_t>(), iidx, nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), c,
         ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(), ws.glwf.as<float>(),
-        d_xyz, d_q, d_f, d_phi, d_psi);
+        d_xyz, d_q, d_f, d_phi);
     return hipGetLastError();
 }
 
@@ -784,9 +791,10 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,
                                 double scale, double *d_out, hipStream_t s,
-                                const std::function<void()> *after_pair)
+                                const std::function<void()> *after_pair, hipStream_t self_s)
 {
     if (n <= 0) return hipSuccess;
+    if (!self_s) self_s = s;
     const int64_t np = nl.npairs;
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     const int32_t *jpos = nl.jpos.as<int32_t>();
@@ -799,10 +807,10 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),
                 nl.pj.as<int32_t>(), jpos, np, d_xyz, d_q, d_v, tpf);
-        if (after_pair) (*after_pair)();
-        k_self_tangent<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_tangent<QEM_PREC_MIXED><<<nblk(n, 256), 256, 0, self_s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);
+        if (after_pair) (*after_pair)();
     } else {
         if (np > 0) {
             if (c.jd_n > 0)
@@ -816,10 +824,10 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                     ws.glwf.as<float>(), nullptr, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(), jpos,
                     np, d_xyz, d_q, d_v, tp);
         }
-        if (after_pair) (*after_pair)();
-        k_self_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, s>>>(
+        k_self_tangent<QEM_PREC_FP64><<<nblk(n, 256), 256, 0, self_s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
             ws.glwf.as<float>(), n, d_xyz, d_q, d_v, scale, d_out);
+        if (after_pair) (*after_pair)();
     }
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     if (precision == QEM_PREC_MIXED)

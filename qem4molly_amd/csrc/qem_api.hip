@@ -981,12 +981,9 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     const LongConsts lc = long_consts(ctx, weight, QEM_LONG_ALL);
     int64_t Ptot = 0;
     int lst = QEM_OK;
-    if (overlap) {
-        API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
-        API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
-    }
-    // queued once the pair kernel is, and run beside it: k rows, z sort and
-    // the k = 0 term; then, once it is done, the potential passes
+    // queued once the pair kernel and the self term (on the side stream) are,
+    // and run beside the pair kernel: k rows, z sort and the k = 0 term;
+    // then, once it is done, the potential passes
     const std::function<void()> passes = [&]() {
         lst = stage_kvecs(ctx, kvecs, P, weight, true, ctx->side, &Ptot);
         if (lst) return;
@@ -1016,13 +1013,22 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
         const double *px, *pq;
         if ((st = slot_inputs(ctx, n, d_xyz, d_q, &px, &pq, s))) return st;
+        if (overlap) {
+            // the side stream sees the slot-ordered inputs
+            API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
+            API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+        }
         NeighborList &nl = ctx->nl;
         // slot order: phi and psi rows land in sout first
         double *phs = nl.slot_order ? nl.sout.as<double>() : phi;
         double *pss = nl.slot_order ? nl.sout.as<double>() + n : psi;
+        // with the overlap the self term runs on the side stream beside the
+        // pair kernel; the gather waits for the side stream's chunk sums,
+        // queued after it
         API_HIP(ctx, launch_short_energy(c, ctx->p.precision, ctx->sw, nl, n, px, pq,
                                          ctx->sw.fpair.as<double>(), phs, pss, s,
-                                         overlap ? &passes : nullptr));
+                                         overlap ? &passes : nullptr,
+                                         overlap ? ctx->side : nullptr));
         if (nl.slot_order) {
             API_HIP(ctx, launch_rows_from_slots(nl, n, 1, phs, phi, s));
             API_HIP(ctx, launch_rows_from_slots(nl, n, 2, pss, psi, s));
@@ -1093,12 +1099,9 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     const bool overlap = P > 0 && P <= kLongBatch && ctx->p.precision == QEM_PREC_MIXED;
     int64_t Ptot = 0;
     int lst = QEM_OK;
-    if (overlap) {
-        API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
-        API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
-    }
-    // queued once the pair tangent kernel is, and run beside it: k rows and
-    // z sort; then, once it is done, the tangent passes
+    // queued once the pair tangent kernel and the self term (on the side
+    // stream) are, and run beside the pair kernel: k rows and z sort; then,
+    // once it is done, the tangent passes
     const std::function<void()> lanes = [&]() {
         lst = stage_kvecs(ctx, kvecs, P, weight, false, ctx->side, &Ptot);
         if (lst) return;
@@ -1137,9 +1140,18 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
             pv = nl.sv.as<double>();
             po = nl.sout.as<double>();
         }
+        if (overlap) {
+            // the side stream sees the slot-ordered inputs
+            API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
+            API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+        }
+        // with the overlap the self term runs on the side stream beside the
+        // pair kernel; the gather waits for the side stream's chunk sums,
+        // queued after it
         API_HIP(ctx, launch_short_tangent(c, ctx->p.precision, ctx->sw, nl, n, px, pq, pv,
                                           ctx->sw.fpair.as<double>(), -1.0, po, s,
-                                          overlap ? &lanes : nullptr));
+                                          overlap ? &lanes : nullptr,
+                                          overlap ? ctx->side : nullptr));
         if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, po, d_hv, s));
     }
     if (overlap) {
@@ -1203,8 +1215,6 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
         API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
         phi = ctx->d_f2.as<double>();
     }
-    API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
-    API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
     int64_t Ptot = 0;
     int lst = QEM_OK;
     int st = QEM_OK;
@@ -1213,7 +1223,10 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
         PhaseScope ps(ctx, QEM_PHASE_PAIR, s);
         st = slot_inputs(ctx, n, d_xyz, d_q, &px, &pq, s);
         if (st == QEM_OK) {
-            hipError_t e = ensure(nl.sout2, sizeof(double) * 3 * (size_t)n);
+            // the side stream sees the slot-ordered inputs
+            hipError_t e = hipEventRecord(ctx->fork_ev, s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0);
+            if (e == hipSuccess) e = ensure(nl.sout2, sizeof(double) * 3 * (size_t)n);
             if (e == hipSuccess)
                 e = launch_pair_forces_energy(c, ctx->sw, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
                                               nl.jpos.as<int32_t>(), nl.npairs, px, pq,
@@ -1221,9 +1234,16 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             if (e != hipSuccess) st = hip_err(ctx, e, "fused pair kernel");
         }
     }
-    // side stream, queued after the pair kernel and run beside it: k rows,
-    // z sort, the k = 0 term, per-k constants
-    if (st == QEM_OK)
+    // side stream, queued after the pair kernel and run beside it: the self
+    // energies (phi and psi of every atom, in slot order), k rows, z sort,
+    // the k = 0 term, per-k constants
+    if (st == QEM_OK) {
+        hipError_t e = launch_self_energy(c, QEM_PREC_MIXED, ctx->sw, n, px, pq,
+                                          nl.sout2.as<double>(), nl.sout2.as<double>() + n,
+                                          ctx->side);
+        if (e != hipSuccess) lst = hip_err(ctx, e, "self energy");
+    }
+    if (st == QEM_OK && lst == QEM_OK)
         lst = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
                         &Ptot);
     // side stream, once the pair kernel is done: passes A and S with the GPU
@@ -1248,14 +1268,15 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             if (e != hipSuccess) lst = hip_err(ctx, e, "long-range energy");
         }
     }
-    // call's stream: one gather for the forces, dE/dq and the self terms, in
-    // slot order
+    // call's stream, once passes A and S (queued after the self energies on
+    // the side stream) are done: one gather for the forces (self force
+    // included) and the pairs' share of dE/dq, in slot order
     if (st == QEM_OK) {
         PhaseScope ps(ctx, QEM_PHASE_GATHER, s);
         hipError_t e = launch_gather_forces_energy(c, ctx->sw, nl, n, px, pq,
                                                    ctx->sw.fpair.as<double>(),
                                                    nl.sout.as<double>(), nl.sout2.as<double>(),
-                                                   nl.sout2.as<double>() + n, s);
+                                                   s);
         if (e != hipSuccess) st = hip_err(ctx, e, "gather");
     }
     // joined whatever happened, so that s never runs ahead of the side work

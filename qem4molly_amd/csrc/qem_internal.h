@@ -360,14 +360,19 @@ hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double 
 // have just formed: the two share passes A and S.
 hipError_t launch_long_energy_B(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                 hipStream_t s, std::string &err);
+// Self energies per atom: d_phi = 2 q s_i (N) and d_psi = 2 q ds_i/dgamma_1,2
+// (N x 2), overwritten.
+hipError_t launch_self_energy(const ShortConsts &c, int precision, const ShortWork &ws,
+                              int64_t n, const double *d_xyz, const double *d_q, double *d_phi,
+                              double *d_psi, hipStream_t s);
 // Per atom from the records of launch_pair_forces_energy (mixed precision)
-// and the self terms: the short-range force into d_f (N x 3), dE/dq into
-// d_phi (N) and the self terms' d phi / d gamma_1,2 into d_psi (N x 2), all
-// overwritten.
+// and the self force: the short-range force into d_f (N x 3, overwritten);
+// the pairs' dE/dq added to d_phi (N), which holds the self energies of
+// launch_self_energy.
 hipError_t launch_gather_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                        const NeighborList &nl, int64_t n, const double *d_xyz,
                                        const double *d_q, const double *d_rec, double *d_f,
-                                       double *d_phi, double *d_psi, hipStream_t s);
+                                       double *d_phi, hipStream_t s);
 // Tangent of the long-range force along d_v (N x 3), times scale; between,
 // when given, is called once the chunk sums and offsets are queued, before
 // pass B.  Without with_sum the call stops at the per-group partial
@@ -385,23 +390,30 @@ hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double
 // endpoint the other atom's charge times (G, dG/dgamma_1, dG/dgamma_2, 0),
 // float4 in mixed precision, double4 in FP64), per atom phi (N) and psi
 // (N x 2: d phi / d gamma_1,2), self terms included.
-// after_pair (optional) is called once the pair kernel is queued.
+// The self terms run on self_s (default s) and are queued right after the
+// pair kernel; after_pair (optional) is called after them.  With self_s
+// another stream, the caller orders it: self_s must see the inputs, and
+// the gather (on s) must wait for the self terms.
 hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortWork &ws,
                                const NeighborList &nl, int64_t n, const double *d_xyz,
                                const double *d_q, double *d_pot, double *d_phi,
                                double *d_psi, hipStream_t s,
-                               const std::function<void()> *after_pair = nullptr);
+                               const std::function<void()> *after_pair = nullptr,
+                               hipStream_t self_s = nullptr);
 // Short-range force tangent along d_v, times scale, into d_out (N x 3);
 // d_tp holds the tangent records (pair_rec_bytes: first atom (dfx, dfy,
 // dfz_i, 0), second atom (-dfx, -dfy, dfz_j, 0); float4 in mixed precision,
 // double4 in FP64).
-// after_pair (optional) is called once the pair kernel is queued, before
-// the self term and the gather.
+// The self term runs on self_s (default s) and is queued right after the
+// pair kernel, then after_pair (optional) is called, then the gather is
+// queued on s.  With self_s another stream, the caller orders it: self_s
+// must see the inputs, and the gather must wait for the self term.
 hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,
                                 double scale, double *d_out, hipStream_t s,
-                                const std::function<void()> *after_pair = nullptr);
+                                const std::function<void()> *after_pair = nullptr,
+                                hipStream_t self_s = nullptr);
 // out3 = (1/2 q.phi, 1/2 q.psi_1, 1/2 q.psi_2); psi may be null; d_part is
 // scratch of half_dots_scratch() doubles.  With d_gpart, the sums of its
 // ngpart pairs of doubles are added to out3[1] and out3[2].
