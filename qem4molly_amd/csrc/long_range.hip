@@ -264,7 +264,7 @@ constexpr int kRedBatch = 4;   // a power of two (slots are step & (kRedBatch - 
 constexpr int kRedPitch = 65;
 
 struct LaneReducer {
-    double *buf;   // this wave's [kRedBatch][3][kRedPitch]
+    double *buf;   // this wave's [kRedBatch][NC][kRedPitch], NC = 3 or 4 values per atom
 
     __device__ __forceinline__ void put(int slot, int lane, double x, double y, double z) const
     {
@@ -272,6 +272,16 @@ struct LaneReducer {
         r[lane] = x;
         r[kRedPitch + lane] = y;
         r[2 * kRedPitch + lane] = z;
+    }
+    // four values per atom: the force and the potential
+    __device__ __forceinline__ void put4(int slot, int lane, double x, double y, double z,
+                                         double e) const
+    {
+        double *r = buf + slot * 4 * kRedPitch;
+        r[lane] = x;
+        r[kRedPitch + lane] = y;
+        r[2 * kRedPitch + lane] = z;
+        r[3 * kRedPitch + lane] = e;
     }
 
     // out[3 base + 3 j + c] (=, or += with add) for the first nb slots
@@ -349,17 +359,19 @@ struct LaneSite {
 // of the batch that starts at step t0 is the atom a0 + t0 + j of each
 // sub-chunk (up), or a1 - 1 - (t0 + j) (down); atoms past a sub-chunk's
 // end are not written.
-template <int SPLIT>
+// With NC = 4 the rows are (atom j, component c) = row 4 j + c, and the
+// fourth component (the potential) goes to oute[a].
+template <int SPLIT, int NC = 3>
 __device__ __forceinline__ void lane_flush(const LaneReducer &red, int nb, int lane,
                                            double *__restrict__ out, int64_t wfc, int chk,
                                            int64_t n, int nchunk, int64_t t0, bool down,
-                                           bool add)
+                                           bool add, double *__restrict__ oute = nullptr)
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int r = lane >> 2, q = lane & 3;
-    const bool live = r < 3 * nb;
+    const bool live = r < NC * nb;
     double sum = 0.0;
     if (live) {
         const double *v = red.buf + r * kRedPitch + 16 * q;
@@ -374,10 +386,12 @@ __device__ __forceinline__ void lane_flush(const LaneReducer &red, int nb, int l
         const int64_t ch = wfc + q / QPS;
         if (ch < nchunk) {
             const int64_t a0 = ch * chk, a1 = min(n, a0 + chk);
-            const int j = r / 3, c = r - 3 * j;
+            const int j = r / NC, c = r - NC * j;
             const int64_t a = down ? a1 - 1 - (t0 + j) : a0 + t0 + j;
             if (a >= a0 && a < a1) {
                 double *o = out + 3 * a + c;
+                if constexpr (NC == 4)
+                    if (c == 3) o = oute + a;
                 *o = add ? *o + sum : sum;
             }
         }
@@ -468,16 +482,37 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_A(const KRec *__restri
 // inclusive sum when the chunk is flat in e^{-kz} (k_max dz <= 1 over the
 // chunk, so the subtraction loses at most a factor e in relative accuracy);
 // otherwise a second walk in descending z accumulates them from the top.
-template <int PREC, int SPLIT>
+// Every product that feeds a sum is an explicit fused multiply-add or a
+// plain product, and the remaining sums are not contracted, so that the
+// forces round the same with or without the potential (ENERGY).
+//
+// ENERGY: also the potential phi_i = dE/dq_i of the wave's 64 k rows into
+// partE[group][atom] (the k = 0 potential is k_k0_apply's).  For k != 0,
+//   phi_k(i) = w/(2k) [c e^{-kz} (A + P1) + s e^{-kz} (B + P2)
+//                      + c e^{+kz} (C + P3) + s e^{+kz} (D + P4)]
+// with the prefix/suffix sums A..D (the direct pair term) and P1..P4 the
+// per-k derivative of the image term I_k = [g1 |S-|^2 + g2 e2 |S+|^2
+// + 2 g1 g2 e2 Re(S+ conj S-)] / (2 (1 - g1 g2 e2)) with respect to q_i,
+// divided by the atom's own terms.  With the non-sort row constants N1..N3
+// (g = 1 / 2 (g12 e2 - 1), so 1 / (1 - g12 e2) = -2 g) these are
+// P1 = -2 v2, P2 = -2 v1, P3 = -2 u2, P4 = -2 u1 for the u, v of the
+// non-sort force, so the potential is
+//   w/(2k) [(c e^{-kz} A + s e^{-kz} B) + (c e^{+kz} C + s e^{+kz} D) - 2 (X + Y)],
+// X = c e^{+kz} u2 + s e^{+kz} u1, Y = c e^{-kz} v2 + s e^{-kz} v1, the
+// first two brackets the sorted-sum force's own products.  It needs the
+// non-sort constants (launch with the non-sort part on).
+template <int PREC, int SPLIT, bool ENERGY = false>
 __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t Ptot,
                                             const double4 *__restrict__ satom, int64_t n,
                                             int chk, int nchunk,
                                             const double4 *__restrict__ partA,
                                             const double4 *__restrict__ offA,
                                             const double4 *__restrict__ totA, int own_term,
-                                            double *__restrict__ partB, double *red_buf, int wv,
+                                            double *__restrict__ partB,
+                                            double *__restrict__ partE, double *red_buf, int wv,
                                             int lane)
 {
+    constexpr int NC = ENERGY ? 4 : 3;
     const LaneSite<SPLIT> w(wv, lane, n, chk, nchunk);
     if (w.wfc >= nchunk) return;
     const bool live = w.p < Ptot;
@@ -486,6 +521,8 @@ __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t
     LaneK L = load_lane(kr[pl], PREC == QEM_PREC_MIXED);
     const double kabs = live ? kr[pl].k : 0.0;
     if (!live) L.w = L.wx = L.wy = 0.0;   // a lane past the last row adds zeros
+    // w/(2k), zero on a lane past the last row
+    const double we = ENERGY ? L.w / (2.0 * kr[pl].k) : 0.0;
     const double4 T = totA[pl];
     // Fl_non_sort over all atoms, the atom's own term included: that term
     // is zero in xy (sin 0 = 0) and q e^{+-kz} (c^2 + s^2) times the z
@@ -511,6 +548,7 @@ __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t
     for (int d = 32; d >= RW_of<SPLIT>(); d >>= 1) dz = fmax(dz, __shfl_xor(dz, d));
     const bool flat = __builtin_amdgcn_readfirstlane((int)(kmax * dz <= 1.0)) != 0;
     double *out = partB + ((int64_t)blockIdx.y * n) * 3;
+    double *oute = ENERGY ? partE + (int64_t)blockIdx.y * n : nullptr;
     const LaneReducer red{red_buf};
 
     double A = off.x, B = off.y;   // exclusive prefix below the atom
@@ -523,43 +561,52 @@ __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t
         for (int64_t t = 0; t < w.lmax; ++t) {
             const double4 at = nx;
             if (t + 1 < w.lmax) nx = satom[w.at(t + 1, n)];
-            double fx = 0.0, fy = 0.0, fz = 0.0;
+            double fx = 0.0, fy = 0.0, fz = 0.0, fe = 0.0;
             if (SPLIT == 1 || t < len) {
                 const double q = at.w;
                 double c, s, Ep, Em;
                 lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
                 const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-                const double vx = q * cp, vy = q * sp;
                 // Fl_non_sort (src/Force_long.jl:5-70), own term included
                 // in the products cp, sp, cm, sm: four FMAs each
-                const double gxn = fma(cp, u1, fma(-sp, u2, fma(cm, v1, -sm * v2)));
-                const double gzn = fma(cp, u2, fma(sp, u1, -fma(cm, v2, sm * v1)));
+                const double Y = fma(cm, v2, sm * v1);
+                const double gxn = fma(cp, u1, fma(-sp, u2, fma(cm, v1, -(sm * v2))));
+                const double gzn = fma(cp, u2, fma(sp, u1, -Y));
                 // Fl_sort (src/Force_long.jl:72-127): A, B below the atom; C, D
                 // above it here for a flat chunk, else from the second walk
-                double sxy = sm * A - cm * B, szz = cm * A + sm * B;
+                double sxy = fma(sm, A, -(cm * B));
+                const double z1 = fma(cm, A, sm * B);
+                double szz = z1;
+                double te = 0.0;
+                if constexpr (ENERGY)
+                    te = fma(-2.0, add_nc(fma(cp, u2, sp * u1), Y), z1);
                 if constexpr (FLAT) {
                     PZ = add_nc(PZ, q * cm);
                     PW = add_nc(PW, q * sm);
                     const double C = off.z + (own.z - PZ), D = off.w + (own.w - PW);
-                    sxy += sp * C - cp * D;
-                    szz -= cp * C + sp * D;
+                    const double z2 = fma(cp, C, sp * D);
+                    sxy = add_nc(sxy, fma(sp, C, -(cp * D)));
+                    szz = add_nc(szz, -z2);
+                    if constexpr (ENERGY) te = add_nc(te, z2);
                 }
-                double gzs = gzn + L.hs * szz;
+                double gzs = fma(L.hs, szz, gzn);
                 // Flz_self (src/Force_long.jl:164-184) minus the own term above
-                if constexpr (OWN) gzs += q * (E1 * (Ep * Ep) - E2 * (Em * Em));
-                const double gx = q * (gxn + L.hs * sxy);
+                if constexpr (OWN) gzs = fma(q, fma(E1, Ep * Ep, -(E2 * (Em * Em))), gzs);
+                const double gx = q * fma(L.hs, sxy, gxn);
                 const double gz = q * gzs;
                 fx = L.wx * gx;
                 fy = L.wy * gx;
                 fz = L.w * gz;
-                A += vx;
-                B += vy;
+                fe = we * te;
+                A = fma(q, cp, A);
+                B = fma(q, sp, B);
             }
             const int slot = (int)t & (kRedBatch - 1);
-            red.put(slot, lane, fx, fy, fz);
+            if constexpr (ENERGY) red.put4(slot, lane, fx, fy, fz, fe);
+            else red.put(slot, lane, fx, fy, fz);
             if (slot == kRedBatch - 1 || t + 1 == w.lmax)
-                lane_flush<SPLIT>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk, t - slot,
-                                  false, false);
+                lane_flush<SPLIT, NC>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk,
+                                      t - slot, false, false, oute);
         }
     };
     if (flat) {
@@ -574,30 +621,35 @@ __device__ __forceinline__ void lane_B_body(const KRec *__restrict__ kr, int64_t
     // from the top, added to the first walk's values
     double C = off.z, D = off.w;
     for (int64_t t = 0; t < w.lmax; ++t) {
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        double fx = 0.0, fy = 0.0, fz = 0.0, fe = 0.0;
         if (SPLIT == 1 || t < len) {
             const double4 at = satom[w.a1 - 1 - t];
             const double q = at.w;
             double c, s, Ep, Em;
             lane_terms<PREC>(L, at.x, at.y, at.z, c, s, Ep, Em);
             const double cp = c * Ep, sp = s * Ep, cm = c * Em, sm = s * Em;
-            const double gx = q * (L.hs * (sp * C - cp * D));
-            const double gz = q * (L.hs * (-cp * C - sp * D));
+            const double z2 = fma(cp, C, sp * D);
+            const double gx = q * (L.hs * fma(sp, C, -(cp * D)));
+            const double gz = q * (L.hs * -z2);
             fx = L.wx * gx;
             fy = L.wy * gx;
             fz = L.w * gz;
-            C += q * cm;
-            D += q * sm;
+            fe = we * z2;
+            C = fma(q, cm, C);
+            D = fma(q, sm, D);
         }
         const int slot = (int)t & (kRedBatch - 1);
-        red.put(slot, lane, fx, fy, fz);
+        if constexpr (ENERGY) red.put4(slot, lane, fx, fy, fz, fe);
+        else red.put(slot, lane, fx, fy, fz);
         if (slot == kRedBatch - 1 || t + 1 == w.lmax)
-            lane_flush<SPLIT>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk, t - slot, true,
-                              true);
+            lane_flush<SPLIT, NC>(red, slot + 1, lane, out, w.wfc, chk, n, nchunk, t - slot,
+                                  true, true, oute);
     }
 }
 
-template <int PREC>
+// Pass B over the k-lane grid; the last group of rows takes SPLIT
+// sub-chunks per wave (lane_split_of).  ENERGY: the potential as well.
+template <int PREC, bool ENERGY = false>
 __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restrict__ kr,
                                                              int64_t Ptot,
                                                              const double4 *__restrict__ satom,
@@ -607,21 +659,23 @@ __global__ __launch_bounds__(64 * kLaneWaves) void k_lane_B(const KRec *__restri
                                                              const double4 *__restrict__ offA,
                                                              const double4 *__restrict__ totA,
                                                              int own_term,
-                                                             double *__restrict__ partB)
+                                                             double *__restrict__ partB,
+                                                             double *__restrict__ partE)
 {
-    __shared__ double red_lds[kLaneWaves][kRedBatch * 3 * kRedPitch];
+    constexpr int NC = ENERGY ? 4 : 3;
+    __shared__ double red_lds[kLaneWaves][kRedBatch * NC * kRedPitch];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int split = blockIdx.y + 1 == gridDim.y ? split_last : 1;
     if (split == 4)
-        lane_B_body<PREC, 4>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA, own_term, partB,
-                             red_lds[wv], wv, lane);
+        lane_B_body<PREC, 4, ENERGY>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA,
+                                     own_term, partB, partE, red_lds[wv], wv, lane);
     else if (split == 2)
-        lane_B_body<PREC, 2>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA, own_term, partB,
-                             red_lds[wv], wv, lane);
+        lane_B_body<PREC, 2, ENERGY>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA,
+                                     own_term, partB, partE, red_lds[wv], wv, lane);
     else
-        lane_B_body<PREC, 1>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA, own_term, partB,
-                             red_lds[wv], wv, lane);
+        lane_B_body<PREC, 1, ENERGY>(kr, Ptot, satom, n, chk, nchunk, partA, offA, totA,
+                                     own_term, partB, partE, red_lds[wv], wv, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1359,11 +1413,46 @@ hipError_t launch_long_passes(const LongConsts &c, LongWork &lw, int64_t n, int6
     if (c.precision == QEM_PREC_MIXED)
         k_lane_B<QEM_PREC_MIXED><<<g, 64 * kLaneWaves, 0, s>>>(
             kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>(),
-            lw.offA.as<double4>(), lw.totA.as<double4>(), own_term, lw.partB.as<double>());
+            lw.offA.as<double4>(), lw.totA.as<double4>(), own_term, lw.partB.as<double>(),
+            nullptr);
     else
         k_lane_B<QEM_PREC_FP64><<<g, 64 * kLaneWaves, 0, s>>>(
             kr, Ptot, lw.satom.as<double4>(), n, chk, nchunk, sl, lw.partA.as<double4>(),
-            lw.offA.as<double4>(), lw.totA.as<double4>(), own_term, lw.partB.as<double>());
+            lw.offA.as<double4>(), lw.totA.as<double4>(), own_term, lw.partB.as<double>(),
+            nullptr);
+    LR_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+hipError_t launch_long_forces_energy_B(const LongConsts &c, LongWork &lw, int64_t n,
+                                       int64_t Ptot, hipStream_t s, std::string &err)
+{
+    if (n <= 0 || Ptot <= 0) return hipSuccess;
+    // the potential from the force pass needs the non-sort row constants
+    if (!(c.parts & QEM_LONG_NONSORT)) {
+        const hipError_t e = launch_long_passes(c, lw, n, Ptot, s, err, false, kLongPassB);
+        return e != hipSuccess ? e : launch_long_energy_B(c, lw, n, Ptot, s, err);
+    }
+    const LaneGrid lg = lane_grid(n, Ptot);
+    LR_CHECK(ensure(lw.partB, sizeof(double) * 3 * (size_t)lg.ngroup * (size_t)n));
+    LR_CHECK(ensure(lw.partE, sizeof(double) * (size_t)lg.ngroup * (size_t)n));
+    LR_CHECK(ensure(lw.q0part, sizeof(double2) * (size_t)Ptot));
+    const KRec *kr = lw.kr.as<KRec>();
+    const dim3 g((unsigned)((lg.nchunk + kLaneWaves - 1) / kLaneWaves), (unsigned)lg.ngroup);
+    const int own_term = ((c.parts & QEM_LONG_NONSORT) != 0) != ((c.parts & QEM_LONG_SELF) != 0);
+    if (c.precision == QEM_PREC_MIXED)
+        k_lane_B<QEM_PREC_MIXED, true><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, lw.satom.as<double4>(), n, lg.chk, lg.nchunk, lg.split_last,
+            lw.partA.as<double4>(), lw.offA.as<double4>(), lw.totA.as<double4>(), own_term,
+            lw.partB.as<double>(), lw.partE.as<double>());
+    else
+        k_lane_B<QEM_PREC_FP64, true><<<g, 64 * kLaneWaves, 0, s>>>(
+            kr, Ptot, lw.satom.as<double4>(), n, lg.chk, lg.nchunk, lg.split_last,
+            lw.partA.as<double4>(), lw.offA.as<double4>(), lw.totA.as<double4>(), own_term,
+            lw.partB.as<double>(), lw.partE.as<double>());
+    LR_CHECK(hipGetLastError());
+    k_long_dgamma<<<nblk(Ptot, 256), 256, 0, s>>>(c, lw.kv.as<double4>(), Ptot,
+                                                  lw.totA.as<double4>(), lw.q0part.as<double2>());
     LR_CHECK(hipGetLastError());
     return hipSuccess;
 }

@@ -1247,8 +1247,8 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
         lst = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
                         &Ptot);
     // side stream, once the pair kernel is done: passes A and S with the GPU
-    // to themselves, then (beside the gather, which waits for them) the
-    // force pass B and the potential pass on their chunk sums
+    // to themselves, then (beside the gather, which waits for them) pass B
+    // on their chunk sums, forces and potential in one walk
     if (lst == QEM_OK && st == QEM_OK) {
         hipError_t e = hipEventRecord(ctx->lanes_ev, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
@@ -1260,12 +1260,10 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->sums_ev, 0);
             if (e != hipSuccess) lst = hip_err(ctx, e, "sums event");
         }
-        if (lst == QEM_OK)
-            lst = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassB);
         if (lst == QEM_OK) {
             PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
-            e = launch_long_energy_B(lc, ctx->lw, n, Ptot, ctx->side, ctx->err);
-            if (e != hipSuccess) lst = hip_err(ctx, e, "long-range energy");
+            e = launch_long_forces_energy_B(lc, ctx->lw, n, Ptot, ctx->side, ctx->err);
+            if (e != hipSuccess) lst = hip_err(ctx, e, "long-range forces and energy");
         }
     }
     // call's stream, once passes A and S (queued after the self energies on

@@ -360,6 +360,11 @@ hipError_t launch_long_energy_sum(LongWork &lw, int64_t n, int64_t Ptot, double 
 // have just formed: the two share passes A and S.
 hipError_t launch_long_energy_B(const LongConsts &c, LongWork &lw, int64_t n, int64_t Ptot,
                                 hipStream_t s, std::string &err);
+// Pass B of launch_long_passes and launch_long_energy_B in one: the partial
+// forces (lw.partB), the potential (lw.partE) and the per-k dE_k/dgamma, for
+// k rows whose chunk sums, offsets and totals passes A and S have formed.
+hipError_t launch_long_forces_energy_B(const LongConsts &c, LongWork &lw, int64_t n,
+                                       int64_t Ptot, hipStream_t s, std::string &err);
 // Self energies per atom: d_phi = 2 q s_i (N) and d_psi = 2 q ds_i/dgamma_1,2
 // (N x 2), overwritten.
 hipError_t launch_self_energy(const ShortConsts &c, int precision, const ShortWork &ws,
