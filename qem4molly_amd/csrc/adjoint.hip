@@ -220,6 +220,10 @@ struct Dual4 {
 // multiples of u.x, the tangent is a fixed combination of six (eight for the
 // gaussian set) node sums: S = sum t J, U = sum u.x t J, Q = sum u.x (E/D) t J,
 // R = sum u.x^2 t J, each for J0 and J1; the per-set factors are applied once.
+// With g12 E = D + 1, B E/D = 2 Lz r (1 + 1/D) (g12 = 0 included), so
+// B Q = 2 Lz r (U + Q') with Q' = sum u.x t J / D: one product per node
+// (u.x t times 1/D) instead of two, and the coefficient of U becomes
+// A + 2 Lz r.
 __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a, float da,
                                                      float rho, float drho, bool gauss,
                                                      bool self, const ShortConsts &c,
@@ -247,7 +251,7 @@ __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a,
                                 : __builtin_amdgcn_exp2f(fmaf(e, rP, u.w));
         const float rD = __builtin_amdgcn_rcpf(fmaf(g12, E, -1.0f));
         const float t = num * rD;
-        const float tu = t * u.x, tq = tu * (E * rD);   // u.x t, u.x (E/D) t
+        const float tu = t * u.x, tq = tu * rD;   // u.x t, u.x t / D
         if (self) {  // J0 = 1, J1 = 0
             S0 += t;
             U0 += tu;
@@ -275,8 +279,9 @@ __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a,
             R1 = fmaf(tr, dp, R1);
         }
     }
-    const float A = gauss ? -r * a - da : -r * (a + Lz2) - da;
-    const float B = Lz2 * g12 * r;
+    // A + 2 Lz r and 2 Lz r (the E/D sums folded into U and Q' above)
+    const float A = gauss ? fmaf(-r, a - Lz2, -da) : -r * a - da;
+    const float B = Lz2 * r;
     const float C = gauss ? -r * (float)(0.5 / c.alpha) : 0.0f;
     const float sg = (!self && rho > 0.0f) ? r + drho / rho : 0.0f;
     const float sg2 = sg * c.jt_dx * c.jt_dx;

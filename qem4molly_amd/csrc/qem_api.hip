@@ -718,7 +718,7 @@ int short_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, d
         if (int st = advance_step(ctx, n, d_xyz, s)) return st;
     }
     const ShortConsts c = short_consts(ctx, 0);
-    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, pair_records_f32(c, ctx->p.precision) ? 16 : 32)));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, force_rec_bytes(c, ctx->p.precision))));
     NeighborList &nl = ctx->nl;
     const double *px = d_xyz, *pq = d_q;
     double *pf = d_f;

@@ -211,6 +211,11 @@ __device__ __forceinline__ void store4(double4 *r, double x, double y, double z,
 {
     *r = make_double4(x, y, z, w);
 }
+// three-component records (w is not kept)
+__device__ __forceinline__ void store4(float3 *r, double x, double y, double z, double)
+{
+    *r = make_float3((float)x, (float)y, (float)z);
+}
 
 template <typename Rec, int NC>
 __device__ __forceinline__ void add_rec(double *acc, const Rec &r)
@@ -218,7 +223,7 @@ __device__ __forceinline__ void add_rec(double *acc, const Rec &r)
     acc[0] += (double)r.x;
     acc[1] += (double)r.y;
     acc[2] += (double)r.z;
-    if (NC > 3) acc[3] += (double)r.w;
+    if constexpr (NC > 3) acc[3] += (double)r.w;
 }
 
 // Lane l of an atom's L lanes adds records t = s + l, s + l + L, ... of

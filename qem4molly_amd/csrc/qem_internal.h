@@ -224,11 +224,15 @@ hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
                                      double *d_gpart, hipStream_t s);
 // Pair records of the force path (launch_pair_forces -> launch_gather_self):
-// float4 for the mixed-precision list kernel, double4 otherwise (FP64, and
-// the pole-splitting kernels).
+// float3 (12 bytes) for the mixed-precision list kernel, double4 (32 bytes)
+// otherwise (FP64, and the pole-splitting kernels).
 inline bool pair_records_f32(const ShortConsts &c, int precision)
 {
     return precision == QEM_PREC_MIXED && !c.pole;
+}
+inline int force_rec_bytes(const ShortConsts &c, int precision)
+{
+    return pair_records_f32(c, precision) ? 12 : 32;
 }
 // Per atom: gather pair forces (i side and j side) and add the self term.
 hipError_t launch_gather_self(const ShortConsts &c, int precision,

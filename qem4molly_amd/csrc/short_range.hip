@@ -171,11 +171,11 @@ struct PairForce {
     double fx, fy, fz, fzj;
 };
 
-// The per-pair record of the force path: float4 in mixed precision (the
+// The per-pair record of the force path: float3 in mixed precision (the
 // pair values come from FP32 node sums; the per-atom sums of the gather
-// stay FP64), double4 in FP64.  Half the bytes for the gather to read.
+// stay FP64), double4 in FP64.  12 bytes per endpoint for the gather to read.
 template <int PREC>
-using PairRec = typename std::conditional<PREC == QEM_PREC_MIXED, float4, double4>::type;
+using PairRec = typename std::conditional<PREC == QEM_PREC_MIXED, float3, double4>::type;
 
 // The pair's two endpoint records: the first atom adds (fx, fy, fz), the
 // second (-fx, -fy, fz_j).
@@ -511,7 +511,7 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision, const ShortWo
         k_pair_list<QEM_PREC_MIXED><<<pair_grid(npairs), 256, lds, s>>>(
             c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(),
             ws.jtab.as<float4>(), d_pi, d_pj, d_jpos, npairs, d_xyz, d_q,
-            reinterpret_cast<float4 *>(d_fpair));
+            reinterpret_cast<float3 *>(d_fpair));
     } else {
         if (c.jd_n > 0)
             k_pair_list<QEM_PREC_FP64><<<pair_grid(npairs, 512), 512, dev::lds_bytes64(c), s>>>(
@@ -612,8 +612,8 @@ hipError_t launch_gather_self(const ShortConsts &c, int precision, const ShortWo
             n, np, reinterpret_cast<const double4 *>(d_fpair), is, ie, iidx, js, je, c, glx,
             glw, glxf, glwf, d_xyz, d_q, d_f);
     } else if (pair_records_f32(c, precision)) {
-        k_gather_pairs<float4, QEM_PREC_MIXED, true><<<nb, 256, 0, s>>>(
-            n, np, reinterpret_cast<const float4 *>(d_fpair), is, ie, iidx, js, je, c, glx,
+        k_gather_pairs<float3, QEM_PREC_MIXED, true><<<nb, 256, 0, s>>>(
+            n, np, reinterpret_cast<const float3 *>(d_fpair), is, ie, iidx, js, je, c, glx,
             glw, glxf, glwf, d_xyz, d_q, d_f);
     } else {
         k_gather_pairs<double4, QEM_PREC_FP64, true><<<nb, 256, 0, s>>>(
