@@ -457,15 +457,15 @@ __device__ double self_force_tangent(const ShortConsts &c, const double *__restr
 
 // ---- kernels ------------------------------------------------------------------------
 
-// Per-pair records of the energy and tangent passes: float4 in mixed
+// Per-pair records of the energy and tangent passes: float3 in mixed
 // precision (the values come from FP32 node sums; the per-atom sums of the
-// gathers stay FP64), double4 in FP64.
+// gathers stay FP64), double4 in FP64 (w unused).
 template <int PREC>
-using TanRec = typename std::conditional<PREC == QEM_PREC_MIXED, float4, double4>::type;
+using TanRec = typename std::conditional<PREC == QEM_PREC_MIXED, float3, double4>::type;
 
 
 // Potential records (endpoint records, placed as the force records): the
-// first atom's is q_j (G, dG/dgamma_1, dG/dgamma_2, 0), the second's
+// first atom's is q_j (G, dG/dgamma_1, dG/dgamma_2), the second's
 // q_i (...), so the gather reads one record per pair and endpoint and no
 // charges.
 template <int PREC>
@@ -486,8 +486,8 @@ __global__ __launch_bounds__(256) void k_pair_energy(
     });
 }
 
-// Tangent records: the first atom's (dfx, dfy, dfz_i, 0), the second's
-// (-dfx, -dfy, dfz_j, 0), placed as the force records.
+// Tangent records: the first atom's (dfx, dfy, dfz_i), the second's
+// (-dfx, -dfy, dfz_j), placed as the force records.
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pair_tangent(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
@@ -718,7 +718,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     const int32_t *jpos = nl.jpos.as<int32_t>();
     double4 *pot = reinterpret_cast<double4 *>(d_pot);
-    float4 *potf = reinterpret_cast<float4 *>(d_pot);
+    float3 *potf = reinterpret_cast<float3 *>(d_pot);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     if (precision == QEM_PREC_MIXED) {
         if (np > 0)
@@ -750,7 +750,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
     }
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     if (precision == QEM_PREC_MIXED)
-        k_gather_energy<float4><<<nb, 256, 0, s>>>(
+        k_gather_energy<float3><<<nb, 256, 0, s>>>(
             n, np, potf, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
             nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), d_phi, d_psi);
     else
@@ -804,7 +804,7 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
     const int32_t *iidx = nl.has_i_idx ? nl.i_idx.as<int32_t>() : nullptr;
     const int32_t *jpos = nl.jpos.as<int32_t>();
     double4 *tp = reinterpret_cast<double4 *>(d_tp);
-    float4 *tpf = reinterpret_cast<float4 *>(d_tp);
+    float3 *tpf = reinterpret_cast<float3 *>(d_tp);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     if (precision == QEM_PREC_MIXED) {
         if (np > 0)
@@ -836,7 +836,7 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
     }
     const unsigned nb = (nblk(16 * n, 256) + 7) / 8 * 8;
     if (precision == QEM_PREC_MIXED)
-        k_gather_tangent<float4><<<nb, 256, 0, s>>>(
+        k_gather_tangent<float3><<<nb, 256, 0, s>>>(
             n, np, tpf, nl.i_start.as<int32_t>(), nl.i_end.as<int32_t>(), iidx,
             nl.j_start.as<int32_t>(), nl.j_end.as<int32_t>(), scale, d_out);
     else

@@ -962,7 +962,7 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     int st = ensure_list(ctx, n, d_xyz, s);
     if (st) return st;
     const ShortConsts c = short_consts(ctx, 0);
-    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, ctx->p.precision == QEM_PREC_MIXED ? 16 : 32)));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, ctx->p.precision == QEM_PREC_MIXED ? 12 : 32)));
     API_HIP(ctx, ensure(ctx->d_adj, sizeof(double) * (2 * (size_t)n + 4 + half_dots_scratch())));
     double *psi = ctx->d_adj.as<double>();          // N x 2
     double *out3 = psi + 2 * n;                      // 3 (+1 pad)
@@ -1089,7 +1089,7 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     int st = ensure_list(ctx, n, d_xyz, s);
     if (st) return st;
     const ShortConsts c = short_consts(ctx, 0);
-    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, ctx->p.precision == QEM_PREC_MIXED ? 16 : 32)));
+    API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, ctx->p.precision == QEM_PREC_MIXED ? 12 : 32)));
     const LongConsts lc = long_consts(ctx, weight, QEM_LONG_NONSORT | QEM_LONG_SORT | QEM_LONG_SELF);
     // in mixed precision the long-range tangent runs on the side stream, as
     // in forces_core: k staging and z sort beside the pair tangent kernel,

@@ -199,14 +199,14 @@ struct ShortWork {
 // are then two contiguous runs, [i_start, i_end) of the first half (through
 // i_idx for an installed list) and [j_start, j_end) of the second, which
 // the gathers read in order.  Bytes for the records of npairs pairs, rec
-// bytes each (16 for float4, 32 for double4): two per pair, or four for the
-// forces + energies kernel (two record sets).
-inline size_t pair_rec_bytes(int64_t npairs, size_t rec, int sets = 1)
+// bytes each (12 for float3, 16 for float4, 32 for double4): two per pair.
+inline size_t pair_rec_bytes(int64_t npairs, size_t rec)
 {
-    return 2 * (size_t)sets * rec * (size_t)(npairs > 0 ? npairs : 1);
+    return 2 * rec * (size_t)(npairs > 0 ? npairs : 1);
 }
 // Evaluates every pair of the list into endpoint records in d_fpair: first
-// atom (fx, fy, fz_i, 0), second atom (-fx, -fy, fz_j, 0).
+// atom (fx, fy, fz_i), second atom (-fx, -fy, fz_j) (float3 in mixed
+// precision, double4 with w = 0 otherwise).
 hipError_t launch_pair_forces(const ShortConsts &c, int precision,
                               const ShortWork &ws, const int32_t *d_pi,
                               const int32_t *d_pj, const int32_t *d_jpos, int64_t npairs,
@@ -396,8 +396,8 @@ hipError_t launch_long_tangent_sum(LongWork &lw, int64_t n, int64_t Ptot, double
                                    double *d_out, int accumulate, hipStream_t s,
                                    std::string &err);
 // Short-range energy: potential records in d_pot (pair_rec_bytes: per
-// endpoint the other atom's charge times (G, dG/dgamma_1, dG/dgamma_2, 0),
-// float4 in mixed precision, double4 in FP64), per atom phi (N) and psi
+// endpoint the other atom's charge times (G, dG/dgamma_1, dG/dgamma_2),
+// float3 in mixed precision, double4 in FP64), per atom phi (N) and psi
 // (N x 2: d phi / d gamma_1,2), self terms included.
 // The self terms run on self_s (default s) and are queued right after the
 // pair kernel; after_pair (optional) is called after them.  With self_s
@@ -411,7 +411,7 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
                                hipStream_t self_s = nullptr);
 // Short-range force tangent along d_v, times scale, into d_out (N x 3);
 // d_tp holds the tangent records (pair_rec_bytes: first atom (dfx, dfy,
-// dfz_i, 0), second atom (-dfx, -dfy, dfz_j, 0); float4 in mixed precision,
+// dfz_i), second atom (-dfx, -dfy, dfz_j); float3 in mixed precision,
 // double4 in FP64).
 // The self term runs on self_s (default s) and is queued right after the
 // pair kernel, then after_pair (optional) is called, then the gather is
