@@ -885,13 +885,11 @@ int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         if (st) return st;
         API_HIP(ctx, hipEventRecord(ctx->lanes_ev, s));
         API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0));
-        // passes A and S with the GPU to themselves, then the gather (on s)
-        // beside pass B: beside the gather's waves, the latency-bound chunk
-        // scan waits for wave slots
-        if ((st = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassAS))) return st;
-        API_HIP(ctx, hipEventRecord(ctx->sums_ev, ctx->side));
-        API_HIP(ctx, hipStreamWaitEvent(s, ctx->sums_ev, 0));
-        return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassB);
+        // the gather (on s) beside pass A: the gather is bound by memory
+        // latency and pass A by its transcendentals, so the two overlap
+        // (measured at N = 100k: the tail after the pair kernel 306 -> 269 us
+        // against A and S alone, then the gather beside B)
+        return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassAll);
     };
     bool deferred = false;
     int st = short_dev(ctx, n, d_xyz, d_q, d_f, s, true, addz, &lanes, &deferred);
