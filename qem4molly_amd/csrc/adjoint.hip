@@ -243,6 +243,9 @@ __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a,
     const float hR = self ? 0.0f : fminf(h * rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
     // S0, S1 = sum t p, t dp; U = u.x-weighted; Q = u.x E/D-weighted; R = u.x^2-weighted
     float S0 = 0.f, S1 = 0.f, U0 = 0.f, U1 = 0.f, Q0 = 0.f, Q1 = 0.f, R0 = 0.f, R1 = 0.f;
+    // two nodes per iteration: their scalar and LDS loads overlap (measured
+    // at N = 100k: HVP 4.57 -> 4.35 ms; four per iteration 4.42)
+#pragma unroll 2
     for (int n = 0; n < c.Nt; ++n) {
         const float4 u = ncon[n];
         const float e = hE * u.x;
