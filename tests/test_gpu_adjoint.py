@@ -238,3 +238,30 @@ def test_forces_energy_in_one_call(precision):
         assert rel_rms(phi, phio) < 1e-6
         assert np.allclose(dg, dgo, rtol=1e-5)
         assert E == pytest.approx(E2, rel=1e-6)
+
+
+@pytest.mark.parametrize("n,P", [(2000, 40), (2000, 90), (2000, 200), (300, 90)])
+def test_forces_energy_pass_b_variants(n, P):
+    # the mixed fused call forms the long-range forces and potential in one
+    # pass B walk; cover its row-group layouts (P = 40: one group of 40 rows;
+    # 90: a last group of 26 rows, two chunks per wave; 200: a last group of
+    # 8 rows, four chunks per wave) and, at n = 300, chunks of 16 atoms that
+    # are not flat in e^{-kz} (the descending walk): the forces equal the
+    # force call's bit for bit, the potential and E match the oracle.  E of
+    # a neutral system is a small difference of terms q_i phi_i / 2 (here
+    # ~1 against a sum of magnitudes ~300), so its bar is 1e-6 of that sum:
+    # the mixed energy-only call misses E by up to 1.1e-6 relative too
+    cfg, xyz, q, c = _setup("slab_n10k", n, sem=L.SEM_CORRECTED, precision="mixed")
+    with c, ctx_for(cfg, semantics=L.SEM_CORRECTED, precision="mixed",
+                    iter_period=100000) as c2:
+        K, _, sK = c.kset()
+        kidx = c.sample_k(13, P)
+        F, E, phi, dg = c.forces_energy(xyz, q, kidx)
+        F2 = c2.forces(xyz, q, kidx)
+    assert np.array_equal(F, F2)
+    pairs = O.neighbor_pairs(xyz, cfg.Lx, cfg.Ly, cfg.r_cutoff)
+    Eo, phio, dgo = O.energy(oracle_params(cfg, semantics=L.SEM_CORRECTED), xyz, q, pairs,
+                             K[kidx], weight=sK / P, nthreads=16)
+    assert abs(E - Eo) <= 1e-6 * 0.5 * np.abs(q * phio).sum()
+    assert rel_rms(phi, phio) < 1e-6
+    assert np.allclose(dg, dgo, rtol=1e-5)
