@@ -178,6 +178,24 @@ def test_adjoint_slab_n10k_fd_and_symmetry():
         assert dg[w] == pytest.approx(d, rel=1e-6, abs=1e-9 * abs(E))
 
 
+@pytest.mark.parametrize("g1,g2", [(0.0, 0.9), (0.9, -0.5), (0.95, 0.95)])
+def test_mixed_hvp_against_fp64_across_gammas(g1, g2):
+    # the mixed tangent loop takes its E/D-weighted node sums as 1/D-weighted
+    # ones through gamma_1 gamma_2 E = D + 1; the FP64 tangent does not, so
+    # the two agree only if that holds for a zero, a negative and a large
+    # product of the contrasts
+    base = scaled_config(CONFIGS["slab_n10k"], 1000)
+    cfg = base.__class__(**{**base.__dict__, "gamma_1": g1, "gamma_2": g2})
+    xyz, q = make_system(cfg, seed=4)
+    v = np.random.default_rng(5).normal(size=xyz.shape)
+    with ctx_for(cfg, semantics=L.SEM_CORRECTED, precision="mixed", iter_period=100000) as cm, \
+            ctx_for(cfg, semantics=L.SEM_CORRECTED, precision="fp64", iter_period=100000) as cd:
+        kidx = cm.sample_k(3, 50)
+        hm = cm.hvp(xyz, q, kidx, v)
+        hd = cd.hvp(xyz, q, kidx, v)
+    assert rel_rms(hm, hd) < 1e-6
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_energy_and_hvp_beyond_one_long_range_pass(precision):
     # a batch of 8,292 k vectors runs its long-range passes in two parts of
