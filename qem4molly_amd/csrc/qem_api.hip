@@ -875,20 +875,18 @@ int forces_core(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     API_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
     API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
     int64_t Ptot = 0;
-    // queued once the pair kernel is: the side stream's preparation (k rows,
-    // z sort, k = 0 term, per-k constants) runs beside it, and a caller that
-    // waits for each call does not see the GPU idle while the host queues
-    // that chain ahead of the pair kernel
+    // queued once the pair kernel is (so that a caller that waits for each
+    // call does not see the GPU idle while the host queues this chain ahead
+    // of it), and run beside it: k rows, z sort, k = 0 term, per-k
+    // constants, then the k-lane passes.  The pair kernel leaves issue slots
+    // that the passes' transcendental and FP64 work fills: measured at
+    // N = 100k, the pair kernel +125 us and the tail after it (the gather,
+    // the output pass) 267 -> 105 us against the passes after the pair
+    // kernel beside the gather
     const std::function<int()> lanes = [&]() -> int {
         int st = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
                            &Ptot);
         if (st) return st;
-        API_HIP(ctx, hipEventRecord(ctx->lanes_ev, s));
-        API_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0));
-        // the gather (on s) beside pass A: the gather is bound by memory
-        // latency and pass A by its transcendentals, so the two overlap
-        // (measured at N = 100k: the tail after the pair kernel 306 -> 269 us
-        // against A and S alone, then the gather beside B)
         return long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassAll);
     };
     bool deferred = false;
@@ -980,8 +978,9 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     int64_t Ptot = 0;
     int lst = QEM_OK;
     // queued once the pair kernel and the self term (on the side stream) are,
-    // and run beside the pair kernel: k rows, z sort and the k = 0 term;
-    // then, once it is done, the potential passes
+    // and run beside the pair kernel: k rows, z sort, the k = 0 term and the
+    // potential passes (beside the pair kernel rather than after it: the
+    // call 3.18 -> 3.08 ms at N = 100k)
     const std::function<void()> passes = [&]() {
         lst = stage_kvecs(ctx, kvecs, P, weight, true, ctx->side, &Ptot);
         if (lst) return;
@@ -994,17 +993,15 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                 return;
             }
         }
-        hipError_t e = hipEventRecord(ctx->lanes_ev, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
-        // the self term and the gather on s wait for passes A and S
+        // the gather on s waits for the self term and passes A and S, queued
+        // before it on the side stream
         const std::function<hipError_t()> sums = [&]() -> hipError_t {
             hipError_t r = hipEventRecord(ctx->sums_ev, ctx->side);
             return r == hipSuccess ? hipStreamWaitEvent(s, ctx->sums_ev, 0) : r;
         };
-        if (e == hipSuccess) {
-            PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
-            e = launch_long_energy_passes(lc, ctx->lw, n, Ptot, ctx->side, ctx->err, &sums);
-        }
+        PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
+        const hipError_t e =
+            launch_long_energy_passes(lc, ctx->lw, n, Ptot, ctx->side, ctx->err, &sums);
         if (e != hipSuccess) lst = hip_err(ctx, e, "long-range energy");
     };
     {
@@ -1098,8 +1095,9 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
     int64_t Ptot = 0;
     int lst = QEM_OK;
     // queued once the pair tangent kernel and the self term (on the side
-    // stream) are, and run beside the pair kernel: k rows and z sort; then,
-    // once it is done, the tangent passes
+    // stream) are, and run beside the pair kernel: k rows, z sort and the
+    // tangent passes (beside the pair kernel rather than after it: the HVP
+    // 4.35 -> 4.25 ms at N = 100k)
     const std::function<void()> lanes = [&]() {
         lst = stage_kvecs(ctx, kvecs, P, weight, false, ctx->side, &Ptot);
         if (lst) return;
@@ -1111,18 +1109,15 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
                 return;
             }
         }
-        hipError_t e = hipEventRecord(ctx->lanes_ev, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
-        // the self term and the gather on s wait for passes A and S
+        // the gather on s waits for the self term and passes A and S, queued
+        // before it on the side stream
         const std::function<hipError_t()> sums = [&]() -> hipError_t {
             hipError_t r = hipEventRecord(ctx->sums_ev, ctx->side);
             return r == hipSuccess ? hipStreamWaitEvent(s, ctx->sums_ev, 0) : r;
         };
-        if (e == hipSuccess) {
-            PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
-            e = launch_long_tangent(lc, ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1, ctx->side,
-                                    ctx->err, false, &sums);
-        }
+        PhaseScope ps(ctx, QEM_PHASE_LONG, ctx->side);
+        const hipError_t e = launch_long_tangent(lc, ctx->lw, n, Ptot, d_v, -1.0, d_hv, 1,
+                                                 ctx->side, ctx->err, false, &sums);
         if (e != hipSuccess) lst = hip_err(ctx, e, "long-range tangent");
     };
     {
@@ -1244,15 +1239,13 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
     if (st == QEM_OK && lst == QEM_OK)
         lst = long_prep(ctx, n, d_xyz, d_q, kv.data(), P, w, QEM_LONG_ALL, ctx->side, false,
                         &Ptot);
-    // side stream, once the pair kernel is done: passes A and S with the GPU
-    // to themselves, then (beside the gather, which waits for them) pass B
-    // on their chunk sums, forces and potential in one walk
+    // side stream, beside the pair kernel as well (measured at N = 100k:
+    // the call 3.97 -> 3.94 ms against waiting for the pair kernel): passes
+    // A and S, which the gather waits for, then pass B on their chunk sums,
+    // forces and potential in one walk
     if (lst == QEM_OK && st == QEM_OK) {
-        hipError_t e = hipEventRecord(ctx->lanes_ev, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->lanes_ev, 0);
-        if (e != hipSuccess) lst = hip_err(ctx, e, "lanes event");
-        if (lst == QEM_OK)
-            lst = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassAS);
+        hipError_t e = hipSuccess;
+        lst = long_lanes(ctx, n, Ptot, w, QEM_LONG_ALL, ctx->side, kLongPassAS);
         if (lst == QEM_OK) {
             e = hipEventRecord(ctx->sums_ev, ctx->side);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->sums_ev, 0);
@@ -1394,7 +1387,6 @@ int qem_create(const qem_params *p, qem_ctx **out)
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->lanes_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->sums_ev, hipEventDisableTiming) != hipSuccess)
         return fail(QEM_ERR_HIP);
@@ -1554,7 +1546,6 @@ int qem_destroy(qem_ctx *ctx)
     }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-    if (ctx->lanes_ev) (void)hipEventDestroy(ctx->lanes_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     if (ctx->sums_ev) (void)hipEventDestroy(ctx->sums_ev);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);

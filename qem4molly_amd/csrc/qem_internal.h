@@ -460,10 +460,11 @@ struct qem_ctx {
     std::string err;
     hipStream_t stream = nullptr;
     // the force call's long range runs on `side` alongside the short range
-    // on the call's stream: fork_ev orders it after the inputs, lanes_ev its
-    // k-lane passes after the pair kernel, join_ev the final sum after it
+    // on the call's stream: fork_ev orders it after the inputs, join_ev the
+    // final sum after it, and sums_ev (adjoint calls) the gather after the
+    // side stream's self terms and chunk sums
     hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, lanes_ev = nullptr, join_ev = nullptr, sums_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, sums_ev = nullptr;
     // per-k dE_k/dgamma of the energy call, copied on the call's stream; kept
     // here, not on the stack, so an early error return leaves no copy
     // pending into freed memory
