@@ -568,8 +568,10 @@ def main(argv=None):
     if a.gpus != 1 or world != 1:
         if int(os.environ.get("RANK", "0")) == 0:
             print("bench.py: this build runs the QEM force path on one GPU only; "
-                  f"--gpus {a.gpus} (WORLD_SIZE={world}) is not supported. How the path "
-                  "would shard is described in DESIGN.md.", file=sys.stderr)
+                  f"--gpus {a.gpus} (WORLD_SIZE={world}) is not supported. Across GPUs the "
+                  "short range would shard by xy domain with an r_cutoff halo and the long "
+                  "range by k batch (positions all-gathered, forces reduce-scattered).",
+                  file=sys.stderr)
         return 2
 
     if a.adjoint_only:
