@@ -685,22 +685,27 @@ __global__ __launch_bounds__(256) void k_half_dots_part(int64_t n, const double 
 }
 
 // out3 = (1/2 sum q phi, 1/2 sum q psi_1 + sum gpart_1, 1/2 sum q psi_2 +
-// sum gpart_2) from the block parts, in block order
-__global__ void k_half_dots_final(int nb, const double *__restrict__ part,
-                                  double *__restrict__ out3)
+// sum gpart_2) from the block parts: one wave, lane l adding parts l,
+// l + 64, ... in order, then a butterfly over the 64 lanes (a fixed order;
+// every lane forms the same sums).  One thread summing all 5 nb parts took
+// 25 us of dependent loads.
+__global__ __launch_bounds__(64) void k_half_dots_final(int nb, const double *__restrict__ part,
+                                                        double *__restrict__ out3)
 {
-    if (threadIdx.x != 0) return;
-    double a = 0.0, b = 0.0, d = 0.0, g1 = 0.0, g2 = 0.0;
-    for (int i = 0; i < nb; ++i) {
-        a += part[5 * i];
-        b += part[5 * i + 1];
-        d += part[5 * i + 2];
-        g1 += part[5 * i + 3];
-        g2 += part[5 * i + 4];
+    const int l = threadIdx.x;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = l; i < nb; i += 64)
+#pragma unroll
+        for (int r = 0; r < 5; ++r) v[r] += part[5 * i + r];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v[r] += __shfl_xor(v[r], d);
+    if (l == 0) {
+        out3[0] = 0.5 * v[0];
+        out3[1] = 0.5 * v[1] + v[3];
+        out3[2] = 0.5 * v[2] + v[4];
     }
-    out3[0] = 0.5 * a;
-    out3[1] = 0.5 * b + g1;
-    out3[2] = 0.5 * d + g2;
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
