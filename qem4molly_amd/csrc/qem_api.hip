@@ -934,6 +934,21 @@ LongConsts long_consts(const qem_ctx *ctx, double weight, uint32_t parts)
     return c;
 }
 
+// ctx->h_out with room for nd doubles (grown, never shrunk)
+static int host_out(qem_ctx *ctx, size_t nd, double **out)
+{
+    if (ctx->h_out_cap < nd) {
+        if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+        ctx->h_out = nullptr;
+        ctx->h_out_cap = 0;
+        API_HIP(ctx, hipHostMalloc(reinterpret_cast<void **>(&ctx->h_out), nd * sizeof(double),
+                                   hipHostMallocDefault));
+        ctx->h_out_cap = nd;
+    }
+    *out = ctx->h_out;
+    return QEM_OK;
+}
+
 // E, phi = dE/dq (device, N) and dE/dgamma for the current list + k batch
 // the energy and tangent kernels implement the image-series forms for
 // |gamma_1 gamma_2| < 1 only
@@ -1030,15 +1045,16 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
         }
     }
     double dg_long[2] = {0.0, 0.0};
-    std::vector<double> &hq0 = ctx->h_q0;
-    hq0.clear();
+    double *hout = nullptr;
+    size_t nq0 = 0;   // per-k dE_k/dgamma values copied to hout + 4
     if (overlap) {
         API_HIP(ctx, hipEventRecord(ctx->join_ev, ctx->side));
         API_HIP(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
         if (lst) return lst;
-        hq0.resize(2 * (size_t)Ptot);
+        nq0 = 2 * (size_t)Ptot;
+        if ((st = host_out(ctx, 4 + nq0, &hout))) return st;
         PhaseScope ps(ctx, QEM_PHASE_LONG, s);
-        hipError_t e = launch_long_energy_sum(ctx->lw, n, Ptot, phi, 1, hq0.data(), s,
+        hipError_t e = launch_long_energy_sum(ctx->lw, n, Ptot, phi, 1, hout + 4, s,
                                               ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "long-range energy");
     } else if (P >= 0) {
@@ -1059,18 +1075,18 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
             if (last) break;
         }
     }
+    if (!hout && (st = host_out(ctx, 4, &hout))) return st;
     API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s));
-    double h[3];
-    API_HIP(ctx, hipMemcpyAsync(h, out3, sizeof h, hipMemcpyDeviceToHost, s));
+    API_HIP(ctx, hipMemcpyAsync(hout, out3, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     API_HIP(ctx, hipStreamSynchronize(s));
-    for (size_t p = 0; p < hq0.size(); p += 2) {
-        dg_long[0] += hq0[p];
-        dg_long[1] += hq0[p + 1];
+    for (size_t p = 0; p < nq0; p += 2) {
+        dg_long[0] += hout[4 + p];
+        dg_long[1] += hout[4 + p + 1];
     }
-    if (E) *E = h[0];
+    if (E) *E = hout[0];
     if (dEdg) {
-        dEdg[0] = h[1] + dg_long[0];
-        dEdg[1] = h[2] + dg_long[1];
+        dEdg[0] = hout[1] + dg_long[0];
+        dEdg[1] = hout[2] + dg_long[1];
     }
     return QEM_OK;
 }
@@ -1275,8 +1291,9 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
     if (lst) return lst;
     if (ej != hipSuccess) return hip_err(ctx, ej, "join event");
     if (ew != hipSuccess) return hip_err(ctx, ew, "join wait");
-    std::vector<double> &hq0 = ctx->h_q0;
-    hq0.assign(2 * (size_t)Ptot, 0.0);
+    const size_t nq0 = 2 * (size_t)Ptot;   // per-k dE_k/dgamma values copied to hout + 4
+    double *hout = nullptr;
+    if (int st2 = host_out(ctx, 4 + nq0, &hout)) return st2;
     {
         PhaseScope ps(ctx, QEM_PHASE_LONG, s);
         hipError_t e = launch_long_sum_slots(ctx->lw, n, Ptot, nl.slot_of.as<int32_t>(),
@@ -1285,22 +1302,21 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
         if (e == hipSuccess)
             e = launch_rows_from_slots(nl, n, 2, nl.sout2.as<double>() + n, psi, s);
         if (e == hipSuccess)
-            e = launch_long_energy_sum(ctx->lw, n, Ptot, phi, 1, hq0.data(), s, ctx->err);
+            e = launch_long_energy_sum(ctx->lw, n, Ptot, phi, 1, hout + 4, s, ctx->err);
         if (e != hipSuccess) return hip_err(ctx, e, "output passes");
     }
     API_HIP(ctx, launch_half_dots(n, d_q, phi, psi, out3, part, s, gpart, ngp));
-    double h[3];
-    API_HIP(ctx, hipMemcpyAsync(h, out3, sizeof h, hipMemcpyDeviceToHost, s));
+    API_HIP(ctx, hipMemcpyAsync(hout, out3, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     API_HIP(ctx, hipStreamSynchronize(s));
     double dg_long[2] = {0.0, 0.0};
-    for (size_t q = 0; q < hq0.size(); q += 2) {
-        dg_long[0] += hq0[q];
-        dg_long[1] += hq0[q + 1];
+    for (size_t q = 0; q < nq0; q += 2) {
+        dg_long[0] += hout[4 + q];
+        dg_long[1] += hout[4 + q + 1];
     }
-    if (E) *E = h[0];
+    if (E) *E = hout[0];
     if (dEdg) {
-        dEdg[0] = h[1] + dg_long[0];
-        dEdg[1] = h[2] + dg_long[1];
+        dEdg[0] = hout[1] + dg_long[0];
+        dEdg[1] = hout[2] + dg_long[1];
     }
     return QEM_OK;
 }
@@ -1539,6 +1555,7 @@ int qem_destroy(qem_ctx *ctx)
         if (ctx->lw.kv_ev[t]) (void)hipEventDestroy(ctx->lw.kv_ev[t]);
     }
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+    if (ctx->h_out) (void)hipHostFree(ctx->h_out);
     if (ctx->wall_ev) (void)hipEventDestroy(ctx->wall_ev);
     for (auto &m : ctx->marks) {
         (void)hipEventDestroy(m.a);

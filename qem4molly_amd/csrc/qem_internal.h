@@ -465,10 +465,13 @@ struct qem_ctx {
     // side stream's self terms and chunk sums
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr, sums_ev = nullptr;
-    // per-k dE_k/dgamma of the energy call, copied on the call's stream; kept
-    // here, not on the stack, so an early error return leaves no copy
-    // pending into freed memory
-    std::vector<double> h_q0;
+    // pinned host memory for the energy calls' small device-to-host copies
+    // (the three half-dot sums at [0, 3), the per-k dE_k/dgamma from [4)): a
+    // copy into pageable memory holds the host until it is done, before it
+    // can queue the next launch.  Kept here, not on the stack, so an early
+    // error return leaves no copy pending into freed memory
+    double *h_out = nullptr;
+    size_t h_out_cap = 0;   // doubles
     // K set (host)
     std::vector<double> kset;   // |K| x 3
     std::vector<float> kprob;   // |K|
