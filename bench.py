@@ -274,6 +274,18 @@ def adjoint_cost(name, precision, reps=10):
     t_e = timed(lambda: ctx.energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, pd.ptr))
     t_fe = timed(lambda: ctx.forces_energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr, pd.ptr))
     mem = ctx.device_bytes()
+    # the same step with the forward keeping the pair quadrature's tangent
+    # node sums (qem_forces_energy_save_dev) and the HVP combining them
+    # (qem_hvp_saved_dev): each call alone, then the pair as one step
+    t_fes = timed(lambda: ctx.forces_energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr, pd.ptr,
+                                                save=True))
+    t_hs = timed(lambda: ctx.hvp_dev(cfg.n, xd.ptr, qd.ptr, kidx, vd.ptr, hd.ptr, saved=True))
+
+    def step_saved():
+        ctx.forces_energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr, pd.ptr, save=True)
+        ctx.hvp_dev(cfg.n, xd.ptr, qd.ptr, kidx, vd.ptr, hd.ptr, saved=True)
+    t_ss = timed(step_saved)
+    mem_saved = ctx.device_bytes()
     ctx.close()
     # the differentiable op's forward is the one-call forces + energy (F, E,
     # dE/dq, dE/dgamma); its backward is the HVP
@@ -283,7 +295,9 @@ def adjoint_cost(name, precision, reps=10):
             "backward_over_forward": t_h / t_fe,
             "forward_plus_backward_over_forward": (t_fe + t_h) / t_fe,
             "adjoint_step_over_force_step": (t_fe + t_h) / t_f,
-            "device_bytes": mem}
+            "forces_energy_save_ms": t_fes, "hvp_saved_ms": t_hs, "saved_step_ms": t_ss,
+            "saved_step_over_force_step": t_ss / t_f,
+            "device_bytes": mem, "device_bytes_saved": mem_saved}
 
 
 def general_step(cfg, xyz_d, q_d, f_d, steps, warmup, seed0=1000):

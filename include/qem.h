@@ -301,6 +301,32 @@ int qem_hvp(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
 int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                 const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv,
                 void *stream);
+/* The adjoint step with one pair quadrature in the forward (no reference
+ * counterpart; for callers that take forces, energy and then a
+ * Hessian-vector product at the same positions, as an autograd backward
+ * does).  qem_forces_energy_save(_dev) is qem_forces_energy(_dev) whose
+ * pair kernel also keeps, in mixed precision, every pair's force-tangent
+ * node sums in the context (256 bytes per pair of device memory, grown
+ * once); qem_hvp_saved(_dev) is qem_hvp(_dev) that combines those sums
+ * instead of running the tangent quadrature.  The caller passes
+ * qem_hvp_saved(_dev) the positions, charges and k indices of the last save
+ * call (the library cannot tell changed positions apart); it uses the sums
+ * when they belong to the current atom count and neighbour list (no
+ * rebuild since) and the context is mixed precision, and otherwise
+ * computes the product as qem_hvp(_dev) does.  Any forces + energy call
+ * without saving drops the sums.  The product equals qem_hvp's within the
+ * mixed-precision tolerance (the sums are taken at the forward's nodes). */
+int qem_forces_energy_save_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                               const int64_t *k_idx, int64_t P, double *d_f, double *d_dE_dq,
+                               double *E, double *dE_dgamma, void *stream);
+int qem_forces_energy_save(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                           const int64_t *k_idx, int64_t P, double *f, double *E,
+                           double *dE_dq, double *dE_dgamma);
+int qem_hvp_saved_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                      const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv,
+                      void *stream);
+int qem_hvp_saved(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                  const int64_t *k_idx, int64_t P, const double *v, double *hv);
 
 /* ---- allocation-free long range ------------------------------------------
  * No counterpart in the reference.  qem_long_forces_dev for callers that

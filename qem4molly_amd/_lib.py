@@ -56,6 +56,7 @@ EXPORTS = (
     "qem_device_bytes", "qem_substrate_lj", "qem_substrate_lj_dev", "qem_general_forces_dev",
     "qem_long_variance", "qem_long_variance_dev", "qem_long_terms_dev",
     "qem_long_forces_noalloc_dev", "qem_forces_energy_dev", "qem_forces_energy",
+    "qem_forces_energy_save_dev", "qem_forces_energy_save", "qem_hvp_saved_dev", "qem_hvp_saved",
 )
 
 NOALLOC_MAX_N = 131072
@@ -157,6 +158,11 @@ _SIGS = {
     "qem_forces_energy_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _pd, _pd,
                                         _vp]),
     "qem_forces_energy": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd, _pd, _pd]),
+    "qem_forces_energy_save_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _pd,
+                                             _pd, _vp]),
+    "qem_forces_energy_save": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd, _pd, _pd]),
+    "qem_hvp_saved": (C.c_int, [_vp, _i64, _pd, _pd, _pi64, _i64, _pd, _pd]),
+    "qem_hvp_saved_dev": (C.c_int, [_vp, _i64, _vp, _vp, _pi64, _i64, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -439,32 +445,38 @@ class Context:
                                      k_idx.shape[0], C.byref(E), ptr(phi), ptr(dg)), "qem_energy")
         return E.value, phi, dg
 
-    def forces_energy(self, xyz, q, k_idx):
-        """(F [N x 3], E, dE/dq [N], (dE/dgamma_1, dE/dgamma_2)) in one call."""
+    def forces_energy(self, xyz, q, k_idx, save=False):
+        """(F [N x 3], E, dE/dq [N], (dE/dgamma_1, dE/dgamma_2)) in one call; save
+        keeps the pair quadrature's tangent sums for hvp(..., saved=True)."""
         xyz = f64(xyz, (-1, 3)); q = f64(q)
         k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
         f = np.zeros_like(xyz); E = _d(); phi = np.zeros(xyz.shape[0]); dg = np.zeros(2)
-        self._check(lib().qem_forces_energy(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
-                                            k_idx.shape[0], ptr(f), C.byref(E), ptr(phi),
-                                            ptr(dg)), "qem_forces_energy")
+        name = "qem_forces_energy_save" if save else "qem_forces_energy"
+        self._check(getattr(lib(), name)(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
+                                         k_idx.shape[0], ptr(f), C.byref(E), ptr(phi),
+                                         ptr(dg)), name)
         return f, E.value, phi, dg
 
-    def forces_energy_dev(self, n, d_xyz, d_q, k_idx, d_f, d_phi=None, stream=None):
+    def forces_energy_dev(self, n, d_xyz, d_q, k_idx, d_f, d_phi=None, stream=None,
+                          save=False):
         """(E, dE/dgamma); forces into d_f, dE/dq into d_phi (device)."""
         k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
         E = _d(); dg = np.zeros(2)
-        self._check(lib().qem_forces_energy_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx),
-                                                k_idx.shape[0], d_f, d_phi, C.byref(E), ptr(dg),
-                                                stream), "qem_forces_energy_dev")
+        name = "qem_forces_energy_save_dev" if save else "qem_forces_energy_dev"
+        self._check(getattr(lib(), name)(self._h, int(n), d_xyz, d_q, ptr(k_idx),
+                                         k_idx.shape[0], d_f, d_phi, C.byref(E), ptr(dg),
+                                         stream), name)
         return E.value, dg
 
-    def hvp(self, xyz, q, k_idx, v) -> np.ndarray:
-        """-(dF/dx) v of the force as computed (N x 3)."""
+    def hvp(self, xyz, q, k_idx, v, saved=False) -> np.ndarray:
+        """-(dF/dx) v of the force as computed (N x 3); saved uses the sums of the
+        last forces_energy(..., save=True) at the same xyz, q, k_idx."""
         xyz = f64(xyz, (-1, 3)); q = f64(q); v = f64(v, (-1, 3))
         k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
         hv = np.zeros_like(xyz)
-        self._check(lib().qem_hvp(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
-                                  k_idx.shape[0], ptr(v), ptr(hv)), "qem_hvp")
+        name = "qem_hvp_saved" if saved else "qem_hvp"
+        self._check(getattr(lib(), name)(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
+                                         k_idx.shape[0], ptr(v), ptr(hv)), name)
         return hv
 
     def energy_dev(self, n, d_xyz, d_q, k_idx, d_phi=None, stream=None):
@@ -474,10 +486,11 @@ class Context:
                                          d_phi, C.byref(E), ptr(dg), stream), "qem_energy_dev")
         return E.value, dg
 
-    def hvp_dev(self, n, d_xyz, d_q, k_idx, d_v, d_hv, stream=None):
+    def hvp_dev(self, n, d_xyz, d_q, k_idx, d_v, d_hv, stream=None, saved=False):
         k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
-        self._check(lib().qem_hvp_dev(self._h, int(n), d_xyz, d_q, ptr(k_idx), k_idx.shape[0],
-                                      d_v, d_hv, stream), "qem_hvp_dev")
+        name = "qem_hvp_saved_dev" if saved else "qem_hvp_dev"
+        self._check(getattr(lib(), name)(self._h, int(n), d_xyz, d_q, ptr(k_idx),
+                                         k_idx.shape[0], d_v, d_hv, stream), name)
 
     def timing_enable(self, on=True):
         self._check(lib().qem_timing_enable(self._h, int(bool(on))), "qem_timing_enable")

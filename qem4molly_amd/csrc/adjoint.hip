@@ -203,27 +203,18 @@ __device__ __forceinline__ double3 self_energy(const ShortConsts &c,
 // Value and first-order tangent of sum_n w k t(k) {J1, J0}(k rho), where the
 // limit K (so every node k_n = K (1 + x_n)/2 and weight) moves with dK, the
 // image distance a with da and rho with drho.
-struct Dual4 {
-    double sr, sz, dsr, dsz;
-};
+using dev::Dual4;
+using dev::TanSums;
 
 // Mixed-precision value and tangent node sums.  With the node constants
 // u = (1 + x_n, (1 + x_n)^2, log2 w_n, log2(w_n (1 + x_n))) the node is
 // k = h u.x (h = K/2) and the summand t J is taken as in the force kernel
-// (t = w k e^{..}/D / h^2, J0 and J1 = -J0' from the cubic table).  Along the
-// tangent the nodes move with K (dk = r k, r = dK/K), the image distance
-// with da and rho with drho, so
-//   dt = t k (A + B E/D [+ C k]),   A = -r (a [+ 2 Lz]) - da,
-//   B = 2 Lz g12 r,  C = -r / (2 alpha) (gaussian set),
-//   dJ0 = -J1 x s,  dJ1 = (x J0 - J1) s,  s = r + drho/rho,  x = k rho,
-// and d(h^2 S) = h^2 (2 r S + dS).  Since k and the table argument are both
-// multiples of u.x, the tangent is a fixed combination of six (eight for the
-// gaussian set) node sums: S = sum t J, U = sum u.x t J, Q = sum u.x (E/D) t J,
-// R = sum u.x^2 t J, each for J0 and J1; the per-set factors are applied once.
-// With g12 E = D + 1, B E/D = 2 Lz r (1 + 1/D) (g12 = 0 included), so
-// B Q = 2 Lz r (U + Q') with Q' = sum u.x t J / D: one product per node
-// (u.x t times 1/D) instead of two, and the coefficient of U becomes
-// A + 2 Lz r.
+// (t = w k e^{..}/D / h^2, J0 and J1 = -J0' from the cubic table).  Since k
+// and the table argument are both multiples of u.x, the tangent is a fixed
+// combination of six (eight for the gaussian set) node sums, TanSums, which
+// dev::tangent_of_sums combines with the per-set factors (the derivation is
+// there); the E/D-weighted sums are taken as 1/D-weighted ones: one product
+// per node (u.x t times 1/D) instead of two.
 __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a, float da,
                                                      float rho, float drho, bool gauss,
                                                      bool self, const ShortConsts &c,
@@ -231,7 +222,6 @@ __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a,
 {
     const float4 *__restrict__ ncon = c.ncon;
     const float h = 0.5f * K;
-    const float r = K != 0.0f ? dK / K : 0.0f;
     const float Lz2 = 2.0f * (float)c.Lz;
     const float c2 = -Lz2 * kLog2e;
     const float cg = -kLog2e * (float)(0.25 / c.alpha);
@@ -282,23 +272,8 @@ __device__ __forceinline__ Dual4 nodes_tangent_mixed(float K, float dK, float a,
             R1 = fmaf(tr, dp, R1);
         }
     }
-    // A + 2 Lz r and 2 Lz r (the E/D sums folded into U and Q' above)
-    const float A = gauss ? fmaf(-r, a - Lz2, -da) : -r * a - da;
-    const float B = Lz2 * r;
-    const float C = gauss ? -r * (float)(0.5 / c.alpha) : 0.0f;
-    const float sg = (!self && rho > 0.0f) ? r + drho / rho : 0.0f;
-    const float sg2 = sg * c.jt_dx * c.jt_dx;
-    const float hA = h * A, hB = h * B, h2C = h * h * C;
-    // tangents of the sums over nodes, before the h^2 factor
-    const float dS0 = hA * U0 + hB * Q0 + h2C * R0 + sg * hR * U1;
-    const float dS1 = hA * U1 + hB * Q1 + h2C * R1 - sg * S1 - sg2 * hR * U0;
-    const float h2 = h * h;
-    Dual4 o;
-    o.sr = -h2 * c.jt_inv_dx * S1;
-    o.sz = h2 * S0;
-    o.dsr = -h2 * c.jt_inv_dx * fmaf(2.0f * r, S1, dS1);
-    o.dsz = h2 * fmaf(2.0f * r, S0, dS0);
-    return o;
+    return dev::tangent_of_sums(TanSums{S0, S1, U0, U1, Q0, Q1, R0, R1}, K, dK, a, da, rho,
+                                drho, gauss, self, c);
 }
 
 template <int PREC>
@@ -359,21 +334,33 @@ __device__ __forceinline__ Dual4 nodes_tangent(double K, double dK, double a, do
 }
 
 // Tangent of F_short_ij (force on i; z force on j) along (vi, vj).
-template <int PREC>
+// SAVED (mixed precision): the node sums of every set come from the saved
+// arrays sv (np pairs, this pair at sp) that the fused forces + energy
+// kernel wrote at the same positions; rho and the limits are then formed
+// as that kernel formed them, so the sums belong to these nodes.
+template <int PREC, bool SAVED = false>
 __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, const double *__restrict__ glx,
                                       const double *__restrict__ glw,
                                       const float *__restrict__ glxf,
                                       const float *__restrict__ glwf,
                                       const float4 *__restrict__ tab, double qi, double qj,
                                       const double *ri, const double *rj, const double *vi,
-                                      const double *vj)
+                                      const double *vj, const float4 *__restrict__ sv = nullptr,
+                                      int64_t np = 0, int64_t sp = 0)
 {
+    static_assert(!SAVED || PREC == QEM_PREC_MIXED, "the saved sums are mixed precision");
     const double dx = ri[0] - rj[0], dy = ri[1] - rj[1];
     double mx, my;
     const double d2 = dev::min_image_d2(c, dx, dy, mx, my);
     if (d2 > c.rc2) return make_double4(0.0, 0.0, 0.0, 0.0);
     const double ddx = vi[0] - vj[0], ddy = vi[1] - vj[1];
-    const double rho = sqrt(d2);
+    double rho;
+    if constexpr (SAVED) {
+        double irho;
+        dev::mixed_rho(d2, rho, irho);
+    } else {
+        rho = sqrt(d2);
+    }
     const double drho = rho > 0.0 ? (mx * ddx + my * ddy) / rho : 0.0;
     const double zn = fabs(ri[2] - rj[2]), zp = ri[2] + rj[2], sg = jl_sign(ri[2] - rj[2]);
     const double dzn = sg * (vi[2] - vj[2]), dzp = vi[2] + vj[2];
@@ -385,7 +372,13 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
         dev::image(c, l, zn, zp, sg, a, b, s);
         const double da = l == 0 ? dzn : (l == 1 ? dzp : (l == 2 ? -dzp : -dzn));
         const dev::Limits L = dev::limits(c, a);
-        const double Kp = swap ? L.kf1 : L.kf2, Kg = swap ? L.kf2 : L.kf1;
+        double Kp = swap ? L.kf1 : L.kf2, Kg = swap ? L.kf2 : L.kf1;
+        if constexpr (SAVED) {
+            float kp, kg;
+            dev::mixed_limits(c, a, swap, kp, kg);
+            Kp = kp;
+            Kg = kg;
+        }
         const double dKp = (swap ? L.dkf1 : L.dkf2) * da, dKg = (swap ? L.dkf2 : L.dkf1) * da;
         const double r2 = a * a + rho * rho;
         const double inv32 = 1.0 / (r2 * sqrt(r2));
@@ -395,11 +388,21 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
         const double dcz = hb * s * (da * inv32 + a * dinv32);
         Dual4 P{0.0, 0.0, 0.0, 0.0}, G{0.0, 0.0, 0.0, 0.0};
         if (b != 0.0) {
-            P = nodes_tangent<PREC>(Kp, dKp, a, da, rho, drho, false, c, glx, glw, glxf, glwf,
-                                    tab, false);
-            if (!c.single)
-                G = nodes_tangent<PREC>(Kg, dKg, a, da, rho, drho, true, c, glx, glw, glxf,
+            if constexpr (SAVED) {
+                P = dev::tangent_of_sums(dev::load_tan_sums(sv, np, sp, l, 0), (float)Kp,
+                                         (float)dKp, (float)a, (float)da, (float)rho,
+                                         (float)drho, false, false, c);
+                if (!c.single)
+                    G = dev::tangent_of_sums(dev::load_tan_sums(sv, np, sp, l, 1), (float)Kg,
+                                             (float)dKg, (float)a, (float)da, (float)rho,
+                                             (float)drho, true, false, c);
+            } else {
+                P = nodes_tangent<PREC>(Kp, dKp, a, da, rho, drho, false, c, glx, glw, glxf,
                                         glwf, tab, false);
+                if (!c.single)
+                    G = nodes_tangent<PREC>(Kg, dKg, a, da, rho, drho, true, c, glx, glw, glxf,
+                                            glwf, tab, false);
+            }
         }
         Fsr += -(hb * c.g12) * P.sr + cr + hb * G.sr;
         dFsr += -(hb * c.g12) * P.dsr + dcr + hb * G.dsr;
@@ -506,6 +509,24 @@ __global__ __launch_bounds__(256) void k_pair_tangent(
         const int64_t i = o.i, j = o.j;
         const double4 d = pair_force_tangent<PREC>(c, glx, glw, glxf, glwf, tab, o.qi, o.qj,
                                                    xyz + 3 * i, xyz + 3 * j, v + 3 * i, v + 3 * j);
+        dev::store4(out + p, d.x, d.y, d.z, 0.0);
+        dev::store4(out + npairs + o.jp, -d.x, -d.y, d.w, 0.0);
+    });
+}
+
+// k_pair_tangent (mixed precision) from the node sums that the fused forces
+// + energy kernel kept for these pairs (no node loop, no Bessel table)
+__global__ __launch_bounds__(256) void k_pair_tangent_saved(
+    ShortConsts c, const float4 *__restrict__ sv, const int32_t *__restrict__ pi,
+    const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
+    const double *__restrict__ xyz, const double *__restrict__ q, const double *__restrict__ v,
+    float3 *__restrict__ out)
+{
+    dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
+        const int64_t i = o.i, j = o.j;
+        const double4 d = pair_force_tangent<QEM_PREC_MIXED, true>(
+            c, nullptr, nullptr, nullptr, nullptr, nullptr, o.qi, o.qj, xyz + 3 * i, xyz + 3 * j,
+            v + 3 * i, v + 3 * j, sv, npairs, p);
         dev::store4(out + p, d.x, d.y, d.z, 0.0);
         dev::store4(out + npairs + o.jp, -d.x, -d.y, d.w, 0.0);
     });
@@ -804,7 +825,8 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,
                                 double scale, double *d_out, hipStream_t s,
-                                const std::function<void()> *after_pair, hipStream_t self_s)
+                                const std::function<void()> *after_pair, hipStream_t self_s,
+                                const float *d_sums)
 {
     if (n <= 0) return hipSuccess;
     if (!self_s) self_s = s;
@@ -815,7 +837,11 @@ hipError_t launch_short_tangent(const ShortConsts &c, int precision, const Short
     float3 *tpf = reinterpret_cast<float3 *>(d_tp);
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
     if (precision == QEM_PREC_MIXED) {
-        if (np > 0)
+        if (np > 0 && d_sums)
+            k_pair_tangent_saved<<<pair_grid(np), 256, 0, s>>>(
+                c, reinterpret_cast<const float4 *>(d_sums), nl.pi.as<int32_t>(),
+                nl.pj.as<int32_t>(), jpos, np, d_xyz, d_q, d_v, tpf);
+        else if (np > 0)
             k_pair_tangent<QEM_PREC_MIXED><<<pair_grid(np), 256, lds, s>>>(
                 c, ws.glx.as<double>(), ws.glw.as<double>(), ws.glxf.as<float>(),
                 ws.glwf.as<float>(), ws.jtab.as<float4>(), nl.pi.as<int32_t>(),

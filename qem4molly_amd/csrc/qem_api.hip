@@ -1092,13 +1092,22 @@ int energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
 }
 
 // hv = -(dF/dx) v for the current list + k batch
+// saved: use the node sums of the last saving forces + energy call when they
+// belong to this atom count and neighbour list (mixed precision); the caller
+// passes the positions, charges and k rows of that call
 int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, const double *kvecs,
-            int64_t P, double weight, const double *d_v, double *d_hv, hipStream_t s)
+            int64_t P, double weight, const double *d_v, double *d_hv, hipStream_t s,
+            bool saved = false)
 {
     if (int st0 = check_adjoint(ctx)) return st0;
     if (n == 0) return QEM_OK;
     int st = ensure_list(ctx, n, d_xyz, s);
     if (st) return st;
+    const float *sums =
+        (saved && ctx->tsums_ok && ctx->tsums_n == n && ctx->tsums_np == ctx->nl.npairs &&
+         ctx->tsums_builds == ctx->nl.builds && ctx->p.precision == QEM_PREC_MIXED)
+            ? ctx->tsums.as<float>()
+            : nullptr;
     const ShortConsts c = short_consts(ctx, 0);
     API_HIP(ctx, ensure(ctx->sw.fpair, pair_rec_bytes(ctx->nl.npairs, ctx->p.precision == QEM_PREC_MIXED ? 12 : 32)));
     const LongConsts lc = long_consts(ctx, weight, QEM_LONG_NONSORT | QEM_LONG_SORT | QEM_LONG_SELF);
@@ -1160,7 +1169,7 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
         API_HIP(ctx, launch_short_tangent(c, ctx->p.precision, ctx->sw, nl, n, px, pq, pv,
                                           ctx->sw.fpair.as<double>(), -1.0, po, s,
                                           overlap ? &lanes : nullptr,
-                                          overlap ? ctx->side : nullptr));
+                                          overlap ? ctx->side : nullptr, sums));
         if (nl.slot_order) API_HIP(ctx, launch_rows_from_slots(nl, n, 3, po, d_hv, s));
     }
     if (overlap) {
@@ -1193,9 +1202,10 @@ int hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q, con
 // then energy_dev.  The step counter advances as for a force call.
 int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
                       const std::vector<double> &kv, int64_t P, double *d_f, double *d_phi,
-                      double *E, double *dEdg, hipStream_t s)
+                      double *E, double *dEdg, hipStream_t s, bool save = false)
 {
     if (int st0 = check_adjoint(ctx)) return st0;
+    ctx->tsums_ok = false;
     const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
     if (n == 0) {
         if (E) *E = 0.0;
@@ -1224,6 +1234,11 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
         API_HIP(ctx, ensure(ctx->d_f2, sizeof(double) * (size_t)n));
         phi = ctx->d_f2.as<double>();
     }
+    float *sums = nullptr;
+    if (save && nl.npairs > 0) {
+        API_HIP(ctx, ensure(ctx->tsums, tan_sums_bytes(nl.npairs)));
+        sums = ctx->tsums.as<float>();
+    }
     int64_t Ptot = 0;
     int lst = QEM_OK;
     int st = QEM_OK;
@@ -1239,7 +1254,7 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
             if (e == hipSuccess)
                 e = launch_pair_forces_energy(c, ctx->sw, nl.pi.as<int32_t>(), nl.pj.as<int32_t>(),
                                               nl.jpos.as<int32_t>(), nl.npairs, px, pq,
-                                              ctx->sw.fpair.as<double>(), gpart, s);
+                                              ctx->sw.fpair.as<double>(), gpart, s, sums);
             if (e != hipSuccess) st = hip_err(ctx, e, "fused pair kernel");
         }
     }
@@ -1317,6 +1332,12 @@ int forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double
     if (dEdg) {
         dEdg[0] = hout[1] + dg_long[0];
         dEdg[1] = hout[2] + dg_long[1];
+    }
+    if (save) {
+        ctx->tsums_ok = true;
+        ctx->tsums_n = n;
+        ctx->tsums_np = nl.npairs;
+        ctx->tsums_builds = nl.builds;
     }
     return QEM_OK;
 }
@@ -1527,7 +1548,7 @@ static std::vector<DevBuf *> all_buffers(qem_ctx *ctx)
                       &ctx->lw.perm, &ctx->lw.tmp,
                       &ctx->lw.satom, &ctx->lw.k0tile, &ctx->lw.k0out,
                       &ctx->lw.kr, &ctx->lw.kv,
-                      &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part,
+                      &ctx->lw.partA, &ctx->lw.offA, &ctx->lw.totA, &ctx->lw.q0part, &ctx->tsums,
                       &ctx->lw.partB, &ctx->lw.svx, &ctx->lw.svy,
                       &ctx->lw.svz, &ctx->lw.partT, &ctx->lw.offT, &ctx->lw.totT, &ctx->d_xyz,
                       &ctx->d_q, &ctx->d_f, &ctx->d_f2, &ctx->d_aux, &ctx->d_adj,
@@ -2016,9 +2037,10 @@ int qem_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d
                       pick_stream(ctx, stream));
 }
 
-int qem_forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
-                          const int64_t *k_idx, int64_t P, double *d_f, double *d_dE_dq,
-                          double *E, double *dE_dgamma, void *stream)
+static int forces_energy_dev_entry(qem_ctx *ctx, int64_t n, const double *d_xyz,
+                                   const double *d_q, const int64_t *k_idx, int64_t P,
+                                   double *d_f, double *d_dE_dq, double *E, double *dE_dgamma,
+                                   void *stream, bool save)
 {
     CTX_GUARD(ctx);
     if (n < 0 || P < 0 || (P > 0 && !k_idx) || (n > 0 && (!d_xyz || !d_q || !d_f)))
@@ -2027,12 +2049,28 @@ int qem_forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const do
     int st = kidx_to_vecs(ctx, k_idx, P, kv);
     if (st) return st;
     return forces_energy_dev(ctx, n, d_xyz, d_q, kv, P, d_f, d_dE_dq, E, dE_dgamma,
-                             pick_stream(ctx, stream));
+                             pick_stream(ctx, stream), save);
 }
 
-int qem_forces_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
-                      const int64_t *k_idx, int64_t P, double *f, double *E, double *dE_dq,
-                      double *dE_dgamma)
+int qem_forces_energy_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                          const int64_t *k_idx, int64_t P, double *d_f, double *d_dE_dq,
+                          double *E, double *dE_dgamma, void *stream)
+{
+    return forces_energy_dev_entry(ctx, n, d_xyz, d_q, k_idx, P, d_f, d_dE_dq, E, dE_dgamma,
+                                   stream, false);
+}
+
+int qem_forces_energy_save_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                               const int64_t *k_idx, int64_t P, double *d_f, double *d_dE_dq,
+                               double *E, double *dE_dgamma, void *stream)
+{
+    return forces_energy_dev_entry(ctx, n, d_xyz, d_q, k_idx, P, d_f, d_dE_dq, E, dE_dgamma,
+                                   stream, true);
+}
+
+static int forces_energy_host(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                              const int64_t *k_idx, int64_t P, double *f, double *E,
+                              double *dE_dq, double *dE_dgamma, bool save)
 {
     CTX_GUARD(ctx);
     if ((n > 0 && (!q || !f)) || P < 0 || (P > 0 && !k_idx))
@@ -2052,7 +2090,7 @@ int qem_forces_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *
     API_HIP(ctx, ensure(ctx->d_aux, sizeof(double) * (size_t)n));
     st = forces_energy_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv, P,
                            ctx->d_f.as<double>(), ctx->d_aux.as<double>(), E, dE_dgamma,
-                           ctx->stream);
+                           ctx->stream, save);
     if (st) return st;
     API_HIP(ctx, hipMemcpyAsync(f, ctx->d_f.p, b3, hipMemcpyDeviceToHost, ctx->stream));
     if (dE_dq)
@@ -2062,8 +2100,22 @@ int qem_forces_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *
     return QEM_OK;
 }
 
-int qem_hvp(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, const int64_t *k_idx,
-            int64_t P, const double *v, double *hv)
+int qem_forces_energy(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                      const int64_t *k_idx, int64_t P, double *f, double *E, double *dE_dq,
+                      double *dE_dgamma)
+{
+    return forces_energy_host(ctx, n, xyz, q, k_idx, P, f, E, dE_dq, dE_dgamma, false);
+}
+
+int qem_forces_energy_save(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                           const int64_t *k_idx, int64_t P, double *f, double *E, double *dE_dq,
+                           double *dE_dgamma)
+{
+    return forces_energy_host(ctx, n, xyz, q, k_idx, P, f, E, dE_dq, dE_dgamma, true);
+}
+
+static int hvp_host(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                    const int64_t *k_idx, int64_t P, const double *v, double *hv, bool saved)
 {
     CTX_GUARD(ctx);
     if ((n > 0 && (!q || !v || !hv)) || P < 0 || (P > 0 && !k_idx))
@@ -2082,15 +2134,28 @@ int qem_hvp(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, const i
     API_HIP(ctx, ensure(ctx->d_f, b3));
     const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
     st = hvp_dev(ctx, n, ctx->d_xyz.as<double>(), ctx->d_q.as<double>(), kv.data(), P, w,
-                 ctx->d_aux.as<double>(), ctx->d_f.as<double>(), ctx->stream);
+                 ctx->d_aux.as<double>(), ctx->d_f.as<double>(), ctx->stream, saved);
     if (st) return st;
     API_HIP(ctx, hipMemcpyAsync(hv, ctx->d_f.p, b3, hipMemcpyDeviceToHost, ctx->stream));
     API_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return QEM_OK;
 }
 
-int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
-                const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv, void *stream)
+int qem_hvp(qem_ctx *ctx, int64_t n, const double *xyz, const double *q, const int64_t *k_idx,
+            int64_t P, const double *v, double *hv)
+{
+    return hvp_host(ctx, n, xyz, q, k_idx, P, v, hv, false);
+}
+
+int qem_hvp_saved(qem_ctx *ctx, int64_t n, const double *xyz, const double *q,
+                  const int64_t *k_idx, int64_t P, const double *v, double *hv)
+{
+    return hvp_host(ctx, n, xyz, q, k_idx, P, v, hv, true);
+}
+
+static int hvp_dev_entry(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                         const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv,
+                         void *stream, bool saved)
 {
     CTX_GUARD(ctx);
     if (n < 0 || P < 0 || (P > 0 && !k_idx) || (n > 0 && (!d_xyz || !d_q || !d_v || !d_hv)))
@@ -2099,7 +2164,21 @@ int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
     int st = kidx_to_vecs(ctx, k_idx, P, kv);
     if (st) return st;
     const double w = P > 0 ? ctx->sum_K / (double)P : 0.0;
-    return hvp_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, d_v, d_hv, pick_stream(ctx, stream));
+    return hvp_dev(ctx, n, d_xyz, d_q, kv.data(), P, w, d_v, d_hv, pick_stream(ctx, stream),
+                   saved);
+}
+
+int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv, void *stream)
+{
+    return hvp_dev_entry(ctx, n, d_xyz, d_q, k_idx, P, d_v, d_hv, stream, false);
+}
+
+int qem_hvp_saved_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
+                      const int64_t *k_idx, int64_t P, const double *d_v, double *d_hv,
+                      void *stream)
+{
+    return hvp_dev_entry(ctx, n, d_xyz, d_q, k_idx, P, d_v, d_hv, stream, true);
 }
 
 int qem_timing_enable(qem_ctx *ctx, int32_t on)

@@ -148,6 +148,37 @@ __device__ __forceinline__ Limits limits(const ShortConsts &c, double a)
     return L;
 }
 
+// Mixed precision: rho and 1/rho from the FP64 reciprocal square root and
+// two Newton steps (no FP64 square root or division)
+__device__ __forceinline__ void mixed_rho(double d2, double &rho, double &irho)
+{
+    rho = 0.0;
+    irho = 0.0;
+    if (d2 > 0.0) {
+        double y = __builtin_amdgcn_rsq(d2);
+        y = y * fma(-0.5 * d2 * y, y, 1.5);
+        y = y * fma(-0.5 * d2 * y, y, 1.5);
+        irho = y;
+        rho = d2 * y;
+    }
+}
+
+// Mixed precision: the point-charge and gaussian limits Kp, Kg of image
+// distance a in FP32 (they only place FP32 nodes), with k_f2 in the form
+// Y / (sqrt(X^2 + Y) + X) = sqrt(X^2 + Y) - X, X = 2 alpha a,
+// Y = -4 alpha ln(acc), which does not cancel; swap: the limits exchanged
+// (greens_element_ij_init, src/greens_function.jl:26-27)
+__device__ __forceinline__ void mixed_limits(const ShortConsts &c, double a, bool swap, float &Kp,
+                                             float &Kg)
+{
+    const float af = (float)a, X = c.f_2alpha * af;
+    const float kf1 = c.f_nlnacc * __builtin_amdgcn_rcpf(c.f_2Lz + af);
+    const float kf2 =
+        c.f_m4alnacc * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(X, X, c.f_m4alnacc)) + X);
+    Kp = swap ? kf1 : kf2;
+    Kg = swap ? kf2 : kf1;
+}
+
 // image l of a pair: distance a, factor b, sign s (src/greens_function.jl:17-24)
 __device__ __forceinline__ void image(const ShortConsts &c, int l, double zn, double zp, double sg,
                                       double &a, double &b, double &s)
@@ -475,6 +506,76 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red)
             b += red[2 * k + 1];
         }
     }
+}
+
+// ---- force tangent of one node set from its node sums ------------------------
+// Value and first-order tangent of sum_n w k t(k) {J1, J0}(k rho) (mixed
+// precision).
+struct Dual4 {
+    double sr, sz, dsr, dsz;
+};
+
+// The direction-independent node sums of one node set (mixed precision)
+// that its force tangent combines: with u = (1 + x_n, ...) the node
+// constants, t the summand of the force loop and p = J0, dp = -J1 dx from
+// the table, S = sum t (p, dp), U = sum u.x t (p, dp), Q = sum u.x t / D
+// (p, dp), R = sum u.x^2 t (p, dp) (R only for the gaussian set).
+struct TanSums {
+    float S0, S1, U0, U1, Q0, Q1, R0, R1;
+};
+
+// The node set's value and tangent from its sums.  Along the tangent the
+// nodes move with K (dk = r k, r = dK/K), the image distance with da and
+// rho with drho:
+//   dt = t k (A + B E/D [+ C k]),   A = -r (a [+ 2 Lz]) - da,
+//   B = 2 Lz g12 r,  C = -r / (2 alpha) (gaussian set),
+//   dJ0 = -J1 x s,  dJ1 = (x J0 - J1) s,  s = r + drho/rho,  x = k rho,
+// and d(h^2 S) = h^2 (2 r S + dS), h = K/2.  Since g12 E = D + 1,
+// B E/D = 2 Lz r (1 + 1/D) (g12 = 0 included): the E/D-weighted sums are
+// the 1/D-weighted Q and the coefficient of U becomes A + 2 Lz r.
+__device__ __forceinline__ Dual4 tangent_of_sums(const TanSums &t, float K, float dK, float a,
+                                                 float da, float rho, float drho, bool gauss,
+                                                 bool self, const ShortConsts &c)
+{
+    const float h = 0.5f * K;
+    const float r = K != 0.0f ? dK / K : 0.0f;
+    const float Lz2 = 2.0f * (float)c.Lz;
+    const float hR = self ? 0.0f : fminf(h * rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
+    const float A = gauss ? fmaf(-r, a - Lz2, -da) : -r * a - da;
+    const float B = Lz2 * r;
+    const float C = gauss ? -r * (float)(0.5 / c.alpha) : 0.0f;
+    const float sg = (!self && rho > 0.0f) ? r + drho / rho : 0.0f;
+    const float sg2 = sg * c.jt_dx * c.jt_dx;
+    const float hA = h * A, hB = h * B, h2C = h * h * C;
+    // tangents of the sums over nodes, before the h^2 factor
+    const float dS0 = hA * t.U0 + hB * t.Q0 + h2C * t.R0 + sg * hR * t.U1;
+    const float dS1 = hA * t.U1 + hB * t.Q1 + h2C * t.R1 - sg * t.S1 - sg2 * hR * t.U0;
+    const float h2 = h * h;
+    Dual4 o;
+    o.sr = -h2 * c.jt_inv_dx * t.S1;
+    o.sz = h2 * t.S0;
+    o.dsr = -h2 * c.jt_inv_dx * fmaf(2.0f * r, t.S1, dS1);
+    o.dsz = h2 * fmaf(2.0f * r, t.S0, dS0);
+    return o;
+}
+
+// Saved node sums: 16 float4 per pair, array k = 4 l + 2 set + half for
+// image l, set 0 (point) or 1 (gaussian), half 0 = (S0, S1, U0, U1),
+// 1 = (Q0, Q1, R0, R1); element p of array k at sv[k npairs + p].
+constexpr int kTanSumArrays = 16;
+__device__ __forceinline__ void store_tan_sums(float4 *__restrict__ sv, int64_t npairs,
+                                               int64_t p, int l, int set, const TanSums &t)
+{
+    const int64_t k = 4 * l + 2 * set;
+    sv[k * npairs + p] = make_float4(t.S0, t.S1, t.U0, t.U1);
+    sv[(k + 1) * npairs + p] = make_float4(t.Q0, t.Q1, t.R0, t.R1);
+}
+__device__ __forceinline__ TanSums load_tan_sums(const float4 *__restrict__ sv, int64_t npairs,
+                                                 int64_t p, int l, int set)
+{
+    const int64_t k = 4 * l + 2 * set;
+    const float4 x = sv[k * npairs + p], y = sv[(k + 1) * npairs + p];
+    return TanSums{x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
 }
 
 }  // namespace dev

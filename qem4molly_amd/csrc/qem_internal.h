@@ -218,11 +218,18 @@ hipError_t launch_pair_forces(const ShortConsts &c, int precision,
 // the launch (pair_fe_blocks of them) the pairs' sum of
 // (q_i q_j dG/dgamma_1, q_i q_j dG/dgamma_2) in d_gpart (2 doubles each).
 int pair_fe_blocks(int64_t npairs);
+// With d_sums (tan_sums_bytes(npairs)) the kernel also keeps every pair's
+// force-tangent node sums there, for launch_short_tangent's saved path.
 hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                      const int32_t *d_pi, const int32_t *d_pj,
                                      const int32_t *d_jpos, int64_t npairs,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
-                                     double *d_gpart, hipStream_t s);
+                                     double *d_gpart, hipStream_t s, float *d_sums = nullptr);
+// Bytes of the saved tangent node sums of npairs pairs (16 float4 per pair).
+inline size_t tan_sums_bytes(int64_t npairs)
+{
+    return 16 * 16 * (size_t)(npairs > 0 ? npairs : 1);
+}
 // Pair records of the force path (launch_pair_forces -> launch_gather_self):
 // float3 (12 bytes) for the mixed-precision list kernel, double4 (32 bytes)
 // otherwise (FP64, and the pole-splitting kernels).
@@ -417,12 +424,16 @@ hipError_t launch_short_energy(const ShortConsts &c, int precision, const ShortW
 // pair kernel, then after_pair (optional) is called, then the gather is
 // queued on s.  With self_s another stream, the caller orders it: self_s
 // must see the inputs, and the gather must wait for the self term.
+// d_sums (mixed precision): the pairs' node sums that
+// launch_pair_forces_energy kept at these positions, used instead of the
+// node loops.
 hipError_t launch_short_tangent(const ShortConsts &c, int precision, const ShortWork &ws,
                                 const NeighborList &nl, int64_t n, const double *d_xyz,
                                 const double *d_q, const double *d_v, double *d_tp,
                                 double scale, double *d_out, hipStream_t s,
                                 const std::function<void()> *after_pair = nullptr,
-                                hipStream_t self_s = nullptr);
+                                hipStream_t self_s = nullptr,
+                                const float *d_sums = nullptr);
 // out3 = (1/2 q.phi, 1/2 q.psi_1, 1/2 q.psi_2); psi may be null; d_part is
 // scratch of half_dots_scratch() doubles.  With d_gpart, the sums of its
 // ngpart pairs of doubles are added to out3[1] and out3[2].
@@ -472,6 +483,12 @@ struct qem_ctx {
     // error return leaves no copy pending into freed memory
     double *h_out = nullptr;
     size_t h_out_cap = 0;   // doubles
+    // the pairs' force-tangent node sums kept by the last saving forces +
+    // energy call (tan_sums_bytes), valid for the atom count, pair count and
+    // neighbour-list build it was made with
+    qem::DevBuf tsums;
+    bool tsums_ok = false;
+    int64_t tsums_n = -1, tsums_np = -1, tsums_builds = -1;
     // K set (host)
     std::vector<double> kset;   // |K| x 3
     std::vector<float> kprob;   // |K|

@@ -119,11 +119,15 @@ __device__ __forceinline__ void nodes_fp64(double K, double a, double rho, bool 
 //   ev = h sum (t / u.x) J0,   ed = h sum (t / u.x) J0 {1/D point, E/D gauss},
 // ed being what d/dgamma of 1/D brings down; the force operations are the
 // same, so the forces are the same bits with or without it.
-template <bool ENERGY>
+// With SAVE the loop also forms the node sums of the set's force tangent
+// (dev::TanSums: the u.x-, u.x/D- and u.x^2-weighted sums beside the
+// force's own), into *ts, so that a later Hessian-vector product at the same
+// positions combines them instead of running the quadrature again.
+template <bool ENERGY, bool SAVE = false>
 __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool gauss,
                                             const ShortConsts &c, const float4 *__restrict__ ncon,
                                             const float4 *__restrict__ tab, float &sr, float &sz,
-                                            float &ev, float &ed)
+                                            float &ev, float &ed, dev::TanSums *ts = nullptr)
 {
     const float h = 0.5f * K;
     const float c2 = c.f_c2;                       // -2 Lz log2(e)
@@ -135,6 +139,7 @@ __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool ga
     // Bessel argument in table units; u.x < 2 keeps m <= jt_n - 1
     const float hR = fminf(h * rho * c.jt_inv_dx, 0.5f * (float)(c.jt_n - 1));
     float ar = 0.f, az = 0.f, av = 0.f, ad = 0.f;
+    float U0 = 0.f, U1 = 0.f, Q0 = 0.f, Q1 = 0.f, R0 = 0.f, R1 = 0.f;
     for (int n = 0; n < c.Nt; ++n) {
         const float4 u = ncon[n];
         const float e = hE * u.x;
@@ -157,7 +162,20 @@ __device__ __forceinline__ void nodes_mixed(float K, float a, float rho, bool ga
             av += tp;
             ad = fmaf(tp, gauss ? E * rD : rD, ad);
         }
+        if constexpr (SAVE) {
+            const float tu = t * u.x, tq = tu * rD;   // u.x t, u.x t / D
+            U0 = fmaf(tu, p, U0);
+            U1 = fmaf(tu, dp, U1);
+            Q0 = fmaf(tq, p, Q0);
+            Q1 = fmaf(tq, dp, Q1);
+            if (gauss) {
+                const float tr = tu * u.x;   // u.x^2 t
+                R0 = fmaf(tr, p, R0);
+                R1 = fmaf(tr, dp, R1);
+            }
+        }
     }
+    if constexpr (SAVE) *ts = dev::TanSums{az, ar, U0, U1, Q0, Q1, R0, R1};
     const float h2 = h * h;
     sr = -h2 * c.jt_inv_dx * ar;
     sz = h2 * az;
@@ -205,14 +223,20 @@ __device__ __forceinline__ double db_dgamma(const ShortConsts &c, int l, int w)
 // rho^2)) minus the point-charge J0 transform of Gamma_2 plus the gaussian
 // one of Gamma_1 e^{-k^2/4 alpha}; the derivatives take d b / d gamma and
 // d(1/D)/d gamma = -g_other E / D^2 (src/greens_function.jl:68-88).
-template <int PREC, bool ENERGY = false>
+// With SAVE (mixed precision) the node sums of the pair's force tangent go
+// to the saved arrays sav (dev::store_tan_sums, snp pairs, this pair at sp;
+// zeros for a set that is not evaluated).
+template <int PREC, bool ENERGY = false, bool SAVE = false>
 __device__ __forceinline__ PairForce pair_force(const ShortConsts &c, const double *__restrict__ glx,
                                 const double *__restrict__ glw, const float4 *__restrict__ ncon,
                                 const float4 *__restrict__ tab,
                                 double qi, double qj, double xi, double yi, double zi, double xj,
-                                double yj, double zj, double4 *pot = nullptr)
+                                double yj, double zj, double4 *pot = nullptr,
+                                float4 *__restrict__ sav = nullptr, int64_t snp = 0,
+                                int64_t sp = 0)
 {
     static_assert(!ENERGY || PREC == QEM_PREC_MIXED, "the fused energy is mixed precision");
+    static_assert(!SAVE || PREC == QEM_PREC_MIXED, "the saved tangent sums are mixed precision");
     PairForce out{0.0, 0.0, 0.0, 0.0};
     if (ENERGY) *pot = make_double4(0.0, 0.0, 0.0, 0.0);
     const double dx = xi - xj, dy = yi - yj;
@@ -229,18 +253,10 @@ __device__ __forceinline__ PairForce pair_force(const ShortConsts &c, const doub
     // rho and 1/rho: in mixed precision from the FP64 reciprocal square root
     // and two Newton steps (no FP64 square root or division in the set-up)
     double rho, irho = 0.0;
-    if (PREC == QEM_PREC_MIXED) {
-        rho = 0.0;
-        if (d2 > 0.0) {
-            double y = __builtin_amdgcn_rsq(d2);
-            y = y * fma(-0.5 * d2 * y, y, 1.5);
-            y = y * fma(-0.5 * d2 * y, y, 1.5);
-            irho = y;
-            rho = d2 * y;
-        }
-    } else {
+    if (PREC == QEM_PREC_MIXED)
+        dev::mixed_rho(d2, rho, irho);
+    else
         rho = sqrt(d2);
-    }
     const double zn = fabs(zi - zj), zp = zi + zj;
     const double sg = jl_sign(zi - zj);
     const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
@@ -257,15 +273,11 @@ __device__ __forceinline__ PairForce pair_force(const ShortConsts &c, const doub
         double Kp, Kg, inv32, inv_r = 0.0;
         const double r2 = a * a + (PREC == QEM_PREC_MIXED ? d2 : rho * rho);
         if (PREC == QEM_PREC_MIXED) {
-            // the limits only place FP32 nodes: FP32, with k_f2 in the form
-            // Y / (sqrt(X^2 + Y) + X) = sqrt(X^2 + Y) - X, X = 2 alpha a,
-            // Y = -4 alpha ln(acc), which does not cancel
-            const float af = (float)a, X = c.f_2alpha * af;
-            const float kf1 = c.f_nlnacc * __builtin_amdgcn_rcpf(c.f_2Lz + af);
-            const float kf2 =
-                c.f_m4alnacc * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(X, X, c.f_m4alnacc)) + X);
-            Kp = swap ? kf1 : kf2;
-            Kg = swap ? kf2 : kf1;
+            // the limits only place FP32 nodes: FP32
+            float kp, kg;
+            dev::mixed_limits(c, a, swap, kp, kg);
+            Kp = kp;
+            Kg = kg;
             // r2^{-3/2} from the FP64 reciprocal square root and two Newton steps
             double y = __builtin_amdgcn_rsq(r2);
             y = y * fma(-0.5 * r2 * y, y, 1.5);
@@ -295,21 +307,29 @@ __device__ __forceinline__ PairForce pair_force(const ShortConsts &c, const doub
                 if (!c.single) nodes_fp64<TAB>(Kg, a, rho, true, c, glx, glw, tab64, gr, gz);
             } else {
                 float fr, fz, fv = 0.f, fd = 0.f;
-                nodes_mixed<ENERGY>((float)Kp, (float)a, (float)rho, false, c, ncon, tab, fr, fz,
-                                    fv, fd);
+                dev::TanSums ts{};
+                nodes_mixed<ENERGY, SAVE>((float)Kp, (float)a, (float)rho, false, c, ncon, tab,
+                                          fr, fz, fv, fd, &ts);
+                if constexpr (SAVE) dev::store_tan_sums(sav, snp, sp, l, 0, ts);
                 pr = fr;
                 pz = fz;
                 Pv = fv;
                 Pd = fd;
                 if (!c.single) {
-                    nodes_mixed<ENERGY>((float)Kg, (float)a, (float)rho, true, c, ncon, tab, fr,
-                                        fz, fv, fd);
+                    nodes_mixed<ENERGY, SAVE>((float)Kg, (float)a, (float)rho, true, c, ncon, tab,
+                                              fr, fz, fv, fd, &ts);
                     gr = fr;
                     gz = fz;
                     Gv = fv;
                     Gd = fd;
+                } else {
+                    ts = dev::TanSums{};
                 }
+                if constexpr (SAVE) dev::store_tan_sums(sav, snp, sp, l, 1, ts);
             }
+        } else if constexpr (SAVE) {
+            dev::store_tan_sums(sav, snp, sp, l, 0, dev::TanSums{});
+            dev::store_tan_sums(sav, snp, sp, l, 1, dev::TanSums{});
         }
         const double hb = 0.5 * b;
         // F_sr: -sum GL[k Gamma_2 J1] + closed + sum GL[k Gamma_1 e^{-k^2/4a} J1]
@@ -385,12 +405,14 @@ __attribute__((amdgpu_waves_per_eu(PREC == QEM_PREC_MIXED ? 5 : 1))) void k_pair
 // into gpart[blockIdx.x].
 // Four waves per SIMD: at 128 VGPRs the compiler spills 28 registers to
 // scratch, and the kernel still runs 4% faster than at three waves (156).
+// SAVE: the pairs' tangent node sums into sums as well (dev::store_tan_sums).
+template <bool SAVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pair_list_fe(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,
     const float4 *__restrict__ ncon, const float4 *__restrict__ gtab, const int32_t *__restrict__ pi,
     const int32_t *__restrict__ pj, const int32_t *__restrict__ jpos, int64_t npairs,
     const double *__restrict__ xyz, const double *__restrict__ q, float4 *__restrict__ fpair,
-    double2 *__restrict__ gpart)
+    double2 *__restrict__ gpart, float4 *__restrict__ sums)
 {
     extern __shared__ float4 lds_tab[];
     __shared__ double red[8];
@@ -398,8 +420,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     double g1 = 0.0, g2 = 0.0;
     dev::walk_pairs(pi, pj, jpos, npairs, xyz, q, [&](int64_t p, const dev::PairOps &o) {
         double4 u;
-        const PairForce f = pair_force<QEM_PREC_MIXED, true>(
-            c, glx, glw, ncon, tab, o.qi, o.qj, o.xi, o.yi, o.zi, o.xj, o.yj, o.zj, &u);
+        const PairForce f = pair_force<QEM_PREC_MIXED, true, SAVE>(
+            c, glx, glw, ncon, tab, o.qi, o.qj, o.xi, o.yi, o.zi, o.xj, o.yj, o.zj, &u, sums,
+            npairs, p);
         dev::store4(fpair + p, f.fx, f.fy, f.fz, o.qj * u.x);
         dev::store4(fpair + npairs + o.jp, -f.fx, -f.fy, f.fzj, o.qi * u.x);
         const double qq = o.qi * o.qj;
@@ -533,15 +556,22 @@ hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                      const int32_t *d_pi, const int32_t *d_pj,
                                      const int32_t *d_jpos, int64_t npairs,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
-                                     double *d_gpart, hipStream_t s)
+                                     double *d_gpart, hipStream_t s, float *d_sums)
 {
     if (npairs <= 0) return hipSuccess;
     if (c.pole || !c.nconi) return hipErrorInvalidValue;
     const size_t lds = sizeof(float4) * (size_t)ws.jt_n;
-    k_pair_list_fe<<<pair_grid(npairs), 256, lds, s>>>(
-        c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(), ws.jtab.as<float4>(),
-        d_pi, d_pj, d_jpos, npairs, d_xyz, d_q, reinterpret_cast<float4 *>(d_fpair),
-        reinterpret_cast<double2 *>(d_gpart));
+    float4 *sums = reinterpret_cast<float4 *>(d_sums);
+    if (sums)
+        k_pair_list_fe<true><<<pair_grid(npairs), 256, lds, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(),
+            ws.jtab.as<float4>(), d_pi, d_pj, d_jpos, npairs, d_xyz, d_q,
+            reinterpret_cast<float4 *>(d_fpair), reinterpret_cast<double2 *>(d_gpart), sums);
+    else
+        k_pair_list_fe<false><<<pair_grid(npairs), 256, lds, s>>>(
+            c, ws.glx.as<double>(), ws.glw.as<double>(), ws.ncon.as<float4>(),
+            ws.jtab.as<float4>(), d_pi, d_pj, d_jpos, npairs, d_xyz, d_q,
+            reinterpret_cast<float4 *>(d_fpair), reinterpret_cast<double2 *>(d_gpart), nullptr);
     return hipGetLastError();
 }
 

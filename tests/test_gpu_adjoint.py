@@ -178,6 +178,37 @@ def test_adjoint_slab_n10k_fd_and_symmetry():
         assert dg[w] == pytest.approx(d, rel=1e-6, abs=1e-9 * abs(E))
 
 
+@pytest.mark.parametrize("sem", [0, L.SEM_CORRECTED])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_hvp_from_saved_sums(sem, precision):
+    # qem_forces_energy_save keeps the pair quadrature's force-tangent node
+    # sums and qem_hvp_saved combines them instead of running the tangent
+    # quadrature: the forces are those of the call without saving bit for
+    # bit, the product is qem_hvp's within the mixed tolerance (the sums are
+    # taken at the forward's FP32 limits), and without valid sums (FP64, or
+    # after a forces + energy call that does not save) qem_hvp_saved is
+    # qem_hvp bit for bit
+    cfg, xyz, q, c = _setup("slab_n10k", 2000, sem=sem, precision=precision)
+    v = np.random.default_rng(11).normal(size=xyz.shape)
+    with c:
+        kidx = c.sample_k(17, cfg.P)
+        F0, E0, phi0, dg0 = c.forces_energy(xyz, q, kidx)
+        F, E, phi, dg = c.forces_energy(xyz, q, kidx, save=True)
+        hs = c.hvp(xyz, q, kidx, v, saved=True)
+        hd = c.hvp(xyz, q, kidx, v)
+        c.forces_energy(xyz, q, kidx)
+        hs2 = c.hvp(xyz, q, kidx, v, saved=True)
+    assert np.array_equal(F, F0) and E == E0
+    assert np.array_equal(phi, phi0) and np.array_equal(dg, dg0)
+    assert np.array_equal(hs2, hd)
+    if precision == "fp64":
+        assert np.array_equal(hs, hd)
+    else:
+        # (not the same bits: the sums were taken at the forward's limits)
+        assert not np.array_equal(hs, hd)
+        assert rel_rms(hs, hd) < 1e-6
+
+
 @pytest.mark.parametrize("g1,g2", [(0.0, 0.9), (0.9, -0.5), (0.95, 0.95)])
 def test_mixed_hvp_against_fp64_across_gammas(g1, g2):
     # the mixed tangent loop takes its E/D-weighted node sums as 1/D-weighted
