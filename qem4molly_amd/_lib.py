@@ -231,6 +231,10 @@ class Context:
             raise QEMError(st, f"qem_create: {msg}")
         self._h = h
         self.params = params
+        # counts the forces + energy calls: each one replaces (save) or drops
+        # the kept tangent sums, so a caller that recorded the count after its
+        # saving call knows the sums are still its own while the count holds
+        self.sums_gen = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -452,6 +456,7 @@ class Context:
         k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
         f = np.zeros_like(xyz); E = _d(); phi = np.zeros(xyz.shape[0]); dg = np.zeros(2)
         name = "qem_forces_energy_save" if save else "qem_forces_energy"
+        self.sums_gen += 1
         self._check(getattr(lib(), name)(self._h, xyz.shape[0], ptr(xyz), ptr(q), ptr(k_idx),
                                          k_idx.shape[0], ptr(f), C.byref(E), ptr(phi),
                                          ptr(dg)), name)
@@ -463,6 +468,7 @@ class Context:
         k_idx = np.ascontiguousarray(k_idx, dtype=np.int64)
         E = _d(); dg = np.zeros(2)
         name = "qem_forces_energy_save_dev" if save else "qem_forces_energy_dev"
+        self.sums_gen += 1
         self._check(getattr(lib(), name)(self._h, int(n), d_xyz, d_q, ptr(k_idx),
                                          k_idx.shape[0], d_f, d_phi, C.byref(E), ptr(dg),
                                          stream), name)

@@ -405,7 +405,9 @@ __attribute__((amdgpu_waves_per_eu(PREC == QEM_PREC_MIXED ? 5 : 1))) void k_pair
 // into gpart[blockIdx.x].
 // Four waves per SIMD: at 128 VGPRs the compiler spills 28 registers to
 // scratch, and the kernel still runs 4% faster than at three waves (156).
-// SAVE: the pairs' tangent node sums into sums as well (dev::store_tan_sums).
+// SAVE: the pairs' tangent node sums into sums as well (dev::store_tan_sums);
+// 44 registers spill at four waves, and three waves (156 VGPRs, no spill)
+// run the saving call 4% slower.
 template <bool SAVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pair_list_fe(
     ShortConsts c, const double *__restrict__ glx, const double *__restrict__ glw,

@@ -18,7 +18,13 @@ backward (no reference counterpart; the reference has forces only):
            respect to gamma is not provided either (raises, rather than
            returning the energy term alone).  `gamma` must equal the
            context's contrasts; it is only the handle autograd differentiates.
-The quadrature is recomputed in the backward pass: nothing per pair is kept.
+With keep_sums (the default when xyz requires grad) the forward keeps the
+pair quadrature's tangent node sums (qem_forces_energy_save_dev, 256 B per
+pair in mixed precision) and the backward's HVP combines them
+(qem_hvp_saved_dev) instead of running the quadrature again: about 1.3 ms
+more forward and 2.9 ms less backward at N = 100k.  The context keeps one set
+of sums; when another forces + energy call on the same context has replaced
+them before this backward runs, the HVP recomputes the quadrature.
 """
 from __future__ import annotations
 
@@ -43,7 +49,7 @@ def _stream():
 
 class QEMEnergyForces(torch.autograd.Function):
     @staticmethod
-    def forward(fctx, xyz, q, gamma, qctx, k_idx):
+    def forward(fctx, xyz, q, gamma, qctx, k_idx, keep_sums=False):
         if xyz.dtype != torch.float64 or q.dtype != torch.float64 or not xyz.is_cuda:
             raise TypeError("xyz and q must be float64 tensors on a ROCm device")
         n = xyz.shape[0]
@@ -53,7 +59,8 @@ class QEMEnergyForces(torch.autograd.Function):
         phi = torch.empty_like(qq)
         s = _stream()
         E, dg = qctx.forces_energy_dev(n, x.data_ptr(), qq.data_ptr(), k_idx, F.data_ptr(),
-                                       phi.data_ptr(), s)
+                                       phi.data_ptr(), s, save=keep_sums)
+        fctx.sums_gen = qctx.sums_gen if keep_sums else None
         fctx.save_for_backward(x, qq, F, phi)
         fctx.qctx = qctx
         fctx.k_idx = np.array(k_idx, dtype=np.int64)
@@ -73,7 +80,8 @@ class QEMEnergyForces(torch.autograd.Function):
                 hv = torch.empty_like(x)
                 gFc = gF.contiguous()
                 fctx.qctx.hvp_dev(n, x.data_ptr(), qq.data_ptr(), fctx.k_idx, gFc.data_ptr(),
-                                  hv.data_ptr(), _stream())
+                                  hv.data_ptr(), _stream(),
+                                  saved=fctx.sums_gen == fctx.qctx.sums_gen)
                 gx = gx - hv
         if fctx.needs_input_grad[1]:
             if gF is not None and bool(torch.any(gF != 0)):
@@ -85,19 +93,23 @@ class QEMEnergyForces(torch.autograd.Function):
                 raise NotImplementedError(
                     "gradient of the forces with respect to gamma is not provided")
             gg = gE * x.new_tensor(fctx.dg)
-        return gx, gq, gg, None, None
+        return gx, gq, gg, None, None, None
 
 
-def qem_energy_forces(qctx: L.Context, xyz, q, k_idx, gamma=None):
-    """(E, F) for one electrostatic evaluation; differentiable (see module)."""
+def qem_energy_forces(qctx: L.Context, xyz, q, k_idx, gamma=None, keep_sums=None):
+    """(E, F) for one electrostatic evaluation; differentiable (see module).
+    keep_sums (default: xyz.requires_grad under grad mode) keeps the pair
+    quadrature's tangent sums for the backward's HVP."""
     p = qctx.params
+    if keep_sums is None:
+        keep_sums = bool(xyz.requires_grad and torch.is_grad_enabled())
     if gamma is None:
         gamma = xyz.new_tensor([p.gamma_1, p.gamma_2])
     else:
         g = gamma.detach().cpu().numpy()
         if not np.allclose(g, [p.gamma_1, p.gamma_2], rtol=0, atol=0):
             raise ValueError("gamma must equal the context's gamma_1, gamma_2")
-    return QEMEnergyForces.apply(xyz, q, gamma, qctx, k_idx)
+    return QEMEnergyForces.apply(xyz, q, gamma, qctx, k_idx, bool(keep_sums))
 
 
 def qem_forces(qctx: L.Context, xyz, q, k_idx=None, walls=(), full_sum=False, out=None):

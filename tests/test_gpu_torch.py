@@ -66,6 +66,31 @@ with ctx_for(cfg, semantics=L.SEM_CORRECTED, iter_period=100000) as c:
                  torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     out["dev"] = rel_rms(f.cpu().numpy(), c.forces(xyz_h, q_h, kidx))
+# mixed precision: the forward keeps the tangent sums and the backward's HVP
+# combines them; a second forward on the context before the first backward
+# replaces the sums, and that backward then recomputes the quadrature
+with ctx_for(cfg, precision="mixed", semantics=L.SEM_CORRECTED, iter_period=100000) as c:
+    kidx = c.sample_k(3, cfg.P)
+    v = np.random.default_rng(0).normal(size=xyz_h.shape)
+    c.forces_energy(xyz_h, q_h, kidx, save=True)
+    hs_ref = c.hvp(xyz_h, q_h, kidx, v, saved=True)
+    hd_ref = c.hvp(xyz_h, q_h, kidx, v)
+    vt = torch.tensor(v, device=dev)
+    xa = torch.tensor(xyz_h, device=dev, requires_grad=True)
+    Ea, Fa = qem_energy_forces(c, xa, torch.tensor(q_h, device=dev), kidx)
+    (Fa * vt).sum().backward()
+    out["mixed_saved_exact"] = bool(np.array_equal(xa.grad.cpu().numpy(), -hs_ref))
+    xb = torch.tensor(xyz_h, device=dev, requires_grad=True)
+    Eb, Fb = qem_energy_forces(c, xb, torch.tensor(q_h, device=dev), kidx, keep_sums=False)
+    (Fb * vt).sum().backward()
+    out["mixed_direct_exact"] = bool(np.array_equal(xb.grad.cpu().numpy(), -hd_ref))
+    xc = torch.tensor(xyz_h, device=dev, requires_grad=True)
+    xd = torch.tensor(xyz_h + 0.01, device=dev, requires_grad=True)
+    Ec, Fc = qem_energy_forces(c, xc, torch.tensor(q_h, device=dev), kidx)
+    Ed, Fd = qem_energy_forces(c, xd, torch.tensor(q_h, device=dev), kidx)
+    (Fc * vt).sum().backward()
+    out["mixed_replaced_exact"] = bool(np.array_equal(xc.grad.cpu().numpy(), -hd_ref))
+    out["mixed_saved_vs_direct"] = rel_rms(hs_ref, hd_ref)
 print("RESULT " + json.dumps(out), flush=True)
 """
 
@@ -82,3 +107,6 @@ def test_torch_autograd_binding():
     assert res["hv"] < 1e-13
     assert res["dev"] < 1e-13
     assert res["raise_gamma"] and res["raise_q"]
+    assert res["mixed_saved_exact"] and res["mixed_direct_exact"]
+    assert res["mixed_replaced_exact"]
+    assert res["mixed_saved_vs_direct"] < 1e-6
