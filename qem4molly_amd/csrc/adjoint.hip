@@ -354,14 +354,17 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
     const double d2 = dev::min_image_d2(c, dx, dy, mx, my);
     if (d2 > c.rc2) return make_double4(0.0, 0.0, 0.0, 0.0);
     const double ddx = vi[0] - vj[0], ddy = vi[1] - vj[1];
-    double rho;
+    double rho, irho = 0.0;
     if constexpr (SAVED) {
-        double irho;
         dev::mixed_rho(d2, rho, irho);
     } else {
         rho = sqrt(d2);
     }
-    const double drho = rho > 0.0 ? (mx * ddx + my * ddy) / rho : 0.0;
+    double drho;
+    if constexpr (SAVED)
+        drho = (mx * ddx + my * ddy) * irho;
+    else
+        drho = rho > 0.0 ? (mx * ddx + my * ddy) / rho : 0.0;
     const double zn = fabs(ri[2] - rj[2]), zp = ri[2] + rj[2], sg = jl_sign(ri[2] - rj[2]);
     const double dzn = sg * (vi[2] - vj[2]), dzp = vi[2] + vj[2];
     const bool swap = (c.sem & QEM_SEM_SWAP_LIMITS) != 0;
@@ -371,17 +374,29 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
         double a, b, s;
         dev::image(c, l, zn, zp, sg, a, b, s);
         const double da = l == 0 ? dzn : (l == 1 ? dzp : (l == 2 ? -dzp : -dzn));
-        const dev::Limits L = dev::limits(c, a);
-        double Kp = swap ? L.kf1 : L.kf2, Kg = swap ? L.kf2 : L.kf1;
+        double Kp, Kg, dKp, dKg, inv32;
+        const double r2 = a * a + rho * rho;
         if constexpr (SAVED) {
-            float kp, kg;
-            dev::mixed_limits(c, a, swap, kp, kg);
+            // FP32 limits and limit derivatives, 1/r^3 from the reciprocal
+            // square root and two Newton steps
+            float kp, kg, dkp, dkg;
+            dev::mixed_limits_d(c, a, swap, kp, kg, dkp, dkg);
             Kp = kp;
             Kg = kg;
+            dKp = (double)dkp * da;
+            dKg = (double)dkg * da;
+            double y = __builtin_amdgcn_rsq(r2);
+            y = y * fma(-0.5 * r2 * y, y, 1.5);
+            y = y * fma(-0.5 * r2 * y, y, 1.5);
+            inv32 = y * y * y;
+        } else {
+            const dev::Limits L = dev::limits(c, a);
+            Kp = swap ? L.kf1 : L.kf2;
+            Kg = swap ? L.kf2 : L.kf1;
+            dKp = (swap ? L.dkf1 : L.dkf2) * da;
+            dKg = (swap ? L.dkf2 : L.dkf1) * da;
+            inv32 = 1.0 / (r2 * sqrt(r2));
         }
-        const double dKp = (swap ? L.dkf1 : L.dkf2) * da, dKg = (swap ? L.dkf2 : L.dkf1) * da;
-        const double r2 = a * a + rho * rho;
-        const double inv32 = 1.0 / (r2 * sqrt(r2));
         const double dinv32 = -3.0 * inv32 * (a * da + rho * drho) / r2;
         const double hb = 0.5 * b;
         const double cr = hb * rho * inv32, dcr = hb * (drho * inv32 + rho * dinv32);
@@ -416,7 +431,11 @@ __device__ __forceinline__ double4 pair_force_tangent(const ShortConsts &c, cons
     if (rho != 0.0) {
         const bool mi = (c.sem & QEM_SEM_MINIMAGE_DIR) != 0;
         const double ux = mi ? mx : dx, uy = mi ? my : dy;
-        const double ir = 1.0 / rho;
+        double ir;
+        if constexpr (SAVED)
+            ir = irho;
+        else
+            ir = 1.0 / rho;
         o.x = pref * (dFsr * ux * ir + Fsr * (ddx * ir - ux * drho * ir * ir));
         o.y = pref * (dFsr * uy * ir + Fsr * (ddy * ir - uy * drho * ir * ir));
     }

@@ -179,6 +179,23 @@ __device__ __forceinline__ void mixed_limits(const ShortConsts &c, double a, boo
     Kg = swap ? kf2 : kf1;
 }
 
+// mixed_limits with the limits' derivatives in a (FP32, the tangent of the
+// saved node sums only uses them in FP32): d k_f1/da = -k_f1/(2Lz + a) and
+// d k_f2/da = -2 alpha k_f2 / sqrt(X^2 + Y), the non-cancelling form of
+// 2 alpha (X / sqrt(X^2 + Y) - 1).  Kp, Kg are those of mixed_limits.
+__device__ __forceinline__ void mixed_limits_d(const ShortConsts &c, double a, bool swap,
+                                               float &Kp, float &Kg, float &dKp, float &dKg)
+{
+    mixed_limits(c, a, swap, Kp, Kg);
+    const float af = (float)a, X = c.f_2alpha * af;
+    const float kf1 = swap ? Kp : Kg, kf2 = swap ? Kg : Kp;
+    const float dkf1 = -kf1 * __builtin_amdgcn_rcpf(c.f_2Lz + af);
+    const float dkf2 = -c.f_2alpha * kf2 *
+                       __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(X, X, c.f_m4alnacc)));
+    dKp = swap ? dkf1 : dkf2;
+    dKg = swap ? dkf2 : dkf1;
+}
+
 // image l of a pair: distance a, factor b, sign s (src/greens_function.jl:17-24)
 __device__ __forceinline__ void image(const ShortConsts &c, int l, double zn, double zp, double sg,
                                       double &a, double &b, double &s)
