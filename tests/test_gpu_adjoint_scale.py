@@ -14,7 +14,10 @@ every step's outputs are finite, the last step's are bitwise repeatable,
 the one-call forces + energy (qem_forces_energy_dev) gives the same forces
 bit for bit and the energy within the same bars, and the context's device
 memory stays under 1.6 GB (measured 1.49 GB: the endpoint records take
-32 bytes per pair, 0.5 GB at 15.7 million pairs).
+32 bytes per pair, 0.5 GB at 15.7 million pairs).  At the end the same
+call keeping the tangent node sums (qem_forces_energy_save_dev) repeats
+its outputs bit for bit, the HVP from those sums (qem_hvp_saved_dev) is
+within 1e-6 of FP64, and the sums add at most 1.25 x 256 bytes per pair.
 """
 import numpy as np
 import pytest
@@ -93,3 +96,21 @@ def test_adjoint_slab_n100k_perturbed_steps():
         mem = c.device_bytes()
         print(f"peak device bytes (mixed context): {mem}", flush=True)
         assert mem < MAX_DEVICE_BYTES
+        # the same call keeping the tangent node sums: its outputs bit for bit;
+        # the HVP from the sums within the mixed bar of FP64 (FP32 sums in
+        # another order than the direct HVP's, so not bitwise the same)
+        E4, dg4 = c.forces_energy_dev(cfg.n, xd.ptr, qd.ptr, kidx, fd.ptr, pd.ptr, save=True)
+        L.synchronize()
+        assert np.array_equal(fd.download(), F3) and E4 == E3
+        assert np.array_equal(pd.download(), phi3) and np.array_equal(np.array(dg4), dg3)
+        c.hvp_dev(cfg.n, xd.ptr, qd.ptr, kidx, vd.ptr, hd.ptr, saved=True)
+        L.synchronize()
+        hs = hd.download()
+        print(f"saved-sums HVP vs fp64: {rel_rms(hs, hv64):.2e}, vs direct {rel_rms(hs, hv):.2e}",
+              flush=True)
+        assert rel_rms(hs, hv64) < 1e-6 and not np.array_equal(hs, hv)
+        # the sums: 256 bytes per pair, allocated with 25% headroom
+        npairs = c.short_state()[0]
+        mem_saved = c.device_bytes()
+        print(f"device bytes with the saved sums: {mem_saved} ({npairs} pairs)", flush=True)
+        assert mem_saved <= mem + 1.25 * 256 * npairs + 4096
