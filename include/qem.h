@@ -306,7 +306,7 @@ int qem_hvp_dev(qem_ctx *ctx, int64_t n, const double *d_xyz, const double *d_q,
  * Hessian-vector product at the same positions, as an autograd backward
  * does).  qem_forces_energy_save(_dev) is qem_forces_energy(_dev) whose
  * pair kernel also keeps, in mixed precision, every pair's force-tangent
- * node sums in the context (256 bytes per pair of device memory, grown
+ * node sums in the context (224 bytes per pair of device memory, grown
  * once); qem_hvp_saved(_dev) is qem_hvp(_dev) that combines those sums
  * instead of running the tangent quadrature.  The caller passes
  * qem_hvp_saved(_dev) the positions, charges and k indices of the last save

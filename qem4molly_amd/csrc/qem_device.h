@@ -576,23 +576,31 @@ __device__ __forceinline__ Dual4 tangent_of_sums(const TanSums &t, float K, floa
     return o;
 }
 
-// Saved node sums: 16 float4 per pair, array k = 4 l + 2 set + half for
-// image l, set 0 (point) or 1 (gaussian), half 0 = (S0, S1, U0, U1),
-// 1 = (Q0, Q1, R0, R1); element p of array k at sv[k npairs + p].
-constexpr int kTanSumArrays = 16;
+// Saved node sums, 224 B per pair: per image l three float4 arrays,
+// 3 l + 0 = (S0, S1, U0, U1) of the point set, 3 l + 1 = the same of the
+// gaussian set, 3 l + 2 = (Q0, Q1, R0, R1) of the gaussian set, then after
+// the twelve float4 arrays four float2 arrays, l = (Q0, Q1) of the point set
+// (whose R0, R1 are zero: its tangent has no u.x^2 term); element p of each
+// array at [array npairs + p]; tan_sums_bytes(npairs) in all.
 __device__ __forceinline__ void store_tan_sums(float4 *__restrict__ sv, int64_t npairs,
                                                int64_t p, int l, int set, const TanSums &t)
 {
-    const int64_t k = 4 * l + 2 * set;
-    sv[k * npairs + p] = make_float4(t.S0, t.S1, t.U0, t.U1);
-    sv[(k + 1) * npairs + p] = make_float4(t.Q0, t.Q1, t.R0, t.R1);
+    sv[(3 * l + set) * npairs + p] = make_float4(t.S0, t.S1, t.U0, t.U1);
+    if (set)
+        sv[(3 * l + 2) * npairs + p] = make_float4(t.Q0, t.Q1, t.R0, t.R1);
+    else
+        reinterpret_cast<float2 *>(sv + 12 * npairs)[l * npairs + p] = make_float2(t.Q0, t.Q1);
 }
 __device__ __forceinline__ TanSums load_tan_sums(const float4 *__restrict__ sv, int64_t npairs,
                                                  int64_t p, int l, int set)
 {
-    const int64_t k = 4 * l + 2 * set;
-    const float4 x = sv[k * npairs + p], y = sv[(k + 1) * npairs + p];
-    return TanSums{x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+    const float4 x = sv[(3 * l + set) * npairs + p];
+    if (set) {
+        const float4 y = sv[(3 * l + 2) * npairs + p];
+        return TanSums{x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+    }
+    const float2 y = reinterpret_cast<const float2 *>(sv + 12 * npairs)[l * npairs + p];
+    return TanSums{x.x, x.y, x.z, x.w, y.x, y.y, 0.0f, 0.0f};
 }
 
 }  // namespace dev

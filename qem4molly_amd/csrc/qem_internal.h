@@ -225,10 +225,12 @@ hipError_t launch_pair_forces_energy(const ShortConsts &c, const ShortWork &ws,
                                      const int32_t *d_jpos, int64_t npairs,
                                      const double *d_xyz, const double *d_q, double *d_fpair,
                                      double *d_gpart, hipStream_t s, float *d_sums = nullptr);
-// Bytes of the saved tangent node sums of npairs pairs (16 float4 per pair).
+// Bytes of the saved tangent node sums of npairs pairs (224 per pair:
+// twelve float4 and four float2 arrays).
+constexpr int kTanSumBytesPerPair = 12 * 16 + 4 * 8;
 inline size_t tan_sums_bytes(int64_t npairs)
 {
-    return 16 * 16 * (size_t)(npairs > 0 ? npairs : 1);
+    return (size_t)kTanSumBytesPerPair * (size_t)(npairs > 0 ? npairs : 1);
 }
 // Pair records of the force path (launch_pair_forces -> launch_gather_self):
 // float3 (12 bytes) for the mixed-precision list kernel, double4 (32 bytes)

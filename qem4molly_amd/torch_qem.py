@@ -19,7 +19,7 @@ backward (no reference counterpart; the reference has forces only):
            returning the energy term alone).  `gamma` must equal the
            context's contrasts; it is only the handle autograd differentiates.
 With keep_sums (the default when xyz requires grad) the forward keeps the
-pair quadrature's tangent node sums (qem_forces_energy_save_dev, 256 B per
+pair quadrature's tangent node sums (qem_forces_energy_save_dev, 224 B per
 pair in mixed precision) and the backward's HVP combines them
 (qem_hvp_saved_dev) instead of running the quadrature again: about 1.3 ms
 more forward and 2.9 ms less backward at N = 100k.  The context keeps one set

@@ -17,7 +17,7 @@ memory stays under 1.6 GB (measured 1.49 GB: the endpoint records take
 32 bytes per pair, 0.5 GB at 15.7 million pairs).  At the end the same
 call keeping the tangent node sums (qem_forces_energy_save_dev) repeats
 its outputs bit for bit, the HVP from those sums (qem_hvp_saved_dev) is
-within 1e-6 of FP64, and the sums add at most 1.25 x 256 bytes per pair.
+within 1e-6 of FP64, and the sums add at most 1.25 x 224 bytes per pair.
 """
 import numpy as np
 import pytest
@@ -109,8 +109,8 @@ def test_adjoint_slab_n100k_perturbed_steps():
         print(f"saved-sums HVP vs fp64: {rel_rms(hs, hv64):.2e}, vs direct {rel_rms(hs, hv):.2e}",
               flush=True)
         assert rel_rms(hs, hv64) < 1e-6 and not np.array_equal(hs, hv)
-        # the sums: 256 bytes per pair, allocated with 25% headroom
+        # the sums: 224 bytes per pair, allocated with 25% headroom
         npairs = c.short_state()[0]
         mem_saved = c.device_bytes()
         print(f"device bytes with the saved sums: {mem_saved} ({npairs} pairs)", flush=True)
-        assert mem_saved <= mem + 1.25 * 256 * npairs + 4096
+        assert mem_saved <= mem + 1.25 * 224 * npairs + 4096
